@@ -1,0 +1,419 @@
+"""ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module, and only as the checker.  It wraps two
+libraries built by ``oracle/Makefile``:
+
+* ``liboracle.so`` -- the plain-C restatement of the reference path
+  (scalit_oracle.c, every routine cites the Fortran it follows);
+* ``_ref/libscalit_ref.so`` -- the reference's own LAPACK-free Fortran routines
+  compiled where they lie (built only where the reference sources exist; tests
+  that need it skip elsewhere).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libscalit_ref.so")
+
+FMAX = 8
+TA0, TA1 = 0, 1
+TOSB, TOSBD1, TOSBD2 = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcOSB(C.Structure):
+    _fields_ = [
+        ("sF", C.c_int),
+        ("sN", C.c_int * FMAX),
+        ("sDEP", C.c_int * FMAX),
+        ("myDim", C.c_int * FMAX),
+        ("myBlk", C.c_int * FMAX),
+        ("myLen", C.c_long),
+        ("H0", _dp * FMAX),
+        ("DEP", _dp * FMAX),
+        ("RES", _dp),
+        ("VOSB", _dp * FMAX),
+        ("EIG0", _dp),
+        ("E0", C.c_double),
+        ("DE", C.c_double),
+        ("beta", C.c_double),
+        ("osb_kind", C.c_int),
+        ("hc", C.c_int),
+        ("pc", C.c_int),
+    ]
+
+
+class OrcSolverCtx(C.Structure):
+    _fields_ = [
+        ("o", C.POINTER(OrcOSB)),
+        ("qmr_type", C.c_int),
+        ("qmr_max", C.c_int),
+        ("qmr_tol", C.c_double),
+        ("hx_calls", C.c_long),
+        ("px_calls", C.c_long),
+        ("solves", C.c_long),
+        ("qmr_iters", C.c_long),
+    ]
+
+
+OP_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_long, _dp, _dp)
+LS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_long, _dp, _dp, _dp)
+
+_lib = None
+_ref = None
+
+
+def lib():
+    """The oracle library; raises if it was not built (``make -C oracle``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.orc_dot.restype = C.c_double
+        _lib.orc_sum_off_diag.restype = C.c_double
+        _lib.orc_mjacobi.restype = C.c_double
+        _lib.orc_get_diff_max.restype = C.c_double
+        _lib.orc_get_sep_max.restype = C.c_double
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference checker library (lazy binding: see oracle/Makefile)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"reference checker missing: {REF_PATH} (run `make -C oracle ref`)")
+        _ref = C.CDLL(REF_PATH, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        for f in ("getdiffmax_", "getsepmax_", "mjacobi_", "sumoffdiag_"):
+            getattr(_ref, f).restype = C.c_double
+        for f in ("qmr_", "pisth0_", "lanh0_", "mgs_orth_", "gs_orth_", "lan_mgs_orth_", "refdrv_linsolv",
+                  "refdrv_diag"):
+            getattr(_ref, f).restype = C.c_int
+    return _ref
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags["C_CONTIGUOUS"] or a.flags["F_CONTIGUOUS"])
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32
+    return a.ctypes.data_as(_ip)
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def fmat(a) -> np.ndarray:
+    """Column-major (Fortran order) float64 copy."""
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# closed-form DVR pieces
+# --------------------------------------------------------------------------
+def sinc2_dvr(n: int, a0: float, b0: float, mass: float):
+    x0 = np.zeros(n)
+    h0 = np.zeros((n, n), order="F")
+    lib().orc_sinc2_dvr(C.c_int(n), C.c_double(a0), C.c_double(b0), C.c_double(mass), dptr(x0), dptr(h0))
+    return x0, h0
+
+
+def gau_legendre(n: int, x1: float = -1.0, x2: float = 1.0):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_gau_legendre(C.c_double(x1), C.c_double(x2), C.c_int(n), dptr(x), dptr(w))
+    return x, w
+
+
+def yj_polys(x: np.ndarray, l: int):
+    x = f64(x)
+    yj = np.zeros((len(x), l + 1), order="F")
+    lib().orc_yj_polys(C.c_int(len(x)), C.c_int(l), dptr(x), dptr(yj))
+    return yj
+
+
+def legendre_l2(n: int):
+    x = np.zeros(n)
+    l2 = np.zeros((n, n), order="F")
+    lib().orc_legendre_l2(C.c_int(n), dptr(x), dptr(l2))
+    return x, l2
+
+
+# --------------------------------------------------------------------------
+# small dense helpers
+# --------------------------------------------------------------------------
+def sym_eigvals(a: np.ndarray) -> np.ndarray:
+    a = fmat(a)
+    n = a.shape[0]
+    w = np.zeros(n)
+    lib().orc_sym_eigvals(C.c_int(n), dptr(a), C.c_int(n), dptr(w))
+    return w
+
+
+def tridiag_eigvals(d: np.ndarray, e: np.ndarray) -> np.ndarray:
+    d = f64(d)
+    e = f64(np.concatenate([e, [0.0]]))
+    w = np.zeros(len(d))
+    lib().orc_tridiag_eigvals(C.c_int(len(d)), dptr(d), dptr(e), dptr(w))
+    return w
+
+
+def get_window(e0: float, vec: np.ndarray, m: int):
+    vec = f64(vec)
+    ind = np.zeros(m, dtype=np.int32)
+    out = np.zeros(m)
+    lib().orc_get_window0(C.c_double(e0), C.c_int(len(vec)), dptr(vec), C.c_int(m), iptr(ind), dptr(out))
+    return ind, out
+
+
+def get_diff_max(v1, v2) -> float:
+    v1, v2 = f64(v1), f64(v2)
+    return lib().orc_get_diff_max(C.c_int(len(v1)), dptr(v1), dptr(v2))
+
+
+def get_sep_max(v1, v2) -> float:
+    v1, v2 = f64(v1), f64(v2)
+    return lib().orc_get_sep_max(C.c_int(len(v1)), dptr(v1), C.c_int(len(v2)), dptr(v2))
+
+
+def mgs_orth(wj: np.ndarray, vj: np.ndarray, m: int) -> tuple[bool, np.ndarray]:
+    """VJ(:, m+1) from WJ; vj is N x (>= m+1) column-major and is updated in place."""
+    n = vj.shape[0]
+    ok = lib().orc_mgs_orth(C.c_long(n), dptr(f64(wj)), C.c_int(m), dptr(vj))
+    return bool(ok), vj
+
+
+# --------------------------------------------------------------------------
+# the OSB operator
+# --------------------------------------------------------------------------
+class Operator:
+    """Direct-product DVR Hamiltonian with its OSB preconditioner.
+
+    ``h0`` is a list of per-level n x n matrices, ``dep`` a list with either
+    None or a vector of length myDim(level), ``res`` the diagonal potential on
+    the full grid (level 1 fastest)."""
+
+    def __init__(self, sn, h0, res, dep=None, e0: float = 0.0):
+        self.sF = len(sn)
+        self.sN = [int(v) for v in sn]
+        self.H0 = [fmat(h) for h in h0]
+        dep = dep if dep is not None else [None] * self.sF
+        self.DEP = [None if d is None else f64(d) for d in dep]
+        self.RES = f64(res).reshape(-1)
+        self.s = OrcOSB()
+        self.s.sF = self.sF
+        for i in range(self.sF):
+            self.s.sN[i] = self.sN[i]
+            self.s.sDEP[i] = 0 if self.DEP[i] is None else 1
+            self.s.H0[i] = dptr(self.H0[i])
+            if self.DEP[i] is not None:
+                self.s.DEP[i] = dptr(self.DEP[i])
+        lib().orc_cal_osb(C.byref(self.s))
+        self.N = int(self.s.myLen)
+        assert self.RES.size == self.N
+        self.s.RES = dptr(self.RES)
+        self.s.E0 = e0
+        self.s.DE = 1.0
+        self.VOSB = None
+        self.EIG0 = None
+
+    @property
+    def myDim(self):
+        return [self.s.myDim[i] for i in range(self.sF)]
+
+    @property
+    def myBlk(self):
+        return [self.s.myBlk[i] for i in range(self.sF)]
+
+    def set_e0(self, e0: float):
+        self.s.E0 = e0
+
+    def hx(self, x: np.ndarray, ntype: int = TA0) -> np.ndarray:
+        x = f64(x)
+        y = np.zeros(self.N)
+        lib().orc_myhx(C.byref(self.s), C.c_int(ntype), dptr(x), dptr(y))
+        return y
+
+    def hx_t(self, x: np.ndarray, ntype: int = TA0) -> np.ndarray:
+        x = f64(x)
+        y = np.zeros(self.N)
+        lib().orc_myhx_t(C.byref(self.s), C.c_int(ntype), dptr(x), dptr(y))
+        return y
+
+    def hx_omp(self, x: np.ndarray, y: np.ndarray, tmp: np.ndarray, ntype: int = TA0):
+        lib().orc_myhx_omp(C.byref(self.s), C.c_int(ntype), dptr(x), dptr(y), dptr(tmp))
+
+    def diag(self, bjmax: int = 1000, bjtol: float = 1e-5):
+        self.VOSB = [np.zeros(self.myBlk[l] * self.sN[l] ** 2) for l in range(self.sF)]
+        self.EIG0 = np.zeros(self.N)
+        for l in range(self.sF):
+            self.s.VOSB[l] = dptr(self.VOSB[l])
+        self.s.EIG0 = dptr(self.EIG0)
+        ok = lib().orc_my_diag(C.byref(self.s), C.c_int(bjmax), C.c_double(bjtol))
+        assert ok
+        return self.VOSB, self.EIG0
+
+    def set_precond(self, vosb, eig0):
+        self.VOSB = [f64(v) for v in vosb]
+        self.EIG0 = f64(eig0)
+        for l in range(self.sF):
+            self.s.VOSB[l] = dptr(self.VOSB[l])
+        self.s.EIG0 = dptr(self.EIG0)
+
+    def px(self, x: np.ndarray, npc: int = TA1, kind: int = TOSB) -> np.ndarray:
+        assert self.VOSB is not None
+        x = f64(x)
+        y = np.zeros(self.N)
+        lib().orc_mypx(C.byref(self.s), C.c_int(kind), C.c_int(npc), dptr(x), dptr(y))
+        return y
+
+    def _ctx(self, qmr_type, qmr_max, qmr_tol):
+        c = OrcSolverCtx()
+        c.o = C.pointer(self.s)
+        c.qmr_type, c.qmr_max, c.qmr_tol = qmr_type, qmr_max, qmr_tol
+        return c
+
+    def qmr(self, b, qmr_type: int = -1, qmr_max: int = 1000, qmr_tol: float = 1e-3, x0=None):
+        L = lib()
+        b = f64(b)
+        x = np.zeros(self.N) if x0 is None else f64(x0).copy()
+        res = C.c_double(0.0)
+        c = self._ctx(qmr_type, qmr_max, qmr_tol)
+        hx = C.cast(L.orc_cb_hx_e, OP_FN)
+        px = C.cast(L.orc_cb_px, OP_FN)
+        r = L.orc_qmr(C.c_int(qmr_type), C.c_int(qmr_max), C.c_double(qmr_tol), C.c_long(self.N), dptr(b), hx, px,
+                      C.byref(c), dptr(x), C.byref(res))
+        return r, x, res.value, c
+
+    def pist_conv(self, x, e0, etol, m, m_max, start, step, stepeig=1, ntype=0, qmr_type=-1, qmr_max=1000,
+                  qmr_tol=1e-3):
+        L = lib()
+        self.set_e0(e0)
+        x = f64(x)
+        eig = np.zeros(m)
+        res = C.c_double(0.0)
+        c = self._ctx(qmr_type, qmr_max, qmr_tol)
+        iters = C.c_int(0)
+        r = L.orc_pist_conv(C.c_double(e0), C.c_double(etol), C.c_int(ntype), C.c_long(self.N), dptr(x),
+                            C.c_int(start), C.c_int(step), C.c_int(stepeig), C.c_int(m), C.c_int(m_max),
+                            C.cast(L.orc_cb_hx0, OP_FN), C.cast(L.orc_cb_linsolv, LS_FN), C.byref(c), dptr(eig),
+                            C.byref(res), C.byref(iters))
+        return r, eig, res.value, c
+
+    def lan_conv(self, x, e0, etol, m, m_max, start, step, stepeig=1, ntype=0):
+        L = lib()
+        x = f64(x)
+        eig = np.zeros(m)
+        res = C.c_double(0.0)
+        c = self._ctx(-1, 1, 1.0)
+        r = L.orc_lan_conv(C.c_double(e0), C.c_double(etol), C.c_int(ntype), C.c_long(self.N), dptr(x),
+                           C.c_int(start), C.c_int(step), C.c_int(stepeig), C.c_int(m), C.c_int(m_max),
+                           C.cast(L.orc_cb_hx0, OP_FN), C.byref(c), dptr(eig), C.byref(res))
+        return r, eig, res.value
+
+    def dense(self) -> np.ndarray:
+        """The full N x N matrix (tiny grids only): column k is H e_k."""
+        n = self.N
+        m = np.zeros((n, n))
+        e = np.zeros(n)
+        for k in range(n):
+            e[:] = 0.0
+            e[k] = 1.0
+            m[:, k] = self.hx(e)
+        return m
+
+
+# --------------------------------------------------------------------------
+# reference checker wrappers (Fortran by-reference calling convention)
+# --------------------------------------------------------------------------
+def _ci(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _cd(v):
+    return C.byref(C.c_double(float(v)))
+
+
+class RefOperator:
+    """The same operator driven through the reference's compiled routines."""
+
+    def __init__(self, op: Operator):
+        self.op = op
+        R = ref()
+        sn = (C.c_int * FMAX)(*op.sN)
+        sdep = (C.c_int * FMAX)(*[0 if d is None else 1 for d in op.DEP])
+        h0 = (_dp * FMAX)(*[dptr(h) for h in op.H0])
+        dep = (_dp * FMAX)(*[dptr(d) if d is not None else _dp() for d in op.DEP])
+        R.refdrv_setup(C.c_int(op.sF), sn, sdep, h0, dep, dptr(op.RES), C.c_double(op.s.E0))
+        self.vosb = None
+        self.eig0 = None
+
+    def set_e0(self, e0):
+        op = self.op
+        op.set_e0(e0)
+        self.__init__(op)
+
+    def hx(self, x, shifted=False):
+        x = f64(x)
+        y = np.zeros_like(x)
+        f = ref().refdrv_hx if shifted else ref().refdrv_hx0
+        f(_ci(len(x)), dptr(x), dptr(y))
+        return y
+
+    def diag(self, bjmax=1000, bjtol=1e-5):
+        op = self.op
+        self.vosb = [np.zeros(op.myBlk[l] * op.sN[l] ** 2) for l in range(op.sF)]
+        self.eig0 = np.zeros(op.N)
+        vp = (_dp * FMAX)(*[dptr(v) for v in self.vosb])
+        ok = ref().refdrv_diag(C.c_int(bjmax), C.c_double(bjtol), vp, dptr(self.eig0))
+        assert ok
+        ref().refdrv_set_precond((_dp * FMAX)(*[dptr(v) for v in self.vosb]), dptr(self.eig0))
+        return self.vosb, self.eig0
+
+    def px(self, x):
+        x = f64(x)
+        y = np.zeros_like(x)
+        ref().refdrv_px(_ci(len(x)), dptr(x), dptr(y))
+        return y
+
+    def qmr(self, b, qmr_type=-1, qmr_max=1000, qmr_tol=1e-3):
+        R = ref()
+        b = f64(b)
+        x = np.zeros_like(b)
+        res = C.c_double(0.0)
+        FOP = C.CFUNCTYPE(None, _ip, _dp, _dp)
+        r = R.qmr_(_ci(qmr_type), _ci(qmr_max), _cd(qmr_tol), _ci(len(b)), dptr(b), C.cast(R.refdrv_hx, FOP),
+                   C.cast(R.refdrv_px, FOP), dptr(x), C.byref(res))
+        return r, x, res.value
+
+    def pist_h0(self, x, m_max, m_low, m_high, vj, hmat, qmr_type=-1, qmr_max=1000, qmr_tol=1e-3):
+        R = ref()
+        R.refdrv_set_qmr(C.c_int(qmr_type), C.c_int(qmr_max), C.c_double(qmr_tol))
+        FOP = C.CFUNCTYPE(None, _ip, _dp, _dp)
+        FLS = C.CFUNCTYPE(C.c_int, _ip, _dp, _dp, _dp)
+        x = f64(x)
+        return R.pisth0_(_ci(len(x)), dptr(x), _ci(m_max), _ci(m_low), _ci(m_high), C.cast(R.refdrv_hx0, FOP),
+                         C.cast(R.refdrv_linsolv, FLS), dptr(vj), dptr(hmat))
+
+    def lan_h0(self, x, vj, m_low, m_high, alpha, beta):
+        R = ref()
+        FOP = C.CFUNCTYPE(None, _ip, _dp, _dp)
+        x = f64(x)
+        return R.lanh0_(_ci(len(x)), dptr(x), dptr(vj), _ci(m_low), _ci(m_high), C.cast(R.refdrv_hx0, FOP),
+                        dptr(alpha), dptr(beta))

@@ -1,0 +1,180 @@
+"""Pin the oracle where no reference run is possible here, by what the
+mathematics fixes: brute-force Kronecker sums, separable spectra, symmetry,
+dense eigensolves, residual identities.  The PIST_CONV / Lan_Conv outer loops
+call LAPACK in the reference (DSYEV / DSTEV, not buildable in this image), so
+their eigenvalues are checked against dense numpy eigensolves instead."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import oracle as O
+from scalit_amd import problems as P
+
+
+def tiny(grid=(5, 6, 4), seed=3, kind="gauss"):
+    pot = P.gaussian_potential(seed) if kind == "gauss" else P.jittered_potential(seed)
+    return P.build_problem(*grid, pot)
+
+
+def kron_dense(prob):
+    n1, n2, n3 = prob.sN
+    t1, t2, l2 = prob.h0
+    i1, i2, i3 = np.eye(n1), np.eye(n2), np.eye(n3)
+    # vector index i1 + n1*(i2 + n2*i3): kron ordering is (i3, i2, i1)
+    h = np.kron(i3, np.kron(i2, t1)) + np.kron(i3, np.kron(t2, i1))
+    h = h + np.diag(np.tile(prob.dep[2], n3)) @ np.kron(l2, np.kron(i2, i1))
+    h = h + np.diag(prob.res)
+    return h
+
+
+def test_hx_equals_kronecker_sum():
+    prob = tiny()
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    h = kron_dense(prob)
+    assert np.allclose(op.dense(), h, rtol=0, atol=1e-15 * np.abs(h).max())
+
+
+def test_hx_shift_and_symmetry():
+    prob = tiny(grid=(7, 6, 5))
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=-0.03)
+    g = np.random.default_rng(1)
+    x, y = g.standard_normal((2, op.N))
+    hx, hy = op.hx(x), op.hx(y)
+    assert abs(x @ hy - y @ hx) <= 1e-13 * abs(x @ hy)
+    assert np.allclose(op.hx(x, O.TA1), hx + 0.03 * x, rtol=1e-14, atol=1e-16)
+    assert np.allclose(op.hx_t(x), hx, rtol=1e-13, atol=1e-17)
+
+
+def test_separable_spectrum():
+    # DEP constant and V separable: H is a Kronecker sum, eigenvalues are sums of 1-D ones
+    n = (5, 4, 3)
+    g = np.random.default_rng(2)
+    mats = []
+    for k in n:
+        a = g.standard_normal((k, k))
+        mats.append(a + a.T)
+    dep = [None, None, np.full(n[0] * n[1], 0.7)]
+    op = O.Operator(n, mats, np.zeros(int(np.prod(n))), dep)
+    ev = np.linalg.eigvalsh(op.dense())
+    e1, e2, e3 = (np.linalg.eigvalsh(m) for m in mats)
+    e3 = 0.7 * e3
+    expect = np.sort((e1[:, None, None] + e2[None, :, None] + e3[None, None, :]).reshape(-1))
+    assert np.allclose(ev, expect, atol=1e-12)
+
+
+def test_sym_eigvals_against_lapack():
+    g = np.random.default_rng(5)
+    for n in (1, 2, 7, 40, 120):
+        a = g.standard_normal((n, n))
+        a = a + a.T
+        w = O.sym_eigvals(a)
+        w_ref = scipy.linalg.eigvalsh(a)
+        assert np.allclose(w, w_ref, rtol=0, atol=1e-13 * max(1.0, abs(w_ref).max()))
+    d = g.standard_normal(30)
+    e = g.standard_normal(29)
+    w = O.tridiag_eigvals(d, e)
+    w_ref = scipy.linalg.eigvalsh_tridiagonal(d, e)
+    assert np.allclose(w, w_ref, atol=1e-13)
+
+
+def test_osb_preconditioner_identity():
+    # V (EIG0-E0)^-1 V^T with the rotations V orthogonal: P^-1 = V diag(EIG0-E0) V^T
+    prob = tiny(grid=(6, 5, 4))
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=-0.5)
+    op.diag(bjmax=200, bjtol=1e-12)
+    x = np.random.default_rng(3).standard_normal(op.N)
+    y = op.px(x)
+    # orthogonality of each level's rotations
+    for l in range(3):
+        n = op.sN[l]
+        v = op.VOSB[l].reshape(op.myBlk[l], n, n)
+        for b in range(op.myBlk[l]):
+            vb = v[b].T   # column-major block
+            assert np.allclose(vb.T @ vb, np.eye(n), atol=1e-13)
+    # P is symmetric positive definite for E0 below the spectrum of the OSB diagonal
+    z = np.random.default_rng(4).standard_normal(op.N)
+    assert abs(z @ op.px(x) - x @ op.px(z)) <= 1e-12 * abs(z @ op.px(x))
+    assert x @ y > 0
+
+
+def test_osb_diag_approximates_spectrum():
+    # with tight Jacobi convergence the OSB diagonal's lowest value bounds the true ground state from above
+    prob = tiny(grid=(6, 5, 4), kind="jitter")
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    _, eig0 = op.diag(bjmax=500, bjtol=1e-14)
+    ev = np.linalg.eigvalsh(kron_dense(prob))
+    assert eig0.min() >= ev[0] - 1e-12
+
+
+@pytest.mark.parametrize("qmr_type", [-1, 0, 2])
+def test_qmr_solves(qmr_type):
+    prob = tiny(grid=(8, 7, 6))
+    e0 = -0.02
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=e0)
+    op.diag()
+    b = np.random.default_rng(6).standard_normal(op.N)
+    r, x, res, _ = op.qmr(b, qmr_type=qmr_type, qmr_max=500, qmr_tol=1e-10)
+    assert r > 0
+    h = kron_dense(prob) - e0 * np.eye(op.N)
+    assert np.linalg.norm(h @ x - b) <= 1e-8 * np.linalg.norm(b)
+
+
+def test_qmr_breakdowns_and_limits():
+    prob = tiny(grid=(6, 5, 4))
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=-0.02)
+    op.diag()
+    # zero right-hand side: VNRM == 0, the loop never runs
+    r, x, _, _ = op.qmr(np.zeros(op.N), qmr_type=-1, qmr_max=50, qmr_tol=1e-10)
+    assert r == 0 and not x.any()
+    # too few iterations: -QMRMAX
+    b = np.random.default_rng(7).standard_normal(op.N)
+    r, _, _, _ = op.qmr(b, qmr_type=-1, qmr_max=2, qmr_tol=1e-14)
+    assert r == -2
+
+
+def test_pist_conv_against_dense():
+    prob = tiny(grid=(8, 9, 6), kind="jitter")
+    h = kron_dense(prob)
+    ev = np.linalg.eigvalsh(h)
+    e0 = float(prob.res.min())
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=e0)
+    op.diag()
+    x = P.random_vectors(op.N, 1, 8)[0]
+    m = 6
+    r, eig, res, ctx = op.pist_conv(x, e0, 1e-10, m, m_max=60, start=2 * m, step=4, stepeig=2, qmr_tol=1e-10)
+    assert r > 0, r
+    assert np.allclose(eig, ev[:m], rtol=1e-10, atol=0)
+    assert ctx.solves > 0
+
+
+def test_pist_conv_parameter_errors():
+    prob = tiny()
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    op.diag()
+    x = np.ones(op.N)
+    assert op.pist_conv(x, 0.0, 1e-8, m=5, m_max=5, start=2, step=1, stepeig=5)[0] == 0
+    assert op.pist_conv(x, 0.0, 1e-8, m=6, m_max=5, start=2, step=1)[0] == 0
+
+
+def test_lan_conv_against_dense():
+    prob = tiny(grid=(8, 9, 6), kind="jitter")
+    ev = np.linalg.eigvalsh(kron_dense(prob))
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    x = P.random_vectors(op.N, 1, 9)[0]
+    m = 5
+    r, eig, res = op.lan_conv(x, float(ev[0]) - 0.01, 1e-11, m, m_max=300, start=40, step=20, stepeig=5)
+    assert r > 0, r
+    assert np.allclose(eig, ev[:m], rtol=1e-10, atol=0)
+
+
+def test_mgs_orthogonality_criterion():
+    # the reference's own test criterion: max |V^T V - I| (orth/testing/gsorth.drv.f90)
+    g = np.random.default_rng(8)
+    n, m = 200, 30
+    vj = np.zeros((n, m), order="F")
+    v0 = g.standard_normal(n)
+    vj[:, 0] = v0 / np.linalg.norm(v0)
+    for j in range(1, m):
+        ok, _ = O.mgs_orth(g.standard_normal(n), vj, j)
+        assert ok
+    assert np.abs(vj.T @ vj - np.eye(m)).max() < 1e-14
