@@ -1,0 +1,215 @@
+/*
+ * scalit.h -- C ABI of the MI355X implementation of ScalIT's iterate hot path:
+ * the H*X product of a direct-product DVR Hamiltonian, the OSB block
+ * preconditioner P*X, and the QMR / PIST / Lanczos iterations that use them.
+ *
+ * Two layers:
+ *
+ * 1. Handle API (scalit_*): a context owns the operator on one GPU (it replaces
+ *    the reference's module globals H0/RES/DEP/VOSB/EIG0,
+ *    src/iterate/module/osb.data.h:28-69).  Vector arguments are DEVICE
+ *    pointers; nvec vectors of length N are stored with leading dimension ld.
+ *    Kernels are enqueued on the given HIP stream (NULL = the context's own).
+ *
+ * 2. Fortran drop-in layer (lower case + trailing underscore, all arguments by
+ *    reference, HOST arrays, column-major, 1-based semantics): the callbacks
+ *    and external library functions the reference's solvers bind.  They act on
+ *    the context made current with scalit_bind().
+ *
+ * Return codes: the reference's own conventions where a reference function is
+ * replaced (QMR: i / -i / 0 / -1; PIST_CONV: cnt / -cnt / 0; ...).  Handle
+ * API functions return SCALIT_OK (0) or a negative SCALIT_E* code; they never
+ * throw across the ABI.
+ */
+#ifndef SCALIT_H
+#define SCALIT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCALIT_FMAX 8
+
+#define SCALIT_OK 0
+#define SCALIT_EINVAL (-1001)   /* bad argument */
+#define SCALIT_EHIP (-1002)     /* HIP runtime error */
+#define SCALIT_ENOMEM (-1003)   /* device allocation failed */
+#define SCALIT_ESTATE (-1004)   /* operator incomplete (e.g. no preconditioner yet) */
+
+/* H*X shift selector: src/iterate/module/osb.hx.h:82-89 */
+#define SCALIT_TA0 0   /* Y = H X */
+#define SCALIT_TA1 1   /* Y = (H - E0) X */
+/* preconditioner kernels: src/iterate/module/osb.px.h:156-211 */
+#define SCALIT_TOSB 0
+#define SCALIT_TOSBD1 1
+#define SCALIT_TOSBD2 2
+/* arithmetic of an operator application */
+#define SCALIT_FP64 0
+#define SCALIT_FP32 1
+#define SCALIT_BF16 2  /* vectors stored in bf16, arithmetic in fp32 */
+
+typedef struct scalit_ctx scalit_ctx;
+typedef void *scalit_stream; /* hipStream_t */
+
+/* ------------------------------------------------------------------------ */
+/* context and operator data                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces initOSB/calOSB (src/iterate/module/osb.f90:17, osb.init.h:2-50):
+ * sF levels of sizes sN[0..sF-1] (level 1 fastest), sDEP[l] != 0 when level
+ * l+1 is multiplied by the coordinate-dependent factor DEP (sDEP[0] must be 0,
+ * src/iterate/module/mosb.print.h:241-242).  Allocates nothing large yet. */
+int scalit_create(scalit_ctx **ctx, int sF, const int *sN, const int *sDEP, int device);
+int scalit_destroy(scalit_ctx *ctx);
+int scalit_get_len(const scalit_ctx *ctx, long *n);
+
+/* Operator pieces, host arrays (replaces loadH0/loadRES/loadDep,
+ * src/iterate/module/osb.io.h:2-60):  h0 is n x n column-major for level
+ * `level` (1-based); dep has myDim(level) = prod(sN[0..level-2]) entries; res
+ * has N entries. */
+int scalit_set_h0(scalit_ctx *ctx, int level, const double *h0);
+int scalit_set_dep(scalit_ctx *ctx, int level, const double *dep);
+int scalit_set_res(scalit_ctx *ctx, const double *res);
+/* sOSBW%mE0 / mDE / mBeta (src/iterate/module/a_osbtype.f90:26-29) */
+int scalit_set_shift(scalit_ctx *ctx, double e0, double de, double beta);
+
+/* OSB set-up on the GPU: myDiag (src/iterate/module/osb.diag.h:53-151) with
+ * HosbDiag/MJACOBI (src/iterate/libiterate/diag/hosbdiag.f90:5-75,
+ * jacobi.f90:2-83).  Leaves VOSB and EIG0 resident. */
+int scalit_block_diag(scalit_ctx *ctx, int bjmax, double bjtol);
+/* Load / fetch VOSB and EIG0 (host arrays, the sVOSB<0 / >0 file modes,
+ * src/iterate/module/osb.progDiag.h:6-25): vosb[l] holds myBlk(l)*n_l^2. */
+int scalit_set_precond(scalit_ctx *ctx, const double *const *vosb, const double *eig0);
+int scalit_get_precond(scalit_ctx *ctx, double *const *vosb, double *eig0);
+
+/* ------------------------------------------------------------------------ */
+/* operator applications (device pointers)                                  */
+/* ------------------------------------------------------------------------ */
+
+/* MYHX (src/iterate/module/osb.hx.h:71-110):
+ * Y_b = (RES - shift_b) o X_b + sum_l (I x H0_l [x DEP_l] x I) X_b.
+ * ntype TA0: shift 0; TA1: shift E0.  `shifts` (device, nvec doubles) when not
+ * NULL overrides the shift per vector (used for batched PIST windows). */
+int scalit_hx(scalit_ctx *ctx, int ntype, int nvec, const double *x, long ldx, double *y, long ldy,
+              const double *shifts, scalit_stream stream);
+/* The same product in fp32 (prec SCALIT_FP32: float vectors) or with bf16
+ * storage (SCALIT_BF16: unsigned short vectors), fp32 accumulation. */
+int scalit_hx_lp(scalit_ctx *ctx, int prec, int ntype, int nvec, const void *x, long ldx, void *y, long ldy,
+                 const float *shifts, scalit_stream stream);
+/* Adjoint of H*X for gradients: dx_b = H^T g_b (H0 transposed per level; RES
+ * and DEP diagonal) and, when dv != NULL, dv += sum_b x_b o g_b. */
+int scalit_hx_adjoint(scalit_ctx *ctx, int ntype, int nvec, const double *x, const double *g, long ld,
+                      double *dx, double *dv, scalit_stream stream);
+
+/* MYPX (src/iterate/module/osb.px.h:125-219): Y = V (EIG0 - E0)^-1 V^T X with
+ * the TOSB / TOSBD1 / TOSBD2 kernels; npc TA0 uses E0 = 0. */
+int scalit_px(scalit_ctx *ctx, int nosb, int npc, int nvec, const double *x, long ldx, double *y, long ldy,
+              const double *shifts, scalit_stream stream);
+int scalit_px_lp(scalit_ctx *ctx, int prec, int npc, int nvec, const void *x, long ldx, void *y, long ldy,
+                 const float *shifts, scalit_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* solvers on the context's operator (device vectors)                       */
+/* ------------------------------------------------------------------------ */
+
+/* QMR (src/iterate/libiterate/qmr/qmr.f90:5-175) on (H - E0) with the OSB
+ * preconditioner, for nvec right-hand sides at once (shifts per vector when
+ * not NULL; host array).  ret[b] gets the reference's return code for vector
+ * b, res[b] its final residual.  Returns SCALIT_OK or an error code. */
+int scalit_qmr(scalit_ctx *ctx, int ntype, int qmrmax, double qmrtol, int nvec, const double *b, double *x,
+               long ld, const double *shifts, int *ret, double *res, scalit_stream stream);
+/* Mixed-precision QMR: inner iterations in fp32 (prec SCALIT_FP32) or with
+ * bf16-stored vectors (SCALIT_BF16), outer fp64 iterative refinement until
+ * ||b - (H-E0)x|| <= tol ||b||.  ret[b] as scalit_qmr (total inner steps). */
+int scalit_qmr_mixed(scalit_ctx *ctx, int prec, int qmrmax, double qmrtol, int inner_max, double inner_tol, int nvec,
+                     const double *b, double *x, long ld, const double *shifts, int *ret, double *res,
+                     scalit_stream stream);
+
+/* PIST_CONV (src/iterate/libiterate/pist/pist.f90:7-98) with pistH0
+ * (pistHij.f90:328-383), LinSolv = OSB QMR (src/iterate/module/osbw.qmr.h:13),
+ * H0X = H (TA0).  x is the device start vector; eig (host, m entries)
+ * receives the eigenvalues ascending.  The return value is the reference's.
+ * prec selects the inner solve (SCALIT_FP64 = plain QMR, FP32/BF16 = mixed
+ * with fp64 refinement); stats (host, 4 ints, may be NULL): total QMR
+ * iterations, linear solves, final Krylov size, H*X applications. */
+int scalit_pist_conv(scalit_ctx *ctx, double e0, double etol, int ntype, const double *x, int startcnt, int stepcnt,
+                     int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int prec, double *eig,
+                     double *res, int *stats);
+/* Batched PIST: nwin windows with their own E0 (host array) and start
+ * vectors (device, leading dimension ld), solved in lock-step so that every
+ * QMR iteration is one batched H*X / P*X launch.  eig is nwin x m (host),
+ * ret / res per window. */
+int scalit_pist_conv_batched(scalit_ctx *ctx, int nwin, const double *e0, double etol, int ntype, const double *x,
+                             long ld, int startcnt, int stepcnt, int stepeig, int m, int m_max, int qmr_type,
+                             int qmrmax, double qmrtol, int prec, double *eig, int *ret, double *res, int *stats);
+/* Ritz vectors of the last PIST solve of window `win` for its m eigenvalues:
+ * psi (device, N x m, leading dimension ld), unit norm. */
+int scalit_pist_vectors(scalit_ctx *ctx, int win, double *psi, long ld);
+
+/* Lan_Conv (src/iterate/libiterate/lanczos/lan.f90:7-105) with lanH0
+ * (lanHij.f90:6-45): FP64 Lanczos with full re-orthogonalisation. */
+int scalit_lan_conv(scalit_ctx *ctx, double e0, double etol, int ntype, const double *x, int startcnt, int stepcnt,
+                    int stepeig, int m, int m_max, double *eig, double *res);
+
+/* Hellmann-Feynman gradients: dE_k/dp_j = sum_grid psi_k^2 dV/dp_j.
+ * psi device N x nk (ld), dvdp device N x np (lddv); out host nk x np
+ * (row-major).  Uses the adjoint product's dV = x o g with x = g = psi_k. */
+int scalit_eig_gradients(scalit_ctx *ctx, int nk, const double *psi, long ld, int np, const double *dvdp, long lddv,
+                         double *out);
+
+/* ------------------------------------------------------------------------ */
+/* Fortran drop-in layer (host arrays, arguments by reference)               */
+/* ------------------------------------------------------------------------ */
+
+/* Make ctx the operator behind hx_/px_/... (replaces the module state). */
+int scalit_bind(scalit_ctx *ctx);
+
+typedef void (*scalit_fop)(const int *n, const double *x, double *y);
+typedef int (*scalit_flinsolv)(const int *n, const double *b, double *x, double *res);
+
+/* HX(N,X,Y) with sHC (src/iterate/module/osb.hx.h:13-20), HX0 (TA0, :41-48),
+ * EHX (TA1, :56-63) */
+void hx_(const int *n, const double *x, double *y);
+void hx0_(const int *n, const double *x, double *y);
+void ehx_(const int *n, const double *x, double *y);
+/* PX(N,X,Y) with sOSB/sPC (src/iterate/module/osb.px.h:12-19), EPX (TA1) */
+void px_(const int *n, const double *x, double *y);
+void epx_(const int *n, const double *x, double *y);
+/* OSB_QMR(N,B,X,RES): the PIST LinSolv callback (src/iterate/module/osbw.qmr.h:13-23) */
+int osb_qmr_(const int *n, const double *b, double *x, double *eres);
+/* Set the QMR parameters OSB_QMR uses (sQMRConvType, sQMR%mMax, sQMR%mTol). */
+void scalit_set_qmr_params(int qmr_type, int qmrmax, double qmrtol);
+
+/* QMR(nTYPE,QMRMAX,QMRTOL,N,B,HX,PX,X,RES) (src/iterate/libiterate/qmr/qmr.f90:5).
+ * When HX/PX are this library's own callbacks the whole solve stays on the
+ * GPU; other callbacks are called on host copies of the vectors. */
+int qmr_(const int *ntype, const int *qmrmax, const double *qmrtol, const int *n, const double *b, scalit_fop hx,
+         scalit_fop px, double *x, double *res);
+/* PIST_CONV(E0,ETOL,NTYPE,N,X,STARTCNT,STEPCNT,STEPEIG,M,M_MAX,H0X,LinSolv,EIG,RES)
+ * (src/iterate/libiterate/pist/pist.f90:7) */
+int pist_conv_(const double *e0, const double *etol, const int *ntype, const int *n, const double *x,
+               const int *startcnt, const int *stepcnt, const int *stepeig, const int *m, const int *m_max,
+               scalit_fop h0x, scalit_flinsolv linsolv, double *eig, double *res);
+/* Lan_Conv(E0,ETOL,nType,N,X,startCnt,stepCnt,stepEig,M,M_MAX,HX,EIG,RES)
+ * (src/iterate/libiterate/lanczos/lan.f90:7) */
+int lan_conv_(const double *e0, const double *etol, const int *ntype, const int *n, const double *x,
+              const int *startcnt, const int *stepcnt, const int *stepeig, const int *m, const int *m_max,
+              scalit_fop hx, double *eig, double *res);
+/* MGS_ORTH(N,WJ,M,VJ) (src/iterate/libiterate/orth/gsorth.f90:42): returns 1/0 */
+int mgs_orth_(const int *n, const double *wj, const int *m, double *vj);
+
+/* ------------------------------------------------------------------------ */
+/* test and profiling hooks (no counterpart in the reference)                */
+/* ------------------------------------------------------------------------ */
+/* Force (on != 0) the generic any-level H*X kernels instead of the fused
+ * 3-level path, for parity tests of both. */
+int scalit_debug_force_generic(int on);
+/* Launch one pass of the fused FP64 H*X alone (1: modes 1+2 and the diagonal,
+ * 2: mode 3 accumulate) so that each kernel can be timed by itself. */
+int scalit_profile_hx_pass(scalit_ctx *ctx, int pass, int nvec, const double *x, long ldx, double *y, long ldy,
+                           scalit_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

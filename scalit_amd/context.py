@@ -1,0 +1,242 @@
+"""Python front end of the C ABI: a `Context` owns one operator on one GPU.
+
+Vectors are torch tensors on the GPU of shape (nvec, N), contiguous, with the
+grid's level 1 as the fastest index; torch is only the allocator and stream
+provider here, every operation runs in libscalit_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._lib import check
+
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _np_dp(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Context:
+    def __init__(self, sN, sDEP, device: int = 0):
+        self.sN = [int(n) for n in sN]
+        self.sF = len(self.sN)
+        self.sDEP = [1 if d else 0 for d in sDEP]
+        self.device = device
+        self._h = C.c_void_p()
+        sn = (C.c_int * len(self.sN))(*self.sN)
+        sd = (C.c_int * len(self.sN))(*self.sDEP)
+        check(L.lib().scalit_create(C.byref(self._h), self.sF, sn, sd, device), "scalit_create")
+        n = C.c_long()
+        check(L.lib().scalit_get_len(self._h, C.byref(n)))
+        self.N = n.value
+        self.dim = [int(np.prod(self.sN[:l])) for l in range(self.sF)]
+        self.blk = [int(np.prod(self.sN[l + 1:])) for l in range(self.sF)]
+        self.dev = torch.device("cuda", device)
+
+    @classmethod
+    def from_problem(cls, prob, device: int = 0, e0: float = 0.0):
+        ctx = cls(prob.sN, [d is not None for d in prob.dep], device)
+        for l, h in enumerate(prob.h0):
+            ctx.set_h0(l + 1, h)
+        for l, d in enumerate(prob.dep):
+            if d is not None:
+                ctx.set_dep(l + 1, d)
+        ctx.set_res(prob.res)
+        ctx.set_shift(e0)
+        return ctx
+
+    def close(self):
+        if self._h:
+            L.lib().scalit_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ------------------------------------------------------------ operator data
+    def set_h0(self, level: int, h0: np.ndarray):
+        a = np.asfortranarray(h0, dtype=np.float64)
+        check(L.lib().scalit_set_h0(self._h, level, _np_dp(a)), "scalit_set_h0")
+
+    def set_dep(self, level: int, dep: np.ndarray):
+        a = np.ascontiguousarray(dep, dtype=np.float64)
+        check(L.lib().scalit_set_dep(self._h, level, _np_dp(a)), "scalit_set_dep")
+
+    def set_res(self, res: np.ndarray):
+        a = np.ascontiguousarray(res, dtype=np.float64)
+        check(L.lib().scalit_set_res(self._h, _np_dp(a)), "scalit_set_res")
+
+    def set_shift(self, e0: float, de: float = 1.0, beta: float = 0.0):
+        self.e0 = e0
+        check(L.lib().scalit_set_shift(self._h, e0, de, beta))
+
+    def block_diag(self, bjmax: int = 1000, bjtol: float = 1e-5):
+        check(L.lib().scalit_block_diag(self._h, bjmax, bjtol), "scalit_block_diag")
+
+    def get_precond(self):
+        vosb = [np.zeros(self.blk[l] * self.sN[l] ** 2) for l in range(self.sF)]
+        eig0 = np.zeros(self.N)
+        arr = (_dp * self.sF)(*[_np_dp(v) for v in vosb])
+        check(L.lib().scalit_get_precond(self._h, arr, _np_dp(eig0)), "scalit_get_precond")
+        return vosb, eig0
+
+    def set_precond(self, vosb, eig0):
+        vs = [np.ascontiguousarray(v, dtype=np.float64) for v in vosb]
+        e = np.ascontiguousarray(eig0, dtype=np.float64)
+        arr = (_dp * self.sF)(*[_np_dp(v) for v in vs])
+        check(L.lib().scalit_set_precond(self._h, arr, _np_dp(e)), "scalit_set_precond")
+
+    # ------------------------------------------------------------ vectors
+    def vectors(self, nvec: int, dtype=torch.float64):
+        return torch.empty((nvec, self.N), dtype=dtype, device=self.dev)
+
+    def to_device(self, a: np.ndarray, dtype=torch.float64):
+        a = np.atleast_2d(a)
+        return torch.from_numpy(np.ascontiguousarray(a)).to(device=self.dev, dtype=dtype)
+
+    def _shifts(self, shifts, dtype):
+        if shifts is None:
+            return None
+        return torch.as_tensor(np.asarray(shifts), dtype=dtype, device=self.dev).contiguous()
+
+    def hx(self, x: torch.Tensor, ntype: int = L.TA0, shifts=None, out=None):
+        x = x.contiguous()
+        nv = x.shape[0]
+        y = torch.empty_like(x) if out is None else out
+        sh = self._shifts(shifts, torch.float64)
+        check(L.lib().scalit_hx(self._h, ntype, nv, _ptr(x), self.N, _ptr(y), self.N, _ptr(sh), _stream()),
+              "scalit_hx")
+        return y
+
+    def hx_lp(self, x: torch.Tensor, ntype: int = L.TA0, shifts=None, out=None):
+        prec = {torch.float32: L.FP32, torch.bfloat16: L.BF16}[x.dtype]
+        x = x.contiguous()
+        y = torch.empty_like(x) if out is None else out
+        sh = self._shifts(shifts, torch.float32)
+        check(L.lib().scalit_hx_lp(self._h, prec, ntype, x.shape[0], _ptr(x), self.N, _ptr(y), self.N, _ptr(sh),
+                                   _stream()), "scalit_hx_lp")
+        return y
+
+    def hx_adjoint(self, x: torch.Tensor, g: torch.Tensor, ntype: int = L.TA0, with_dv: bool = True):
+        g = g.contiguous()
+        x = x.contiguous()
+        dx = torch.empty_like(g)
+        dv = torch.zeros(self.N, dtype=torch.float64, device=self.dev) if with_dv else None
+        check(L.lib().scalit_hx_adjoint(self._h, ntype, g.shape[0], _ptr(x), _ptr(g), self.N, _ptr(dx), _ptr(dv),
+                                        _stream()), "scalit_hx_adjoint")
+        return dx, dv
+
+    def px(self, x: torch.Tensor, nosb: int = L.TOSB, npc: int = L.TA1, shifts=None, out=None):
+        x = x.contiguous()
+        y = torch.empty_like(x) if out is None else out
+        sh = self._shifts(shifts, torch.float64)
+        check(L.lib().scalit_px(self._h, nosb, npc, x.shape[0], _ptr(x), self.N, _ptr(y), self.N, _ptr(sh),
+                                _stream()), "scalit_px")
+        return y
+
+    def px_lp(self, x: torch.Tensor, npc: int = L.TA1, shifts=None, out=None):
+        prec = {torch.float32: L.FP32, torch.bfloat16: L.BF16}[x.dtype]
+        x = x.contiguous()
+        y = torch.empty_like(x) if out is None else out
+        sh = self._shifts(shifts, torch.float32)
+        check(L.lib().scalit_px_lp(self._h, prec, npc, x.shape[0], _ptr(x), self.N, _ptr(y), self.N, _ptr(sh),
+                                   _stream()), "scalit_px_lp")
+        return y
+
+    # ------------------------------------------------------------ solvers
+    def qmr(self, b: torch.Tensor, ntype: int = -1, qmrmax: int = 1000, qmrtol: float = 1e-3, shifts=None, x0=None):
+        b = b.contiguous()
+        nv = b.shape[0]
+        x = torch.zeros_like(b) if x0 is None else x0.clone().contiguous()
+        ret = (C.c_int * nv)()
+        res = (C.c_double * nv)()
+        sh = None if shifts is None else (C.c_double * nv)(*[float(s) for s in shifts])
+        check(L.lib().scalit_qmr(self._h, ntype, qmrmax, qmrtol, nv, _ptr(b), _ptr(x), self.N, sh, ret, res,
+                                 _stream()), "scalit_qmr")
+        return np.array(ret[:]), x, np.array(res[:])
+
+    def qmr_mixed(self, b: torch.Tensor, prec: int = L.FP32, qmrmax: int = 10, qmrtol: float = 1e-10,
+                  inner_max: int = 1000, inner_tol: float = 1e-5, shifts=None):
+        b = b.contiguous()
+        nv = b.shape[0]
+        x = torch.zeros_like(b)
+        ret = (C.c_int * nv)()
+        res = (C.c_double * nv)()
+        sh = None if shifts is None else (C.c_double * nv)(*[float(s) for s in shifts])
+        check(L.lib().scalit_qmr_mixed(self._h, prec, qmrmax, qmrtol, inner_max, inner_tol, nv, _ptr(b), _ptr(x),
+                                       self.N, sh, ret, res, _stream()), "scalit_qmr_mixed")
+        return np.array(ret[:]), x, np.array(res[:])
+
+    def pist_conv(self, x0: torch.Tensor, e0: float, etol: float, m: int, m_max: int, startcnt: int, stepcnt: int,
+                  stepeig: int = 1, ntype: int = 0, qmr_type: int = -1, qmrmax: int = 1000, qmrtol: float = 1e-3,
+                  prec: int = L.FP64):
+        x0 = x0.contiguous().reshape(-1)
+        eig = (C.c_double * m)()
+        res = C.c_double()
+        stats = (C.c_int * 4)()
+        r = L.lib().scalit_pist_conv(self._h, e0, etol, ntype, _ptr(x0), startcnt, stepcnt, stepeig, m, m_max,
+                                     qmr_type, qmrmax, qmrtol, prec, eig, C.byref(res), stats)
+        if r <= L.SCALIT_EINVAL and r >= L.SCALIT_ESTATE:
+            check(r, "scalit_pist_conv")
+        return r, np.array(eig[:]), res.value, np.array(stats[:])
+
+    def pist_conv_batched(self, x0: torch.Tensor, e0s, etol: float, m: int, m_max: int, startcnt: int, stepcnt: int,
+                          stepeig: int = 1, ntype: int = 0, qmr_type: int = -1, qmrmax: int = 1000,
+                          qmrtol: float = 1e-3, prec: int = L.FP64):
+        x0 = x0.contiguous()
+        nw = x0.shape[0]
+        e = (C.c_double * nw)(*[float(v) for v in e0s])
+        eig = (C.c_double * (nw * m))()
+        ret = (C.c_int * nw)()
+        res = (C.c_double * nw)()
+        stats = (C.c_int * 4)()
+        check(L.lib().scalit_pist_conv_batched(self._h, nw, e, etol, ntype, _ptr(x0), self.N, startcnt, stepcnt,
+                                               stepeig, m, m_max, qmr_type, qmrmax, qmrtol, prec, eig, ret, res,
+                                               stats), "scalit_pist_conv_batched")
+        return np.array(ret[:]), np.array(eig[:]).reshape(nw, m), np.array(res[:]), np.array(stats[:])
+
+    def pist_vectors(self, win: int, m: int):
+        psi = torch.empty((m, self.N), dtype=torch.float64, device=self.dev)
+        check(L.lib().scalit_pist_vectors(self._h, win, _ptr(psi), self.N), "scalit_pist_vectors")
+        return psi
+
+    def lan_conv(self, x0: torch.Tensor, e0: float, etol: float, m: int, m_max: int, startcnt: int, stepcnt: int,
+                 stepeig: int = 1, ntype: int = 0):
+        x0 = x0.contiguous().reshape(-1)
+        eig = (C.c_double * m)()
+        res = C.c_double()
+        r = L.lib().scalit_lan_conv(self._h, e0, etol, ntype, _ptr(x0), startcnt, stepcnt, stepeig, m, m_max, eig,
+                                    C.byref(res))
+        return r, np.array(eig[:]), res.value
+
+    def eig_gradients(self, psi: torch.Tensor, dvdp: torch.Tensor):
+        psi = psi.contiguous()
+        dvdp = dvdp.contiguous()
+        nk, np_ = psi.shape[0], dvdp.shape[0]
+        out = np.zeros((nk, np_))
+        check(L.lib().scalit_eig_gradients(self._h, nk, _ptr(psi), self.N, np_, _ptr(dvdp), self.N, _np_dp(out)),
+              "scalit_eig_gradients")
+        return out
+
+    def bind(self):
+        check(L.lib().scalit_bind(self._h))

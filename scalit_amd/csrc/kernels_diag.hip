@@ -1,0 +1,573 @@
+// GPU set-up of the OSB preconditioner and the small dense eigensolves.
+//
+// OSB set-up: myDiag (src/iterate/module/osb.diag.h:53-151) for levels sF..1:
+//   HOSB(d,k,j,b) = H0(k,j) [* DEP(d)] + delta_kj EIG0(d,k,b)
+//     (src/iterate/libiterate/diag/hinitxyz.f90:7-29, hinitdep.f90:10-34)
+//   joint Jacobi sweeps per block b with one rotation angle per pair (p,q)
+//   shared by all inner points d (MJACOBI, jacobi.f90:2-83), pairs in the
+//   reference's order (i = 1..MAK-1, j = 1..MAK-i, pair (j, j+i)), until
+//   |d offdiag| <= tol * initial or BJMAX sweeps (hosbdiag.f90:5-75);
+//   VOSB accumulates the rotations (vupdate.f90:21-36), EIG0 <- diag(HOSB).
+//
+// Level 1 (NDK = 1): one wave per block runs the whole Jacobi in LDS.
+// Levels with NDK > 1: one launch per rotation; each workgroup owns a slice of
+// the inner points d, reduces the A/B/C sums of its pair from the per-slice
+// partials written by the previous rotation (every workgroup in the same fixed
+// order, so all agree on the angle), applies the rotation to its slice, and
+// writes the partial sums of the next pair.  A sweep of launches is recorded
+// once in a HIP graph and replayed.
+#include "scalit_impl.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace scalit {
+
+__device__ __forceinline__ double jacobi_angle(double A, double B, double C)
+{
+    const double PIE = 3.14159265358979323846264338328;
+    double X;
+    if (A == C) {
+        if (B == 0.0) X = 0.0;
+        else X = -copysign(PIE / 8.0, B);
+    } else {
+        X = atan(B / (A - C)) / 4.0;
+        if (A > C) {
+            if (X < 0.0) X = X + PIE / 4.0;
+            else X = X - PIE / 4.0;
+        }
+    }
+    return X;
+}
+
+// ---------------------------------------------------------------- level 1
+// One wave per block b.  H (n x n) built in LDS from H0 and EIG0; V written
+// to VOSB (global); EIG0 overwritten with the diagonal at the end.
+__global__ void __launch_bounds__(64) k_osb_jacobi_ndk1(int n, int nblk, const double *__restrict__ h0,
+                                                      double *__restrict__ eig0, double *__restrict__ vosb, int bjmax,
+                                                      double bjtol, int *sweeps_out)
+{
+    extern __shared__ char smem_raw[];
+    double *H = (double *)smem_raw;   // H(r,c) at r + n c
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    double *eb = eig0 + (long)b * n;
+    double *V = vosb + (long)b * n * n;
+    for (int e = lane; e < n * n; e += 64) {
+        int r = e % n, c = e / n;
+        double v = h0[e];
+        if (r == c) v = v + eb[r];
+        H[e] = v;
+        V[e] = (r == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    auto offdiag = [&]() {
+        double s = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < i; j++) {
+                double h = H[j + n * i];
+                s = s + (0.0 + h * h);
+            }
+        return 2.0 * s;
+    };
+    double initye = 0.0;
+    if (lane == 0) initye = offdiag();
+    initye = __shfl(initye, 0);
+    double ye = initye, dye = initye;
+    int k = 1;
+    while (fabs(dye) > initye * bjtol && k <= bjmax) {
+        for (int i = 1; i <= n - 1; i++) {
+            for (int j = 1; j <= n - i; j++) {
+                const int p = j - 1, q = j + i - 1;
+                const double hpq = H[p + n * q], hpp = H[p + n * p], hqq = H[q + n * q];
+                const double odg = hpq;
+                const double ddg = 0.5 * hpp - 0.5 * hqq;
+                const double A = 0.0 + odg * odg;
+                const double B = 2.0 * (0.0 + odg * ddg);
+                const double C = 0.0 + ddg * ddg;
+                const double X = jacobi_angle(A, B, C);
+                const double SN = sin(X), CS = cos(X);
+                const double C2 = CS * CS, S2 = SN * SN, SC = SN * CS;
+                __syncthreads();
+                if (lane == 0) {
+                    H[p + n * p] = C2 * hpp + S2 * hqq - 2 * SC * odg;
+                    H[q + n * q] = C2 * hqq + S2 * hpp + 2 * SC * odg;
+                    const double nqp = (C2 - S2) * odg + 2 * SC * ddg;
+                    H[p + n * q] = nqp;
+                    H[q + n * p] = nqp;
+                }
+                for (int r = lane; r < n; r += 64) {
+                    if (r != p && r != q) {
+                        const double hrp = H[r + n * p], hrq = H[r + n * q];
+                        const double np = CS * hrp - SN * hrq;
+                        const double nq = CS * hrq + SN * hrp;
+                        H[r + n * p] = np;
+                        H[p + n * r] = np;
+                        H[r + n * q] = nq;
+                        H[q + n * r] = nq;
+                    }
+                    // VUPDATE(p, q, X): columns p and q of V
+                    const double vp = V[r + n * p], vq = V[r + n * q];
+                    V[r + n * p] = CS * vp - SN * vq;
+                    V[r + n * q] = CS * vq + SN * vp;
+                }
+                __syncthreads();
+            }
+        }
+        double s = 0.0;
+        if (lane == 0) s = offdiag();
+        s = __shfl(s, 0);
+        dye = s - ye;
+        ye = ye + dye;
+        k = k + 1;
+    }
+    for (int r = lane; r < n; r += 64) eb[r] = H[r + n * r];
+    if (lane == 0 && sweeps_out) atomicAdd(sweeps_out, k - 1);
+}
+
+// ---------------------------------------------------------------- NDK > 1
+// HOSB(d,k,j,b) = H0(k,j) * [DEP(d)] + delta_kj EIG0(d,k,b)
+__global__ void k_hosb_init(int ndk, int mak, int nblk, const double *__restrict__ h0, const double *__restrict__ dep,
+                            const double *__restrict__ eig0, double *__restrict__ hosb)
+{
+    long total = (long)ndk * mak * mak * nblk;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        int d = (int)(e % ndk);
+        long t = e / ndk;
+        int kk = (int)(t % mak);
+        t /= mak;
+        int jj = (int)(t % mak);
+        long b = t / mak;
+        double v = h0[kk + (long)mak * jj];
+        if (dep) v = v * dep[d];
+        if (kk == jj) v = v + eig0[d + (long)ndk * (kk + (long)mak * b)];
+        hosb[e] = v;
+    }
+}
+
+struct JPart {
+    double a, b, c;
+};
+
+__device__ __forceinline__ double block_sum_256(double v, double *red)
+{
+    // deterministic: fixed tree over the 256 threads
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = red[tid] + red[tid + s];
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Partial A/B/C sums of pair (p,q) over each 256-point slice of d.
+__global__ void __launch_bounds__(256) k_jpart(int ndk, int mak, int p, int q, const double *__restrict__ hosb,
+                                               JPart *__restrict__ part, const int *active)
+{
+    __shared__ double red[256];
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    const double *hb = hosb + (long)ndk * mak * mak * b;
+    double odg = 0, ddg = 0;
+    if (d < ndk) {
+        odg = hb[d + (long)ndk * (p + (long)mak * q)];
+        ddg = 0.5 * hb[d + (long)ndk * (p + (long)mak * p)] - 0.5 * hb[d + (long)ndk * (q + (long)mak * q)];
+    }
+    double a = block_sum_256(odg * odg, red);
+    double bb = block_sum_256(odg * ddg, red);
+    double c = block_sum_256(ddg * ddg, red);
+    if (threadIdx.x == 0) part[(long)b * gridDim.x + blockIdx.x] = {a, bb, c};
+}
+
+// One rotation (p,q) of every active block, then the partials of (pn,qn).
+__global__ void __launch_bounds__(256) k_jrot(int ndk, int mak, int nch, int p, int q, int pn, int qn,
+                                              double *__restrict__ hosb, double *__restrict__ vosb,
+                                              const JPart *__restrict__ part_in, JPart *__restrict__ part_out,
+                                              const int *active)
+{
+    __shared__ double red[256];
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    double A = 0, B = 0, C = 0;
+    const JPart *pb = part_in + (long)b * nch;
+    for (int i = 0; i < nch; i++) {
+        A += pb[i].a;
+        B += pb[i].b;
+        C += pb[i].c;
+    }
+    B = 2.0 * B;
+    const double X = jacobi_angle(A, B, C);
+    const double SN = sin(X), CS = cos(X);
+    const double C2 = CS * CS, S2 = SN * SN, SC = SN * CS;
+    const long m2 = (long)mak * mak;
+    double *hb = hosb + (long)ndk * m2 * b;
+    const int d = blockIdx.x * 256 + threadIdx.x;
+#define HO(r, c) hb[d + (long)ndk * ((r) + (long)mak * (c))]
+    if (d < ndk) {
+        const double hpp = HO(p, p), hqq = HO(q, q), odg = HO(p, q);
+        const double ddg = 0.5 * hpp - 0.5 * hqq;
+        HO(p, p) = C2 * hpp + S2 * hqq - 2 * SC * odg;
+        HO(q, q) = C2 * hqq + S2 * hpp + 2 * SC * odg;
+        const double nqp = (C2 - S2) * odg + 2 * SC * ddg;
+        HO(p, q) = nqp;
+        HO(q, p) = nqp;
+        for (int r = 0; r < mak; r++) {
+            if (r == p || r == q) continue;
+            const double hrp = HO(r, p), hrq = HO(r, q);
+            const double np = CS * hrp - SN * hrq;
+            const double nq = CS * hrq + SN * hrp;
+            HO(r, p) = np;
+            HO(p, r) = np;
+            HO(r, q) = nq;
+            HO(q, r) = nq;
+        }
+    }
+    if (blockIdx.x == 0) {
+        double *V = vosb + m2 * b;
+        for (int r = threadIdx.x; r < mak; r += 256) {
+            const double vp = V[r + (long)mak * p], vq = V[r + (long)mak * q];
+            V[r + (long)mak * p] = CS * vp - SN * vq;
+            V[r + (long)mak * q] = CS * vq + SN * vp;
+        }
+    }
+    // partial sums of the next pair from this slice's updated values
+    double odg = 0, ddg = 0;
+    if (d < ndk && pn >= 0) {
+        odg = HO(pn, qn);
+        ddg = 0.5 * HO(pn, pn) - 0.5 * HO(qn, qn);
+    }
+#undef HO
+    if (pn < 0) return;
+    double a = block_sum_256(odg * odg, red);
+    double bb = block_sum_256(odg * ddg, red);
+    double c = block_sum_256(ddg * ddg, red);
+    if (threadIdx.x == 0) part_out[(long)b * nch + blockIdx.x] = {a, bb, c};
+}
+
+// per-slice partial of sum_{i>j} H(d,j,i)^2
+__global__ void __launch_bounds__(256) k_offdiag_part(int ndk, int mak, const double *__restrict__ hosb,
+                                                      double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int b = blockIdx.y;
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    const double *hb = hosb + (long)ndk * mak * mak * b;
+    double s = 0;
+    if (d < ndk)
+        for (int i = 0; i < mak; i++)
+            for (int j = 0; j < i; j++) {
+                double h = hb[d + (long)ndk * (j + (long)mak * i)];
+                s += h * h;
+            }
+    double t = block_sum_256(s, red);
+    if (threadIdx.x == 0) part[(long)b * gridDim.x + blockIdx.x] = t;
+}
+
+// convergence bookkeeping per block b (hosbdiag.f90:23-43)
+__global__ void k_jconv(int nblk, int nch, int first, const double *__restrict__ part, double *__restrict__ ye,
+                        double *__restrict__ initye, int *__restrict__ kcount, int *__restrict__ active, int bjmax,
+                        double bjtol)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    double s = 0;
+    for (int i = 0; i < nch; i++) s += part[(long)b * nch + i];
+    s = 2.0 * s;
+    if (first) {
+        initye[b] = s;
+        ye[b] = s;
+        kcount[b] = 1;
+        active[b] = (fabs(s) > s * bjtol && 1 <= bjmax) ? 1 : 0;
+        return;
+    }
+    if (!active[b]) return;
+    double dye = s - ye[b];
+    ye[b] = ye[b] + dye;
+    kcount[b] += 1;
+    active[b] = (fabs(dye) > initye[b] * bjtol && kcount[b] <= bjmax) ? 1 : 0;
+}
+
+__global__ void k_vinit(int mak, int nblk, double *vosb)
+{
+    long total = (long)mak * mak * nblk;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        long m = e % ((long)mak * mak);
+        vosb[e] = (m % mak == m / mak) ? 1.0 : 0.0;
+    }
+}
+
+__global__ void k_eig_extract(int ndk, int mak, int nblk, const double *__restrict__ hosb, double *__restrict__ eig0)
+{
+    long total = (long)ndk * mak * nblk;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        int d = (int)(e % ndk);
+        long t = e / ndk;
+        int i = (int)(t % mak);
+        long b = t / mak;
+        eig0[e] = hosb[d + (long)ndk * (i + (long)mak * (i + (long)mak * b))];
+    }
+}
+
+static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, hipStream_t s)
+{
+    Level &L = c->lv[l];
+    const int ndk = L.dim, mak = L.n, nblk = L.blk;
+    const int nch = (ndk + 255) / 256;
+    DBuf<double> hosb, part_od, ye, initye;
+    DBuf<JPart> pa, pb;
+    DBuf<int> active, kcount;
+    SC_TRY(hosb.alloc((size_t)ndk * mak * mak * nblk));
+    SC_TRY(part_od.alloc((size_t)nch * nblk));
+    SC_TRY(ye.alloc(nblk));
+    SC_TRY(initye.alloc(nblk));
+    SC_TRY(pa.alloc((size_t)nch * nblk));
+    SC_TRY(pb.alloc((size_t)nch * nblk));
+    SC_TRY(active.alloc(nblk));
+    SC_TRY(kcount.alloc(nblk));
+    int rc = SCALIT_OK;
+    {
+        hipLaunchKernelGGL(k_hosb_init, dim3(4096), dim3(256), 0, s, ndk, mak, nblk, L.h0.p, L.dep ? L.depv.p : nullptr,
+                           c->eig0.p, hosb.p);
+        hipLaunchKernelGGL(k_vinit, dim3(1024), dim3(256), 0, s, mak, nblk, L.vosb.p);
+        hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, hosb.p, part_od.p);
+        hipLaunchKernelGGL(k_jconv, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, nch, 1, part_od.p, ye.p, initye.p,
+                           kcount.p, active.p, bjmax, bjtol);
+        // the pair sequence of one sweep
+        std::vector<int> pp, qq;
+        for (int i = 1; i <= mak - 1; i++)
+            for (int j = 1; j <= mak - i; j++) {
+                pp.push_back(j - 1);
+                qq.push_back(j + i - 1);
+            }
+        const int npair = (int)pp.size();
+        std::vector<int> hact(nblk);
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        bool captured = false;
+        for (int sweep = 0; sweep < bjmax && npair > 0; sweep++) {
+            if (hipMemcpyAsync(hact.data(), active.p, sizeof(int) * nblk, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = SCALIT_EHIP;
+                break;
+            }
+            bool any = false;
+            for (int b = 0; b < nblk; b++) any = any || hact[b];
+            if (!any) break;
+            if (!captured) {
+                // record one sweep: partials of the first pair, every rotation, convergence
+                if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                    rc = SCALIT_EHIP;
+                    break;
+                }
+                hipLaunchKernelGGL(k_jpart, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, pp[0], qq[0], hosb.p, pa.p,
+                                   active.p);
+                JPart *pin = pa.p, *pout = pb.p;
+                for (int r = 0; r < npair; r++) {
+                    int pn = (r + 1 < npair) ? pp[r + 1] : -1;
+                    int qn = (r + 1 < npair) ? qq[r + 1] : -1;
+                    hipLaunchKernelGGL(k_jrot, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, nch, pp[r], qq[r], pn, qn,
+                                       hosb.p, L.vosb.p, pin, pout, active.p);
+                    JPart *t = pin;
+                    pin = pout;
+                    pout = t;
+                }
+                hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, hosb.p, part_od.p);
+                hipLaunchKernelGGL(k_jconv, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, nch, 0, part_od.p, ye.p,
+                                   initye.p, kcount.p, active.p, bjmax, bjtol);
+                if (hipStreamEndCapture(s, &graph) != hipSuccess ||
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                    rc = SCALIT_EHIP;
+                    break;
+                }
+                captured = true;
+            }
+            if (hipGraphLaunch(exec, s) != hipSuccess) {
+                rc = SCALIT_EHIP;
+                break;
+            }
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc == SCALIT_OK) {
+            hipLaunchKernelGGL(k_eig_extract, dim3(4096), dim3(256), 0, s, ndk, mak, nblk, hosb.p, c->eig0.p);
+            if (hipStreamSynchronize(s) != hipSuccess) rc = SCALIT_EHIP;
+        }
+    }
+    hosb.release();
+    part_od.release();
+    ye.release();
+    initye.release();
+    pa.release();
+    pb.release();
+    active.release();
+    kcount.release();
+    return rc;
+}
+
+int block_diag(scalit_ctx *c, int bjmax, double bjtol)
+{
+    hipStream_t s = c->stream;
+    for (int l = 0; l < c->sF; l++) {
+        if (!c->lv[l].has_h0) return SCALIT_ESTATE;
+        if (c->lv[l].dep && !c->lv[l].has_dep) return SCALIT_ESTATE;
+        SC_TRY(c->lv[l].vosb.alloc((size_t)c->lv[l].blk * c->lv[l].n * c->lv[l].n));
+    }
+    if (!c->has_res) return SCALIT_ESTATE;
+    SC_TRY(c->eig0.alloc(c->N));
+    HIP_TRY(hipMemcpyAsync(c->eig0.p, c->res.p, sizeof(double) * c->N, hipMemcpyDeviceToDevice, s));
+    for (int l = c->sF - 1; l >= 0; l--) {
+        Level &L = c->lv[l];
+        if (L.dim == 1) {
+            size_t lds = sizeof(double) * (size_t)L.n * L.n;
+            if (lds > 160 * 1024) return SCALIT_EINVAL;
+            (void)hipFuncSetAttribute((const void *)k_osb_jacobi_ndk1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      160 * 1024);
+            hipLaunchKernelGGL(k_osb_jacobi_ndk1, dim3(L.blk), dim3(64), lds, s, L.n, L.blk, L.h0.p, c->eig0.p,
+                               L.vosb.p, bjmax, bjtol, (int *)nullptr);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(s));
+        } else {
+            SC_TRY(diag_level_ndk(c, l, bjmax, bjtol, s));
+        }
+    }
+    c->has_precond = true;
+    // fp32 copies are rebuilt on demand
+    for (int l = 0; l < c->sF; l++) c->lv[l].vosb_f.release();
+    c->eig0_f.release();
+    return SCALIT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// small dense symmetric eigensolver (one workgroup): cyclic Jacobi in the
+// round-robin (tournament) ordering, n/2 disjoint rotations per round.  It
+// stands in for LAPACK DSYEV('N'/'V') of the Rayleigh-Ritz matrices.
+// work layout: A (n x n, ld n) | Z (n x n) | cs (2 * n) | pairs (2 * n ints)
+// ---------------------------------------------------------------------------
+size_t small_sym_eig_work(int n) { return (size_t)n * n * 2 + 4 * (size_t)n + 8; }
+
+__global__ void __launch_bounds__(1024) k_small_sym_eig(int n, const double *__restrict__ a_in, int lda,
+                                                        double *__restrict__ w, double *__restrict__ z_out, int ldz,
+                                                        double *__restrict__ work)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double *A = work;
+    double *Z = A + (long)n * n;
+    double *cs = Z + (long)n * n;   // c, s per pair
+    __shared__ int pairs[2 * 512];
+    __shared__ double red[1024];
+    __shared__ int flag;
+    const int m = (n + 1) / 2 * 2;   // even count with a dummy index n when n is odd
+    for (int e = tid; e < n * n; e += nt) {
+        int i = e % n, j = e / n;
+        // symmetric from the upper triangle (DSYEV 'U')
+        A[e] = (i <= j) ? a_in[i + (long)lda * j] : a_in[j + (long)lda * i];
+        Z[e] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 60; sweep++) {
+        // off-diagonal and total Frobenius norms
+        double off = 0, tot = 0;
+        for (int e = tid; e < n * n; e += nt) {
+            double v = A[e] * A[e];
+            tot += v;
+            if (e % n != e / n) off += v;
+        }
+        red[tid] = off;
+        __syncthreads();
+        for (int s2 = nt / 2; s2 > 0; s2 >>= 1) {
+            if (tid < s2) red[tid] += red[tid + s2];
+            __syncthreads();
+        }
+        off = red[0];
+        __syncthreads();
+        red[tid] = tot;
+        __syncthreads();
+        for (int s2 = nt / 2; s2 > 0; s2 >>= 1) {
+            if (tid < s2) red[tid] += red[tid + s2];
+            __syncthreads();
+        }
+        tot = red[0];
+        __syncthreads();
+        if (off <= 1e-34 * tot || off == 0.0) break;
+        for (int round = 0; round < m - 1; round++) {
+            // tournament pairing: position 0 fixed, others rotate
+            for (int k = tid; k < m / 2; k += nt) {
+                int a = (k == 0) ? 0 : 1 + (k - 1 + round) % (m - 1);
+                int b = 1 + (m - 2 - k + round) % (m - 1);
+                if (k == 0) b = 1 + (m - 2 + round) % (m - 1);
+                int p = a < b ? a : b, q = a < b ? b : a;
+                pairs[2 * k] = p;
+                pairs[2 * k + 1] = q;
+                double c = 1.0, s = 0.0;
+                if (q < n) {
+                    double apq = A[p + (long)n * q];
+                    if (apq != 0.0) {
+                        double app = A[p + (long)n * p], aqq = A[q + (long)n * q];
+                        double theta = (aqq - app) / (2.0 * apq);
+                        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        s = t * c;
+                    }
+                }
+                cs[2 * k] = c;
+                cs[2 * k + 1] = s;
+            }
+            __syncthreads();
+            // rows: A <- J^T A  (each pair touches rows p and q)
+            for (int e = tid; e < (m / 2) * n; e += nt) {
+                int k = e / n, col = e % n;
+                int p = pairs[2 * k], q = pairs[2 * k + 1];
+                if (q >= n) continue;
+                double c = cs[2 * k], s = cs[2 * k + 1];
+                if (s == 0.0) continue;
+                double ap = A[p + (long)n * col], aq = A[q + (long)n * col];
+                A[p + (long)n * col] = c * ap - s * aq;
+                A[q + (long)n * col] = s * ap + c * aq;
+            }
+            __syncthreads();
+            // columns: A <- A J ; Z <- Z J
+            for (int e = tid; e < (m / 2) * n; e += nt) {
+                int k = e / n, row = e % n;
+                int p = pairs[2 * k], q = pairs[2 * k + 1];
+                if (q >= n) continue;
+                double c = cs[2 * k], s = cs[2 * k + 1];
+                if (s == 0.0) continue;
+                double ap = A[row + (long)n * p], aq = A[row + (long)n * q];
+                A[row + (long)n * p] = c * ap - s * aq;
+                A[row + (long)n * q] = s * ap + c * aq;
+                double zp = Z[row + (long)n * p], zq = Z[row + (long)n * q];
+                Z[row + (long)n * p] = c * zp - s * zq;
+                Z[row + (long)n * q] = s * zp + c * zq;
+            }
+            __syncthreads();
+        }
+    }
+    (void)flag;
+    // sort ascending (selection by rank: stable on ties by index)
+    for (int i = tid; i < n; i += nt) {
+        double di = A[i + (long)n * i];
+        int rank = 0;
+        for (int j = 0; j < n; j++) {
+            double dj = A[j + (long)n * j];
+            if (dj < di || (dj == di && j < i)) rank++;
+        }
+        w[rank] = di;
+        if (z_out)
+            for (int r = 0; r < n; r++) z_out[r + (long)ldz * rank] = Z[r + (long)n * i];
+    }
+}
+
+int small_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz, double *work, hipStream_t s)
+{
+    if (n < 1 || n > 1024) return SCALIT_EINVAL;
+    hipLaunchKernelGGL(k_small_sym_eig, dim3(1), dim3(1024), 0, s, n, a, lda, w, z, ldz, work);
+    HIP_TRY(hipGetLastError());
+    return SCALIT_OK;
+}
+
+}  // namespace scalit

@@ -1,0 +1,729 @@
+// Vector kernels of the Krylov solvers.
+//
+// Reductions are deterministic: a kernel writes NPART per-block partial sums
+// per vector (fixed grid, fixed tree inside a block) and every consumer adds
+// the partials in index order.  The QMR recurrence of
+// src/iterate/libiterate/qmr/qmr.f90:5-175 is split into fused
+// reduce-then-update kernels whose scalars live in device memory (struct
+// QmrScal), so that an iteration needs no host round trip and can be recorded
+// once in a HIP graph.  Every kernel of an iteration skips vectors whose
+// status is no longer "running", which reproduces the reference's `exit`
+// semantics when more iterations are replayed than a vector needs.
+#include "scalit_impl.hpp"
+#include "solver_state.hpp"
+
+namespace scalit {
+
+__device__ __forceinline__ float v_bf16_to_f32(bf16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
+__device__ __forceinline__ bf16_t v_f32_to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)(u >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+template <typename S>
+__device__ __forceinline__ double lv(const S *p, long i) { return (double)p[i]; }
+template <>
+__device__ __forceinline__ double lv<bf16_t>(const bf16_t *p, long i) { return (double)v_bf16_to_f32(p[i]); }
+template <typename S>
+__device__ __forceinline__ void sv(S *p, long i, double v) { p[i] = (S)v; }
+template <>
+__device__ __forceinline__ void sv<bf16_t>(bf16_t *p, long i, double v) { p[i] = v_f32_to_bf16((float)v); }
+
+// deterministic block reduction over 256 threads (warp shuffles + LDS)
+__device__ __forceinline__ double bsum(double v, double *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0) r = (red[0] + red[1]) + (red[2] + red[3]);
+    return r;   // valid on thread 0
+}
+
+__device__ __forceinline__ double sum_parts(const double *p)
+{
+    double s = 0;
+    for (int i = 0; i < NPART; i++) s += p[i];
+    return s;
+}
+
+#define GSTRIDE(i, n) for (long i = blockIdx.x * 256L + threadIdx.x; i < (n); i += (long)gridDim.x * 256L)
+
+// ---------------------------------------------------------------- generic
+__global__ void __launch_bounds__(256) k_dot_part(long n, const double *x, long ldx, const double *y, long ldy,
+                                                  double *part)
+{
+    __shared__ double red[4];
+    const int v = blockIdx.y;
+    const double *xv = x + (long)v * ldx, *yv = y + (long)v * ldy;
+    double s = 0;
+    GSTRIDE(i, n) s += xv[i] * yv[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
+}
+
+__global__ void k_reduce_parts(int nvec, const double *part, double *out)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nvec) out[v] = sum_parts(part + (long)v * NPART);
+}
+
+void launch_dot_partials(int nvec, long n, const double *x, long ldx, const double *y, long ldy, double *part,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dot_part, dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy, part);
+}
+
+void launch_reduce_partials(int nvec, const double *part, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_parts, dim3((nvec + 63) / 64), dim3(64), 0, s, nvec, part, out);
+}
+
+int dot_host(int nvec, long n, const double *x, long ldx, const double *y, long ldy, double *out_host,
+             double *part_dev, double *red_dev, hipStream_t s)
+{
+    launch_dot_partials(nvec, n, x, ldx, y, ldy, part_dev, s);
+    launch_reduce_partials(nvec, part_dev, red_dev, s);
+    HIP_TRY(hipMemcpyAsync(out_host, red_dev, sizeof(double) * nvec, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SCALIT_OK;
+}
+
+// y = a x + b y  (per-vector scalars on the host side: a, b broadcast)
+__global__ void k_axpby(long n, double a, const double *x, long ldx, double b, double *y, long ldy)
+{
+    const int v = blockIdx.y;
+    GSTRIDE(i, n) y[(long)v * ldy + i] = a * x[(long)v * ldx + i] + b * y[(long)v * ldy + i];
+}
+
+// casts between storage types
+template <typename A, typename B>
+__global__ void k_cast(long n, const A *x, long ldx, B *y, long ldy)
+{
+    const int v = blockIdx.y;
+    GSTRIDE(i, n) sv<B>(y + (long)v * ldy, i, lv<A>(x + (long)v * ldx, i));
+}
+
+void launch_cast_d2f(int nvec, long n, const double *x, long ldx, float *y, long ldy, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
+}
+void launch_cast_f2d(int nvec, long n, const float *x, long ldx, double *y, long ldy, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cast<float, double>), dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
+}
+void launch_cast_d2b(int nvec, long n, const double *x, long ldx, bf16_t *y, long ldy, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cast<double, bf16_t>), dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
+}
+void launch_cast_b2d(int nvec, long n, const bf16_t *x, long ldx, double *y, long ldy, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cast<bf16_t, double>), dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
+}
+void launch_axpby(int nvec, long n, double a, const double *x, long ldx, double b, double *y, long ldy,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpby, dim3(NPART, nvec), dim3(256), 0, s, n, a, x, ldx, b, y, ldy);
+}
+
+// ---------------------------------------------------------------- QMR
+// Vectors of one QMR: nvec columns each of R, UX, UR, Q, S, V, Z, HX0 with
+// leading dimension ld (storage type S); B and X are the caller's (leading
+// dimension ldb / ldx, storage type S).
+
+__device__ __forceinline__ bool running(const QmrScal *q, int v) { return q[v].status == QMR_RUNNING; }
+
+// init part 1: R = V = B, X = 0 (ntype <= 0); partial B.B
+template <typename S>
+__global__ void __launch_bounds__(256) k_qmr_init(long n, int zero_x, QmrVecs<S> w, double *part)
+{
+    __shared__ double red[4];
+    const int v = blockIdx.y;
+    const S *b = w.b + (long)v * w.ldb;
+    S *x = w.x + (long)v * w.ldx;
+    S *R = w.R + (long)v * w.ld, *V = w.V + (long)v * w.ld;
+    double s = 0;
+    GSTRIDE(i, n)
+    {
+        S bi = b[i];
+        R[i] = bi;
+        V[i] = bi;
+        if (zero_x) x[i] = (S)0;
+        double d = lv<S>(b, i);
+        s += d * d;
+    }
+    s = bsum(s, red);
+    if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
+}
+
+// init part 2 (after Z = P B): partial Z.Z
+template <typename S>
+__global__ void __launch_bounds__(256) k_qmr_init_z(long n, QmrVecs<S> w, double *part)
+{
+    __shared__ double red[4];
+    const int v = blockIdx.y;
+    const S *Z = w.Z + (long)v * w.ld;
+    double s = 0;
+    GSTRIDE(i, n)
+    {
+        double d = lv<S>(Z, i);
+        s += d * d;
+    }
+    s = bsum(s, red);
+    if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
+}
+
+// init scalars: BNRM, VNRM, ZNRM, GAMAMO=1, ETA=-1, ZVRAT=1
+__global__ void k_qmr_init_scal(int nvec, const double *part_bb, const double *part_zz, QmrScal *q, int *active,
+                                int qmrmax)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvec) return;
+    QmrScal &s = q[v];
+    s.bnrm = sqrt(sum_parts(part_bb + (long)v * NPART));
+    s.vnrm = s.bnrm;
+    s.znrm = sqrt(sum_parts(part_zz + (long)v * NPART));
+    s.gamamo = 1.0;
+    s.eta = -1.0;
+    s.zvrat = 1.0;
+    s.thetamo = 0.0;
+    s.qsdot = 0.0;
+    s.ret = 0;
+    s.iter = 1;
+    s.res = 0.0;
+    s.status = (qmrmax >= 1) ? QMR_RUNNING : QMR_EXIT;
+    active[v] = s.status == QMR_RUNNING;
+}
+
+// k2: exit check on VNRM/ZNRM, ZVRAT *= ZNRM/VNRM, V /= VNRM, Z /= ZNRM, partial V.Z
+template <typename S>
+__global__ void __launch_bounds__(256) k_qmr_a(long n, QmrVecs<S> w, QmrScal *q, int *active, double *part)
+{
+    __shared__ double red[4];
+    const int v = blockIdx.y;
+    if (!running(q, v)) return;
+    const double vnrm = q[v].vnrm, znrm = q[v].znrm;
+    if (vnrm == 0.0 || znrm == 0.0) {
+        __syncthreads();
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            q[v].status = QMR_EXIT;
+            active[v] = 0;
+        }
+        return;
+    }
+    S *V = w.V + (long)v * w.ld, *Z = w.Z + (long)v * w.ld;
+    double s = 0;
+    GSTRIDE(i, n)
+    {
+        double a = lv<S>(V, i) / vnrm;
+        double z = lv<S>(Z, i) / znrm;
+        sv<S>(V, i, a);
+        sv<S>(Z, i, z);
+        s += lv<S>(V, i) * lv<S>(Z, i);
+    }
+    s = bsum(s, red);
+    if (threadIdx.x == 0) {
+        part[(long)v * NPART + blockIdx.x] = s;
+        if (blockIdx.x == 0) q[v].zvrat_next = q[v].zvrat * znrm / vnrm;
+    }
+}
+
+// k3: VZDOT, breakdown check, Q = Z - (VNRM VZDOT / QSDOT) Q
+template <typename S>
+__global__ void __launch_bounds__(256) k_qmr_b(long n, QmrVecs<S> w, QmrScal *q, int *active, const double *part)
+{
+    const int v = blockIdx.y;
+    if (!running(q, v)) return;
+    const double vzdot = sum_parts(part + (long)v * NPART);
+    if (vzdot == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            q[v].status = QMR_EXIT;
+            active[v] = 0;
+        }
+        return;
+    }
+    S *Q = w.Q + (long)v * w.ld;
+    const S *Z = w.Z + (long)v * w.ld;
+    const bool first = q[v].iter == 1;
+    const double c = first ? 0.0 : q[v].vnrm * vzdot / q[v].qsdot;
+    if (first) {
+        GSTRIDE(i, n) Q[i] = Z[i];
+    } else {
+        GSTRIDE(i, n) sv<S>(Q, i, lv<S>(Z, i) - c * lv<S>(Q, i));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        q[v].vzdot = vzdot;
+        q[v].zvrat = q[v].zvrat_next;
+    }
+}
+
+// k4: S = ZVRAT * S, partial Q.S
+template <typename S>
+__global__ void __launch_bounds__(256) k_qmr_c(long n, QmrVecs<S> w, const QmrScal *q, double *part)
+{
+    __shared__ double red[4];
+    const int v = blockIdx.y;
+    if (!running(q, v)) return;
+    const double zvrat = q[v].zvrat;
+    S *Sv = w.Sv + (long)v * w.ld;
+    const S *Q = w.Q + (long)v * w.ld;
+    double s = 0;
+    GSTRIDE(i, n)
+    {
+        sv<S>(Sv, i, zvrat * lv<S>(Sv, i));
+        s += lv<S>(Q, i) * lv<S>(Sv, i);
+    }
+    s = bsum(s, red);
+    if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
+}
+
+// k5: QSDOT, BETA, breakdown, V = S - BETA V, partial V.V
+template <typename S>
+__global__ void __launch_bounds__(256) k_qmr_d(long n, QmrVecs<S> w, QmrScal *q, int *active, const double *part_in,
+                                               double *part)
+{
+    __shared__ double red[4];
+    const int v = blockIdx.y;
+    if (!running(q, v)) return;
+    const double qsdot = sum_parts(part_in + (long)v * NPART);
+    const double beta = qsdot / q[v].vzdot;
+    if (qsdot == 0.0 || beta == 0.0) {
+        __syncthreads();
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            q[v].status = QMR_EXIT;
+            active[v] = 0;
+        }
+        return;
+    }
+    S *V = w.V + (long)v * w.ld;
+    const S *Sv = w.Sv + (long)v * w.ld;
+    double s = 0;
+    GSTRIDE(i, n)
+    {
+        sv<S>(V, i, lv<S>(Sv, i) - beta * lv<S>(V, i));
+        double a = lv<S>(V, i);
+        s += a * a;
+    }
+    s = bsum(s, red);
+    if (threadIdx.x == 0) {
+        part[(long)v * NPART + blockIdx.x] = s;
+        if (blockIdx.x == 0) {
+            q[v].qsdot = qsdot;
+            q[v].beta = beta;
+        }
+    }
+}
+
+// k6 (after Z = P V): VNRMPO, Z = Z / ZVRAT, partial Z.Z
+template <typename S>
+__global__ void __launch_bounds__(256) k_qmr_e(long n, QmrVecs<S> w, QmrScal *q, const double *part_vv, double *part)
+{
+    __shared__ double red[4];
+    const int v = blockIdx.y;
+    if (!running(q, v)) return;
+    const double zvrat = q[v].zvrat;
+    S *Z = w.Z + (long)v * w.ld;
+    double s = 0;
+    GSTRIDE(i, n)
+    {
+        sv<S>(Z, i, lv<S>(Z, i) / zvrat);
+        double a = lv<S>(Z, i);
+        s += a * a;
+    }
+    s = bsum(s, red);
+    if (threadIdx.x == 0) {
+        part[(long)v * NPART + blockIdx.x] = s;
+        if (blockIdx.x == 0) q[v].vnrmpo = sqrt(sum_parts(part_vv + (long)v * NPART));
+    }
+}
+
+// k7: ZNRM, THETA, GAMA, ETA, breakdown on GAMA; UX, UR, X, R updates and the
+// partial sums the convergence test of type |nType| = 0 / 2 needs.
+template <typename S>
+__global__ void __launch_bounds__(256) k_qmr_f(long n, int conv, QmrVecs<S> w, QmrScal *q, int *active,
+                                               const double *part_zz, double *part_rr, double *part_uxux,
+                                               double *part_xx, double *part_urur)
+{
+    __shared__ double red[4];
+    const int v = blockIdx.y;
+    if (!running(q, v)) return;
+    const QmrScal &o = q[v];
+    const double znrm = sqrt(sum_parts(part_zz + (long)v * NPART));
+    const double theta = o.vnrmpo / (o.gamamo * fabs(o.beta));
+    const double gama = 1.0 / sqrt(1.0 + theta * theta);
+    const double eta = -o.eta * o.vnrm * gama * gama / (o.beta * o.gamamo * o.gamamo);
+    if (gama == 0.0) {
+        __syncthreads();
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            q[v].status = QMR_EXIT;
+            active[v] = 0;
+        }
+        return;
+    }
+    const bool first = o.iter == 1;
+    const double a = eta * o.zvrat;
+    const double tg = (o.thetamo * gama) * (o.thetamo * gama);
+    S *UX = w.UX + (long)v * w.ld, *UR = w.UR + (long)v * w.ld, *R = w.R + (long)v * w.ld;
+    const S *Q = w.Q + (long)v * w.ld, *Sv = w.Sv + (long)v * w.ld;
+    S *X = w.x + (long)v * w.ldx;
+    double srr = 0, sux = 0, sxx = 0, sur = 0;
+    GSTRIDE(i, n)
+    {
+        double ux = first ? a * lv<S>(Q, i) : a * lv<S>(Q, i) + tg * lv<S>(UX, i);
+        double ur = first ? eta * lv<S>(Sv, i) : eta * lv<S>(Sv, i) + tg * lv<S>(UR, i);
+        sv<S>(UX, i, ux);
+        sv<S>(UR, i, ur);
+        sv<S>(X, i, lv<S>(X, i) + lv<S>(UX, i));
+        sv<S>(R, i, lv<S>(R, i) - lv<S>(UR, i));
+        if (conv != 1) {
+            double r = lv<S>(R, i);
+            srr += r * r;
+            if (conv == 2) {
+                double u = lv<S>(UX, i), xx = lv<S>(X, i), uu = lv<S>(UR, i);
+                sux += u * u;
+                sxx += xx * xx;
+                sur += uu * uu;
+            }
+        }
+    }
+    if (conv != 1) {
+        srr = bsum(srr, red);
+        if (threadIdx.x == 0) part_rr[(long)v * NPART + blockIdx.x] = srr;
+        if (conv == 2) {
+            sux = bsum(sux, red);
+            if (threadIdx.x == 0) part_uxux[(long)v * NPART + blockIdx.x] = sux;
+            sxx = bsum(sxx, red);
+            if (threadIdx.x == 0) part_xx[(long)v * NPART + blockIdx.x] = sxx;
+            sur = bsum(sur, red);
+            if (threadIdx.x == 0) part_urur[(long)v * NPART + blockIdx.x] = sur;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        q[v].znrm_next = znrm;
+        q[v].theta = theta;
+        q[v].gama = gama;
+        q[v].eta_next = eta;
+    }
+}
+
+// k8 (conv type 1, after HX0 = H X): partial HX0.HX0 and (HX0-B).(HX0-B)
+template <typename S>
+__global__ void __launch_bounds__(256) k_qmr_g(long n, QmrVecs<S> w, const QmrScal *q, double *part_hh,
+                                               double *part_rr)
+{
+    __shared__ double red[4];
+    const int v = blockIdx.y;
+    if (!running(q, v)) return;
+    const S *H = w.HX0 + (long)v * w.ld;
+    const S *b = w.b + (long)v * w.ldb;
+    double s1 = 0, s2 = 0;
+    GSTRIDE(i, n)
+    {
+        double h = lv<S>(H, i);
+        double r = h - lv<S>(b, i);
+        s1 += h * h;
+        s2 += r * r;
+    }
+    s1 = bsum(s1, red);
+    if (threadIdx.x == 0) part_hh[(long)v * NPART + blockIdx.x] = s1;
+    s2 = bsum(s2, red);
+    if (threadIdx.x == 0) part_rr[(long)v * NPART + blockIdx.x] = s2;
+}
+
+// k9: residual, convergence, roll the two-level variables, next iteration
+__global__ void k_qmr_h(int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *part_hh,
+                        const double *part_rr, const double *part_uxux, const double *part_xx,
+                        const double *part_urur)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvec) return;
+    QmrScal &s = q[v];
+    if (s.status != QMR_RUNNING) return;
+    double res;
+    if (conv == 1) {
+        double xnrm = sqrt(sum_parts(part_hh + (long)v * NPART));
+        res = sqrt(sum_parts(part_rr + (long)v * NPART)) / xnrm;
+    } else if (conv == 2) {
+        double rr = sum_parts(part_rr + (long)v * NPART);
+        double r1 = sqrt(rr) / s.bnrm;
+        double r2 = sqrt(sum_parts(part_uxux + (long)v * NPART) / sum_parts(part_xx + (long)v * NPART));
+        double r3 = sqrt(sum_parts(part_urur + (long)v * NPART) / rr);
+        res = r1 > r2 ? r1 : r2;
+        res = res > r3 ? res : r3;
+    } else {
+        res = sqrt(sum_parts(part_rr + (long)v * NPART)) / s.bnrm;
+    }
+    s.res = res;
+    if (fabs(res) < tol) {
+        s.ret = s.iter;
+        s.status = QMR_CONVERGED;
+        active[v] = 0;
+        return;
+    }
+    s.ret = -s.iter;
+    s.vnrm = s.vnrmpo;
+    s.gamamo = s.gama;
+    s.thetamo = s.theta;
+    s.eta = s.eta_next;
+    s.znrm = s.znrm_next;
+    s.iter += 1;
+    if (s.iter > qmrmax) {
+        s.status = QMR_EXIT;
+        active[v] = 0;
+    }
+}
+
+template <typename S>
+void launch_qmr_init(int nvec, long n, int zero_x, const QmrVecs<S> &w, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_qmr_init<S>), dim3(NPART, nvec), dim3(256), 0, s, n, zero_x, w, part);
+}
+template <typename S>
+void launch_qmr_init_z(int nvec, long n, const QmrVecs<S> &w, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_qmr_init_z<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, part);
+}
+void launch_qmr_init_scal(int nvec, const double *pbb, const double *pzz, QmrScal *q, int *active, int qmrmax,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qmr_init_scal, dim3((nvec + 63) / 64), dim3(64), 0, s, nvec, pbb, pzz, q, active, qmrmax);
+}
+template <typename S>
+void launch_qmr_a(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_qmr_a<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, active, part);
+}
+template <typename S>
+void launch_qmr_b(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, const double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_qmr_b<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, active, part);
+}
+template <typename S>
+void launch_qmr_c(int nvec, long n, const QmrVecs<S> &w, const QmrScal *q, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_qmr_c<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, part);
+}
+template <typename S>
+void launch_qmr_d(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, const double *pin, double *part,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL((k_qmr_d<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, active, pin, part);
+}
+template <typename S>
+void launch_qmr_e(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, const double *pvv, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_qmr_e<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, pvv, part);
+}
+template <typename S>
+void launch_qmr_f(int nvec, long n, int conv, const QmrVecs<S> &w, QmrScal *q, int *active, const double *pzz,
+                  double *prr, double *pux, double *pxx, double *pur, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_qmr_f<S>), dim3(NPART, nvec), dim3(256), 0, s, n, conv, w, q, active, pzz, prr, pux, pxx,
+                       pur);
+}
+template <typename S>
+void launch_qmr_g(int nvec, long n, const QmrVecs<S> &w, const QmrScal *q, double *phh, double *prr, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_qmr_g<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, phh, prr);
+}
+void launch_qmr_h(int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *phh,
+                  const double *prr, const double *pux, const double *pxx, const double *pur, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qmr_h, dim3((nvec + 63) / 64), dim3(64), 0, s, nvec, conv, qmrmax, tol, q, active, phh, prr,
+                       pux, pxx, pur);
+}
+
+#define QMR_INST(S)                                                                                               \
+    template void launch_qmr_init<S>(int, long, int, const QmrVecs<S> &, double *, hipStream_t);                  \
+    template void launch_qmr_init_z<S>(int, long, const QmrVecs<S> &, double *, hipStream_t);                     \
+    template void launch_qmr_a<S>(int, long, const QmrVecs<S> &, QmrScal *, int *, double *, hipStream_t);        \
+    template void launch_qmr_b<S>(int, long, const QmrVecs<S> &, QmrScal *, int *, const double *, hipStream_t);  \
+    template void launch_qmr_c<S>(int, long, const QmrVecs<S> &, const QmrScal *, double *, hipStream_t);         \
+    template void launch_qmr_d<S>(int, long, const QmrVecs<S> &, QmrScal *, int *, const double *, double *,      \
+                                  hipStream_t);                                                                  \
+    template void launch_qmr_e<S>(int, long, const QmrVecs<S> &, QmrScal *, const double *, double *, hipStream_t); \
+    template void launch_qmr_f<S>(int, long, int, const QmrVecs<S> &, QmrScal *, int *, const double *, double *, \
+                                  double *, double *, double *, hipStream_t);                                    \
+    template void launch_qmr_g<S>(int, long, const QmrVecs<S> &, const QmrScal *, double *, double *, hipStream_t);
+QMR_INST(double)
+QMR_INST(float)
+QMR_INST(bf16_t)
+
+// ---------------------------------------------------------------- MGS / PIST
+// One modified Gram-Schmidt step (src/iterate/libiterate/orth/gsorth.f90:60-63)
+// for nw independent windows: w -= d_i v_i with d_i from the partials of the
+// previous launch, then the partials of w . v_{i+1}.  Window k's basis is
+// vj + k * ldw (columns of length n, stride ldv); w_k at w + k * ldwk.
+__global__ void __launch_bounds__(256) k_mgs_step(long n, int i, int inext, const double *vj, long ldv, long ldw,
+                                                  double *w, long ldwk, const double *part_in, double *part_out,
+                                                  const int *active)
+{
+    __shared__ double red[4];
+    const int k = blockIdx.y;
+    if (active && !active[k]) return;
+    const double *V = vj + (long)k * ldw;
+    double *W = w + (long)k * ldwk;
+    double s = 0;
+    if (i >= 0) {
+        const double d = sum_parts(part_in + (long)k * NPART);
+        const double *vi = V + (long)i * ldv;
+        if (inext >= 0) {
+            const double *vn = V + (long)inext * ldv;
+            GSTRIDE(t, n)
+            {
+                double a = W[t] - d * vi[t];
+                W[t] = a;
+                s += a * vn[t];
+            }
+        } else {
+            GSTRIDE(t, n)
+            {
+                double a = W[t] - d * vi[t];
+                W[t] = a;
+                s += a * a;
+            }
+        }
+    } else {
+        const double *vn = V + (long)inext * ldv;
+        GSTRIDE(t, n) s += W[t] * vn[t];
+    }
+    s = bsum(s, red);
+    if (threadIdx.x == 0) part_out[(long)k * NPART + blockIdx.x] = s;
+}
+
+// normalise: dst = w / ||w|| (norm from partials); status[k] = 0 when the norm is 0
+__global__ void __launch_bounds__(256) k_normalize(long n, const double *w, long ldwk, double *dst, long ldd,
+                                                   const double *part, int *ok, const int *active)
+{
+    const int k = blockIdx.y;
+    if (active && !active[k]) return;
+    const double nrm = sqrt(sum_parts(part + (long)k * NPART));
+    if (blockIdx.x == 0 && threadIdx.x == 0 && ok) ok[k] = nrm != 0.0;
+    if (nrm == 0.0) return;
+    const double *W = w + (long)k * ldwk;
+    double *D = dst + (long)k * ldd;
+    GSTRIDE(t, n) D[t] = W[t] / nrm;
+}
+
+void launch_mgs_step(int nw, long n, int i, int inext, const double *vj, long ldv, long ldw, double *w, long ldwk,
+                     const double *pin, double *pout, const int *active, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mgs_step, dim3(NPART, nw), dim3(256), 0, s, n, i, inext, vj, ldv, ldw, w, ldwk, pin, pout,
+                       active);
+}
+
+void launch_normalize(int nw, long n, const double *w, long ldwk, double *dst, long ldd, const double *part, int *ok,
+                      const int *active, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_normalize, dim3(NPART, nw), dim3(256), 0, s, n, w, ldwk, dst, ldd, part, ok, active);
+}
+
+// Multi-dot: part[k][i][blk] = partial of <V_k(:, i0+i), w_k> for i < m
+// (the Rayleigh-Ritz column of pistH0, pistHij.f90:375-381).
+__global__ void __launch_bounds__(256) k_multidot(long n, int i0, int m, const double *vj, long ldv, long ldw,
+                                                  const double *w, long ldwk, double *part, const int *active)
+{
+    __shared__ double red[4];
+    const int k = blockIdx.y;
+    if (active && !active[k]) return;
+    const double *V = vj + (long)k * ldw;
+    const double *W = w + (long)k * ldwk;
+    for (int i = 0; i < m; i++) {
+        const double *vi = V + (long)(i0 + i) * ldv;
+        double s = 0;
+        GSTRIDE(t, n) s += vi[t] * W[t];
+        s = bsum(s, red);
+        if (threadIdx.x == 0) part[((long)k * m + i) * NPART + blockIdx.x] = s;
+    }
+}
+
+// hmat_k(i0+i, j) = hmat_k(j, i0+i) = sum of partials   (upper and lower)
+__global__ void k_multidot_fin(int nw, int m, int i0, int j, const double *part, double *hmat, int ldh, long ldhk,
+                               const int *active)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nw * m) return;
+    int k = t / m, i = t % m;
+    if (active && !active[k]) return;
+    double s = sum_parts(part + ((long)k * m + i) * NPART);
+    double *H = hmat + (long)k * ldhk;
+    H[(i0 + i) + (long)ldh * j] = s;
+    H[j + (long)ldh * (i0 + i)] = s;
+}
+
+void launch_multidot(int nw, long n, int i0, int m, const double *vj, long ldv, long ldw, const double *w, long ldwk,
+                     double *part, const int *active, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_multidot, dim3(NPART, nw), dim3(256), 0, s, n, i0, m, vj, ldv, ldw, w, ldwk, part, active);
+}
+
+void launch_multidot_fin(int nw, int m, int i0, int j, const double *part, double *hmat, int ldh, long ldhk,
+                         const int *active, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_multidot_fin, dim3((nw * m + 63) / 64), dim3(64), 0, s, nw, m, i0, j, part, hmat, ldh, ldhk,
+                       active);
+}
+
+// Ritz vectors: psi(:, c) = sum_i V(:, i) Z(i, col[c])   (one window)
+__global__ void k_ritz(long n, int m, int nc, const double *vj, long ldv, const double *z, int ldz, const int *cols,
+                       double *psi, long ldp)
+{
+    const int c = blockIdx.y;
+    const int col = cols[c];
+    GSTRIDE(t, n)
+    {
+        double s = 0;
+        for (int i = 0; i < m; i++) s += vj[t + (long)ldv * i] * z[i + (long)ldz * col];
+        psi[t + (long)ldp * c] = s;
+    }
+}
+
+void launch_ritz(long n, int m, int nc, const double *vj, long ldv, const double *z, int ldz, const int *cols,
+                 double *psi, long ldp, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ritz, dim3(NPART, nc), dim3(256), 0, s, n, m, nc, vj, ldv, z, ldz, cols, psi, ldp);
+}
+
+// Hellmann-Feynman: part[(k*np + j)][blk] = partial of sum psi_k^2 dvdp_j
+__global__ void __launch_bounds__(256) k_hf_part(long n, int np, const double *psi, long ld, const double *dvdp,
+                                                 long lddv, double *part)
+{
+    __shared__ double red[4];
+    const int k = blockIdx.y;
+    const double *p = psi + (long)k * ld;
+    for (int j = 0; j < np; j++) {
+        const double *g = dvdp + (long)j * lddv;
+        double s = 0;
+        GSTRIDE(t, n) s += p[t] * p[t] * g[t];
+        s = bsum(s, red);
+        if (threadIdx.x == 0) part[((long)k * np + j) * NPART + blockIdx.x] = s;
+    }
+}
+
+void launch_hf_part(int nk, long n, int np, const double *psi, long ld, const double *dvdp, long lddv, double *part,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hf_part, dim3(NPART, nk), dim3(256), 0, s, n, np, psi, ld, dvdp, lddv, part);
+}
+
+// dv += sum_b x_b o g_b
+__global__ void k_dv_accum(long n, int nvec, const double *x, const double *g, long ld, double *dv)
+{
+    GSTRIDE(t, n)
+    {
+        double s = dv[t];
+        for (int b = 0; b < nvec; b++) s += x[t + (long)ld * b] * g[t + (long)ld * b];
+        dv[t] = s;
+    }
+}
+
+void launch_dv_accum(long n, int nvec, const double *x, const double *g, long ld, double *dv, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dv_accum, dim3(NPART), dim3(256), 0, s, n, nvec, x, g, ld, dv);
+}
+
+}  // namespace scalit

@@ -1,0 +1,133 @@
+// Internal definitions shared by the HIP sources of libscalit_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/scalit.h"
+
+#define SC_FMAX SCALIT_FMAX
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            std::fprintf(stderr, "scalit: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                             \
+            return SCALIT_EHIP;                                                                 \
+        }                                                                                       \
+    } while (0)
+
+#define SC_TRY(expr)             \
+    do {                         \
+        int r_ = (expr);         \
+        if (r_ != SCALIT_OK) return r_; \
+    } while (0)
+
+namespace scalit {
+
+using bf16_t = uint16_t;
+
+// A device buffer owned by the context.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t cnt) {
+        if (cnt <= n && p) return SCALIT_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (cnt == 0) return SCALIT_OK;
+        if (hipMalloc((void **)&p, cnt * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return SCALIT_ENOMEM;
+        }
+        n = cnt;
+        return SCALIT_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct Level {
+    int n = 0;          // sN(level)
+    bool dep = false;   // sDEP(level)
+    int dim = 1;        // myDim(level): product of inner sizes (NIN)
+    int blk = 1;        // myBlk(level): product of outer sizes (BLK)
+    DBuf<double> h0, h0t, depv, vosb;   // h0t: transposed copy for the adjoint
+    DBuf<float> h0_f, depv_f, vosb_f;
+    bool has_h0 = false, has_dep = false;
+};
+
+struct PistState;   // defined in solvers.hip
+
+}  // namespace scalit
+
+struct scalit_ctx {
+    int device = 0;
+    int sF = 0;
+    long N = 0;
+    scalit::Level lv[SC_FMAX];
+    scalit::DBuf<double> res, eig0;
+    scalit::DBuf<float> res_f, eig0_f;
+    bool has_res = false, has_precond = false;
+    double e0 = 0.0, de = 1.0, beta = 0.0;
+    hipStream_t stream = nullptr;
+    // work buffers
+    scalit::DBuf<double> work;      // generic-path ping-pong / mode-product temporaries
+    scalit::DBuf<float> work_f;
+    scalit::DBuf<double> shifts_d;  // per-vector shift staging
+    scalit::DBuf<float> shifts_f;
+    // fused 3-level fast path is available (sF==3, sDEP=(F,F,T), sizes fit)
+    bool fast3 = false;
+    scalit::PistState *pist = nullptr;
+    // bumped whenever a device buffer that operator kernels read is
+    // (re)allocated: recorded graphs of older generations are re-recorded
+    long gen = 0;
+};
+
+namespace scalit {
+
+// ---------------------------------------------------------------- kernels
+// H*X (kernels_hx.hip)
+int hx_apply_f64(scalit_ctx *c, const double *shifts_dev, double shift0, int nvec, const double *x, long ldx,
+                 double *y, long ldy, bool transpose, const int *active, hipStream_t s);
+int hx_apply_f32(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const float *x, long ldx,
+                 float *y, long ldy, const int *active, hipStream_t s);
+int hx_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const bf16_t *x, long ldx,
+                  bf16_t *y, long ldy, const int *active, hipStream_t s);
+// force the generic (any sF) path, for testing
+extern int g_force_generic;
+
+// P*X (kernels_px.hip)
+int px_apply_f64(scalit_ctx *c, int nosb, const double *shifts_dev, double shift0, int nvec, const double *x,
+                 long ldx, double *y, long ldy, const int *active, hipStream_t s);
+int px_apply_f32(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const float *x, long ldx, float *y,
+                 long ldy, const int *active, hipStream_t s);
+int px_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const bf16_t *x, long ldx,
+                  bf16_t *y, long ldy, const int *active, hipStream_t s);
+int precond_to_f32(scalit_ctx *c);
+
+// OSB set-up (kernels_diag.hip)
+int block_diag(scalit_ctx *c, int bjmax, double bjtol);
+// symmetric eigen-solve of a small dense matrix on the GPU (one workgroup).
+// a: n x n device, lda; w: n device (ascending); z: eigenvectors (device, ldz) or nullptr
+int small_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz, double *work, hipStream_t s);
+size_t small_sym_eig_work(int n);
+
+// vector helpers (kernels_vec.hip)
+constexpr int NPART = 512;   // partial sums per vector in the deterministic reductions
+void launch_dot_partials(int nvec, long n, const double *x, long ldx, const double *y, long ldy, double *part,
+                         hipStream_t s);
+void launch_reduce_partials(int nvec, const double *part, double *out, hipStream_t s);   // out[v] = sum
+int dot_host(int nvec, long n, const double *x, long ldx, const double *y, long ldy, double *out_host,
+             double *part_dev, double *red_dev, hipStream_t s);
+
+}  // namespace scalit

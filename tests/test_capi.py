@@ -1,0 +1,90 @@
+"""The C-ABI library (CPU-side checks; no device calls): it loads, exports every
+function include/scalit.h declares, and rejects bad arguments before touching
+a GPU."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from scalit_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "scalit.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = "\n".join(l for l in text.splitlines() if not l.lstrip().startswith(("typedef", "#")))
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("defined",)))
+
+
+def test_header_parses():
+    names = declared_functions()
+    assert "scalit_hx" in names and "qmr_" in names and "pist_conv_" in names and len(names) > 30
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = set(line.split()[-1] for line in out.stdout.splitlines() if line.strip())
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+    for n in declared_functions():
+        getattr(lib, n)
+    # and the ctypes signature table covers the whole header
+    assert set(declared_functions()) <= set(L.SIGNATURES)
+
+
+def test_create_rejects_bad_arguments():
+    lib = L.lib()
+    h = C.c_void_p()
+    sn = (C.c_int * 3)(4, 5, 6)
+    assert lib.scalit_create(C.byref(h), 0, sn, (C.c_int * 3)(0, 0, 1), 0) == L.SCALIT_EINVAL
+    assert lib.scalit_create(C.byref(h), 3, sn, (C.c_int * 3)(1, 0, 1), 0) == L.SCALIT_EINVAL   # sDEP(1) must be F
+    assert lib.scalit_create(C.byref(h), 3, (C.c_int * 3)(4, 0, 6), (C.c_int * 3)(0, 0, 1), 0) == L.SCALIT_EINVAL
+    assert lib.scalit_create(C.byref(h), 9, sn, (C.c_int * 3)(0, 0, 1), 0) == L.SCALIT_EINVAL
+
+
+def test_null_context_is_rejected():
+    lib = L.lib()
+    assert lib.scalit_hx(None, 0, 1, None, 1, None, 1, None, None) == L.SCALIT_EINVAL
+    assert lib.scalit_px(None, 0, 1, 1, None, 1, None, 1, None, None) == L.SCALIT_EINVAL
+    assert lib.scalit_block_diag(None, 10, 1e-5) == L.SCALIT_EINVAL
+    assert lib.scalit_destroy(None) == L.SCALIT_EINVAL
+    n = C.c_long()
+    assert lib.scalit_get_len(None, C.byref(n)) == L.SCALIT_EINVAL
+
+
+def test_fortran_layer_without_bound_context():
+    # no context bound: the drop-in entry points report failure in the reference's codes
+    lib = L.lib()
+    lib.scalit_bind(None)
+    import numpy as np
+
+    n = 5
+    w = np.ones(n)
+    vj = np.zeros((n, 2), order="F")
+    ci = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    assert lib.mgs_orth_(ci(n), dp(w), ci(1), dp(vj)) == 0
+    eig = np.zeros(2)
+    res = C.c_double()
+    r = lib.pist_conv_(C.byref(C.c_double(0.0)), C.byref(C.c_double(1e-8)), ci(0), ci(n), dp(w), ci(2), ci(1), ci(1),
+                       ci(2), ci(4), L.FOP(lib.hx0_), L.FLINSOLV(lib.osb_qmr_), dp(eig), C.byref(res))
+    assert r == 0
+
+
+def test_no_reference_or_oracle_in_product():
+    # the product library never links the oracle nor the reference checker
+    out = subprocess.run(["ldd", L.LIB_PATH], capture_output=True, text=True).stdout
+    assert "oracle" not in out and "scalit_ref" not in out
+    src = os.path.join(ROOT, "scalit_amd")
+    for dp_, _, files in os.walk(src):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
+                t = open(os.path.join(dp_, f)).read()
+                assert "import oracle" not in t and "from oracle" not in t, f

@@ -1,0 +1,239 @@
+"""GPU parity of QMR / PIST / Lanczos against the oracle and the golden
+eigenvalues, through the C ABI.
+
+Eigenvalue tolerance: 1e-10 relative (FP64 PIST and Lanczos against the
+fixture), 1e-8 relative for mixed precision.  QMR: the same return code as
+the oracle's restatement of qmr.f90 (+-1 where the last residual sits within
+rounding of the tolerance) and solutions that agree to the solve tolerance."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from scalit_amd import _lib as L
+from scalit_amd import problems as P
+from scalit_amd.context import Context
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "pist_fp64_config1.json")
+
+
+def gold():
+    with open(GOLD) as f:
+        return json.load(f)
+
+
+def op_ctx(grid, e0, seed=1234, kind="jitter"):
+    pot = P.jittered_potential(seed) if kind == "jitter" else P.gaussian_potential(seed)
+    prob = P.build_problem(*grid, pot, seed=seed)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=e0)
+    ctx = Context.from_problem(prob, e0=e0)
+    return prob, op, ctx
+
+
+@pytest.mark.parametrize("qmr_type", [-1, 0, -2])
+def test_qmr_matches_oracle(qmr_type):
+    print(f"qmr type={qmr_type}", flush=True)
+    prob, op, ctx = op_ctx((24, 32, 24), e0=-0.003)
+    vosb, eig0 = op.diag()
+    ctx.set_precond(vosb, eig0)
+    b = P.random_vectors(prob.N, 1, 31)
+    r_o, x_o, res_o, _ = op.qmr(b[0], qmr_type=qmr_type, qmr_max=300, qmr_tol=1e-9)
+    ret, x, res = ctx.qmr(ctx.to_device(b), ntype=qmr_type, qmrmax=300, qmrtol=1e-9)
+    assert r_o > 0 and abs(int(ret[0]) - r_o) <= 1
+    x = x.cpu().numpy()[0]
+    assert np.abs(x - x_o).max() <= 1e-7 * np.abs(x_o).max()
+    h = op.hx(x, O.TA1)
+    assert np.linalg.norm(h - b[0]) <= 1e-8 * np.linalg.norm(h)
+    ctx.close()
+
+
+def test_qmr_batched_shifts_and_breakdown():
+    prob, op, ctx = op_ctx((24, 32, 24), e0=-0.003)
+    vosb, eig0 = op.diag()
+    ctx.set_precond(vosb, eig0)
+    b = P.random_vectors(prob.N, 4, 32)
+    b[2] = 0.0                       # zero right-hand side: QMR returns 0 at once
+    shifts = [-0.003, -0.003, -0.003, -0.001]
+    ret, x, res = ctx.qmr(ctx.to_device(b), ntype=-1, qmrmax=300, qmrtol=1e-8, shifts=shifts)
+    assert ret[2] == 0
+    x = x.cpu().numpy()
+    for v in (0, 1, 3):
+        op.set_e0(shifts[v])
+        r_o, x_o, _, _ = op.qmr(b[v], qmr_type=-1, qmr_max=300, qmr_tol=1e-8)
+        assert abs(int(ret[v]) - r_o) <= 1
+        assert np.abs(x[v] - x_o).max() <= 1e-6 * np.abs(x_o).max()
+    # iteration limit: -QMRMAX
+    ret, _, _ = ctx.qmr(ctx.to_device(b[:1]), ntype=-1, qmrmax=2, qmrtol=1e-14)
+    assert ret[0] == -2
+    ctx.close()
+
+
+def test_pist_fp64_config1_against_golden():
+    g = gold()
+    prob = P.config_problem("pist_fp64")
+    ctx = Context.from_problem(prob, e0=g["e0"])
+    ctx.block_diag(1000, 1e-5)
+    x0 = ctx.to_device(P.random_vectors(prob.N, 1, g["start_seed"]))
+    r, eig, res, stats = ctx.pist_conv(x0, g["e0"], g["etol"], g["m"], g["m_max"], g["startcnt"], g["stepcnt"],
+                                       g["stepeig"], qmrtol=g["qmr_tol"])
+    print("pist", r, res, stats, flush=True)
+    assert r > 0
+    ref = np.array(g["eigsh"])
+    assert np.abs(eig - ref).max() / np.abs(ref).max() <= 1e-10
+    assert (np.abs(eig - ref) / np.abs(ref)).max() <= 1e-9
+    # Ritz vectors: unit norm, Rayleigh quotients reproduce the eigenvalues
+    psi = ctx.pist_vectors(0, g["m"])
+    hp = ctx.hx(psi)
+    rq = (psi * hp).sum(dim=1).cpu().numpy()
+    nrm = (psi * psi).sum(dim=1).cpu().numpy()
+    assert np.allclose(nrm, 1.0, atol=1e-10)
+    assert np.abs(rq - eig).max() <= 1e-9 * np.abs(eig).max()
+    ctx.close()
+
+
+def test_pist_parameter_errors():
+    prob = P.config_problem("pist_fp64", grid=(8, 9, 6))
+    ctx = Context.from_problem(prob, e0=0.0)
+    ctx.block_diag()
+    x0 = ctx.to_device(np.ones((1, prob.N)))
+    assert ctx.pist_conv(x0, 0.0, 1e-8, 5, 5, 2, 1, stepeig=5)[0] == 0
+    assert ctx.pist_conv(x0, 0.0, 1e-8, 6, 5, 2, 1)[0] == 0
+    ctx.close()
+
+
+def test_pist_mixed_precision_config1():
+    g = gold()
+    prob = P.config_problem("pist_fp64")
+    ctx = Context.from_problem(prob, e0=g["e0"])
+    ctx.block_diag(1000, 1e-5)
+    x0 = ctx.to_device(P.random_vectors(prob.N, 1, g["start_seed"]))
+    ref = np.array(g["eigsh"])
+    for prec in (L.FP32, L.BF16):
+        r, eig, res, stats = ctx.pist_conv(x0, g["e0"], 1e-10, g["m"], g["m_max"], g["startcnt"], g["stepcnt"],
+                                           g["stepeig"], qmrtol=1e-6, prec=prec)
+        print("mixed", prec, r, res, stats, flush=True)
+        assert r > 0
+        assert (np.abs(eig - ref) / np.abs(ref)).max() <= 1e-8
+    ctx.close()
+
+
+def test_pist_batched_windows():
+    g = gold()
+    prob = P.config_problem("pist_fp64")
+    ctx = Context.from_problem(prob, e0=g["e0"])
+    ctx.block_diag(1000, 1e-5)
+    ref = np.array(g["eigsh"])
+    e0s = [g["e0"], 0.0040, 0.0055]
+    x0 = ctx.to_device(P.random_vectors(prob.N, 3, 5))
+    ret, eig, res, stats = ctx.pist_conv_batched(x0, e0s, 1e-10, 4, 120, 20, 10, 3, qmrtol=1e-6)
+    print("batched", ret, res, stats, flush=True)
+    assert (ret > 0).all()
+    for k, e0 in enumerate(e0s):
+        # the 4 eigenvalues nearest E0 of the golden set
+        want = np.sort(ref[np.argsort(np.abs(ref - e0), kind="stable")[:4]])
+        assert (np.abs(eig[k] - want) / np.abs(want)).max() <= 1e-9
+    ctx.close()
+
+
+def test_lanczos_small_grid():
+    prob = P.config_problem("pist_fp64", grid=(12, 14, 10))
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    ev = np.linalg.eigvalsh(op.dense())
+    ctx = Context.from_problem(prob)
+    x0 = ctx.to_device(P.random_vectors(prob.N, 1, 3))
+    r, eig, res = ctx.lan_conv(x0, float(ev[0]) - 0.01, 1e-12, 5, 600, 60, 30, 5)
+    assert r > 0, r
+    assert (np.abs(eig - ev[:5]) / np.abs(ev[:5])).max() <= 1e-10
+    r_o, eig_o, _ = op.lan_conv(P.random_vectors(prob.N, 1, 3)[0], float(ev[0]) - 0.01, 1e-12, 5, 600, 60, 30, 5)
+    assert np.abs(eig - eig_o).max() <= 1e-10 * np.abs(eig_o).max()
+    ctx.close()
+
+
+def test_hellmann_feynman_gradients():
+    # dE_k/dp by HF on the GPU against central finite differences of the oracle's dense spectrum
+    grid = (10, 12, 8)
+    pot = P.gaussian_potential(9)
+    prob = P.build_problem(*grid, pot, seed=9)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    ev = np.linalg.eigvalsh(op.dense())
+    e0 = float(ev[0]) - 0.002
+    ctx = Context.from_problem(prob, e0=e0)
+    ctx.block_diag(1000, 1e-8)
+    x0 = ctx.to_device(P.random_vectors(prob.N, 1, 4))
+    m = 4
+    r, eig, _, _ = ctx.pist_conv(x0, e0, 1e-12, m, 150, 20, 10, 4, qmrtol=1e-9)
+    assert r > 0
+    psi = ctx.pist_vectors(0, m)
+    dvdp = ctx.to_device(pot.gradient(prob.r, prob.R, prob.cos_theta))
+    grad = ctx.eig_gradients(psi, dvdp)
+    p = pot.params()
+    h = 1e-5
+    for j in (0, 2, 4, 6, 7):
+        pp, pm = p.copy(), p.copy()
+        pp[j] += h
+        pm[j] -= h
+        ep = np.linalg.eigvalsh(O.Operator(prob.sN, prob.h0, prob.with_potential(pot.with_params(pp)).res,
+                                           prob.dep).dense())[:m]
+        em = np.linalg.eigvalsh(O.Operator(prob.sN, prob.h0, prob.with_potential(pot.with_params(pm)).res,
+                                           prob.dep).dense())[:m]
+        fd = (ep - em) / (2 * h)
+        assert np.abs(grad[:, j] - fd).max() <= 1e-6 * max(1.0, np.abs(fd).max()), (j, grad[:, j], fd)
+    ctx.close()
+
+
+def test_fortran_layer_drop_in():
+    # the reference's call pattern: QMR(nType,QMRMAX,QMRTOL,N,B,HX,PX,X,RES) on host arrays
+    prob, op, ctx = op_ctx((12, 14, 10), e0=-0.002)
+    vosb, eig0 = op.diag()
+    ctx.set_precond(vosb, eig0)
+    ctx.bind()
+    lib = L.lib()
+    n = prob.N
+    b = P.random_vectors(n, 1, 2)[0].copy()
+    x = np.zeros(n)
+    res = C.c_double(0)
+    ci = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    ret = lib.qmr_(ci(-1), ci(300), C.byref(C.c_double(1e-9)), ci(n), O.dptr(b), L.FOP(lib.hx_), L.FOP(lib.px_),
+                   O.dptr(x), C.byref(res))
+    r_o, x_o, _, _ = op.qmr(b, qmr_type=-1, qmr_max=300, qmr_tol=1e-9)
+    assert abs(ret - r_o) <= 1 and np.abs(x - x_o).max() <= 1e-7 * np.abs(x_o).max()
+    # foreign host callbacks (the oracle's operator): the recurrence stays on the GPU
+    x2 = np.zeros(n)
+
+    def hx_cb(pn, px_in, py):
+        xi = np.ctypeslib.as_array(px_in, shape=(pn[0],))
+        np.ctypeslib.as_array(py, shape=(pn[0],))[:] = op.hx(xi, O.TA1)
+
+    def px_cb(pn, px_in, py):
+        xi = np.ctypeslib.as_array(px_in, shape=(pn[0],))
+        np.ctypeslib.as_array(py, shape=(pn[0],))[:] = op.px(xi)
+
+    f_hx, f_px = L.FOP(hx_cb), L.FOP(px_cb)
+    ret2 = lib.qmr_(ci(-1), ci(300), C.byref(C.c_double(1e-9)), ci(n), O.dptr(b), f_hx, f_px, O.dptr(x2),
+                    C.byref(res))
+    assert abs(ret2 - r_o) <= 1 and np.abs(x2 - x_o).max() <= 1e-7 * np.abs(x_o).max()
+    # MGS_ORTH
+    vj = np.zeros((n, 4), order="F")
+    q, _ = np.linalg.qr(np.random.default_rng(0).standard_normal((n, 3)))
+    vj[:, :3] = q
+    w = np.random.default_rng(1).standard_normal(n)
+    vo = vj.copy(order="F")
+    assert lib.mgs_orth_(ci(n), O.dptr(w), ci(3), O.dptr(vj)) == 1
+    O.mgs_orth(w, vo, 3)
+    assert np.abs(vj - vo).max() <= 1e-12
+    # PIST_CONV through the Fortran layer with HX0 / OSB_QMR
+    lib.scalit_set_qmr_params(-1, 300, 1e-8)
+    eig = np.zeros(3)
+    x0 = P.random_vectors(n, 1, 6)[0].copy()
+    r = lib.pist_conv_(C.byref(C.c_double(-0.002)), C.byref(C.c_double(1e-10)), ci(0), ci(n), O.dptr(x0), ci(12),
+                       ci(4), ci(2), ci(3), ci(100), L.FOP(lib.hx0_), L.FLINSOLV(lib.osb_qmr_), O.dptr(eig),
+                       C.byref(res))
+    ev = np.linalg.eigvalsh(op.dense())
+    want = np.sort(ev[np.argsort(np.abs(ev + 0.002), kind="stable")[:3]])
+    assert r > 0 and np.abs(eig - want).max() <= 1e-10 * np.abs(want).max()
+    ctx.close()
