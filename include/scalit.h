@@ -9,7 +9,9 @@
  *    the reference's module globals H0/RES/DEP/VOSB/EIG0,
  *    src/iterate/module/osb.data.h:28-69).  Vector arguments are DEVICE
  *    pointers; nvec vectors of length N are stored with leading dimension ld.
- *    Kernels are enqueued on the given HIP stream (NULL = the context's own).
+ *    Kernels are enqueued on the given HIP stream (NULL = the default stream);
+ *    the solvers, which return host results, synchronise the device on entry
+ *    and run on the context's own stream.
  *
  * 2. Fortran drop-in layer (lower case + trailing underscore, all arguments by
  *    reference, HOST arrays, column-major, 1-based semantics): the callbacks
@@ -204,6 +206,10 @@ int mgs_orth_(const int *n, const double *wj, const int *m, double *vj);
 /* Force (on != 0) the generic any-level H*X kernels instead of the fused
  * 3-level path, for parity tests of both. */
 int scalit_debug_force_generic(int on);
+/* Lane layout of the 16x16x4 MFMA builtins (prec SCALIT_FP64 / SCALIT_FP32):
+ * out (host, 256 doubles) receives accumulator q of lane l at 4 l + q for
+ * D(i,j) = 100 (i+1) + (j+1). */
+int scalit_debug_mfma_probe(int prec, double *out);
 /* Launch one pass of the fused FP64 H*X alone (1: modes 1+2 and the diagonal,
  * 2: mode 3 accumulate) so that each kernel can be timed by itself. */
 int scalit_profile_hx_pass(scalit_ctx *ctx, int pass, int nvec, const double *x, long ldx, double *y, long ldy,

@@ -69,6 +69,7 @@ SIGNATURES = {
     "mgs_orth_": (_i, [_ip, _dp, _ip, _dp]),
     "scalit_debug_force_generic": (_i, [_i]),
     "scalit_profile_hx_pass": (_i, [_vp, _i, _i, _vp, _l, _vp, _l, _vp]),
+    "scalit_debug_mfma_probe": (_i, [_i, _dp]),
 }
 
 _lib = None

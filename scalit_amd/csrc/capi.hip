@@ -32,6 +32,7 @@ int qmr_host_cb(scalit_ctx *c, int ntype, int qmrmax, double tol, long n, const 
                 scalit_fop hx, scalit_fop px, int *ret, double *res);
 int mgs_host(scalit_ctx *c, long n, const double *wj, int m, double *vj, int *ok);
 int hx_pass_f64(scalit_ctx *c, int pass, int nvec, const double *x, long ldx, double *y, long ldy, hipStream_t s);
+int mfma_probe(int prec, double *out_host);
 
 __global__ void k_d2f(long n, const double *a, float *b)
 {
@@ -99,7 +100,12 @@ static void drop_f32(scalit_ctx *c)
 
 using namespace scalit;
 
-static hipStream_t pick(scalit_ctx *c, scalit_stream s) { return s ? (hipStream_t)s : c->stream; }
+// NULL is the legacy default stream (what torch reports for its default stream)
+static hipStream_t pick(scalit_ctx *c, scalit_stream s)
+{
+    (void)c;
+    return (hipStream_t)s;
+}
 
 #define CTX_GUARD(c)                                    \
     do {                                                \
@@ -361,7 +367,8 @@ int scalit_qmr(scalit_ctx *c, int ntype, int qmrmax, double qmrtol, int nvec, co
     SC_TRY(operator_ready(c));
     if (!c->has_precond) return SCALIT_ESTATE;
     if (nvec < 1 || !b || !x || !ret || ld < c->N || qmrmax < 0) return SCALIT_EINVAL;
-    if (stream) HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    (void)stream;
+    HIP_TRY(hipDeviceSynchronize());   // order after whatever the caller enqueued on any stream
     return qmr_f64(c, ntype, qmrmax, qmrtol, nvec, b, x, ld, shifts, ret, res, c->stream);
 }
 
@@ -374,7 +381,8 @@ int scalit_qmr_mixed(scalit_ctx *c, int prec, int qmrmax, double qmrtol, int inn
     if (!c->has_precond) return SCALIT_ESTATE;
     if (prec != SCALIT_FP32 && prec != SCALIT_BF16) return SCALIT_EINVAL;
     if (nvec < 1 || !b || !x || !ret || ld < c->N || qmrmax < 1 || inner_max < 1) return SCALIT_EINVAL;
-    if (stream) HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    (void)stream;
+    HIP_TRY(hipDeviceSynchronize());   // order after whatever the caller enqueued on any stream
     return qmr_mixed(c, prec, -1, qmrmax, qmrtol, inner_max, inner_tol, nvec, b, ld, x, ld, shifts, ret, res,
                      c->stream);
 }
@@ -589,6 +597,12 @@ int scalit_debug_force_generic(int on)
 {
     g_force_generic = on;
     return SCALIT_OK;
+}
+
+int scalit_debug_mfma_probe(int prec, double *out)
+{
+    if (!out) return SCALIT_EINVAL;
+    return mfma_probe(prec, out);
 }
 
 int scalit_profile_hx_pass(scalit_ctx *c, int pass, int nvec, const double *x, long ldx, double *y, long ldy,

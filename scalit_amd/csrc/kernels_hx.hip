@@ -24,6 +24,10 @@ int g_force_generic = 0;
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
+// 16x16x4 MFMA: A(i,k) comes from lane i + 16 k, B(k,j) from lane j + 16 k;
+// accumulator q of lane l holds D(row(l/16, q), l % 16), with the rows
+// interleaved for FP64 and blocked for FP32 (checked on the device by
+// tests/test_gpu_mfma_layout.py).
 template <typename T>
 struct Mfma;
 template <>
@@ -33,6 +37,7 @@ struct Mfma<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int q) { return lk + 4 * q; }
 };
 template <>
 struct Mfma<float> {
@@ -41,6 +46,7 @@ struct Mfma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int q) { return 4 * lk + q; }
 };
 
 // storage <-> compute conversions
@@ -230,7 +236,7 @@ __global__ void __launch_bounds__(256) k_hx_modes12(int n1, int n2, int n3, int 
         const int i1 = c0 + lr;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int i2 = r0 + 4 * lk + q;
+            const int i2 = r0 + M::row(lk, q);
             if (i1 < n1 && i2 < n2) {
                 long o = i1 + (long)n1 * i2;
                 T sv = sS[i1 + ldS * i2];
@@ -291,7 +297,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_mode3(int n1, int n2, int n3,
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int i3 = r0 + 4 * lk + q;
+            const int i3 = r0 + M::row(lk, q);
             if (pin && i3 < n3) {
                 long o = p + P * i3;
                 st_cvt<T, S>(yv, o, ld_cvt<T, S>(yv, o) + dp * acc[q]);
@@ -396,6 +402,34 @@ int hx_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec
 }
 
 bool fast3_ok(const scalit_ctx *c) { return fast3_supported<double>(c); }
+
+// Lane-layout probe of the 16x16x4 MFMA builtins: one wave computes
+// D = A B with A(i,0) = 100 (i+1), A(i,1) = 1, B(0,j) = 1, B(1,j) = j+1 (other
+// k zero) from the fragment mapping the kernels use, so D(i,j) = 100(i+1) +
+// (j+1) names its own row and column; out[4 lane + q] = accumulator q.
+template <typename T>
+__global__ void __launch_bounds__(64) k_mfma_probe(double *out)
+{
+    using M = Mfma<T>;
+    const int lane = threadIdx.x, r = lane & 15, k = lane >> 4;
+    T a = (k == 0) ? (T)(100.0 * (r + 1)) : (k == 1 ? (T)1 : (T)0);
+    T b = (k == 0) ? (T)1 : (k == 1 ? (T)(r + 1) : (T)0);
+    typename M::acc_t acc = {0, 0, 0, 0};
+    acc = M::run(a, b, acc);
+    for (int q = 0; q < 4; q++) out[4 * lane + q] = (double)acc[q];
+}
+
+int mfma_probe(int prec, double *out_host)
+{
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, 256 * sizeof(double)));
+    if (prec == SCALIT_FP32) hipLaunchKernelGGL(k_mfma_probe<float>, dim3(1), dim3(64), 0, 0, d);
+    else hipLaunchKernelGGL(k_mfma_probe<double>, dim3(1), dim3(64), 0, 0, d);
+    hipError_t e = hipMemcpy(out_host, d, 256 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    HIP_TRY(e);
+    return SCALIT_OK;
+}
 
 // One pass of the fused FP64 path alone (pass 1: modes 1+2+diagonal, pass 2:
 // mode 3), for per-kernel timing.

@@ -74,7 +74,7 @@ def test_fortran_layer_without_bound_context():
     eig = np.zeros(2)
     res = C.c_double()
     r = lib.pist_conv_(C.byref(C.c_double(0.0)), C.byref(C.c_double(1e-8)), ci(0), ci(n), dp(w), ci(2), ci(1), ci(1),
-                       ci(2), ci(4), L.FOP(lib.hx0_), L.FLINSOLV(lib.osb_qmr_), dp(eig), C.byref(res))
+                       ci(2), ci(4), C.cast(lib.hx0_, L.FOP), C.cast(lib.osb_qmr_, L.FLINSOLV), dp(eig), C.byref(res))
     assert r == 0
 
 

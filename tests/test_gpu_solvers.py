@@ -198,7 +198,7 @@ def test_fortran_layer_drop_in():
     x = np.zeros(n)
     res = C.c_double(0)
     ci = lambda v: C.byref(C.c_int(v))  # noqa: E731
-    ret = lib.qmr_(ci(-1), ci(300), C.byref(C.c_double(1e-9)), ci(n), O.dptr(b), L.FOP(lib.hx_), L.FOP(lib.px_),
+    ret = lib.qmr_(ci(-1), ci(300), C.byref(C.c_double(1e-9)), ci(n), O.dptr(b), C.cast(lib.hx_, L.FOP), C.cast(lib.px_, L.FOP),
                    O.dptr(x), C.byref(res))
     r_o, x_o, _, _ = op.qmr(b, qmr_type=-1, qmr_max=300, qmr_tol=1e-9)
     assert abs(ret - r_o) <= 1 and np.abs(x - x_o).max() <= 1e-7 * np.abs(x_o).max()
@@ -231,7 +231,7 @@ def test_fortran_layer_drop_in():
     eig = np.zeros(3)
     x0 = P.random_vectors(n, 1, 6)[0].copy()
     r = lib.pist_conv_(C.byref(C.c_double(-0.002)), C.byref(C.c_double(1e-10)), ci(0), ci(n), O.dptr(x0), ci(12),
-                       ci(4), ci(2), ci(3), ci(100), L.FOP(lib.hx0_), L.FLINSOLV(lib.osb_qmr_), O.dptr(eig),
+                       ci(4), ci(2), ci(3), ci(100), C.cast(lib.hx0_, L.FOP), C.cast(lib.osb_qmr_, L.FLINSOLV), O.dptr(eig),
                        C.byref(res))
     ev = np.linalg.eigvalsh(op.dense())
     want = np.sort(ev[np.argsort(np.abs(ev + 0.002), kind="stable")[:3]])
