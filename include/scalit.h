@@ -210,6 +210,13 @@ int scalit_debug_force_generic(int on);
  * out (host, 256 doubles) receives accumulator q of lane l at 4 l + q for
  * D(i,j) = 100 (i+1) + (j+1). */
 int scalit_debug_mfma_probe(int prec, double *out);
+/* MFMA throughput probe: `blocks` workgroups of 4 waves, each wave `iters`
+ * rounds of 8 independent 16x16x4 chains from registers; *ms = kernel time
+ * (FLOP = blocks * 4 * iters * 8 * 2048). */
+int scalit_debug_mfma_peak(int prec, int iters, int blocks, float *ms);
+/* The fused H*X kernel variants (pass A block shape, pass B width) the
+ * autotuner picked for this precision and batch size (tuning them if needed). */
+int scalit_debug_hx_variant(scalit_ctx *ctx, int prec, int nvec, int *va, int *vb);
 /* Launch one pass of the fused FP64 H*X alone (1: modes 1+2 and the diagonal,
  * 2: mode 3 accumulate) so that each kernel can be timed by itself. */
 int scalit_profile_hx_pass(scalit_ctx *ctx, int pass, int nvec, const double *x, long ldx, double *y, long ldy,

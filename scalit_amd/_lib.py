@@ -70,6 +70,8 @@ SIGNATURES = {
     "scalit_debug_force_generic": (_i, [_i]),
     "scalit_profile_hx_pass": (_i, [_vp, _i, _i, _vp, _l, _vp, _l, _vp]),
     "scalit_debug_mfma_probe": (_i, [_i, _dp]),
+    "scalit_debug_mfma_peak": (_i, [_i, _i, _i, C.POINTER(C.c_float)]),
+    "scalit_debug_hx_variant": (_i, [_vp, _i, _i, _ip, _ip]),
 }
 
 _lib = None

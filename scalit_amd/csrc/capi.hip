@@ -33,6 +33,7 @@ int qmr_host_cb(scalit_ctx *c, int ntype, int qmrmax, double tol, long n, const 
 int mgs_host(scalit_ctx *c, long n, const double *wj, int m, double *vj, int *ok);
 int hx_pass_f64(scalit_ctx *c, int pass, int nvec, const double *x, long ldx, double *y, long ldy, hipStream_t s);
 int mfma_probe(int prec, double *out_host);
+int mfma_peak(int prec, int iters, int blocks, float *ms_out);
 
 __global__ void k_d2f(long n, const double *a, float *b)
 {
@@ -147,6 +148,9 @@ int scalit_create(scalit_ctx **out, int sF, const int *sN, const int *sDEP, int 
         return SCALIT_EHIP;
     }
     c->fast3 = fast3_ok(c);
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        g_num_cu = ncu;
     *out = c;
     return SCALIT_OK;
 }
@@ -597,6 +601,20 @@ int scalit_debug_force_generic(int on)
 {
     g_force_generic = on;
     return SCALIT_OK;
+}
+
+int scalit_debug_hx_variant(scalit_ctx *c, int prec, int nvec, int *va, int *vb)
+{
+    CTX_GUARD(c);
+    if (!va || !vb || nvec < 1) return SCALIT_EINVAL;
+    SC_TRY(hx_prepare(c, prec, nvec, nullptr));
+    return hx_variants(c, prec, nvec, va, vb);
+}
+
+int scalit_debug_mfma_peak(int prec, int iters, int blocks, float *ms)
+{
+    if (!ms || iters < 1 || blocks < 1) return SCALIT_EINVAL;
+    return mfma_peak(prec, iters, blocks, ms);
 }
 
 int scalit_debug_mfma_probe(int prec, double *out)

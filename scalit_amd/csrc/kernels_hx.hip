@@ -10,16 +10,24 @@
 //    DEP(r,R) x L^2_theta -- on FP64 / FP32 MFMA:
 //      pass A, one workgroup per (theta point, vector) slab S(n1 x n2) staged
 //      in LDS: Y = T1 S + S T2^T + (V - s) o S   (modes 1 and 2 + diagonal)
-//      pass B, one wave per 16 grid points p=(r,R) across all theta:
+//      pass B, one wave per 16*PT grid points p=(r,R) across all theta:
 //      Y(p,:) += DEP(p) * L2 X(p,:)              (mode 3)
 //    Tiles are computed transposed (D = O^T) so that the 16 lanes of a
 //    fragment column run along the contiguous r index: every global load and
-//    store of X, Y and V moves 128-byte runs.
+//    store of X, Y and V moves 128-byte runs.  Each wave accumulates a block
+//    of RB x CB output tiles at once (independent MFMA chains, fragment
+//    reuse); the block shape and wave count of each pass are chosen per
+//    problem and batch size by timing the candidates on the device (autotune).
+#include <cstdlib>
+#include <algorithm>
+#include <cstring>
+
 #include "scalit_impl.hpp"
 
 namespace scalit {
 
 int g_force_generic = 0;
+int g_num_cu = 256;   // compute units of the device, set when a context is created
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
@@ -138,7 +146,7 @@ static int hx_generic(scalit_ctx *c, const T *res, const T *shifts, T shift0, in
             h = (const T *)(transpose ? L.h0t.p : L.h0.p);
             if (L.dep) dep = (const T *)L.depv.p;
         } else {
-            h = (const T *)L.h0_f.p;   // fp32 copies are symmetric-agnostic: only the forward product
+            h = (const T *)L.h0_f.p;   // the fp32 copies serve the forward product only
             if (L.dep) dep = (const T *)L.depv_f.p;
         }
         hipLaunchKernelGGL((k_mode_generic<T, S>), g, dim3(th), 0, s, L.dim, L.n, L.blk, h, dep, transpose, x, ldx,
@@ -162,15 +170,39 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk)
     return (bid & 7) * (nblk >> 3) + (bid >> 3);
 }
 
-// Pass A.  Grid: n3 * nvec workgroups of 256 threads (4 waves).
-// LDS: S (n1p+1) x n2p, and T1 n1p x n1p, T2 n2p x n2p when T_LDS.
-template <typename T, typename S, bool T_LDS>
-__global__ void __launch_bounds__(256) k_hx_modes12(int n1, int n2, int n3, int nvec, const T *__restrict__ t1,
-                                                   const T *__restrict__ t2, const T *__restrict__ res,
-                                                   const T *__restrict__ shifts, T shift0, const S *__restrict__ x,
-                                                   long ldx, S *__restrict__ y, long ldy, const int *active)
+// Slab s of the persistent pass-A schedule processed by workgroup g of G:
+// workgroups are spread round-robin over the 8 XCDs, so XCD x gets the
+// contiguous slab range [x S/8, (x+1) S/8) and the vectors sharing a theta
+// point (and its slice of V) stay in one XCD's L2.
+__device__ __forceinline__ int slab_of(int g, int k, int G, int S)
+{
+    if ((G & 7) == 0) {
+        const int x = g & 7, loc = g >> 3, nloc = G >> 3;
+        const int lo = (int)(((long)S * x) / 8), hi = (int)(((long)S * (x + 1)) / 8);
+        const int s = lo + loc + k * nloc;
+        return s < hi ? s : -1;
+    }
+    const int s = g + k * G;
+    return s < S ? s : -1;
+}
+
+// Pass A.  Persistent grid of G workgroups of 64*WAVES threads; workgroup g
+// walks its slabs (theta point i3, vector v), slab s = i3 * nvec + v.  LDS:
+// S (n1p+1) x n2p, plus T1 n1p x n1p and T2 n2p x n2p when T_LDS, staged once
+// per workgroup.  The next slab is loaded into registers while the current one
+// is computed.  Each wave owns blocks of RB (i2) x CB (i1) output tiles.
+// ACCUM adds the incoming Y (the mode-3 pass runs first).
+template <typename T, typename S, bool T_LDS, int WAVES, int RB, int CB, bool ACCUM>
+__global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n3, int nvec, const T *__restrict__ t1,
+                                                          const T *__restrict__ t2, const T *__restrict__ res,
+                                                          const T *__restrict__ shifts, T shift0,
+                                                          const S *__restrict__ x, long ldx, S *__restrict__ y,
+                                                          long ldy, const int *active)
 {
     using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    constexpr int NT = 64 * WAVES;
+    constexpr int PFMAX = 16;   // slab values per thread prefetched in registers; the rest load at slab start
     extern __shared__ char smem_raw[];
     T *sm = (T *)smem_raw;
     const int n1p = rup16(n1), n2p = rup16(n2);
@@ -178,82 +210,157 @@ __global__ void __launch_bounds__(256) k_hx_modes12(int n1, int n2, int n3, int 
     T *sS = sm;
     T *sT1 = sS + ldS * n2p;
     T *sT2 = sT1 + n1p * n1p;
-
-    const int logical = xcd_remap(blockIdx.x, gridDim.x);
-    const int v = logical % nvec;
-    const int i3 = logical / nvec;
-    if (!vec_active(active, v)) return;
     const long P = (long)n1 * n2;
-    const S *xs = x + (long)v * ldx + P * i3;
-    S *ys = y + (long)v * ldy + P * i3;
-    const T *rs = res + P * i3;
-    const T sh = shifts ? shifts[v] : shift0;
     const int tid = threadIdx.x;
+    const int nslab = n3 * nvec;
+    const int G = gridDim.x, g = blockIdx.x;
+    const int npf = (n1p * n2p + NT - 1) / NT;
 
-    // stage the slab (coalesced along r), zero the padding
-    for (int e = tid; e < n1p * n2p; e += 256) {
-        int i1 = e % n1p, i2 = e / n1p;
-        T val = 0;
-        if (i1 < n1 && i2 < n2) val = ld_cvt<T, S>(xs, i1 + (long)n1 * i2);
-        sS[i1 + ldS * i2] = val;
-    }
     if constexpr (T_LDS) {
-        for (int e = tid; e < n1p * n1p; e += 256) {
+        for (int e = tid; e < n1p * n1p; e += NT) {
             int a = e % n1p, b = e / n1p;
             sT1[e] = (a < n1 && b < n1) ? t1[a + (long)n1 * b] : (T)0;
         }
-        for (int e = tid; e < n2p * n2p; e += 256) {
+        for (int e = tid; e < n2p * n2p; e += NT) {
             int a = e % n2p, b = e / n2p;
             sT2[e] = (a < n2 && b < n2) ? t2[a + (long)n2 * b] : (T)0;
         }
     }
-    __syncthreads();
+    // first slab of this workgroup that is active
+    int k = 0, s = slab_of(g, 0, G, nslab);
+    while (s >= 0 && !vec_active(active, s % nvec)) s = slab_of(g, ++k, G, nslab);
+    T pf[PFMAX];
+    auto prefetch = [&](int sl) {
+        const S *xs = x + (long)(sl % nvec) * ldx + P * (sl / nvec);
+#pragma unroll
+        for (int j = 0; j < PFMAX; j++) {
+            const int e = tid + j * NT;
+            if (j < npf) {
+                const int i1 = e % n1p, i2 = e / n1p;
+                pf[j] = (i1 < n1 && i2 < n2 && e < n1p * n2p) ? ld_cvt<T, S>(xs, i1 + (long)n1 * i2) : (T)0;
+            }
+        }
+    };
+    if (s >= 0) prefetch(s);
 
     const int lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int nrt = n2p / 16, nct = n1p / 16;
-    for (int t = wave; t < nrt * nct; t += 4) {
-        const int r0 = (t / nct) * 16;   // i2 rows of the transposed tile
-        const int c0 = (t % nct) * 16;   // i1 columns
-        typename M::acc_t acc = {0, 0, 0, 0};
-        // mode 1: D(i2,i1) += sum_k S(k,i2) T1(i1,k)
-        for (int k0 = 0; k0 < n1p; k0 += 4) {
-            T a = sS[(k0 + lk) + ldS * (r0 + lr)];
-            T b;
-            if constexpr (T_LDS) b = sT1[(c0 + lr) + n1p * (k0 + lk)];
-            else b = (c0 + lr < n1 && k0 + lk < n1) ? t1[(c0 + lr) + (long)n1 * (k0 + lk)] : (T)0;
-            acc = M::run(a, b, acc);
-        }
-        // mode 2: D(i2,i1) += sum_k T2(i2,k) S(i1,k)
-        for (int k0 = 0; k0 < n2p; k0 += 4) {
-            T a;
-            if constexpr (T_LDS) a = sT2[(r0 + lr) + n2p * (k0 + lk)];
-            else a = (r0 + lr < n2 && k0 + lk < n2) ? t2[(r0 + lr) + (long)n2 * (k0 + lk)] : (T)0;
-            T b = sS[(c0 + lr) + ldS * (k0 + lk)];
-            acc = M::run(a, b, acc);
-        }
-        // epilogue: + (V - s) S, store
-        const int i1 = c0 + lr;
+    const int nrg = (nrt + RB - 1) / RB, ncg = (nct + CB - 1) / CB;
+    while (s >= 0) {
+        // the prefetched slab into LDS (padding zero)
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int i2 = r0 + M::row(lk, q);
-            if (i1 < n1 && i2 < n2) {
-                long o = i1 + (long)n1 * i2;
-                T sv = sS[i1 + ldS * i2];
-                st_cvt<T, S>(ys, o, acc[q] + (rs[o] - sh) * sv);
+        for (int j = 0; j < PFMAX; j++) {
+            const int e = tid + j * NT;
+            if (j < npf && e < n1p * n2p) sS[(e % n1p) + ldS * (e / n1p)] = pf[j];
+        }
+        if (npf > PFMAX) {
+            const S *xs = x + (long)(s % nvec) * ldx + P * (s / nvec);
+            for (int e = tid + PFMAX * NT; e < n1p * n2p; e += NT) {
+                const int i1 = e % n1p, i2 = e / n1p;
+                sS[i1 + ldS * i2] = (i1 < n1 && i2 < n2) ? ld_cvt<T, S>(xs, i1 + (long)n1 * i2) : (T)0;
             }
         }
+        __syncthreads();
+        const int v = s % nvec, i3 = s / nvec;
+        int sn = slab_of(g, ++k, G, nslab);
+        while (sn >= 0 && !vec_active(active, sn % nvec)) sn = slab_of(g, ++k, G, nslab);
+        if (sn >= 0) prefetch(sn);
+        S *ys = y + (long)v * ldy + P * i3;
+        const T *rs = res + P * i3;
+        const T sh = shifts ? shifts[v] : shift0;
+    for (int g = wave; g < nrg * ncg; g += WAVES) {
+        const int rg = g / ncg, cg = g % ncg;
+        acc_t acc[RB][CB];
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+#pragma unroll
+            for (int c = 0; c < CB; c++) acc[r][c] = acc_t{0, 0, 0, 0};
+        // Tiles past the edge of an RB x CB block read a clamped (valid) tile;
+        // their results are never stored.  No predicate guards a load, and the
+        // k loops go in groups of 4 steps (n1p, n2p are multiples of 16) so the
+        // fragment reads of a group are all in flight before its MFMAs.
+        int rti[RB], cti[CB];
+#pragma unroll
+        for (int r = 0; r < RB; r++) rti[r] = min(rg * RB + r, nrt - 1) * 16 + lr;
+#pragma unroll
+        for (int c = 0; c < CB; c++) cti[c] = min(cg * CB + c, nct - 1) * 16 + lr;
+        // mode 1: D(i2,i1) += sum_k S(k,i2) T1(i1,k)
+        for (int kb = 0; kb < n1p; kb += 16) {
+            T a[4][RB], b[4][CB];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = kb + 4 * u + lk;
+#pragma unroll
+                for (int r = 0; r < RB; r++) a[u][r] = sS[k + ldS * rti[r]];
+#pragma unroll
+                for (int c = 0; c < CB; c++) {
+                    if constexpr (T_LDS) b[u][c] = sT1[cti[c] + n1p * k];
+                    else b[u][c] = (cti[c] < n1 && k < n1) ? t1[cti[c] + (long)n1 * k] : (T)0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int r = 0; r < RB; r++)
+#pragma unroll
+                    for (int c = 0; c < CB; c++) acc[r][c] = M::run(a[u][r], b[u][c], acc[r][c]);
+        }
+        // mode 2: D(i2,i1) += sum_k T2(i2,k) S(i1,k)
+        for (int kb = 0; kb < n2p; kb += 16) {
+            T a[4][RB], b[4][CB];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = kb + 4 * u + lk;
+#pragma unroll
+                for (int r = 0; r < RB; r++) {
+                    if constexpr (T_LDS) a[u][r] = sT2[rti[r] + n2p * k];
+                    else a[u][r] = (rti[r] < n2 && k < n2) ? t2[rti[r] + (long)n2 * k] : (T)0;
+                }
+#pragma unroll
+                for (int c = 0; c < CB; c++) b[u][c] = sS[cti[c] + ldS * k];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int r = 0; r < RB; r++)
+#pragma unroll
+                    for (int c = 0; c < CB; c++) acc[r][c] = M::run(a[u][r], b[u][c], acc[r][c]);
+        }
+        // epilogue: + (V - s) S, store
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+#pragma unroll
+            for (int c = 0; c < CB; c++) {
+                const int i1 = (cg * CB + c) * 16 + lr;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i2 = (rg * RB + r) * 16 + M::row(lk, q);
+                    if (i1 < n1 && i2 < n2) {
+                        const long o = i1 + (long)n1 * i2;
+                        const T sv = sS[i1 + ldS * i2];
+                        T out = acc[r][c][q] + (rs[o] - sh) * sv;
+                        if constexpr (ACCUM) out = ld_cvt<T, S>(ys, o) + out;
+                        st_cvt<T, S>(ys, o, out);
+                    }
+                }
+            }
+    }
+        __syncthreads();   // every wave is done with sS before the next slab lands
+        s = sn;
     }
 }
 
-// Pass B.  Grid: ceil(P / (16*WAVES)) x nvec workgroups of 64*WAVES threads.
-// LDS: L2 n3p x n3p.
-template <typename T, typename S, int WAVES>
+// Pass B.  Grid: ceil(P / (16*PT*WAVES)) x nvec workgroups of 64*WAVES
+// threads.  LDS: L2 n3p x n3p.  Each wave owns PT tiles of 16 grid points.
+// It runs first and writes Y = DEP L2 X (ACC adds to Y instead).
+template <typename T, typename S, int WAVES, int PT, bool ACC>
 __global__ void __launch_bounds__(64 * WAVES) k_hx_mode3(int n1, int n2, int n3, const T *__restrict__ l2,
                                                         const T *__restrict__ dep, const S *__restrict__ x, long ldx,
                                                         S *__restrict__ y, long ldy, const int *active)
 {
     using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
     extern __shared__ char smem_raw[];
     T *sL = (T *)smem_raw;
     const int n3p = rup16(n3);
@@ -270,37 +377,61 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_mode3(int n1, int n2, int n3,
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lk = lane >> 4;
-    const long p0 = ((long)blockIdx.x * WAVES + wave) * 16;
+    const long p0 = ((long)blockIdx.x * WAVES + wave) * 16 * PT;
     if (p0 >= P) return;
-    const long p = p0 + lr;
-    const bool pin = p < P;
-    // B fragments X^T(k, p) for all k: lane holds k = k0 + lk
     constexpr int KMAX = 128 / 4;
-    T bfr[KMAX];
-    const int nk = n3p / 4;
+    constexpr int RMAX = 128 / 16;
+    const int nk = n3p / 4, nr = n3p / 16;
+    // B fragments X^T(k, p) for all k: lane holds k = 4 s + lk
+    T bfr[PT][KMAX];
+    T dp[PT];
 #pragma unroll
-    for (int s = 0; s < KMAX; s++) {
-        if (s < nk) {
-            int k = 4 * s + lk;
-            bfr[s] = (pin && k < n3) ? ld_cvt<T, S>(xv, p + P * k) : (T)0;
-        }
-    }
-    const T dp = pin ? dep[p] : (T)0;
-    for (int r0 = 0; r0 < n3p; r0 += 16) {
-        typename M::acc_t acc = {0, 0, 0, 0};
+    for (int t = 0; t < PT; t++) {
+        const long p = p0 + 16 * t + lr;
+        const bool pin = p < P;
+        dp[t] = pin ? dep[p] : (T)0;
 #pragma unroll
         for (int s = 0; s < KMAX; s++) {
             if (s < nk) {
-                T a = sL[(r0 + lr) + n3p * (4 * s + lk)];
-                acc = M::run(a, bfr[s], acc);
+                const int k = 4 * s + lk;
+                bfr[t][s] = (pin && k < n3) ? ld_cvt<T, S>(xv, p + P * k) : (T)0;
             }
         }
+    }
+    acc_t acc[PT][RMAX];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int i3 = r0 + M::row(lk, q);
-            if (pin && i3 < n3) {
-                long o = p + P * i3;
-                st_cvt<T, S>(yv, o, ld_cvt<T, S>(yv, o) + dp * acc[q]);
+    for (int t = 0; t < PT; t++)
+#pragma unroll
+        for (int r = 0; r < RMAX; r++) acc[t][r] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < KMAX; s++) {
+        if (s < nk) {
+#pragma unroll
+            for (int r = 0; r < RMAX; r++) {
+                if (r < nr) {
+                    const T a = sL[(16 * r + lr) + n3p * (4 * s + lk)];
+#pragma unroll
+                    for (int t = 0; t < PT; t++) acc[t][r] = M::run(a, bfr[t][s], acc[t][r]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < PT; t++) {
+        const long p = p0 + 16 * t + lr;
+        if (p >= P) continue;
+#pragma unroll
+        for (int r = 0; r < RMAX; r++) {
+            if (r < nr) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i3 = 16 * r + M::row(lk, q);
+                    if (i3 < n3) {
+                        const long o = p + P * i3;
+                        if constexpr (ACC) st_cvt<T, S>(yv, o, ld_cvt<T, S>(yv, o) + dp[t] * acc[t][r][q]);
+                        else st_cvt<T, S>(yv, o, dp[t] * acc[t][r][q]);
+                    }
+                }
             }
         }
     }
@@ -325,43 +456,170 @@ bool fast3_supported(const scalit_ctx *c)
     return modes12_lds<T>(c->lv[0].n, c->lv[1].n, false) <= LDS_MAX;
 }
 
+// ---------------------------------------------------------------- variants
+struct VarA {
+    int waves, rb, cb;
+};
+struct VarB {
+    int waves, pt;
+};
+static const VarA kVarA[] = {{4, 1, 1}, {4, 3, 2}, {8, 3, 1}, {8, 2, 2}, {8, 1, 2}, {4, 2, 2}, {8, 4, 2}, {4, 4, 2}};
+static const VarB kVarB[] = {{4, 1}, {4, 2}, {8, 1}, {8, 2}};
+constexpr int NVA = sizeof(kVarA) / sizeof(kVarA[0]);
+constexpr int NVB = sizeof(kVarB) / sizeof(kVarB[0]);
+
+template <typename T, typename S, bool TL, int W, int RB, int CB>
+static void launch_a1(dim3 g, size_t lds, hipStream_t s, int n1, int n2, int n3, int nvec, const T *t1, const T *t2,
+                      const T *res, const T *sh, T sh0, const S *x, long ldx, S *y, long ldy, const int *act)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_hx_modes12<T, S, TL, W, RB, CB, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX);
+        attr = true;
+    }
+    // persistent grid: a multiple of the 8 XCDs, at most what fits on the CUs
+    const int per_cu = std::max(1, std::min((int)(LDS_MAX / std::max<size_t>(lds, 1)), 32 / W));
+    int G = std::min((int)g.x, g_num_cu * per_cu);
+    if (G >= 8) G &= ~7;
+    hipLaunchKernelGGL((k_hx_modes12<T, S, TL, W, RB, CB, true>), dim3(G), dim3(64 * W), lds, s, n1, n2, n3, nvec, t1,
+                       t2, res, sh, sh0, x, ldx, y, ldy, act);
+}
+
+template <typename T, typename S, bool TL>
+static void launch_a(int va, dim3 g, size_t lds, hipStream_t s, int n1, int n2, int n3, int nvec, const T *t1,
+                     const T *t2, const T *res, const T *sh, T sh0, const S *x, long ldx, S *y, long ldy,
+                     const int *act)
+{
+#define LA(W, RB, CB) \
+    launch_a1<T, S, TL, W, RB, CB>(g, lds, s, n1, n2, n3, nvec, t1, t2, res, sh, sh0, x, ldx, y, ldy, act)
+    switch (va) {
+    case 0: LA(4, 1, 1); break;
+    case 1: LA(4, 3, 2); break;
+    case 2: LA(8, 3, 1); break;
+    case 3: LA(8, 2, 2); break;
+    case 4: LA(8, 1, 2); break;
+    case 5: LA(4, 2, 2); break;
+    case 6: LA(8, 4, 2); break;
+    default: LA(4, 4, 2); break;
+    }
+#undef LA
+}
+
+template <typename T, typename S, int W, int PT>
+static void launch_b1(int n1, int n2, int n3, int nvec, const T *l2, const T *dep, const S *x, long ldx, S *y,
+                      long ldy, const int *act, hipStream_t s)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_hx_mode3<T, S, W, PT, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX);
+        attr = true;
+    }
+    const long P = (long)n1 * n2;
+    dim3 gb((unsigned)((P + 16 * PT * W - 1) / (16 * PT * W)), nvec);
+    size_t ldsb = (size_t)rup16(n3) * rup16(n3) * sizeof(T);
+    hipLaunchKernelGGL((k_hx_mode3<T, S, W, PT, false>), gb, dim3(64 * W), ldsb, s, n1, n2, n3, l2, dep, x, ldx, y, ldy,
+                       act);
+}
+
 template <typename T, typename S>
-static int hx_fast3(scalit_ctx *c, const T *t1, const T *t2, const T *l2, const T *dep, const T *res,
+static void launch_b(int vb, int n1, int n2, int n3, int nvec, const T *l2, const T *dep, const S *x, long ldx, S *y,
+                     long ldy, const int *act, hipStream_t s)
+{
+    switch (vb) {
+    case 0: launch_b1<T, S, 4, 1>(n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, act, s); break;
+    case 1: launch_b1<T, S, 4, 2>(n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, act, s); break;
+    case 2: launch_b1<T, S, 8, 1>(n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, act, s); break;
+    default: launch_b1<T, S, 8, 2>(n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, act, s); break;
+    }
+}
+
+template <typename T, typename S>
+static void pass_a(scalit_ctx *c, int va, const T *t1, const T *t2, const T *res, const T *sh, T sh0, int nvec,
+                   const S *x, long ldx, S *y, long ldy, const int *act, hipStream_t s)
+{
+    const int n1 = c->lv[0].n, n2 = c->lv[1].n, n3 = c->lv[2].n;
+    const bool tl = modes12_lds<T>(n1, n2, true) <= LDS_MAX;
+    const size_t lds = modes12_lds<T>(n1, n2, tl);
+    dim3 ga(n3 * nvec);
+    if (tl) launch_a<T, S, true>(va, ga, lds, s, n1, n2, n3, nvec, t1, t2, res, sh, sh0, x, ldx, y, ldy, act);
+    else launch_a<T, S, false>(va, ga, lds, s, n1, n2, n3, nvec, t1, t2, res, sh, sh0, x, ldx, y, ldy, act);
+}
+
+// ---------------------------------------------------------------- autotune
+// Per (precision, batch size) the fastest pass-A and pass-B variants, found by
+// timing every candidate on the caller's buffers (outputs overwritten, the
+// real call follows).  Skipped while a stream is being captured into a graph.
+static int env_variant(const char *name)
+{
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : -1;
+}
+
+template <typename T, typename S>
+static TuneEntry *tune(scalit_ctx *c, int prec, const T *t1, const T *t2, const T *l2, const T *dep, const T *res,
+                       const T *sh, T sh0, int nvec, const S *x, long ldx, S *y, long ldy, hipStream_t s)
+{
+    for (auto &e : c->hx_tune)
+        if (e.prec == prec && e.nvec == nvec) return &e;
+    int fa = env_variant("SCALIT_HX_VARIANT_A"), fb = env_variant("SCALIT_HX_VARIANT_B");
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(s, &st);
+    const int n1 = c->lv[0].n, n2 = c->lv[1].n, n3 = c->lv[2].n;
+    TuneEntry e{prec, nvec, fa >= 0 && fa < NVA ? fa : 2, fb >= 0 && fb < NVB ? fb : 0};
+    if (st != hipStreamCaptureStatusNone) return nullptr;   // use the defaults, decide later
+    if (fa < 0 || fb < 0) {
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return nullptr;
+        float best = 1e30f;
+        for (int va = 0; va < NVA && fa < 0; va++) {
+            for (int w = 0; w < 2; w++) pass_a<T, S>(c, va, t1, t2, res, sh, sh0, nvec, x, ldx, y, ldy, nullptr, s);
+            (void)hipEventRecord(e0, s);
+            for (int w = 0; w < 3; w++) pass_a<T, S>(c, va, t1, t2, res, sh, sh0, nvec, x, ldx, y, ldy, nullptr, s);
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (hipGetLastError() == hipSuccess && ms < best) {
+                best = ms;
+                e.va = va;
+            }
+        }
+        best = 1e30f;
+        for (int vb = 0; vb < NVB && fb < 0; vb++) {
+            for (int w = 0; w < 2; w++) launch_b<T, S>(vb, n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, nullptr, s);
+            (void)hipEventRecord(e0, s);
+            for (int w = 0; w < 3; w++) launch_b<T, S>(vb, n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, nullptr, s);
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (hipGetLastError() == hipSuccess && ms < best) {
+                best = ms;
+                e.vb = vb;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    c->hx_tune.push_back(e);
+    return &c->hx_tune.back();
+}
+
+template <typename T, typename S>
+static int hx_fast3(scalit_ctx *c, int prec, const T *t1, const T *t2, const T *l2, const T *dep, const T *res,
                     const T *shifts, T shift0, int nvec, const S *x, long ldx, S *y, long ldy, const int *active,
                     hipStream_t s)
 {
     const int n1 = c->lv[0].n, n2 = c->lv[1].n, n3 = c->lv[2].n;
-    const bool tl = modes12_lds<T>(n1, n2, true) <= LDS_MAX;
-    size_t lds = modes12_lds<T>(n1, n2, tl);
-    dim3 ga(n3 * nvec);
-    if (tl)
-        hipLaunchKernelGGL((k_hx_modes12<T, S, true>), ga, dim3(256), lds, s, n1, n2, n3, nvec, t1, t2, res, shifts,
-                           shift0, x, ldx, y, ldy, active);
-    else
-        hipLaunchKernelGGL((k_hx_modes12<T, S, false>), ga, dim3(256), lds, s, n1, n2, n3, nvec, t1, t2, res, shifts,
-                           shift0, x, ldx, y, ldy, active);
-    constexpr int WB = 4;
-    const long P = (long)n1 * n2;
-    dim3 gb((unsigned)((P + 16 * WB - 1) / (16 * WB)), nvec);
-    size_t ldsb = (size_t)rup16(n3) * rup16(n3) * sizeof(T);
-    hipLaunchKernelGGL((k_hx_mode3<T, S, WB>), gb, dim3(64 * WB), ldsb, s, n1, n2, n3, l2, dep, x, ldx, y, ldy,
-                       active);
+    TuneEntry *te = tune<T, S>(c, prec, t1, t2, l2, dep, res, shifts, shift0, nvec, x, ldx, y, ldy, s);
+    const int va = te ? te->va : 2, vb = te ? te->vb : 0;
+    // mode 3 writes Y, then modes 1+2 and the diagonal accumulate into it
+    launch_b<T, S>(vb, n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, active, s);
+    pass_a<T, S>(c, va, t1, t2, res, shifts, shift0, nvec, x, ldx, y, ldy, active, s);
     HIP_TRY(hipGetLastError());
     return SCALIT_OK;
-}
-
-static bool lds_attr_done[4] = {false, false, false, false};
-template <typename T, typename S>
-static void ensure_lds_attr(int slot)
-{
-    if (lds_attr_done[slot]) return;
-    (void)hipFuncSetAttribute((const void *)k_hx_modes12<T, S, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)LDS_MAX);
-    (void)hipFuncSetAttribute((const void *)k_hx_modes12<T, S, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)LDS_MAX);
-    (void)hipFuncSetAttribute((const void *)k_hx_mode3<T, S, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)LDS_MAX);
-    lds_attr_done[slot] = true;
 }
 
 int hx_apply_f64(scalit_ctx *c, const double *shifts_dev, double shift0, int nvec, const double *x, long ldx,
@@ -370,13 +628,12 @@ int hx_apply_f64(scalit_ctx *c, const double *shifts_dev, double shift0, int nve
     if (!c->fast3 || g_force_generic || !fast3_supported<double>(c))
         return hx_generic<double, double>(c, c->res.p, shifts_dev, shift0, nvec, x, ldx, y, ldy, transpose, active,
                                           s);
-    ensure_lds_attr<double, double>(0);
     const Level *L = c->lv;
     const double *t1 = transpose ? L[0].h0t.p : L[0].h0.p;
     const double *t2 = transpose ? L[1].h0t.p : L[1].h0.p;
     const double *l2 = transpose ? L[2].h0t.p : L[2].h0.p;
-    return hx_fast3<double, double>(c, t1, t2, l2, L[2].depv.p, c->res.p, shifts_dev, shift0, nvec, x, ldx, y, ldy,
-                                    active, s);
+    return hx_fast3<double, double>(c, SCALIT_FP64, t1, t2, l2, L[2].depv.p, c->res.p, shifts_dev, shift0, nvec, x,
+                                    ldx, y, ldy, active, s);
 }
 
 int hx_apply_f32(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const float *x, long ldx, float *y,
@@ -384,10 +641,9 @@ int hx_apply_f32(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec,
 {
     if (!c->fast3 || g_force_generic || !fast3_supported<float>(c))
         return hx_generic<float, float>(c, c->res_f.p, shifts_dev, shift0, nvec, x, ldx, y, ldy, false, active, s);
-    ensure_lds_attr<float, float>(1);
     const Level *L = c->lv;
-    return hx_fast3<float, float>(c, L[0].h0_f.p, L[1].h0_f.p, L[2].h0_f.p, L[2].depv_f.p, c->res_f.p, shifts_dev,
-                                  shift0, nvec, x, ldx, y, ldy, active, s);
+    return hx_fast3<float, float>(c, SCALIT_FP32, L[0].h0_f.p, L[1].h0_f.p, L[2].h0_f.p, L[2].depv_f.p, c->res_f.p,
+                                  shifts_dev, shift0, nvec, x, ldx, y, ldy, active, s);
 }
 
 int hx_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const bf16_t *x, long ldx,
@@ -395,13 +651,50 @@ int hx_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec
 {
     if (!c->fast3 || g_force_generic || !fast3_supported<float>(c))
         return hx_generic<float, bf16_t>(c, c->res_f.p, shifts_dev, shift0, nvec, x, ldx, y, ldy, false, active, s);
-    ensure_lds_attr<float, bf16_t>(2);
     const Level *L = c->lv;
-    return hx_fast3<float, bf16_t>(c, L[0].h0_f.p, L[1].h0_f.p, L[2].h0_f.p, L[2].depv_f.p, c->res_f.p, shifts_dev,
-                                   shift0, nvec, x, ldx, y, ldy, active, s);
+    return hx_fast3<float, bf16_t>(c, SCALIT_BF16, L[0].h0_f.p, L[1].h0_f.p, L[2].h0_f.p, L[2].depv_f.p, c->res_f.p,
+                                   shifts_dev, shift0, nvec, x, ldx, y, ldy, active, s);
 }
 
 bool fast3_ok(const scalit_ctx *c) { return fast3_supported<double>(c); }
+
+// Time the pass variants once for this batch size in the given precision
+// (scratch vectors allocated here), so that graph recording finds a decision.
+int hx_prepare(scalit_ctx *c, int prec, int nvec, hipStream_t s)
+{
+    if (!c->fast3 || g_force_generic) return SCALIT_OK;
+    for (auto &e : c->hx_tune)
+        if (e.prec == prec && e.nvec == nvec) return SCALIT_OK;
+    const size_t es = prec == SCALIT_FP64 ? 8 : (prec == SCALIT_FP32 ? 4 : 2);
+    void *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, es * nvec * c->N));
+    if (hipMalloc(&b, es * nvec * c->N) != hipSuccess) {
+        (void)hipFree(a);
+        return SCALIT_ENOMEM;
+    }
+    (void)hipMemsetAsync(a, 0, es * nvec * c->N, s);
+    int rc;
+    if (prec == SCALIT_FP64) rc = hx_apply_f64(c, nullptr, 0.0, nvec, (double *)a, c->N, (double *)b, c->N, false,
+                                               nullptr, s);
+    else if (prec == SCALIT_FP32) rc = hx_apply_f32(c, nullptr, 0.0f, nvec, (float *)a, c->N, (float *)b, c->N,
+                                                    nullptr, s);
+    else rc = hx_apply_bf16(c, nullptr, 0.0f, nvec, (bf16_t *)a, c->N, (bf16_t *)b, c->N, nullptr, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
+int hx_variants(const scalit_ctx *c, int prec, int nvec, int *va, int *vb)
+{
+    for (auto &e : c->hx_tune)
+        if (e.prec == prec && e.nvec == nvec) {
+            *va = e.va;
+            *vb = e.vb;
+            return SCALIT_OK;
+        }
+    return SCALIT_ESTATE;
+}
 
 // Lane-layout probe of the 16x16x4 MFMA builtins: one wave computes
 // D = A B with A(i,0) = 100 (i+1), A(i,1) = 1, B(0,j) = 1, B(1,j) = j+1 (other
@@ -419,6 +712,49 @@ __global__ void __launch_bounds__(64) k_mfma_probe(double *out)
     for (int q = 0; q < 4; q++) out[4 * lane + q] = (double)acc[q];
 }
 
+// MFMA throughput probe: every wave runs `iters` rounds of 8 independent
+// 16x16x4 accumulation chains from registers (no memory traffic); the host
+// divides 8 * 2048 * iters * waves FLOP by the kernel time.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mfma_peak(int iters, double *sink)
+{
+    using M = Mfma<T>;
+    typename M::acc_t acc[8];
+    T a = (T)(1.0 + 1e-3 * threadIdx.x), b = (T)(1.0 - 1e-3 * threadIdx.x);
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = typename M::acc_t{0, 0, 0, 0};
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc[i] = M::run(a, b, acc[i]);
+    }
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s += (double)(acc[i][0] + acc[i][3]);
+    if (s == 12345.678) sink[0] = s;   // keeps the chains live
+}
+
+int mfma_peak(int prec, int iters, int blocks, float *ms_out)
+{
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(double)));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int rep = 0; rep < 2; rep++) {
+        (void)hipEventRecord(e0, 0);
+        if (prec == SCALIT_FP32) hipLaunchKernelGGL(k_mfma_peak<float>, dim3(blocks), dim3(256), 0, 0, iters, d);
+        else hipLaunchKernelGGL(k_mfma_peak<double>, dim3(blocks), dim3(256), 0, 0, iters, d);
+        (void)hipEventRecord(e1, 0);
+        (void)hipEventSynchronize(e1);
+    }
+    (void)hipEventElapsedTime(ms_out, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d);
+    HIP_TRY(hipGetLastError());
+    return SCALIT_OK;
+}
+
 int mfma_probe(int prec, double *out_host)
 {
     double *d = nullptr;
@@ -431,35 +767,22 @@ int mfma_probe(int prec, double *out_host)
     return SCALIT_OK;
 }
 
-// One pass of the fused FP64 path alone (pass 1: modes 1+2+diagonal, pass 2:
-// mode 3), for per-kernel timing.
+// One pass of the fused FP64 path alone with the tuned variant (pass 1: modes
+// 1+2+diagonal, pass 2: mode 3), for per-kernel timing.
 int hx_pass_f64(scalit_ctx *c, int pass, int nvec, const double *x, long ldx, double *y, long ldy, hipStream_t s)
 {
     if (!c->fast3 || !fast3_supported<double>(c)) return SCALIT_EINVAL;
-    ensure_lds_attr<double, double>(0);
     const Level *L = c->lv;
     const int n1 = L[0].n, n2 = L[1].n, n3 = L[2].n;
-    if (pass == 1) {
-        const bool tl = modes12_lds<double>(n1, n2, true) <= LDS_MAX;
-        size_t lds = modes12_lds<double>(n1, n2, tl);
-        if (tl)
-            hipLaunchKernelGGL((k_hx_modes12<double, double, true>), dim3(n3 * nvec), dim3(256), lds, s, n1, n2, n3,
-                               nvec, L[0].h0.p, L[1].h0.p, c->res.p, (const double *)nullptr, 0.0, x, ldx, y, ldy,
-                               (const int *)nullptr);
-        else
-            hipLaunchKernelGGL((k_hx_modes12<double, double, false>), dim3(n3 * nvec), dim3(256), lds, s, n1, n2, n3,
-                               nvec, L[0].h0.p, L[1].h0.p, c->res.p, (const double *)nullptr, 0.0, x, ldx, y, ldy,
-                               (const int *)nullptr);
-    } else if (pass == 2) {
-        constexpr int WB = 4;
-        const long P = (long)n1 * n2;
-        dim3 gb((unsigned)((P + 16 * WB - 1) / (16 * WB)), nvec);
-        size_t ldsb = (size_t)rup16(n3) * rup16(n3) * sizeof(double);
-        hipLaunchKernelGGL((k_hx_mode3<double, double, WB>), gb, dim3(64 * WB), ldsb, s, n1, n2, n3, L[2].h0.p,
-                           L[2].depv.p, x, ldx, y, ldy, (const int *)nullptr);
-    } else {
+    TuneEntry *te = tune<double, double>(c, SCALIT_FP64, L[0].h0.p, L[1].h0.p, L[2].h0.p, L[2].depv.p, c->res.p,
+                                         nullptr, 0.0, nvec, x, ldx, y, ldy, s);
+    const int va = te ? te->va : 2, vb = te ? te->vb : 0;
+    if (pass == 1)
+        pass_a<double, double>(c, va, L[0].h0.p, L[1].h0.p, c->res.p, nullptr, 0.0, nvec, x, ldx, y, ldy, nullptr, s);
+    else if (pass == 2)
+        launch_b<double, double>(vb, n1, n2, n3, nvec, L[2].h0.p, L[2].depv.p, x, ldx, y, ldy, nullptr, s);
+    else
         return SCALIT_EINVAL;
-    }
     HIP_TRY(hipGetLastError());
     return SCALIT_OK;
 }

@@ -68,6 +68,11 @@ struct Level {
 
 struct PistState;   // defined in solvers.hip
 
+// autotuned H*X pass variants for one (precision, batch size)
+struct TuneEntry {
+    int prec, nvec, va, vb;
+};
+
 }  // namespace scalit
 
 struct scalit_ctx {
@@ -91,6 +96,7 @@ struct scalit_ctx {
     // bumped whenever a device buffer that operator kernels read is
     // (re)allocated: recorded graphs of older generations are re-recorded
     long gen = 0;
+    std::vector<scalit::TuneEntry> hx_tune;
 };
 
 namespace scalit {
@@ -105,6 +111,10 @@ int hx_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec
                   bf16_t *y, long ldy, const int *active, hipStream_t s);
 // force the generic (any sF) path, for testing
 extern int g_force_generic;
+extern int g_num_cu;
+// choose the fused-path kernel variants for (prec, nvec) before graph capture
+int hx_prepare(scalit_ctx *c, int prec, int nvec, hipStream_t s);
+int hx_variants(const scalit_ctx *c, int prec, int nvec, int *va, int *vb);
 
 // P*X (kernels_px.hip)
 int px_apply_f64(scalit_ctx *c, int nosb, const double *shifts_dev, double shift0, int nvec, const double *x,

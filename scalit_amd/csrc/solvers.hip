@@ -301,6 +301,11 @@ static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double t
     launch_qmr_init_scal(nv, w.part(0), w.part(1), w.scal.p, w.active.p, qmrmax, s);
     HIP_TRY(hipGetLastError());
     const bool use_graph = g_qmr_use_graph != 0 && cb == nullptr;
+    if (cb == nullptr) {
+        const int prec = std::is_same<S, double>::value ? SCALIT_FP64
+                                                        : (std::is_same<S, float>::value ? SCALIT_FP32 : SCALIT_BF16);
+        SC_TRY(hx_prepare(c, prec, nv, s));
+    }
     if (use_graph && (w.g_conv != conv || w.g_qmrmax != qmrmax || w.g_tol != tol || w.g_gen != c->gen || !w.exec)) {
         w.drop_graph();
         HIP_TRY(hipStreamSynchronize(s));
