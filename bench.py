@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pist", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="workload for rocprofv3 --pmc: three 512 MiB device copies (the counter calibration), "
+                         "then --steps H*X calls; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the H*X / PIST path here is built and timed on a single MI355X "
@@ -69,6 +72,25 @@ def main():
 
     n1, n2, n3 = P.CONFIG_GRIDS["hx_bench"]
     B = args.batch
+    if args.pmc_run:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel
+        # against its 512 MiB read and 512 MiB written give the scale factors
+        src = torch.randn(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        prob = P.config_problem("hx_bench")
+        ctx = Context.from_problem(prob)
+        x = ctx.to_device(P.random_vectors(prob.N, B, 2024))
+        y = torch.empty_like(x)
+        for _ in range(args.steps):
+            ctx.hx(x, out=y)
+        torch.cuda.synchronize()
+        ctx.close()
+        print(f"pmc run done: 3 copies of 512 MiB, {args.steps} H*X of batch {B}", flush=True)
+        return
     prob = P.config_problem("hx_bench")
     ctx = Context.from_problem(prob)
     x = ctx.to_device(P.random_vectors(prob.N, B, 2024))

@@ -91,6 +91,7 @@ static void drop_f32(scalit_ctx *c)
         c->lv[l].h0_f.release();
         c->lv[l].depv_f.release();
         c->lv[l].vosb_f.release();
+        c->lv[l].vosbt_f.release();
     }
     c->res_f.release();
     c->eig0_f.release();
@@ -167,9 +168,11 @@ int scalit_destroy(scalit_ctx *c)
         L.h0t.release();
         L.depv.release();
         L.vosb.release();
+        L.vosbt.release();
         L.h0_f.release();
         L.depv_f.release();
         L.vosb_f.release();
+        L.vosbt_f.release();
     }
     c->res.release();
     c->eig0.release();
@@ -273,6 +276,8 @@ int scalit_set_precond(scalit_ctx *c, const double *const *vosb, const double *e
         SC_TRY(L.vosb.alloc(n));
         HIP_TRY(hipMemcpy(L.vosb.p, vosb[l], sizeof(double) * n, hipMemcpyHostToDevice));
         L.vosb_f.release();
+        L.vosbt.release();
+        L.vosbt_f.release();
     }
     SC_TRY(c->eig0.alloc(c->N));
     HIP_TRY(hipMemcpy(c->eig0.p, eig0, sizeof(double) * c->N, hipMemcpyHostToDevice));

@@ -437,7 +437,11 @@ int block_diag(scalit_ctx *c, int bjmax, double bjtol)
     }
     c->has_precond = true;
     // fp32 copies are rebuilt on demand
-    for (int l = 0; l < c->sF; l++) c->lv[l].vosb_f.release();
+    for (int l = 0; l < c->sF; l++) {
+        c->lv[l].vosb_f.release();
+        c->lv[l].vosbt.release();
+        c->lv[l].vosbt_f.release();
+    }
     c->eig0_f.release();
     return SCALIT_OK;
 }

@@ -22,60 +22,13 @@
 #include <algorithm>
 #include <cstring>
 
-#include "scalit_impl.hpp"
+#include "device_common.hpp"
 
 namespace scalit {
 
 int g_force_generic = 0;
 int g_num_cu = 256;   // compute units of the device, set when a context is created
 
-typedef double d4_t __attribute__((ext_vector_type(4)));
-typedef float f4_t __attribute__((ext_vector_type(4)));
-
-// 16x16x4 MFMA: A(i,k) comes from lane i + 16 k, B(k,j) from lane j + 16 k;
-// accumulator q of lane l holds D(row(l/16, q), l % 16), with the rows
-// interleaved for FP64 and blocked for FP32 (checked on the device by
-// tests/test_gpu_mfma_layout.py).
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-    using acc_t = d4_t;
-    static __device__ __forceinline__ acc_t run(double a, double b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lk, int q) { return lk + 4 * q; }
-};
-template <>
-struct Mfma<float> {
-    using acc_t = f4_t;
-    static __device__ __forceinline__ acc_t run(float a, float b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lk, int q) { return 4 * lk + q; }
-};
-
-// storage <-> compute conversions
-__device__ __forceinline__ float bf16_to_f32(bf16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
-__device__ __forceinline__ bf16_t f32_to_bf16(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)(u >> 16);   // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u);                                   // round to nearest even
-    return (bf16_t)(u >> 16);
-}
-template <typename T, typename S>
-__device__ __forceinline__ T ld_cvt(const S *p, long i) { return (T)p[i]; }
-template <>
-__device__ __forceinline__ float ld_cvt<float, bf16_t>(const bf16_t *p, long i) { return bf16_to_f32(p[i]); }
-template <typename T, typename S>
-__device__ __forceinline__ void st_cvt(S *p, long i, T v) { p[i] = (S)v; }
-template <>
-__device__ __forceinline__ void st_cvt<float, bf16_t>(bf16_t *p, long i, float v) { p[i] = f32_to_bf16(v); }
-
-__device__ __forceinline__ bool vec_active(const int *active, int v) { return active == nullptr || active[v] != 0; }
 
 // ---------------------------------------------------------------------------
 // generic path
@@ -159,16 +112,6 @@ static int hx_generic(scalit_ctx *c, const T *res, const T *shifts, T shift0, in
 // ---------------------------------------------------------------------------
 // fused 3-level path
 // ---------------------------------------------------------------------------
-__host__ __device__ static inline int rup16(int n) { return (n + 15) & ~15; }
-
-// XCD-aware remap: consecutive workgroup ids land on different XCDs (8 on
-// MI355X); give each XCD a contiguous range of logical slabs so that the
-// vectors sharing one theta point (and its slice of V) share that XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nblk)
-{
-    if ((nblk & 7) != 0) return bid;
-    return (bid & 7) * (nblk >> 3) + (bid >> 3);
-}
 
 // Slab s of the persistent pass-A schedule processed by workgroup g of G:
 // workgroups are spread round-robin over the 8 XCDs, so XCD x gets the

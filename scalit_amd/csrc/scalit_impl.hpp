@@ -63,6 +63,10 @@ struct Level {
     int blk = 1;        // myBlk(level): product of outer sizes (BLK)
     DBuf<double> h0, h0t, depv, vosb;   // h0t: transposed copy for the adjoint
     DBuf<float> h0_f, depv_f, vosb_f;
+    // per-block transposes of vosb for levels >= 2 (the V^T rotations of P*X),
+    // built on first use and released whenever vosb changes
+    DBuf<double> vosbt;
+    DBuf<float> vosbt_f;
     bool has_h0 = false, has_dep = false;
 };
 
