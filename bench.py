@@ -37,11 +37,11 @@ def hx_counts(n1, n2, n3, nvec, es=8):
     flops_vec = 2 * N * (n1 + n2 + n3) + 5 * N
     bytes_batch = es * (2 * N * nvec + N)
     # per pass
-    # mode 3 runs first and writes Y; modes 1+2 and the diagonal accumulate into it
-    flops_a = nvec * (2 * N * (n1 + n2) + 4 * N)     # two mode products, (V - s) o X, three adds
-    flops_b = nvec * (2 * N * n3 + N)                # mode 3, DEP scale
-    bytes_a = es * (3 * N * nvec + N)                # X read, Y read + written, V read
-    bytes_b = es * (2 * N * nvec + N // n3)          # X read, Y written, DEP
+    # modes 1+2 and the diagonal write Y; mode 3 then adds to it
+    flops_a = nvec * (2 * N * (n1 + n2) + 3 * N)     # two mode products, (V - s) o X, two adds
+    flops_b = nvec * (2 * N * n3 + 2 * N)            # mode 3, DEP scale, add to Y
+    bytes_a = es * (2 * N * nvec + N)                # X read, Y written, V read
+    bytes_b = es * (3 * N * nvec + N // n3)          # X read, Y read + written, DEP
     return dict(N=N, flops_vec=flops_vec, flops_batch=flops_vec * nvec, bytes_batch=bytes_batch, flops_a=flops_a,
                 flops_b=flops_b, bytes_a=bytes_a, bytes_b=bytes_b)
 

@@ -10,8 +10,8 @@
 //    DEP(r,R) x L^2_theta -- on FP64 / FP32 MFMA:
 //      pass A, one workgroup per (theta point, vector) slab S(n1 x n2) staged
 //      in LDS: Y = T1 S + S T2^T + (V - s) o S   (modes 1 and 2 + diagonal)
-//      pass B, one wave per 16*PT grid points p=(r,R) across all theta:
-//      Y(p,:) += DEP(p) * L2 X(p,:)              (mode 3)
+//      pass B, persistent waves over tiles of 16 grid points p=(r,R) across
+//      all theta: Y(p,:) += DEP(p) * L2 X(p,:)    (mode 3)
 //    Tiles are computed transposed (D = O^T) so that the 16 lanes of a
 //    fragment column run along the contiguous r index: every global load and
 //    store of X, Y and V moves 128-byte runs.  Each wave accumulates a block
@@ -23,6 +23,7 @@
 #include <cstring>
 
 #include "device_common.hpp"
+#include "outer_mode.hpp"
 
 namespace scalit {
 
@@ -132,10 +133,12 @@ __device__ __forceinline__ int slab_of(int g, int k, int G, int S)
 // Pass A.  Persistent grid of G workgroups of 64*WAVES threads; workgroup g
 // walks its slabs (theta point i3, vector v), slab s = i3 * nvec + v.  LDS:
 // S (n1p+1) x n2p, plus T1 n1p x n1p and T2 n2p x n2p when T_LDS, staged once
-// per workgroup.  The next slab is loaded into registers while the current one
-// is computed.  Each wave owns blocks of RB (i2) x CB (i1) output tiles.
-// ACCUM adds the incoming Y (the mode-3 pass runs first).
-template <typename T, typename S, bool T_LDS, int WAVES, int RB, int CB, bool ACCUM>
+// per workgroup.  Each wave owns blocks of RB (i2) x CB (i1) output tiles.
+// It writes Y = T1 S + S T2^T + (V - s) o S; the mode-3 pass then adds to Y.
+// While one slab is computed, the next slab of X and the V values of each
+// wave's first output block are loaded into registers, so neither the slab
+// start nor the epilogue waits on HBM.
+template <typename T, typename S, bool T_LDS, int WAVES, int RB, int CB>
 __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n3, int nvec, const T *__restrict__ t1,
                                                           const T *__restrict__ t2, const T *__restrict__ res,
                                                           const T *__restrict__ shifts, T shift0,
@@ -158,16 +161,20 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
     const int nslab = n3 * nvec;
     const int G = gridDim.x, g = blockIdx.x;
     const int npf = (n1p * n2p + NT - 1) / NT;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int nrt = n2p / 16, nct = n1p / 16;
+    const int nrg = (nrt + RB - 1) / RB, ncg = (nct + CB - 1) / CB;
 
     if constexpr (T_LDS) {
-        for (int e = tid; e < n1p * n1p; e += NT) {
-            int a = e % n1p, b = e / n1p;
-            sT1[e] = (a < n1 && b < n1) ? t1[a + (long)n1 * b] : (T)0;
+        if (n1p != n1 || n2p != n2) {
+            for (int e = tid; e < n1p * n1p; e += NT)
+                if (e % n1p >= n1 || e / n1p >= n1) sT1[e] = (T)0;
+            for (int e = tid; e < n2p * n2p; e += NT)
+                if (e % n2p >= n2 || e / n2p >= n2) sT2[e] = (T)0;
         }
-        for (int e = tid; e < n2p * n2p; e += NT) {
-            int a = e % n2p, b = e / n2p;
-            sT2[e] = (a < n2 && b < n2) ? t2[a + (long)n2 * b] : (T)0;
-        }
+        stage_lds<NT, 16>(t1, n1 * n1, tid, [&](int e) -> T & { return sT1[(e % n1) + n1p * (e / n1)]; });
+        stage_lds<NT, 16>(t2, n2 * n2, tid, [&](int e) -> T & { return sT2[(e % n2) + n2p * (e / n2)]; });
     }
     // first slab of this workgroup that is active
     int k = 0, s = slab_of(g, 0, G, nslab);
@@ -184,12 +191,29 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
             }
         }
     };
-    if (s >= 0) prefetch(s);
+    // V of the wave's first output block (block index = wave)
+    T pv[RB][CB][4];
+    auto prefetch_v = [&](int sl) {
+        const T *rsn = res + P * (sl / nvec);
+        const int rg = wave / ncg, cg = wave % ncg;
+        const bool own = wave < nrg * ncg;
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+#pragma unroll
+            for (int c = 0; c < CB; c++) {
+                const int i1 = (cg * CB + c) * 16 + lr;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i2 = (rg * RB + r) * 16 + M::row(lk, q);
+                    pv[r][c][q] = (own && i1 < n1 && i2 < n2) ? rsn[i1 + (long)n1 * i2] : (T)0;
+                }
+            }
+    };
+    if (s >= 0) {
+        prefetch(s);
+        prefetch_v(s);
+    }
 
-    const int lane = tid & 63, wave = tid >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const int nrt = n2p / 16, nct = n1p / 16;
-    const int nrg = (nrt + RB - 1) / RB, ncg = (nct + CB - 1) / CB;
     while (s >= 0) {
         // the prefetched slab into LDS (padding zero)
 #pragma unroll
@@ -212,173 +236,90 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
         S *ys = y + (long)v * ldy + P * i3;
         const T *rs = res + P * i3;
         const T sh = shifts ? shifts[v] : shift0;
-    for (int g = wave; g < nrg * ncg; g += WAVES) {
-        const int rg = g / ncg, cg = g % ncg;
-        acc_t acc[RB][CB];
+        for (int ob = wave; ob < nrg * ncg; ob += WAVES) {
+            const int rg = ob / ncg, cg = ob % ncg;
+            acc_t acc[RB][CB];
 #pragma unroll
-        for (int r = 0; r < RB; r++)
+            for (int r = 0; r < RB; r++)
 #pragma unroll
-            for (int c = 0; c < CB; c++) acc[r][c] = acc_t{0, 0, 0, 0};
-        // Tiles past the edge of an RB x CB block read a clamped (valid) tile;
-        // their results are never stored.  No predicate guards a load, and the
-        // k loops go in groups of 4 steps (n1p, n2p are multiples of 16) so the
-        // fragment reads of a group are all in flight before its MFMAs.
-        int rti[RB], cti[CB];
+                for (int c = 0; c < CB; c++) acc[r][c] = acc_t{0, 0, 0, 0};
+            // Tiles past the edge of an RB x CB block read a clamped (valid) tile;
+            // their results are never stored.  No predicate guards a load, and the
+            // k loops go in groups of 4 steps (n1p, n2p are multiples of 16) so the
+            // fragment reads of a group are all in flight before its MFMAs.
+            int rti[RB], cti[CB];
 #pragma unroll
-        for (int r = 0; r < RB; r++) rti[r] = min(rg * RB + r, nrt - 1) * 16 + lr;
+            for (int r = 0; r < RB; r++) rti[r] = min(rg * RB + r, nrt - 1) * 16 + lr;
 #pragma unroll
-        for (int c = 0; c < CB; c++) cti[c] = min(cg * CB + c, nct - 1) * 16 + lr;
-        // mode 1: D(i2,i1) += sum_k S(k,i2) T1(i1,k)
-        for (int kb = 0; kb < n1p; kb += 16) {
-            T a[4][RB], b[4][CB];
+            for (int c = 0; c < CB; c++) cti[c] = min(cg * CB + c, nct - 1) * 16 + lr;
+            // mode 1: D(i2,i1) += sum_k S(k,i2) T1(i1,k)
+            for (int kb = 0; kb < n1p; kb += 16) {
+                T a[4][RB], b[4][CB];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int k = kb + 4 * u + lk;
+                for (int u = 0; u < 4; u++) {
+                    const int kk = kb + 4 * u + lk;
 #pragma unroll
-                for (int r = 0; r < RB; r++) a[u][r] = sS[k + ldS * rti[r]];
+                    for (int r = 0; r < RB; r++) a[u][r] = sS[kk + ldS * rti[r]];
 #pragma unroll
-                for (int c = 0; c < CB; c++) {
-                    if constexpr (T_LDS) b[u][c] = sT1[cti[c] + n1p * k];
-                    else b[u][c] = (cti[c] < n1 && k < n1) ? t1[cti[c] + (long)n1 * k] : (T)0;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int r = 0; r < RB; r++)
-#pragma unroll
-                    for (int c = 0; c < CB; c++) acc[r][c] = M::run(a[u][r], b[u][c], acc[r][c]);
-        }
-        // mode 2: D(i2,i1) += sum_k T2(i2,k) S(i1,k)
-        for (int kb = 0; kb < n2p; kb += 16) {
-            T a[4][RB], b[4][CB];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int k = kb + 4 * u + lk;
-#pragma unroll
-                for (int r = 0; r < RB; r++) {
-                    if constexpr (T_LDS) a[u][r] = sT2[rti[r] + n2p * k];
-                    else a[u][r] = (rti[r] < n2 && k < n2) ? t2[rti[r] + (long)n2 * k] : (T)0;
-                }
-#pragma unroll
-                for (int c = 0; c < CB; c++) b[u][c] = sS[cti[c] + ldS * k];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int r = 0; r < RB; r++)
-#pragma unroll
-                    for (int c = 0; c < CB; c++) acc[r][c] = M::run(a[u][r], b[u][c], acc[r][c]);
-        }
-        // epilogue: + (V - s) S, store
-#pragma unroll
-        for (int r = 0; r < RB; r++)
-#pragma unroll
-            for (int c = 0; c < CB; c++) {
-                const int i1 = (cg * CB + c) * 16 + lr;
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int i2 = (rg * RB + r) * 16 + M::row(lk, q);
-                    if (i1 < n1 && i2 < n2) {
-                        const long o = i1 + (long)n1 * i2;
-                        const T sv = sS[i1 + ldS * i2];
-                        T out = acc[r][c][q] + (rs[o] - sh) * sv;
-                        if constexpr (ACCUM) out = ld_cvt<T, S>(ys, o) + out;
-                        st_cvt<T, S>(ys, o, out);
+                    for (int c = 0; c < CB; c++) {
+                        if constexpr (T_LDS) b[u][c] = sT1[cti[c] + n1p * kk];
+                        else b[u][c] = (cti[c] < n1 && kk < n1) ? t1[cti[c] + (long)n1 * kk] : (T)0;
                     }
                 }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < RB; r++)
+#pragma unroll
+                        for (int c = 0; c < CB; c++) acc[r][c] = M::run(a[u][r], b[u][c], acc[r][c]);
             }
-    }
+            // mode 2: D(i2,i1) += sum_k T2(i2,k) S(i1,k)
+            for (int kb = 0; kb < n2p; kb += 16) {
+                T a[4][RB], b[4][CB];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int kk = kb + 4 * u + lk;
+#pragma unroll
+                    for (int r = 0; r < RB; r++) {
+                        if constexpr (T_LDS) a[u][r] = sT2[rti[r] + n2p * kk];
+                        else a[u][r] = (rti[r] < n2 && kk < n2) ? t2[rti[r] + (long)n2 * kk] : (T)0;
+                    }
+#pragma unroll
+                    for (int c = 0; c < CB; c++) b[u][c] = sS[cti[c] + ldS * kk];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < RB; r++)
+#pragma unroll
+                        for (int c = 0; c < CB; c++) acc[r][c] = M::run(a[u][r], b[u][c], acc[r][c]);
+            }
+            // epilogue: + (V - s) S, store
+            const bool first = ob == wave;
+#pragma unroll
+            for (int r = 0; r < RB; r++)
+#pragma unroll
+                for (int c = 0; c < CB; c++) {
+                    const int i1 = (cg * CB + c) * 16 + lr;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int i2 = (rg * RB + r) * 16 + M::row(lk, q);
+                        if (i1 < n1 && i2 < n2) {
+                            const long o = i1 + (long)n1 * i2;
+                            const T sv = sS[i1 + ldS * i2];
+                            const T vv = first ? pv[r][c][q] : rs[o];
+                            st_cvt<T, S>(ys, o, acc[r][c][q] + (vv - sh) * sv);
+                        }
+                    }
+                }
+        }
+        if (sn >= 0) prefetch_v(sn);
         __syncthreads();   // every wave is done with sS before the next slab lands
         s = sn;
     }
 }
 
-// Pass B.  Grid: ceil(P / (16*PT*WAVES)) x nvec workgroups of 64*WAVES
-// threads.  LDS: L2 n3p x n3p.  Each wave owns PT tiles of 16 grid points.
-// It runs first and writes Y = DEP L2 X (ACC adds to Y instead).
-template <typename T, typename S, int WAVES, int PT, bool ACC>
-__global__ void __launch_bounds__(64 * WAVES) k_hx_mode3(int n1, int n2, int n3, const T *__restrict__ l2,
-                                                        const T *__restrict__ dep, const S *__restrict__ x, long ldx,
-                                                        S *__restrict__ y, long ldy, const int *active)
-{
-    using M = Mfma<T>;
-    using acc_t = typename M::acc_t;
-    extern __shared__ char smem_raw[];
-    T *sL = (T *)smem_raw;
-    const int n3p = rup16(n3);
-    const int v = blockIdx.y;
-    if (!vec_active(active, v)) return;
-    const long P = (long)n1 * n2;
-    const S *xv = x + (long)v * ldx;
-    S *yv = y + (long)v * ldy;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < n3p * n3p; e += 64 * WAVES) {
-        int a = e % n3p, b = e / n3p;
-        sL[e] = (a < n3 && b < n3) ? l2[a + (long)n3 * b] : (T)0;
-    }
-    __syncthreads();
-    const int lane = tid & 63, wave = tid >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const long p0 = ((long)blockIdx.x * WAVES + wave) * 16 * PT;
-    if (p0 >= P) return;
-    constexpr int KMAX = 128 / 4;
-    constexpr int RMAX = 128 / 16;
-    const int nk = n3p / 4, nr = n3p / 16;
-    // B fragments X^T(k, p) for all k: lane holds k = 4 s + lk
-    T bfr[PT][KMAX];
-    T dp[PT];
-#pragma unroll
-    for (int t = 0; t < PT; t++) {
-        const long p = p0 + 16 * t + lr;
-        const bool pin = p < P;
-        dp[t] = pin ? dep[p] : (T)0;
-#pragma unroll
-        for (int s = 0; s < KMAX; s++) {
-            if (s < nk) {
-                const int k = 4 * s + lk;
-                bfr[t][s] = (pin && k < n3) ? ld_cvt<T, S>(xv, p + P * k) : (T)0;
-            }
-        }
-    }
-    acc_t acc[PT][RMAX];
-#pragma unroll
-    for (int t = 0; t < PT; t++)
-#pragma unroll
-        for (int r = 0; r < RMAX; r++) acc[t][r] = acc_t{0, 0, 0, 0};
-#pragma unroll
-    for (int s = 0; s < KMAX; s++) {
-        if (s < nk) {
-#pragma unroll
-            for (int r = 0; r < RMAX; r++) {
-                if (r < nr) {
-                    const T a = sL[(16 * r + lr) + n3p * (4 * s + lk)];
-#pragma unroll
-                    for (int t = 0; t < PT; t++) acc[t][r] = M::run(a, bfr[t][s], acc[t][r]);
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < PT; t++) {
-        const long p = p0 + 16 * t + lr;
-        if (p >= P) continue;
-#pragma unroll
-        for (int r = 0; r < RMAX; r++) {
-            if (r < nr) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int i3 = 16 * r + M::row(lk, q);
-                    if (i3 < n3) {
-                        const long o = p + P * i3;
-                        if constexpr (ACC) st_cvt<T, S>(yv, o, ld_cvt<T, S>(yv, o) + dp[t] * acc[t][r][q]);
-                        else st_cvt<T, S>(yv, o, dp[t] * acc[t][r][q]);
-                    }
-                }
-            }
-        }
-    }
-}
+// Pass B (mode 3 x DEP, added to Y) is k_outer_mode of outer_mode.hpp.
 
 static constexpr size_t LDS_MAX = 160 * 1024;
 
@@ -395,7 +336,7 @@ template <typename T>
 bool fast3_supported(const scalit_ctx *c)
 {
     if (c->sF != 3 || c->lv[0].dep || c->lv[1].dep || !c->lv[2].dep) return false;
-    if (c->lv[2].n > 128) return false;   // pass B keeps all of theta in registers
+    if (c->lv[2].n > 128) return false;   // pass B keeps a theta column in registers
     return modes12_lds<T>(c->lv[0].n, c->lv[1].n, false) <= LDS_MAX;
 }
 
@@ -403,13 +344,9 @@ bool fast3_supported(const scalit_ctx *c)
 struct VarA {
     int waves, rb, cb;
 };
-struct VarB {
-    int waves, pt;
-};
 static const VarA kVarA[] = {{4, 1, 1}, {4, 3, 2}, {8, 3, 1}, {8, 2, 2}, {8, 1, 2}, {4, 2, 2}, {8, 4, 2}, {4, 4, 2}};
-static const VarB kVarB[] = {{4, 1}, {4, 2}, {8, 1}, {8, 2}};
 constexpr int NVA = sizeof(kVarA) / sizeof(kVarA[0]);
-constexpr int NVB = sizeof(kVarB) / sizeof(kVarB[0]);
+constexpr int NVB = 2;   // pass B: 4 or 8 waves per workgroup
 
 template <typename T, typename S, bool TL, int W, int RB, int CB>
 static void launch_a1(dim3 g, size_t lds, hipStream_t s, int n1, int n2, int n3, int nvec, const T *t1, const T *t2,
@@ -417,7 +354,7 @@ static void launch_a1(dim3 g, size_t lds, hipStream_t s, int n1, int n2, int n3,
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_hx_modes12<T, S, TL, W, RB, CB, true>,
+        (void)hipFuncSetAttribute((const void *)k_hx_modes12<T, S, TL, W, RB, CB>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX);
         attr = true;
     }
@@ -425,7 +362,7 @@ static void launch_a1(dim3 g, size_t lds, hipStream_t s, int n1, int n2, int n3,
     const int per_cu = std::max(1, std::min((int)(LDS_MAX / std::max<size_t>(lds, 1)), 32 / W));
     int G = std::min((int)g.x, g_num_cu * per_cu);
     if (G >= 8) G &= ~7;
-    hipLaunchKernelGGL((k_hx_modes12<T, S, TL, W, RB, CB, true>), dim3(G), dim3(64 * W), lds, s, n1, n2, n3, nvec, t1,
+    hipLaunchKernelGGL((k_hx_modes12<T, S, TL, W, RB, CB>), dim3(G), dim3(64 * W), lds, s, n1, n2, n3, nvec, t1,
                        t2, res, sh, sh0, x, ldx, y, ldy, act);
 }
 
@@ -449,33 +386,13 @@ static void launch_a(int va, dim3 g, size_t lds, hipStream_t s, int n1, int n2, 
 #undef LA
 }
 
-template <typename T, typename S, int W, int PT>
-static void launch_b1(int n1, int n2, int n3, int nvec, const T *l2, const T *dep, const S *x, long ldx, S *y,
-                      long ldy, const int *act, hipStream_t s)
-{
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_hx_mode3<T, S, W, PT, false>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX);
-        attr = true;
-    }
-    const long P = (long)n1 * n2;
-    dim3 gb((unsigned)((P + 16 * PT * W - 1) / (16 * PT * W)), nvec);
-    size_t ldsb = (size_t)rup16(n3) * rup16(n3) * sizeof(T);
-    hipLaunchKernelGGL((k_hx_mode3<T, S, W, PT, false>), gb, dim3(64 * W), ldsb, s, n1, n2, n3, l2, dep, x, ldx, y, ldy,
-                       act);
-}
-
 template <typename T, typename S>
 static void launch_b(int vb, int n1, int n2, int n3, int nvec, const T *l2, const T *dep, const S *x, long ldx, S *y,
                      long ldy, const int *act, hipStream_t s)
 {
-    switch (vb) {
-    case 0: launch_b1<T, S, 4, 1>(n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, act, s); break;
-    case 1: launch_b1<T, S, 4, 2>(n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, act, s); break;
-    case 2: launch_b1<T, S, 8, 1>(n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, act, s); break;
-    default: launch_b1<T, S, 8, 2>(n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, act, s); break;
-    }
+    const long P = (long)n1 * n2;
+    if (vb == 0) launch_outer_mode<T, S, S, 4, true>(P, n3, nvec, l2, dep, x, ldx, y, ldy, act, s);
+    else launch_outer_mode<T, S, S, 8, true>(P, n3, nvec, l2, dep, x, ldx, y, ldy, act, s);
 }
 
 template <typename T, typename S>
@@ -558,9 +475,9 @@ static int hx_fast3(scalit_ctx *c, int prec, const T *t1, const T *t2, const T *
     const int n1 = c->lv[0].n, n2 = c->lv[1].n, n3 = c->lv[2].n;
     TuneEntry *te = tune<T, S>(c, prec, t1, t2, l2, dep, res, shifts, shift0, nvec, x, ldx, y, ldy, s);
     const int va = te ? te->va : 2, vb = te ? te->vb : 0;
-    // mode 3 writes Y, then modes 1+2 and the diagonal accumulate into it
-    launch_b<T, S>(vb, n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, active, s);
+    // modes 1+2 and the diagonal write Y, then mode 3 adds to it
     pass_a<T, S>(c, va, t1, t2, res, shifts, shift0, nvec, x, ldx, y, ldy, active, s);
+    launch_b<T, S>(vb, n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, active, s);
     HIP_TRY(hipGetLastError());
     return SCALIT_OK;
 }

@@ -20,6 +20,7 @@
 #include <cstdlib>
 
 #include "device_common.hpp"
+#include "outer_mode.hpp"
 
 namespace scalit {
 
@@ -363,6 +364,11 @@ static void rotate(const Level &L, const T *vosb, const T *vosbt, bool trans, in
         return;
     }
     const T *mats = trans ? vosbt : vosb;
+    if (blk == 1) {
+        // the outermost level: one rotation for all of the inner indices
+        launch_outer_mode<T, SI, SO, 4, false>(nin, n, nvec, mats, nullptr, x, ldx, y, ldy, active, s);
+        return;
+    }
     int var = 1;   // 4 waves of one tile each: fastest measured from 1 to 16 vectors on the 64x96x80 workload
     if (const char *e = std::getenv("SCALIT_PX_ROT_VARIANT")) {
         const int f = std::atoi(e);

@@ -1,0 +1,166 @@
+// Product along the outermost index of a P x n array, on MFMA:
+//   Y(p, i) [+]= d(p) * sum_j A(i, j) X(p, j),     p < P (contiguous), i, j < n,
+// for every vector of a batch.  The mode-3 pass of H*X (A = L^2_theta,
+// d = DEP(r, R), added to Y) and the outermost rotation of P*X (A = V or V^T,
+// d = 1, written) are both this shape.
+//
+// Persistent grid: each workgroup stages A in LDS once; each wave then walks
+// its own tiles of 16 consecutive p of one vector (no workgroup barrier after
+// the staging) and loads the next tile's X and d into registers while the
+// current tile's MFMAs run (Y, when adding, at the top of its own tile).  Lane (lr, lk) of a tile holds X(p0 + lr, 4 s + lk)
+// for all s (the B fragments); the A fragments come from LDS.  NR = tiles of
+// 16 along n (n <= 16 NR, A zero padded), a compile-time count so the register
+// arrays hold exactly one column.
+#pragma once
+
+#include <algorithm>
+
+#include "device_common.hpp"
+
+namespace scalit {
+
+template <typename T, typename S, typename SO, int WAVES, int NR, bool ACC>
+__global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nvec, const T *__restrict__ amat,
+                                                          const T *__restrict__ dvec, const S *__restrict__ x,
+                                                          long ldx, SO *__restrict__ y, long ldy, const int *active)
+{
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    constexpr int np = 16 * NR;
+    constexpr int KS = 4 * NR;   // k steps of 4
+    extern __shared__ char smem_raw[];
+    T *sA = (T *)smem_raw;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: tile bases stay scalar
+    const int lr = lane & 15, lk = lane >> 4;
+    const long tpv = (P + 15) / 16;
+    const long ntile = tpv * nvec;
+    const long wstride = (long)gridDim.x * WAVES;
+    long t = (long)blockIdx.x * WAVES + wave;
+
+    // X and d of the current and the next tile; Y (when adding) of the current
+    // tile, loaded at the top of its iteration and used after its MFMAs
+    T bc[KS], dc, bn[KS], dn;
+    T yc[ACC ? NR : 1][4];
+    // tile tt: vector v and first point p0 (wave-uniform); lane point p0 + lr
+    auto tile_of = [&](long tt, int &v, long &p0) {
+        v = (int)(tt / tpv);
+        p0 = (tt % tpv) * 16;
+        return p0 + lr < P && vec_active(active, v);
+    };
+    auto load_x = [&](long tt, T(&b)[KS], T &d) {
+        int v;
+        long p0;
+        const bool pin = tile_of(tt, v, p0);
+        const S *xs = x + (long)v * ldx + p0;   // scalar base, 32-bit lane offsets
+        d = pin ? (dvec ? dvec[p0 + lr] : (T)1) : (T)0;
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int k = 4 * s + lk;
+            b[s] = (pin && k < n) ? ld_cvt<T, S>(xs, (int)(lr + P * k)) : (T)0;
+        }
+    };
+    if (t < ntile) load_x(t, bc, dc);
+
+    if (np != n)
+        for (int e = tid; e < np * np; e += 64 * WAVES)
+            if (e % np >= n || e / np >= n) sA[e] = (T)0;
+    stage_lds<64 * WAVES, 16>(amat, n * n, tid, [&](int e) -> T & { return sA[(e % n) + np * (e / n)]; });
+    __syncthreads();
+
+    while (t < ntile) {
+        const long tn = t + wstride;
+        int v;
+        long p0;
+        const bool pin = tile_of(t, v, p0);
+        SO *ys = y + (long)v * ldy + p0;
+        if constexpr (ACC) {
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i = 16 * r + M::row(lk, q);
+                    yc[r][q] = (pin && i < n) ? ld_cvt<T, SO>(ys, (int)(lr + P * i)) : (T)0;
+                }
+        }
+        if (tn < ntile) load_x(tn, bn, dn);
+        acc_t acc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) acc[r] = acc_t{0, 0, 0, 0};
+        // The A fragments are the same for every tile; an opaque base keeps the
+        // compiler from hoisting all KS x NR of them into registers.
+        int abase = lr + np * lk;
+        asm volatile("" : "+v"(abase));
+        // groups of 4 k steps; the scheduling barrier bounds the fragments in flight
+#pragma unroll
+        for (int s0 = 0; s0 < KS; s0 += 4) {
+#pragma unroll
+            for (int s = s0; s < s0 + 4; s++)
+#pragma unroll
+                for (int r = 0; r < NR; r++) acc[r] = M::run(sA[abase + 16 * r + 4 * np * s], bc[s], acc[r]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (pin) {
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i = 16 * r + M::row(lk, q);
+                    if (i < n) {
+                        T out = dc * acc[r][q];
+                        if constexpr (ACC) out += yc[r][q];
+                        st_cvt<T, SO>(ys, (int)(lr + P * i), out);
+                    }
+                }
+        }
+#pragma unroll
+        for (int s = 0; s < KS; s++) bc[s] = bn[s];
+        dc = dn;
+        t = tn;
+    }
+}
+
+// Launch with WAVES waves per workgroup and as many workgroups as fit on the
+// device at once (at most one per 16 tiles' worth of waves).
+template <typename T, typename S, typename SO, int WAVES, int NR, bool ACC>
+static void launch_outer_mode_nr(long P, int n, int nvec, const T *amat, const T *dvec, const S *x, long ldx, SO *y,
+                                 long ldy, const int *active, hipStream_t s)
+{
+    static int per_cu = 0;
+    const size_t lds = (size_t)(16 * NR) * (16 * NR) * sizeof(T);
+    if (per_cu == 0) {
+        (void)hipFuncSetAttribute((const void *)k_outer_mode<T, S, SO, WAVES, NR, ACC>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_outer_mode<T, S, SO, WAVES, NR, ACC>, 64 * WAVES,
+                                                         lds) != hipSuccess || nb < 1)
+            nb = 1;
+        per_cu = nb;
+    }
+    const long ntile = ((P + 15) / 16) * nvec;
+    long G = std::min<long>((ntile + WAVES - 1) / WAVES, (long)g_num_cu * per_cu);
+    if (G < 1) G = 1;
+    hipLaunchKernelGGL((k_outer_mode<T, S, SO, WAVES, NR, ACC>), dim3((unsigned)G), dim3(64 * WAVES), lds, s, P, n, nvec,
+                       amat, dvec, x, ldx, y, ldy, active);
+}
+
+// n <= 128: the tile count along n rounded up to a compiled one
+template <typename T, typename S, typename SO, int WAVES, bool ACC>
+static void launch_outer_mode(long P, int n, int nvec, const T *amat, const T *dvec, const S *x, long ldx, SO *y,
+                              long ldy, const int *active, hipStream_t s)
+{
+    const int nr = (n + 15) / 16;
+    if (nr <= 2)
+        launch_outer_mode_nr<T, S, SO, WAVES, 2, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+    else if (nr <= 4)
+        launch_outer_mode_nr<T, S, SO, WAVES, 4, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+    else if (nr == 5)
+        launch_outer_mode_nr<T, S, SO, WAVES, 5, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+    else if (nr == 6)
+        launch_outer_mode_nr<T, S, SO, WAVES, 6, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+    else
+        launch_outer_mode_nr<T, S, SO, WAVES, 8, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+}
+
+}  // namespace scalit
