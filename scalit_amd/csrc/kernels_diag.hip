@@ -566,8 +566,145 @@ __global__ void __launch_bounds__(1024) k_small_sym_eig(int n, const double *__r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Eigenvalues only (DSYEV 'N'): Householder reduction to tridiagonal form
+// (Golub & Van Loan, Matrix Computations, alg. 8.3.1), then every eigenvalue
+// by bisection on the Sturm sequence of the tridiagonal matrix (one thread per
+// eigenvalue index, which also gives the ascending order).  One workgroup of
+// 1024 threads; the matrix lives in LDS when it fits (LDS_A) and in the work
+// buffer otherwise.
+// ---------------------------------------------------------------------------
+constexpr int EIGV_NT = 1024;
+constexpr int EIGV_LDS_N = 112;   // largest n whose n x (n+1) matrix sits in LDS with the vectors
+
+__device__ double block_sum_1024(double v, double *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();   // red may still be read by the previous reduction
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0;
+#pragma unroll
+    for (int k = 0; k < EIGV_NT / 64; k++) t += red[k];
+    return t;
+}
+
+template <bool LDS_A>
+__global__ void __launch_bounds__(EIGV_NT) k_sym_eigvals(int n, const double *__restrict__ a_in, int lda,
+                                                         double *__restrict__ w, double *__restrict__ gwork)
+{
+    extern __shared__ double sm_eig[];
+    const int tid = threadIdx.x;
+    const int ld = n + 1;
+    double *A = LDS_A ? sm_eig : gwork;                    // A(i,j) at i + ld j
+    double *vec = LDS_A ? sm_eig + (long)n * ld : sm_eig;  // v | p | d | e (n each) | red (16)
+    double *v = vec, *pw = vec + n, *d = vec + 2 * n, *e = vec + 3 * n, *red = vec + 4 * n;
+    for (int x = tid; x < n * n; x += EIGV_NT) {
+        const int i = x % n, j = x / n;
+        A[i + ld * j] = (i <= j) ? a_in[i + (long)lda * j] : a_in[j + (long)lda * i];   // from the upper triangle
+    }
+    __syncthreads();
+    for (int k = 0; k + 2 < n; k++) {
+        const int m = n - k - 1;   // trailing size; x = A(k+1.., k)
+        double ss = 0;
+        for (int i = tid; i < m; i += EIGV_NT) {
+            const double xi = A[(k + 1 + i) + ld * k];
+            ss += (i > 0) ? xi * xi : 0.0;
+        }
+        const double sigma = block_sum_1024(ss, red);   // sum of squares below the subdiagonal
+        const double x0 = A[(k + 1) + ld * k];
+        if (tid == 0) d[k] = A[k + ld * k];
+        if (sigma == 0.0) {   // column already reduced
+            if (tid == 0) e[k] = x0;
+            __syncthreads();
+            continue;
+        }
+        const double nrm = sqrt(x0 * x0 + sigma);
+        const double alpha = x0 <= 0 ? nrm : -nrm;   // new subdiagonal value, no cancellation in v0
+        if (tid == 0) e[k] = alpha;
+        for (int i = tid; i < m; i += EIGV_NT) v[i] = (i == 0) ? x0 - alpha : A[(k + 1 + i) + ld * k];
+        __syncthreads();
+        const double vtv = (x0 - alpha) * (x0 - alpha) + sigma;
+        const double tau = 2.0 / vtv;
+        // p = tau A22 v
+        for (int i = tid; i < m; i += EIGV_NT) {
+            double acc = 0;
+            for (int j = 0; j < m; j++) acc += A[(k + 1 + i) + ld * (k + 1 + j)] * v[j];
+            pw[i] = tau * acc;
+        }
+        __syncthreads();
+        double pv = 0;
+        for (int i = tid; i < m; i += EIGV_NT) pv += pw[i] * v[i];
+        const double K = 0.5 * tau * block_sum_1024(pv, red);
+        for (int i = tid; i < m; i += EIGV_NT) pw[i] -= K * v[i];   // w = p - K v
+        __syncthreads();
+        // A22 -= v w^T + w v^T
+        for (int x = tid; x < m * m; x += EIGV_NT) {
+            const int i = x % m, j = x / m;
+            A[(k + 1 + i) + ld * (k + 1 + j)] -= v[i] * pw[j] + pw[i] * v[j];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (n >= 2) {
+            d[n - 2] = A[(n - 2) + ld * (n - 2)];
+            e[n - 2] = A[(n - 1) + ld * (n - 2)];
+        }
+        d[n - 1] = A[(n - 1) + ld * (n - 1)];
+    }
+    __syncthreads();
+    // Gershgorin interval of the tridiagonal matrix
+    double lo = 1e300, hi = -1e300;
+    for (int i = 0; i < n; i++) {
+        const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
+        lo = fmin(lo, d[i] - r);
+        hi = fmax(hi, d[i] + r);
+    }
+    const double span = fmax(hi - lo, 1e-300);
+    lo -= 1e-12 * span;
+    hi += 1e-12 * span;
+    const double pivmin = 1e-290;
+    for (int idx = tid; idx < n; idx += EIGV_NT) {
+        // smallest x with count(eigenvalues < x) > idx
+        double a = lo, b = hi;
+        for (int it = 0; it < 200; it++) {
+            const double mid = 0.5 * (a + b);
+            if (mid <= a || mid >= b) break;   // interval at one ulp
+            int cnt = 0;
+            double q = d[0] - mid;
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q < 0;
+            for (int i = 1; i < n; i++) {
+                q = d[i] - mid - e[i - 1] * e[i - 1] / q;
+                if (fabs(q) < pivmin) q = -pivmin;
+                cnt += q < 0;
+            }
+            if (cnt > idx) b = mid;
+            else a = mid;
+        }
+        w[idx] = 0.5 * (a + b);
+    }
+}
+
 int small_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz, double *work, hipStream_t s)
 {
+    if (z == nullptr && n > 0) {
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute((const void *)k_sym_eigvals<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      160 * 1024);
+            attr = true;
+        }
+        const size_t vec_bytes = (4 * (size_t)n + 16) * sizeof(double);
+        if (n <= EIGV_LDS_N)
+            hipLaunchKernelGGL(k_sym_eigvals<true>, dim3(1), dim3(EIGV_NT), (size_t)n * (n + 1) * 8 + vec_bytes, s, n,
+                               a, lda, w, work);
+        else
+            hipLaunchKernelGGL(k_sym_eigvals<false>, dim3(1), dim3(EIGV_NT), vec_bytes, s, n, a, lda, w, work);
+        HIP_TRY(hipGetLastError());
+        return SCALIT_OK;
+    }
     if (n < 1 || n > 1024) return SCALIT_EINVAL;
     hipLaunchKernelGGL(k_small_sym_eig, dim3(1), dim3(1024), 0, s, n, a, lda, w, z, ldz, work);
     HIP_TRY(hipGetLastError());

@@ -53,6 +53,20 @@ __device__ __forceinline__ double sum_parts(const double *p)
 
 #define GSTRIDE(i, n) for (long i = blockIdx.x * 256L + threadIdx.x; i < (n); i += (long)gridDim.x * 256L)
 
+// A grid of nparts(n) <= NPART workgroups writes partials 0..gridDim.x-1;
+// workgroup 0 zeroes the rest, so every consumer still adds all NPART slots in
+// index order (deterministic) while small vectors launch few workgroups.
+__device__ __forceinline__ void zero_tail(double *pv)
+{
+    if (blockIdx.x == 0)
+        for (int i = gridDim.x + threadIdx.x; i < NPART; i += blockDim.x) pv[i] = 0.0;
+}
+static unsigned nparts(long n)
+{
+    const long g = (n + 1023) / 1024;
+    return (unsigned)(g < 1 ? 1 : (g > NPART ? NPART : g));
+}
+
 // ---------------------------------------------------------------- generic
 __global__ void __launch_bounds__(256) k_dot_part(long n, const double *x, long ldx, const double *y, long ldy,
                                                   double *part)
@@ -64,6 +78,7 @@ __global__ void __launch_bounds__(256) k_dot_part(long n, const double *x, long 
     GSTRIDE(i, n) s += xv[i] * yv[i];
     s = bsum(s, red);
     if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
+    zero_tail(part + (long)v * NPART);
 }
 
 __global__ void k_reduce_parts(int nvec, const double *part, double *out)
@@ -75,7 +90,7 @@ __global__ void k_reduce_parts(int nvec, const double *part, double *out)
 void launch_dot_partials(int nvec, long n, const double *x, long ldx, const double *y, long ldy, double *part,
                          hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dot_part, dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy, part);
+    hipLaunchKernelGGL(k_dot_part, dim3(nparts(n), nvec), dim3(256), 0, s, n, x, ldx, y, ldy, part);
 }
 
 void launch_reduce_partials(int nvec, const double *part, double *out, hipStream_t s)
@@ -110,24 +125,24 @@ __global__ void k_cast(long n, const A *x, long ldx, B *y, long ldy)
 
 void launch_cast_d2f(int nvec, long n, const double *x, long ldx, float *y, long ldy, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cast<double, float>), dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(nparts(n), nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
 }
 void launch_cast_f2d(int nvec, long n, const float *x, long ldx, double *y, long ldy, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cast<float, double>), dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
+    hipLaunchKernelGGL((k_cast<float, double>), dim3(nparts(n), nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
 }
 void launch_cast_d2b(int nvec, long n, const double *x, long ldx, bf16_t *y, long ldy, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cast<double, bf16_t>), dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
+    hipLaunchKernelGGL((k_cast<double, bf16_t>), dim3(nparts(n), nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
 }
 void launch_cast_b2d(int nvec, long n, const bf16_t *x, long ldx, double *y, long ldy, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cast<bf16_t, double>), dim3(NPART, nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
+    hipLaunchKernelGGL((k_cast<bf16_t, double>), dim3(nparts(n), nvec), dim3(256), 0, s, n, x, ldx, y, ldy);
 }
 void launch_axpby(int nvec, long n, double a, const double *x, long ldx, double b, double *y, long ldy,
                   hipStream_t s)
 {
-    hipLaunchKernelGGL(k_axpby, dim3(NPART, nvec), dim3(256), 0, s, n, a, x, ldx, b, y, ldy);
+    hipLaunchKernelGGL(k_axpby, dim3(nparts(n), nvec), dim3(256), 0, s, n, a, x, ldx, b, y, ldy);
 }
 
 // ---------------------------------------------------------------- QMR
@@ -158,6 +173,7 @@ __global__ void __launch_bounds__(256) k_qmr_init(long n, int zero_x, QmrVecs<S>
     }
     s = bsum(s, red);
     if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
+    zero_tail(part + (long)v * NPART);
 }
 
 // init part 2 (after Z = P B): partial Z.Z
@@ -175,6 +191,7 @@ __global__ void __launch_bounds__(256) k_qmr_init_z(long n, QmrVecs<S> w, double
     }
     s = bsum(s, red);
     if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
+    zero_tail(part + (long)v * NPART);
 }
 
 // init scalars: BNRM, VNRM, ZNRM, GAMAMO=1, ETA=-1, ZVRAT=1
@@ -226,6 +243,7 @@ __global__ void __launch_bounds__(256) k_qmr_a(long n, QmrVecs<S> w, QmrScal *q,
         s += lv<S>(V, i) * lv<S>(Z, i);
     }
     s = bsum(s, red);
+    zero_tail(part + (long)v * NPART);
     if (threadIdx.x == 0) {
         part[(long)v * NPART + blockIdx.x] = s;
         if (blockIdx.x == 0) q[v].zvrat_next = q[v].zvrat * znrm / vnrm;
@@ -279,6 +297,7 @@ __global__ void __launch_bounds__(256) k_qmr_c(long n, QmrVecs<S> w, const QmrSc
     }
     s = bsum(s, red);
     if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
+    zero_tail(part + (long)v * NPART);
 }
 
 // k5: QSDOT, BETA, breakdown, V = S - BETA V, partial V.V
@@ -309,6 +328,7 @@ __global__ void __launch_bounds__(256) k_qmr_d(long n, QmrVecs<S> w, QmrScal *q,
         s += a * a;
     }
     s = bsum(s, red);
+    zero_tail(part + (long)v * NPART);
     if (threadIdx.x == 0) {
         part[(long)v * NPART + blockIdx.x] = s;
         if (blockIdx.x == 0) {
@@ -335,6 +355,7 @@ __global__ void __launch_bounds__(256) k_qmr_e(long n, QmrVecs<S> w, QmrScal *q,
         s += a * a;
     }
     s = bsum(s, red);
+    zero_tail(part + (long)v * NPART);
     if (threadIdx.x == 0) {
         part[(long)v * NPART + blockIdx.x] = s;
         if (blockIdx.x == 0) q[v].vnrmpo = sqrt(sum_parts(part_vv + (long)v * NPART));
@@ -393,13 +414,17 @@ __global__ void __launch_bounds__(256) k_qmr_f(long n, int conv, QmrVecs<S> w, Q
     if (conv != 1) {
         srr = bsum(srr, red);
         if (threadIdx.x == 0) part_rr[(long)v * NPART + blockIdx.x] = srr;
+        zero_tail(part_rr + (long)v * NPART);
         if (conv == 2) {
             sux = bsum(sux, red);
             if (threadIdx.x == 0) part_uxux[(long)v * NPART + blockIdx.x] = sux;
+            zero_tail(part_uxux + (long)v * NPART);
             sxx = bsum(sxx, red);
             if (threadIdx.x == 0) part_xx[(long)v * NPART + blockIdx.x] = sxx;
+            zero_tail(part_xx + (long)v * NPART);
             sur = bsum(sur, red);
             if (threadIdx.x == 0) part_urur[(long)v * NPART + blockIdx.x] = sur;
+            zero_tail(part_urur + (long)v * NPART);
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -430,8 +455,10 @@ __global__ void __launch_bounds__(256) k_qmr_g(long n, QmrVecs<S> w, const QmrSc
     }
     s1 = bsum(s1, red);
     if (threadIdx.x == 0) part_hh[(long)v * NPART + blockIdx.x] = s1;
+    zero_tail(part_hh + (long)v * NPART);
     s2 = bsum(s2, red);
     if (threadIdx.x == 0) part_rr[(long)v * NPART + blockIdx.x] = s2;
+    zero_tail(part_rr + (long)v * NPART);
 }
 
 // k9: residual, convergence, roll the two-level variables, next iteration
@@ -480,12 +507,12 @@ __global__ void k_qmr_h(int nvec, int conv, int qmrmax, double tol, QmrScal *q, 
 template <typename S>
 void launch_qmr_init(int nvec, long n, int zero_x, const QmrVecs<S> &w, double *part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_init<S>), dim3(NPART, nvec), dim3(256), 0, s, n, zero_x, w, part);
+    hipLaunchKernelGGL((k_qmr_init<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, zero_x, w, part);
 }
 template <typename S>
 void launch_qmr_init_z(int nvec, long n, const QmrVecs<S> &w, double *part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_init_z<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, part);
+    hipLaunchKernelGGL((k_qmr_init_z<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, part);
 }
 void launch_qmr_init_scal(int nvec, const double *pbb, const double *pzz, QmrScal *q, int *active, int qmrmax,
                           hipStream_t s)
@@ -495,40 +522,40 @@ void launch_qmr_init_scal(int nvec, const double *pbb, const double *pzz, QmrSca
 template <typename S>
 void launch_qmr_a(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, double *part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_a<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, active, part);
+    hipLaunchKernelGGL((k_qmr_a<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, active, part);
 }
 template <typename S>
 void launch_qmr_b(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, const double *part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_b<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, active, part);
+    hipLaunchKernelGGL((k_qmr_b<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, active, part);
 }
 template <typename S>
 void launch_qmr_c(int nvec, long n, const QmrVecs<S> &w, const QmrScal *q, double *part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_c<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, part);
+    hipLaunchKernelGGL((k_qmr_c<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, part);
 }
 template <typename S>
 void launch_qmr_d(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, const double *pin, double *part,
                   hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_d<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, active, pin, part);
+    hipLaunchKernelGGL((k_qmr_d<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, active, pin, part);
 }
 template <typename S>
 void launch_qmr_e(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, const double *pvv, double *part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_e<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, pvv, part);
+    hipLaunchKernelGGL((k_qmr_e<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, pvv, part);
 }
 template <typename S>
 void launch_qmr_f(int nvec, long n, int conv, const QmrVecs<S> &w, QmrScal *q, int *active, const double *pzz,
                   double *prr, double *pux, double *pxx, double *pur, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_f<S>), dim3(NPART, nvec), dim3(256), 0, s, n, conv, w, q, active, pzz, prr, pux, pxx,
+    hipLaunchKernelGGL((k_qmr_f<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, conv, w, q, active, pzz, prr, pux, pxx,
                        pur);
 }
 template <typename S>
 void launch_qmr_g(int nvec, long n, const QmrVecs<S> &w, const QmrScal *q, double *phh, double *prr, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_g<S>), dim3(NPART, nvec), dim3(256), 0, s, n, w, q, phh, prr);
+    hipLaunchKernelGGL((k_qmr_g<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, phh, prr);
 }
 void launch_qmr_h(int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *phh,
                   const double *prr, const double *pux, const double *pxx, const double *pur, hipStream_t s)
@@ -593,6 +620,7 @@ __global__ void __launch_bounds__(256) k_mgs_step(long n, int i, int inext, cons
     }
     s = bsum(s, red);
     if (threadIdx.x == 0) part_out[(long)k * NPART + blockIdx.x] = s;
+    zero_tail(part_out + (long)k * NPART);
 }
 
 // normalise: dst = w / ||w|| (norm from partials); status[k] = 0 when the norm is 0
@@ -612,14 +640,14 @@ __global__ void __launch_bounds__(256) k_normalize(long n, const double *w, long
 void launch_mgs_step(int nw, long n, int i, int inext, const double *vj, long ldv, long ldw, double *w, long ldwk,
                      const double *pin, double *pout, const int *active, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mgs_step, dim3(NPART, nw), dim3(256), 0, s, n, i, inext, vj, ldv, ldw, w, ldwk, pin, pout,
+    hipLaunchKernelGGL(k_mgs_step, dim3(nparts(n), nw), dim3(256), 0, s, n, i, inext, vj, ldv, ldw, w, ldwk, pin, pout,
                        active);
 }
 
 void launch_normalize(int nw, long n, const double *w, long ldwk, double *dst, long ldd, const double *part, int *ok,
                       const int *active, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_normalize, dim3(NPART, nw), dim3(256), 0, s, n, w, ldwk, dst, ldd, part, ok, active);
+    hipLaunchKernelGGL(k_normalize, dim3(nparts(n), nw), dim3(256), 0, s, n, w, ldwk, dst, ldd, part, ok, active);
 }
 
 // Multi-dot: part[k][i][blk] = partial of <V_k(:, i0+i), w_k> for i < m
@@ -638,6 +666,7 @@ __global__ void __launch_bounds__(256) k_multidot(long n, int i0, int m, const d
         GSTRIDE(t, n) s += vi[t] * W[t];
         s = bsum(s, red);
         if (threadIdx.x == 0) part[((long)k * m + i) * NPART + blockIdx.x] = s;
+        zero_tail(part + ((long)k * m + i) * NPART);
     }
 }
 
@@ -658,7 +687,7 @@ __global__ void k_multidot_fin(int nw, int m, int i0, int j, const double *part,
 void launch_multidot(int nw, long n, int i0, int m, const double *vj, long ldv, long ldw, const double *w, long ldwk,
                      double *part, const int *active, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_multidot, dim3(NPART, nw), dim3(256), 0, s, n, i0, m, vj, ldv, ldw, w, ldwk, part, active);
+    hipLaunchKernelGGL(k_multidot, dim3(nparts(n), nw), dim3(256), 0, s, n, i0, m, vj, ldv, ldw, w, ldwk, part, active);
 }
 
 void launch_multidot_fin(int nw, int m, int i0, int j, const double *part, double *hmat, int ldh, long ldhk,
@@ -685,7 +714,7 @@ __global__ void k_ritz(long n, int m, int nc, const double *vj, long ldv, const 
 void launch_ritz(long n, int m, int nc, const double *vj, long ldv, const double *z, int ldz, const int *cols,
                  double *psi, long ldp, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ritz, dim3(NPART, nc), dim3(256), 0, s, n, m, nc, vj, ldv, z, ldz, cols, psi, ldp);
+    hipLaunchKernelGGL(k_ritz, dim3(nparts(n), nc), dim3(256), 0, s, n, m, nc, vj, ldv, z, ldz, cols, psi, ldp);
 }
 
 // Hellmann-Feynman: part[(k*np + j)][blk] = partial of sum psi_k^2 dvdp_j
@@ -701,13 +730,14 @@ __global__ void __launch_bounds__(256) k_hf_part(long n, int np, const double *p
         GSTRIDE(t, n) s += p[t] * p[t] * g[t];
         s = bsum(s, red);
         if (threadIdx.x == 0) part[((long)k * np + j) * NPART + blockIdx.x] = s;
+        zero_tail(part + ((long)k * np + j) * NPART);
     }
 }
 
 void launch_hf_part(int nk, long n, int np, const double *psi, long ld, const double *dvdp, long lddv, double *part,
                     hipStream_t s)
 {
-    hipLaunchKernelGGL(k_hf_part, dim3(NPART, nk), dim3(256), 0, s, n, np, psi, ld, dvdp, lddv, part);
+    hipLaunchKernelGGL(k_hf_part, dim3(nparts(n), nk), dim3(256), 0, s, n, np, psi, ld, dvdp, lddv, part);
 }
 
 // dv += sum_b x_b o g_b
@@ -723,7 +753,7 @@ __global__ void k_dv_accum(long n, int nvec, const double *x, const double *g, l
 
 void launch_dv_accum(long n, int nvec, const double *x, const double *g, long ld, double *dv, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dv_accum, dim3(NPART), dim3(256), 0, s, n, nvec, x, g, ld, dv);
+    hipLaunchKernelGGL(k_dv_accum, dim3(nparts(n)), dim3(256), 0, s, n, nvec, x, g, ld, dv);
 }
 
 }  // namespace scalit
