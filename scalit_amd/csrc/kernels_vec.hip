@@ -51,6 +51,22 @@ __device__ __forceinline__ double sum_parts(const double *p)
     return s;
 }
 
+// partials per vector written by a grid sized for vectors of length n (the
+// host's nparts); the slots past them hold zeros, so this sum is bit-identical
+// to sum_parts
+__device__ __forceinline__ int nparts_dev(long n)
+{
+    const long g = (n + 1023) / 1024;
+    return (int)(g < 1 ? 1 : (g > NPART ? NPART : g));
+}
+__device__ __forceinline__ double sum_parts_n(const double *p, long n)
+{
+    const int g = nparts_dev(n);
+    double s = 0;
+    for (int i = 0; i < g; i++) s += p[i];
+    return s;
+}
+
 #define GSTRIDE(i, n) for (long i = blockIdx.x * 256L + threadIdx.x; i < (n); i += (long)gridDim.x * 256L)
 
 // A grid of nparts(n) <= NPART workgroups writes partials 0..gridDim.x-1;
@@ -256,7 +272,7 @@ __global__ void __launch_bounds__(256) k_qmr_b(long n, QmrVecs<S> w, QmrScal *q,
 {
     const int v = blockIdx.y;
     if (!running(q, v)) return;
-    const double vzdot = sum_parts(part + (long)v * NPART);
+    const double vzdot = sum_parts_n(part + (long)v * NPART, n);
     if (vzdot == 0.0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             q[v].status = QMR_EXIT;
@@ -308,7 +324,7 @@ __global__ void __launch_bounds__(256) k_qmr_d(long n, QmrVecs<S> w, QmrScal *q,
     __shared__ double red[4];
     const int v = blockIdx.y;
     if (!running(q, v)) return;
-    const double qsdot = sum_parts(part_in + (long)v * NPART);
+    const double qsdot = sum_parts_n(part_in + (long)v * NPART, n);
     const double beta = qsdot / q[v].vzdot;
     if (qsdot == 0.0 || beta == 0.0) {
         __syncthreads();
@@ -358,7 +374,7 @@ __global__ void __launch_bounds__(256) k_qmr_e(long n, QmrVecs<S> w, QmrScal *q,
     zero_tail(part + (long)v * NPART);
     if (threadIdx.x == 0) {
         part[(long)v * NPART + blockIdx.x] = s;
-        if (blockIdx.x == 0) q[v].vnrmpo = sqrt(sum_parts(part_vv + (long)v * NPART));
+        if (blockIdx.x == 0) q[v].vnrmpo = sqrt(sum_parts_n(part_vv + (long)v * NPART, n));
     }
 }
 
@@ -373,7 +389,7 @@ __global__ void __launch_bounds__(256) k_qmr_f(long n, int conv, QmrVecs<S> w, Q
     const int v = blockIdx.y;
     if (!running(q, v)) return;
     const QmrScal &o = q[v];
-    const double znrm = sqrt(sum_parts(part_zz + (long)v * NPART));
+    const double znrm = sqrt(sum_parts_n(part_zz + (long)v * NPART, n));
     const double theta = o.vnrmpo / (o.gamamo * fabs(o.beta));
     const double gama = 1.0 / sqrt(1.0 + theta * theta);
     const double eta = -o.eta * o.vnrm * gama * gama / (o.beta * o.gamamo * o.gamamo);
@@ -462,7 +478,8 @@ __global__ void __launch_bounds__(256) k_qmr_g(long n, QmrVecs<S> w, const QmrSc
 }
 
 // k9: residual, convergence, roll the two-level variables, next iteration
-__global__ void k_qmr_h(int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *part_hh,
+__global__ void k_qmr_h(long n, int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active,
+                        const double *part_hh,
                         const double *part_rr, const double *part_uxux, const double *part_xx,
                         const double *part_urur)
 {
@@ -472,17 +489,17 @@ __global__ void k_qmr_h(int nvec, int conv, int qmrmax, double tol, QmrScal *q, 
     if (s.status != QMR_RUNNING) return;
     double res;
     if (conv == 1) {
-        double xnrm = sqrt(sum_parts(part_hh + (long)v * NPART));
-        res = sqrt(sum_parts(part_rr + (long)v * NPART)) / xnrm;
+        double xnrm = sqrt(sum_parts_n(part_hh + (long)v * NPART, n));
+        res = sqrt(sum_parts_n(part_rr + (long)v * NPART, n)) / xnrm;
     } else if (conv == 2) {
-        double rr = sum_parts(part_rr + (long)v * NPART);
+        double rr = sum_parts_n(part_rr + (long)v * NPART, n);
         double r1 = sqrt(rr) / s.bnrm;
-        double r2 = sqrt(sum_parts(part_uxux + (long)v * NPART) / sum_parts(part_xx + (long)v * NPART));
-        double r3 = sqrt(sum_parts(part_urur + (long)v * NPART) / rr);
+        double r2 = sqrt(sum_parts_n(part_uxux + (long)v * NPART, n) / sum_parts_n(part_xx + (long)v * NPART, n));
+        double r3 = sqrt(sum_parts_n(part_urur + (long)v * NPART, n) / rr);
         res = r1 > r2 ? r1 : r2;
         res = res > r3 ? res : r3;
     } else {
-        res = sqrt(sum_parts(part_rr + (long)v * NPART)) / s.bnrm;
+        res = sqrt(sum_parts_n(part_rr + (long)v * NPART, n)) / s.bnrm;
     }
     s.res = res;
     if (fabs(res) < tol) {
@@ -557,11 +574,11 @@ void launch_qmr_g(int nvec, long n, const QmrVecs<S> &w, const QmrScal *q, doubl
 {
     hipLaunchKernelGGL((k_qmr_g<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, phh, prr);
 }
-void launch_qmr_h(int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *phh,
+void launch_qmr_h(long n, int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *phh,
                   const double *prr, const double *pux, const double *pxx, const double *pur, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_qmr_h, dim3((nvec + 63) / 64), dim3(64), 0, s, nvec, conv, qmrmax, tol, q, active, phh, prr,
-                       pux, pxx, pur);
+    hipLaunchKernelGGL(k_qmr_h, dim3((nvec + 63) / 64), dim3(64), 0, s, n, nvec, conv, qmrmax, tol, q, active, phh,
+                       prr, pux, pxx, pur);
 }
 
 #define QMR_INST(S)                                                                                               \
@@ -596,7 +613,7 @@ __global__ void __launch_bounds__(256) k_mgs_step(long n, int i, int inext, cons
     double *W = w + (long)k * ldwk;
     double s = 0;
     if (i >= 0) {
-        const double d = sum_parts(part_in + (long)k * NPART);
+        const double d = sum_parts_n(part_in + (long)k * NPART, n);
         const double *vi = V + (long)i * ldv;
         if (inext >= 0) {
             const double *vn = V + (long)inext * ldv;
@@ -629,7 +646,7 @@ __global__ void __launch_bounds__(256) k_normalize(long n, const double *w, long
 {
     const int k = blockIdx.y;
     if (active && !active[k]) return;
-    const double nrm = sqrt(sum_parts(part + (long)k * NPART));
+    const double nrm = sqrt(sum_parts_n(part + (long)k * NPART, n));
     if (blockIdx.x == 0 && threadIdx.x == 0 && ok) ok[k] = nrm != 0.0;
     if (nrm == 0.0) return;
     const double *W = w + (long)k * ldwk;

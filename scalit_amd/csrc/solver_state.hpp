@@ -46,7 +46,7 @@ void launch_qmr_f(int nvec, long n, int conv, const QmrVecs<S> &w, QmrScal *q, i
                   double *prr, double *pux, double *pxx, double *pur, hipStream_t s);
 template <typename S>
 void launch_qmr_g(int nvec, long n, const QmrVecs<S> &w, const QmrScal *q, double *phh, double *prr, hipStream_t s);
-void launch_qmr_h(int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *phh,
+void launch_qmr_h(long n, int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *phh,
                   const double *prr, const double *pux, const double *pxx, const double *pur, hipStream_t s);
 
 void launch_cast_d2f(int nvec, long n, const double *x, long ldx, float *y, long ldy, hipStream_t s);

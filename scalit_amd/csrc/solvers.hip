@@ -16,6 +16,7 @@
 //    lanHij.f90:6-45): fp64 Lanczos with full re-orthogonalisation.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -267,13 +268,21 @@ static int qmr_iteration(scalit_ctx *c, QmrWS<S> &w, int conv, int qmrmax, doubl
         SC_TRY(op_hx<S>(c, w, cb, v.x, v.HX0, act, s));
         launch_qmr_g<S>(nv, n, v, q, w.part(8), w.part(4), s);
     }
-    launch_qmr_h(nv, conv, qmrmax, tol, q, act, w.part(8), w.part(4), w.part(5), w.part(6), w.part(7), s);
+    launch_qmr_h(w.n, nv, conv, qmrmax, tol, q, act, w.part(8), w.part(4), w.part(5), w.part(6), w.part(7), s);
     HIP_TRY(hipGetLastError());
     return SCALIT_OK;
 }
 
-int g_qmr_use_graph = 1;
-int g_qmr_check_every = 4;
+// QMR iterations are replayed from a recorded HIP graph unless
+// SCALIT_QMR_GRAPH=0; the host reads the convergence flags every
+// SCALIT_QMR_CHECK (default 4) iterations.
+static int env_int(const char *name, int def)
+{
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : def;
+}
+int g_qmr_use_graph = env_int("SCALIT_QMR_GRAPH", 1);
+int g_qmr_check_every = std::max(1, env_int("SCALIT_QMR_CHECK", 4));
 
 // Solve (H - shift_v) x_v = b_v for the nvec columns already in w.B(); the
 // solution lands in w.X().  shifts: host, nvec values (the QMR shift E0 per
