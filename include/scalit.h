@@ -214,6 +214,10 @@ int scalit_debug_mfma_probe(int prec, double *out);
  * rounds of 8 independent 16x16x4 chains from registers; *ms = kernel time
  * (FLOP = blocks * 4 * iters * 8 * 2048). */
 int scalit_debug_mfma_peak(int prec, int iters, int blocks, float *ms);
+/* The small dense symmetric eigensolve of the Rayleigh-Ritz steps on the GPU
+ * (host arrays, column-major, upper triangle of a read): w = eigenvalues
+ * ascending; z (n x n, ldz) = eigenvectors when not NULL. */
+int scalit_debug_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz);
 /* The fused H*X kernel variants (pass A block shape, pass B width) the
  * autotuner picked for this precision and batch size (tuning them if needed). */
 int scalit_debug_hx_variant(scalit_ctx *ctx, int prec, int nvec, int *va, int *vb);

@@ -72,6 +72,7 @@ SIGNATURES = {
     "scalit_debug_mfma_probe": (_i, [_i, _dp]),
     "scalit_debug_mfma_peak": (_i, [_i, _i, _i, C.POINTER(C.c_float)]),
     "scalit_debug_hx_variant": (_i, [_vp, _i, _i, _ip, _ip]),
+    "scalit_debug_sym_eig": (_i, [_i, _dp, _i, _dp, _dp, _i]),
 }
 
 _lib = None

@@ -622,6 +622,32 @@ int scalit_debug_mfma_peak(int prec, int iters, int blocks, float *ms)
     return mfma_peak(prec, iters, blocks, ms);
 }
 
+int scalit_debug_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz)
+{
+    if (n < 1 || !a || !w || lda < n || (z && ldz < n)) return SCALIT_EINVAL;
+    DBuf<double> da, dw, dz, work;
+    auto run = [&]() -> int {
+        SC_TRY(da.alloc((size_t)n * n));
+        SC_TRY(dw.alloc(n));
+        SC_TRY(work.alloc(small_sym_eig_work(n)));
+        if (z) SC_TRY(dz.alloc((size_t)n * n));
+        for (int j = 0; j < n; j++)
+            HIP_TRY(hipMemcpy(da.p + (long)j * n, a + (long)j * lda, sizeof(double) * n, hipMemcpyHostToDevice));
+        SC_TRY(small_sym_eig(n, da.p, n, dw.p, z ? dz.p : nullptr, n, work.p, nullptr));
+        HIP_TRY(hipMemcpy(w, dw.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (z)
+            for (int j = 0; j < n; j++)
+                HIP_TRY(hipMemcpy(z + (long)j * ldz, dz.p + (long)j * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+        return SCALIT_OK;
+    };
+    const int rc = run();
+    da.release();
+    dw.release();
+    dz.release();
+    work.release();
+    return rc;
+}
+
 int scalit_debug_mfma_probe(int prec, double *out)
 {
     if (!out) return SCALIT_EINVAL;
