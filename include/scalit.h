@@ -72,6 +72,14 @@ int scalit_get_len(const scalit_ctx *ctx, long *n);
 int scalit_set_h0(scalit_ctx *ctx, int level, const double *h0);
 int scalit_set_dep(scalit_ctx *ctx, int level, const double *dep);
 int scalit_set_res(scalit_ctx *ctx, const double *res);
+/* sNDVR = T outermost layer (OUTH of src/iterate/module/osb.data.h:62, read by
+ * loadOutH, osb.io.h:230-235): outh holds myDim(sF) x sN(sF) x sN(sF) values,
+ * point index fastest.  Level sF of H*X then applies H3X
+ * (src/libutil/serial/base/matrix.f90:188-203, osb.hx.h:93-94) instead of
+ * H0 [x DEP], its OSB block is OUTH itself (osb.diag.h:22-23), and its
+ * H0 / DEP are not needed (sDEP(sF) is forced off, osb.print.h:204).
+ * Requires sF >= 2. */
+int scalit_set_outh(scalit_ctx *ctx, const double *outh);
 /* sOSBW%mE0 / mDE / mBeta (src/iterate/module/a_osbtype.f90:26-29) */
 int scalit_set_shift(scalit_ctx *ctx, double e0, double de, double beta);
 

@@ -48,6 +48,8 @@ class OrcOSB(C.Structure):
         ("osb_kind", C.c_int),
         ("hc", C.c_int),
         ("pc", C.c_int),
+        ("ndvr", C.c_int),
+        ("OUTH", _dp),
     ]
 
 
@@ -209,12 +211,15 @@ class Operator:
     None or a vector of length myDim(level), ``res`` the diagonal potential on
     the full grid (level 1 fastest)."""
 
-    def __init__(self, sn, h0, res, dep=None, e0: float = 0.0):
+    def __init__(self, sn, h0, res, dep=None, e0: float = 0.0, outh=None):
         self.sF = len(sn)
         self.sN = [int(v) for v in sn]
         self.H0 = [fmat(h) for h in h0]
-        dep = dep if dep is not None else [None] * self.sF
+        dep = list(dep) if dep is not None else [None] * self.sF
+        if outh is not None:
+            dep[-1] = None   # sNDVR forces sDEP(sF) = F (src/iterate/module/osb.print.h:204)
         self.DEP = [None if d is None else f64(d) for d in dep]
+        self.OUTH = None if outh is None else f64(outh).reshape(-1)
         self.RES = f64(res).reshape(-1)
         self.s = OrcOSB()
         self.s.sF = self.sF
@@ -228,6 +233,10 @@ class Operator:
         self.N = int(self.s.myLen)
         assert self.RES.size == self.N
         self.s.RES = dptr(self.RES)
+        if self.OUTH is not None:
+            assert self.OUTH.size == self.N * self.sN[-1]
+            self.s.ndvr = 1
+            self.s.OUTH = dptr(self.OUTH)
         self.s.E0 = e0
         self.s.DE = 1.0
         self.VOSB = None
@@ -360,7 +369,8 @@ class RefOperator:
         sdep = (C.c_int * FMAX)(*[0 if d is None else 1 for d in op.DEP])
         h0 = (_dp * FMAX)(*[dptr(h) for h in op.H0])
         dep = (_dp * FMAX)(*[dptr(d) if d is not None else _dp() for d in op.DEP])
-        R.refdrv_setup(C.c_int(op.sF), sn, sdep, h0, dep, dptr(op.RES), C.c_double(op.s.E0))
+        outh = dptr(op.OUTH) if op.OUTH is not None else _dp()
+        R.refdrv_setup(C.c_int(op.sF), sn, sdep, h0, dep, dptr(op.RES), C.c_double(op.s.E0), outh)
         self.vosb = None
         self.eig0 = None
 

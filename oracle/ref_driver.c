@@ -5,7 +5,7 @@
  * and input layer; this file restates those module loops in C
  * (src/iterate/module/osb.hx.h:71-110, osb.px.h:125-219, osbw.qmr.h:13-23,
  * osb.diag.h:111-137) and calls the reference's own compiled library routines
- * (H1X_XYZ, H1X_DEP, VX, VTX, HOSBINIT_*, HosbDiag, QMR) for all arithmetic.
+ * (H1X_XYZ, H1X_DEP, H3X, VX, VTX, HOSBINIT_*, HosbDiag, QMR) for all arithmetic.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -15,6 +15,7 @@
 extern void h1x_xyz_(const int *nin, const int *nout, const int *blk, const double *h1, const double *x, double *y);
 extern void h1x_dep_(const int *nin, const int *nout, const int *blk, const double *h1, const double *corx,
                      const double *x, double *y);
+extern void h3x_(const int *nin, const int *nout, const double *a, const double *x, double *xout);
 extern void vx_(const int *nin, const int *nout, const int *blk, const double *vout, double *x);
 extern void vtx_(const int *nin, const int *nout, const int *blk, const double *vout, double *x);
 extern void hosbinit_xyz_(const int *blk, const int *mak, const int *ndk, const double *h1, const double *res,
@@ -30,6 +31,7 @@ extern int qmr_(const int *ntype, const int *qmrmax, const double *qmrtol, const
 static struct {
     int sF, sN[FMAX], sDEP[FMAX], myDim[FMAX], myBlk[FMAX], myLen;
     const double *H0[FMAX], *DEP[FMAX], *RES;
+    const double *OUTH;   /* sNDVR = T outer layer, or NULL */
     const double *VOSB[FMAX];
     const double *EIG0;
     double E0;
@@ -39,8 +41,9 @@ static struct {
 } S;
 
 void refdrv_setup(int sF, const int *sN, const int *sDEP, const double **H0, const double **DEP, const double *RES,
-                  double E0)
+                  double E0, const double *OUTH)
 {
+    S.OUTH = OUTH;
     S.sF = sF;
     for (int i = 0; i < sF; i++) {
         S.sN[i] = sN[i];
@@ -77,7 +80,8 @@ static void myhx(double c, const double *x, double *y)
     int n = S.myLen;
     for (int k = 0; k < n; k++) y[k] = (S.RES[k] - c) * x[k];
     for (int l = 0; l < S.sF; l++) {
-        if (S.sDEP[l]) h1x_dep_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], S.DEP[l], x, S.tmp);
+        if (S.OUTH && l == S.sF - 1) h3x_(&S.myDim[l], &S.sN[l], S.OUTH, x, S.tmp);   /* osb.hx.h:93-94 */
+        else if (S.sDEP[l]) h1x_dep_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], S.DEP[l], x, S.tmp);
         else h1x_xyz_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], x, S.tmp);
         for (int k = 0; k < n; k++) y[k] = y[k] + S.tmp[k];
     }
@@ -93,7 +97,8 @@ void refdrv_hx(const int *n, const double *x, double *y)
     int nn = S.myLen;
     for (int k = 0; k < nn; k++) y[k] = (S.RES[k] - S.E0) * x[k];
     for (int l = 0; l < S.sF; l++) {
-        if (S.sDEP[l]) h1x_dep_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], S.DEP[l], x, S.tmp);
+        if (S.OUTH && l == S.sF - 1) h3x_(&S.myDim[l], &S.sN[l], S.OUTH, x, S.tmp);   /* osb.hx.h:93-94 */
+        else if (S.sDEP[l]) h1x_dep_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], S.DEP[l], x, S.tmp);
         else h1x_xyz_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], x, S.tmp);
         for (int k = 0; k < nn; k++) y[k] = y[k] + S.tmp[k];
     }
@@ -116,8 +121,8 @@ int refdrv_linsolv(const int *n, const double *b, double *x, double *res)
     return qmr_(&S.qmr_type, &S.qmr_max, &S.qmr_tol, n, b, refdrv_hx, refdrv_px, x, res);
 }
 
-/* myDiag for the real, sNDVR=F case; EIG0 starts as RES.  VOSB[l] and EIG0
- * are caller-allocated. */
+/* myDiag for the real case (osb.diag.h:111-137 with myDiagInit :18-31); EIG0
+ * starts as RES.  VOSB[l] and EIG0 are caller-allocated. */
 int refdrv_diag(int bjmax, double bjtol, double **VOSB, double *EIG0)
 {
     int n = S.myLen;
@@ -125,7 +130,8 @@ int refdrv_diag(int bjmax, double bjtol, double **VOSB, double *EIG0)
     for (int l = S.sF - 1; l >= 0; l--) {
         double *hosb = (double *)malloc(sizeof(double) * (size_t)S.sN[l] * n);
         if (!hosb) return 0;
-        if (S.sDEP[l]) hosbinit_dep_(&S.myBlk[l], &S.sN[l], &S.myDim[l], S.H0[l], S.DEP[l], EIG0, hosb);
+        if (S.OUTH && l == S.sF - 1) memcpy(hosb, S.OUTH, sizeof(double) * (size_t)S.sN[l] * n);   /* osb.diag.h:22-23 */
+        else if (S.sDEP[l]) hosbinit_dep_(&S.myBlk[l], &S.sN[l], &S.myDim[l], S.H0[l], S.DEP[l], EIG0, hosb);
         else hosbinit_xyz_(&S.myBlk[l], &S.sN[l], &S.myDim[l], S.H0[l], EIG0, hosb);
         hosbdiag_(&bjmax, &bjtol, &S.myBlk[l], &S.sN[l], &S.myDim[l], hosb, VOSB[l], EIG0);
         free(hosb);

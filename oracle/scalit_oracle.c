@@ -94,6 +94,19 @@ void orc_h3x(int nin, int nout, const double *a, const double *x, double *xout)
     }
 }
 
+void orc_h3tx(int nin, int nout, const double *a, const double *x, double *xout)
+{
+    for (int i = 0; i < nout; i++) {
+        double *xo = xout + (long)nin * i;
+        for (int k = 0; k < nin; k++) xo[k] = 0.0;
+        for (int j = 0; j < nout; j++) {
+            const double *aji = a + (long)nin * (j + (long)nout * i);
+            const double *xj = x + (long)nin * j;
+            for (int k = 0; k < nin; k++) xo[k] = xo[k] + aji[k] * xj[k];
+        }
+    }
+}
+
 /* src/iterate/libiterate/hpx/hpx.f90:5-18 */
 void orc_h1x_xyz(int nin, int nout, int blk, const double *h1, const double *x, double *y)
 {
@@ -143,7 +156,9 @@ void orc_myhx(const orc_osb *o, int ntype, const double *x, double *y)
         for (long k = 0; k < n; k++) y[k] = (o->RES[k] - o->E0) * x[k];
     }
     for (int l = 0; l < o->sF; l++) {
-        if (o->sDEP[l])
+        if (o->ndvr && l == o->sF - 1)
+            orc_h3x(o->myDim[l], o->sN[l], o->OUTH, x, tmp);
+        else if (o->sDEP[l])
             orc_h1x_dep(o->myDim[l], o->sN[l], o->myBlk[l], o->H0[l], o->DEP[l], x, tmp);
         else
             orc_h1x_xyz(o->myDim[l], o->sN[l], o->myBlk[l], o->H0[l], x, tmp);
@@ -166,6 +181,11 @@ void orc_myhx_t(const orc_osb *o, int ntype, const double *x, double *y)
     for (int l = 0; l < o->sF; l++) {
         int nin = o->myDim[l], nout = o->sN[l], blk = o->myBlk[l];
         long bs = (long)nin * nout;
+        if (o->ndvr && l == o->sF - 1) {
+            orc_h3tx(nin, nout, o->OUTH, x, tmp);
+            for (long k = 0; k < n; k++) y[k] = y[k] + tmp[k];
+            continue;
+        }
         for (int b = 0; b < blk; b++) {
             const double *xb = x + bs * b;
             double *tb = tmp + bs * b;
@@ -388,7 +408,9 @@ int orc_my_diag(orc_osb *o, int bjmax, double bjtol)
     for (int l = o->sF - 1; l >= 0; l--) {
         hosb = (double *)malloc(sizeof(double) * (long)o->sN[l] * n);
         if (!hosb) return 0;
-        if (o->sDEP[l])
+        if (o->ndvr && l == o->sF - 1)   /* myDiagInit: HOSB(sF) = OUTH (osb.diag.h:22-23) */
+            memcpy(hosb, o->OUTH, sizeof(double) * (long)o->sN[l] * n);
+        else if (o->sDEP[l])
             orc_hosbinit_dep(o->myBlk[l], o->sN[l], o->myDim[l], o->H0[l], o->DEP[l], o->EIG0, hosb);
         else
             orc_hosbinit_xyz(o->myBlk[l], o->sN[l], o->myDim[l], o->H0[l], o->EIG0, hosb);
@@ -990,7 +1012,22 @@ void orc_myhx_omp(const orc_osb *o, int ntype, const double *x, double *y, doubl
     for (int l = 0; l < o->sF; l++) {
         int nin = o->myDim[l], nout = o->sN[l], blk = o->myBlk[l];
         long bs = (long)nin * nout;
-        if (blk >= 64 || nin < 64) {
+        if (o->ndvr && l == o->sF - 1) {
+            /* the outer dense layer: H3X with the inner points split over the threads */
+#pragma omp parallel for schedule(static)
+            for (int kc = 0; kc < nin; kc += 64) {
+                int ke = kc + 64 < nin ? kc + 64 : nin;
+                for (int i = 0; i < nout; i++) {
+                    double *xo = tmp + (long)nin * i;
+                    for (int k = kc; k < ke; k++) xo[k] = o->OUTH[(long)nin * i + k] * x[k];
+                    for (int j = 1; j < nout; j++) {
+                        const double *aij = o->OUTH + (long)nin * (i + (long)nout * j);
+                        const double *xj = x + (long)nin * j;
+                        for (int k = kc; k < ke; k++) xo[k] = xo[k] + aij[k] * xj[k];
+                    }
+                }
+            }
+        } else if (blk >= 64 || nin < 64) {
 #pragma omp parallel for schedule(static)
             for (int b = 0; b < blk; b++) {
                 if (o->sDEP[l]) orc_h2x(nin, nout, o->H0[l], o->DEP[l], x + bs * b, tmp + bs * b);

@@ -44,6 +44,10 @@ typedef struct {
     double DE, beta;              /* sOSBW%mDE / mBeta for the OSBD kernels */
     int osb_kind;                 /* ORC_TOSB / ORC_TOSBD1 / ORC_TOSBD2 */
     int hc, pc;                   /* sHC / sPC */
+    /* sNDVR = T: the outermost level is the dense per-point layer OUTH
+     * (myDim(sF) x sN(sF) x sN(sF)), src/iterate/module/osb.hx.h:93-94 */
+    int ndvr;
+    const double *OUTH;
 } orc_osb;
 
 /* ---- sizes: src/iterate/module/osb.init.h:2-50 ---- */
@@ -54,6 +58,8 @@ void orc_h1x(int nin, int nout, const double *h1, const double *x, double *xout)
 void orc_h1tx(int nin, int nout, const double *h1, const double *x, double *xout);
 void orc_h2x(int nin, int nout, const double *h1, const double *corx, const double *x, double *xout);
 void orc_h3x(int nin, int nout, const double *a, const double *x, double *xout);
+/* adjoint of H3X: XOUT(:,i) = sum_j A(:,j,i) X(:,j) */
+void orc_h3tx(int nin, int nout, const double *a, const double *x, double *xout);
 /* ---- src/iterate/libiterate/hpx/hpx.f90:5-73 ---- */
 void orc_h1x_xyz(int nin, int nout, int blk, const double *h1, const double *x, double *y);
 void orc_h1x_dep(int nin, int nout, int blk, const double *h1, const double *corx, const double *x, double *y);

@@ -38,6 +38,7 @@ SIGNATURES = {
     "scalit_set_h0": (_i, [_vp, _i, _dp]),
     "scalit_set_dep": (_i, [_vp, _i, _dp]),
     "scalit_set_res": (_i, [_vp, _dp]),
+    "scalit_set_outh": (_i, [_vp, _dp]),
     "scalit_set_shift": (_i, [_vp, _d, _d, _d]),
     "scalit_block_diag": (_i, [_vp, _i, _d]),
     "scalit_set_precond": (_i, [_vp, C.POINTER(_dp), _dp]),

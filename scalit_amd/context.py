@@ -47,13 +47,21 @@ class Context:
         self.dev = torch.device("cuda", device)
 
     @classmethod
-    def from_problem(cls, prob, device: int = 0, e0: float = 0.0):
-        ctx = cls(prob.sN, [d is not None for d in prob.dep], device)
+    def from_problem(cls, prob, device: int = 0, e0: float = 0.0, outh=None):
+        """The operator of a Problem; with `outh` (the flat OUTH array of
+        problems.outer_layer) the outermost level is the sNDVR = T dense layer."""
+        dep = list(prob.dep)
+        if outh is not None:
+            dep[-1] = None
+        ctx = cls(prob.sN, [d is not None for d in dep], device)
         for l, h in enumerate(prob.h0):
-            ctx.set_h0(l + 1, h)
-        for l, d in enumerate(prob.dep):
+            if outh is None or l < len(prob.h0) - 1:
+                ctx.set_h0(l + 1, h)
+        for l, d in enumerate(dep):
             if d is not None:
                 ctx.set_dep(l + 1, d)
+        if outh is not None:
+            ctx.set_outh(outh)
         ctx.set_res(prob.res)
         ctx.set_shift(e0)
         return ctx
@@ -81,6 +89,11 @@ class Context:
     def set_dep(self, level: int, dep: np.ndarray):
         a = np.ascontiguousarray(dep, dtype=np.float64)
         check(L.lib().scalit_set_dep(self._h, level, _np_dp(a)), "scalit_set_dep")
+
+    def set_outh(self, outh: np.ndarray):
+        a = np.ascontiguousarray(outh, dtype=np.float64).reshape(-1)
+        assert a.size == self.N * self.sN[-1]
+        check(L.lib().scalit_set_outh(self._h, _np_dp(a)), "scalit_set_outh")
 
     def set_res(self, res: np.ndarray):
         a = np.ascontiguousarray(res, dtype=np.float64)

@@ -56,7 +56,7 @@ int ensure_f32(scalit_ctx *c)
     bool changed = false;
     for (int l = 0; l < c->sF; l++) {
         Level &L = c->lv[l];
-        if (!L.h0_f.p) {
+        if (!L.h0_f.p && L.has_h0) {
             SC_TRY(to_f32(L.h0, L.h0_f, (long)L.n * L.n, c->stream));
             changed = true;
         }
@@ -67,6 +67,10 @@ int ensure_f32(scalit_ctx *c)
     }
     if (!c->res_f.p) {
         SC_TRY(to_f32(c->res, c->res_f, c->N, c->stream));
+        changed = true;
+    }
+    if (c->ndvr && !c->outh_f.p) {
+        SC_TRY(to_f32(c->outh, c->outh_f, c->N * (long)c->lv[c->sF - 1].n, c->stream));
         changed = true;
     }
     if (c->has_precond) SC_TRY(precond_to_f32(c));
@@ -95,6 +99,7 @@ static void drop_f32(scalit_ctx *c)
     }
     c->res_f.release();
     c->eig0_f.release();
+    c->outh_f.release();
     c->gen++;
 }
 
@@ -177,6 +182,8 @@ int scalit_destroy(scalit_ctx *c)
     c->res.release();
     c->eig0.release();
     c->res_f.release();
+    c->outh.release();
+    c->outh_f.release();
     c->eig0_f.release();
     c->work.release();
     c->work_f.release();
@@ -225,6 +232,21 @@ int scalit_set_dep(scalit_ctx *c, int level, const double *dep)
     return SCALIT_OK;
 }
 
+int scalit_set_outh(scalit_ctx *c, const double *outh)
+{
+    CTX_GUARD(c);
+    if (!outh || c->sF < 2) return SCALIT_EINVAL;
+    Level &L = c->lv[c->sF - 1];
+    const size_t n = (size_t)c->N * L.n;   // outLen = sN(sF) * myLen (osb.init.h:19)
+    SC_TRY(c->outh.alloc(n));
+    HIP_TRY(hipMemcpy(c->outh.p, outh, sizeof(double) * n, hipMemcpyHostToDevice));
+    c->ndvr = true;
+    L.dep = false;   // sNDVR forces sDEP(sF) = F (osb.print.h:204)
+    c->fast3 = fast3_ok(c);
+    drop_f32(c);
+    return SCALIT_OK;
+}
+
 int scalit_set_res(scalit_ctx *c, const double *res)
 {
     CTX_GUARD(c);
@@ -249,6 +271,7 @@ static int operator_ready(const scalit_ctx *c)
 {
     if (!c->has_res) return SCALIT_ESTATE;
     for (int l = 0; l < c->sF; l++) {
+        if (c->ndvr && l == c->sF - 1) continue;   // the OUTH layer needs no H0
         if (!c->lv[l].has_h0) return SCALIT_ESTATE;
         if (c->lv[l].dep && !c->lv[l].has_dep) return SCALIT_ESTATE;
     }

@@ -313,7 +313,9 @@ __global__ void k_eig_extract(int ndk, int mak, int nblk, const double *__restri
     }
 }
 
-static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, hipStream_t s)
+// hosb_src: the level's HOSB given as is (the sNDVR = T layer, OUTH,
+// osb.diag.h:22-23) instead of H0 [x DEP] + diag(EIG0).
+static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const double *hosb_src, hipStream_t s)
 {
     Level &L = c->lv[l];
     const int ndk = L.dim, mak = L.n, nblk = L.blk;
@@ -331,8 +333,13 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, hipStre
     SC_TRY(kcount.alloc(nblk));
     int rc = SCALIT_OK;
     {
-        hipLaunchKernelGGL(k_hosb_init, dim3(4096), dim3(256), 0, s, ndk, mak, nblk, L.h0.p, L.dep ? L.depv.p : nullptr,
-                           c->eig0.p, hosb.p);
+        if (hosb_src) {
+            if (hipMemcpyAsync(hosb.p, hosb_src, sizeof(double) * (size_t)ndk * mak * mak * nblk,
+                               hipMemcpyDeviceToDevice, s) != hipSuccess)
+                rc = SCALIT_EHIP;
+        } else
+            hipLaunchKernelGGL(k_hosb_init, dim3(4096), dim3(256), 0, s, ndk, mak, nblk, L.h0.p,
+                               L.dep ? L.depv.p : nullptr, c->eig0.p, hosb.p);
         hipLaunchKernelGGL(k_vinit, dim3(1024), dim3(256), 0, s, mak, nblk, L.vosb.p);
         hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, hosb.p, part_od.p);
         hipLaunchKernelGGL(k_jconv, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, nch, 1, part_od.p, ye.p, initye.p,
@@ -349,7 +356,7 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, hipStre
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         bool captured = false;
-        for (int sweep = 0; sweep < bjmax && npair > 0; sweep++) {
+        for (int sweep = 0; rc == SCALIT_OK && sweep < bjmax && npair > 0; sweep++) {
             if (hipMemcpyAsync(hact.data(), active.p, sizeof(int) * nblk, hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess) {
                 rc = SCALIT_EHIP;
@@ -413,7 +420,7 @@ int block_diag(scalit_ctx *c, int bjmax, double bjtol)
 {
     hipStream_t s = c->stream;
     for (int l = 0; l < c->sF; l++) {
-        if (!c->lv[l].has_h0) return SCALIT_ESTATE;
+        if (!c->lv[l].has_h0 && !(c->ndvr && l == c->sF - 1)) return SCALIT_ESTATE;
         if (c->lv[l].dep && !c->lv[l].has_dep) return SCALIT_ESTATE;
         SC_TRY(c->lv[l].vosb.alloc((size_t)c->lv[l].blk * c->lv[l].n * c->lv[l].n));
     }
@@ -432,7 +439,7 @@ int block_diag(scalit_ctx *c, int bjmax, double bjtol)
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(s));
         } else {
-            SC_TRY(diag_level_ndk(c, l, bjmax, bjtol, s));
+            SC_TRY(diag_level_ndk(c, l, bjmax, bjtol, (c->ndvr && l == c->sF - 1) ? c->outh.p : nullptr, s));
         }
     }
     c->has_precond = true;

@@ -86,6 +86,89 @@ static int grid_for(long n, int threads)
     return (int)g;
 }
 
+// sNDVR = T outermost level (H3X, src/libutil/serial/base/matrix.f90:188-203):
+//   Y(p, i) += sum_j A(p, i, j) X(p, j)    (trans: A(p, j, i)),
+// A = OUTH with the point index p fastest, a dense n x n block per inner point.
+// A workgroup takes 16 consecutive points (128-byte runs of A, X and Y) and VB
+// vectors; X of its points for every j sits in LDS; thread (pp, ig) owns
+// point p0 + pp and rows i = ig + 16 r.  A is streamed once per VB vectors.
+template <typename T, typename S, int VB>
+__global__ void __launch_bounds__(256) k_h3x(int nin, int n, int nvec, const T *__restrict__ A, int trans,
+                                             const S *__restrict__ x, long ldx, S *__restrict__ y, long ldy,
+                                             const int *active)
+{
+    extern __shared__ char smem_raw[];
+    T *sX = (T *)smem_raw;   // X(p0 + q, j, v0 + vv) at (j VB + vv) 16 + q
+    const int tid = threadIdx.x, pp = tid & 15, ig = tid >> 4;
+    const long p0 = (long)blockIdx.x * 16;
+    const int v0 = blockIdx.y * VB;
+    const long p = p0 + pp;
+    const bool pin = p < nin;
+    for (int e = tid; e < 16 * n * VB; e += 256) {
+        const int q = e & 15, rest = e >> 4, j = rest % n, vv = rest / n, v = v0 + vv;
+        const long pq = p0 + q;
+        sX[(j * VB + vv) * 16 + q] =
+            (pq < nin && v < nvec && vec_active(active, v)) ? ld_cvt<T, S>(x + (long)v * ldx, pq + (long)nin * j) : (T)0;
+    }
+    __syncthreads();
+    constexpr int RI = 8;   // n <= 128
+    const int nri = (n + 15) / 16;
+    const long si = trans ? (long)nin * n : (long)nin;   // A stride along i
+    const long sj = trans ? (long)nin : (long)nin * n;   // A stride along j
+    T acc[RI][VB];
+#pragma unroll
+    for (int r = 0; r < RI; r++)
+#pragma unroll
+        for (int vv = 0; vv < VB; vv++) acc[r][vv] = (T)0;
+    const T *Ap = A + p;
+#pragma unroll 2
+    for (int j = 0; j < n; j++) {
+        T xv[VB];
+#pragma unroll
+        for (int vv = 0; vv < VB; vv++) xv[vv] = sX[(j * VB + vv) * 16 + pp];
+#pragma unroll
+        for (int r = 0; r < RI; r++) {
+            const int i = ig + 16 * r;
+            if (r < nri && i < n && pin) {
+                const T a = Ap[si * i + sj * j];
+#pragma unroll
+                for (int vv = 0; vv < VB; vv++) acc[r][vv] += a * xv[vv];
+            }
+        }
+    }
+    if (!pin) return;
+#pragma unroll
+    for (int vv = 0; vv < VB; vv++) {
+        const int v = v0 + vv;
+        if (v >= nvec || !vec_active(active, v)) continue;
+        S *yv = y + (long)v * ldy + p;
+#pragma unroll
+        for (int r = 0; r < RI; r++) {
+            const int i = ig + 16 * r;
+            if (r < nri && i < n) st_cvt<T, S>(yv, (long)nin * i, ld_cvt<T, S>(yv, (long)nin * i) + acc[r][vv]);
+        }
+    }
+}
+
+template <typename T, typename S>
+static void launch_h3x(int nin, int n, int nvec, const T *A, bool trans, const S *x, long ldx, S *y, long ldy,
+                       const int *active, hipStream_t s)
+{
+    constexpr int VB = 4;
+    const size_t lds = (size_t)16 * n * VB * sizeof(T);
+    dim3 g((unsigned)((nin + 15) / 16), (unsigned)((nvec + VB - 1) / VB));
+    hipLaunchKernelGGL((k_h3x<T, S, VB>), g, dim3(256), lds, s, nin, n, nvec, A, trans ? 1 : 0, x, ldx, y, ldy,
+                       active);
+}
+
+// the OUTH layer in the compute type
+template <typename T>
+static const T *outh_of(const scalit_ctx *c)
+{
+    if constexpr (sizeof(T) == 8) return (const T *)c->outh.p;
+    else return (const T *)c->outh_f.p;
+}
+
 template <typename T, typename S>
 static int hx_generic(scalit_ctx *c, const T *res, const T *shifts, T shift0, int nvec, const S *x, long ldx, S *y,
                       long ldy, bool transpose, const int *active, hipStream_t s)
@@ -95,6 +178,10 @@ static int hx_generic(scalit_ctx *c, const T *res, const T *shifts, T shift0, in
     hipLaunchKernelGGL((k_diag_init<T, S>), g, dim3(th), 0, s, c->N, res, shifts, shift0, x, ldx, y, ldy, active);
     for (int l = 0; l < c->sF; l++) {
         const Level &L = c->lv[l];
+        if (c->ndvr && l == c->sF - 1) {
+            launch_h3x<T, S>(L.dim, L.n, nvec, outh_of<T>(c), transpose, x, ldx, y, ldy, active, s);
+            continue;
+        }
         const T *h, *dep = nullptr;
         if constexpr (sizeof(T) == 8) {
             h = (const T *)(transpose ? L.h0t.p : L.h0.p);
@@ -335,8 +422,9 @@ static size_t modes12_lds(int n1, int n2, bool t_lds)
 template <typename T>
 bool fast3_supported(const scalit_ctx *c)
 {
-    if (c->sF != 3 || c->lv[0].dep || c->lv[1].dep || !c->lv[2].dep) return false;
-    if (c->lv[2].n > 128) return false;   // pass B keeps a theta column in registers
+    if (c->sF != 3 || c->lv[0].dep || c->lv[1].dep || !(c->lv[2].dep || c->ndvr)) return false;
+    if (c->lv[2].n > 128) return false;   // pass B keeps a theta column in registers, H3X 8 row tiles
+    if (c->ndvr && (size_t)16 * c->lv[2].n * 4 * sizeof(T) > LDS_MAX) return false;
     return modes12_lds<T>(c->lv[0].n, c->lv[1].n, false) <= LDS_MAX;
 }
 
@@ -447,7 +535,7 @@ static TuneEntry *tune(scalit_ctx *c, int prec, const T *t1, const T *t2, const 
             }
         }
         best = 1e30f;
-        for (int vb = 0; vb < NVB && fb < 0; vb++) {
+        for (int vb = 0; vb < NVB && fb < 0 && l2; vb++) {
             for (int w = 0; w < 2; w++) launch_b<T, S>(vb, n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, nullptr, s);
             (void)hipEventRecord(e0, s);
             for (int w = 0; w < 3; w++) launch_b<T, S>(vb, n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, nullptr, s);
@@ -467,17 +555,20 @@ static TuneEntry *tune(scalit_ctx *c, int prec, const T *t1, const T *t2, const 
     return &c->hx_tune.back();
 }
 
+// outh != nullptr: level 3 is the sNDVR = T layer (H3X, transposed for the
+// adjoint when trans_outh) instead of DEP x L2.
 template <typename T, typename S>
-static int hx_fast3(scalit_ctx *c, int prec, const T *t1, const T *t2, const T *l2, const T *dep, const T *res,
-                    const T *shifts, T shift0, int nvec, const S *x, long ldx, S *y, long ldy, const int *active,
-                    hipStream_t s)
+static int hx_fast3(scalit_ctx *c, int prec, const T *t1, const T *t2, const T *l2, const T *dep, const T *outh,
+                    bool trans_outh, const T *res, const T *shifts, T shift0, int nvec, const S *x, long ldx, S *y,
+                    long ldy, const int *active, hipStream_t s)
 {
     const int n1 = c->lv[0].n, n2 = c->lv[1].n, n3 = c->lv[2].n;
     TuneEntry *te = tune<T, S>(c, prec, t1, t2, l2, dep, res, shifts, shift0, nvec, x, ldx, y, ldy, s);
     const int va = te ? te->va : 2, vb = te ? te->vb : 0;
     // modes 1+2 and the diagonal write Y, then mode 3 adds to it
     pass_a<T, S>(c, va, t1, t2, res, shifts, shift0, nvec, x, ldx, y, ldy, active, s);
-    launch_b<T, S>(vb, n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, active, s);
+    if (outh) launch_h3x<T, S>(n1 * n2, n3, nvec, outh, trans_outh, x, ldx, y, ldy, active, s);
+    else launch_b<T, S>(vb, n1, n2, n3, nvec, l2, dep, x, ldx, y, ldy, active, s);
     HIP_TRY(hipGetLastError());
     return SCALIT_OK;
 }
@@ -491,8 +582,9 @@ int hx_apply_f64(scalit_ctx *c, const double *shifts_dev, double shift0, int nve
     const Level *L = c->lv;
     const double *t1 = transpose ? L[0].h0t.p : L[0].h0.p;
     const double *t2 = transpose ? L[1].h0t.p : L[1].h0.p;
-    const double *l2 = transpose ? L[2].h0t.p : L[2].h0.p;
-    return hx_fast3<double, double>(c, SCALIT_FP64, t1, t2, l2, L[2].depv.p, c->res.p, shifts_dev, shift0, nvec, x,
+    const double *l2 = c->ndvr ? nullptr : (transpose ? L[2].h0t.p : L[2].h0.p);
+    return hx_fast3<double, double>(c, SCALIT_FP64, t1, t2, l2, c->ndvr ? nullptr : L[2].depv.p,
+                                    c->ndvr ? c->outh.p : nullptr, transpose, c->res.p, shifts_dev, shift0, nvec, x,
                                     ldx, y, ldy, active, s);
 }
 
@@ -502,8 +594,9 @@ int hx_apply_f32(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec,
     if (!c->fast3 || g_force_generic || !fast3_supported<float>(c))
         return hx_generic<float, float>(c, c->res_f.p, shifts_dev, shift0, nvec, x, ldx, y, ldy, false, active, s);
     const Level *L = c->lv;
-    return hx_fast3<float, float>(c, SCALIT_FP32, L[0].h0_f.p, L[1].h0_f.p, L[2].h0_f.p, L[2].depv_f.p, c->res_f.p,
-                                  shifts_dev, shift0, nvec, x, ldx, y, ldy, active, s);
+    return hx_fast3<float, float>(c, SCALIT_FP32, L[0].h0_f.p, L[1].h0_f.p, c->ndvr ? nullptr : L[2].h0_f.p,
+                                  c->ndvr ? nullptr : L[2].depv_f.p, c->ndvr ? c->outh_f.p : nullptr, false,
+                                  c->res_f.p, shifts_dev, shift0, nvec, x, ldx, y, ldy, active, s);
 }
 
 int hx_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const bf16_t *x, long ldx,
@@ -512,8 +605,9 @@ int hx_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec
     if (!c->fast3 || g_force_generic || !fast3_supported<float>(c))
         return hx_generic<float, bf16_t>(c, c->res_f.p, shifts_dev, shift0, nvec, x, ldx, y, ldy, false, active, s);
     const Level *L = c->lv;
-    return hx_fast3<float, bf16_t>(c, SCALIT_BF16, L[0].h0_f.p, L[1].h0_f.p, L[2].h0_f.p, L[2].depv_f.p, c->res_f.p,
-                                   shifts_dev, shift0, nvec, x, ldx, y, ldy, active, s);
+    return hx_fast3<float, bf16_t>(c, SCALIT_BF16, L[0].h0_f.p, L[1].h0_f.p, c->ndvr ? nullptr : L[2].h0_f.p,
+                                   c->ndvr ? nullptr : L[2].depv_f.p, c->ndvr ? c->outh_f.p : nullptr, false,
+                                   c->res_f.p, shifts_dev, shift0, nvec, x, ldx, y, ldy, active, s);
 }
 
 bool fast3_ok(const scalit_ctx *c) { return fast3_supported<double>(c); }
@@ -631,7 +725,7 @@ int mfma_probe(int prec, double *out_host)
 // 1+2+diagonal, pass 2: mode 3), for per-kernel timing.
 int hx_pass_f64(scalit_ctx *c, int pass, int nvec, const double *x, long ldx, double *y, long ldy, hipStream_t s)
 {
-    if (!c->fast3 || !fast3_supported<double>(c)) return SCALIT_EINVAL;
+    if (!c->fast3 || c->ndvr || !fast3_supported<double>(c)) return SCALIT_EINVAL;
     const Level *L = c->lv;
     const int n1 = L[0].n, n2 = L[1].n, n3 = L[2].n;
     TuneEntry *te = tune<double, double>(c, SCALIT_FP64, L[0].h0.p, L[1].h0.p, L[2].h0.p, L[2].depv.p, c->res.p,

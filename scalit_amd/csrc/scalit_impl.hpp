@@ -94,7 +94,12 @@ struct scalit_ctx {
     scalit::DBuf<float> work_f;
     scalit::DBuf<double> shifts_d;  // per-vector shift staging
     scalit::DBuf<float> shifts_f;
-    // fused 3-level fast path is available (sF==3, sDEP=(F,F,T), sizes fit)
+    // sNDVR = T: the outermost level is the dense per-point layer OUTH
+    // (myDim(sF) x n x n, point index fastest) instead of H0 [x DEP]
+    bool ndvr = false;
+    scalit::DBuf<double> outh;
+    scalit::DBuf<float> outh_f;
+    // fused 3-level fast path is available (sF==3, sDEP=(F,F,T) or (F,F) + OUTH, sizes fit)
     bool fast3 = false;
     scalit::PistState *pist = nullptr;
     // bumped whenever a device buffer that operator kernels read is

@@ -261,6 +261,27 @@ def config_problem(name: str, seed: int = 1234, grid=None) -> Problem:
     return build_problem(n1, n2, n3, pot, name=name, seed=seed)
 
 
+def outer_layer(prob: Problem, seed: int, eps: float = 2e-4, potential: bool = False) -> np.ndarray:
+    """A dense per-point outermost layer (the sNDVR = T form ScalIT's ja3 writes,
+    OUTH(NIN, n3, n3) of src/iterate/module/osb.data.h:62): for every inner grid
+    point p = (r, R), A_p = DEP(p) L^2 + eps S_p with S_p a seeded symmetric
+    N(0,1) matrix, so every block stays symmetric.  With `potential` the grid
+    potential V(p, theta) is added on each block's diagonal (the operator is
+    then used with RES = 0: the OSB block of this level is OUTH itself,
+    src/iterate/module/osb.diag.h:22-23, so the potential must live in OUTH for
+    the preconditioner to see it).  Returned flat, column-major (p fastest)."""
+    n3 = prob.sN[2]
+    nin = prob.sN[0] * prob.sN[1]
+    g = rng(seed)
+    s = g.standard_normal((nin, n3, n3))
+    s = 0.5 * (s + s.transpose(0, 2, 1))
+    a = prob.dep[2][:, None, None] * prob.h0[2][None, :, :] + eps * s
+    if potential:
+        v = prob.res.reshape(n3, nin).T          # V(p, theta)
+        a[:, np.arange(n3), np.arange(n3)] += v
+    return a.reshape(-1, order="F")   # a[p, i, j] at p + NIN (i + n3 j)
+
+
 def random_vectors(n: int, b: int, seed: int) -> np.ndarray:
     """B standard-normal vectors of length n, shape (b, n)."""
     return rng(seed).standard_normal((b, n))

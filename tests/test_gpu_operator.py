@@ -2,6 +2,8 @@
 
 Tolerances: FP64 H*X and P*X within 1e-12 relative (max-norm, relative to the
 oracle's max-norm); FP32 within 2e-6; BF16-stored within 2e-2."""
+import dataclasses
+
 import numpy as np
 import pytest
 import torch
@@ -156,4 +158,77 @@ def test_block_diag_matches_oracle(grid):
         v = vosb_g[l].reshape(-1, n, n)
         for b in range(0, v.shape[0], max(1, v.shape[0] // 7)):
             assert np.allclose(v[b] @ v[b].T, np.eye(n), atol=1e-12)
+    ctx.close()
+
+
+# ---------------------------------------------------------------- sNDVR = T
+def setup_ndvr(grid, seed=5, e0=0.0):
+    prob = P.build_problem(*grid, P.gaussian_potential(seed), seed=seed)
+    outh = P.outer_layer(prob, seed + 100)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=e0, outh=outh)
+    ctx = Context.from_problem(prob, e0=e0, outh=outh)
+    return prob, op, ctx
+
+
+@pytest.mark.parametrize("grid", [(24, 32, 24), (13, 11, 9), (64, 96, 80)])
+def test_ndvr_hx(grid):
+    """The outermost level as the dense per-point layer OUTH (H3X,
+    matrix.f90:188-203 via osb.hx.h:93-94): fused and generic paths, shifts,
+    the adjoint, and FP32."""
+    print(f"ndvr hx grid={grid}", flush=True)
+    prob, op, ctx = setup_ndvr(grid, e0=-0.011)
+    nv = 5 if prob.N < 100_000 else 2
+    x = P.random_vectors(prob.N, nv, 8)
+    xd = ctx.to_device(x)
+    y = ctx.hx(xd, L.TA0).cpu().numpy()
+    y1 = ctx.hx(xd, L.TA1).cpu().numpy()
+    for b in range(nv):
+        assert rel(y[b], op.hx(x[b], O.TA0)) <= 1e-12
+    assert rel(y1[nv - 1], op.hx(x[nv - 1], O.TA1)) <= 1e-12
+    L.lib().scalit_debug_force_generic(1)
+    try:
+        yg = ctx.hx(xd, L.TA0).cpu().numpy()
+    finally:
+        L.lib().scalit_debug_force_generic(0)
+    assert rel(yg[1], op.hx(x[1], O.TA0)) <= 1e-12
+    g = P.random_vectors(prob.N, nv, 9)
+    dx, dv = ctx.hx_adjoint(xd, ctx.to_device(g))
+    dx = dx.cpu().numpy()
+    assert rel(dx[0], op.hx_t(g[0], O.TA0)) <= 1e-12
+    assert rel(dv.cpu().numpy(), (x * g).sum(axis=0)) <= 1e-12
+    yf = ctx.hx_lp(ctx.to_device(x, torch.float32)).double().cpu().numpy()
+    assert rel(yf[0], op.hx(x[0], O.TA0)) <= 2e-6
+    ctx.close()
+
+
+def test_ndvr_block_diag_px_qmr():
+    """OSB set-up with HOSB(sF) = OUTH (osb.diag.h:22-23), then P*X and a QMR
+    solve on the sNDVR operator against the oracle.  The potential sits on the
+    diagonal of the OUTH blocks and RES = 0, the form in which the outer
+    preconditioner block carries it."""
+    prob = P.build_problem(12, 10, 8, P.gaussian_potential(5), seed=5)
+    outh = P.outer_layer(prob, 105, potential=True)
+    zero = np.zeros(prob.N)
+    op = O.Operator(prob.sN, prob.h0, zero, prob.dep, outh=outh)
+    ctx = Context.from_problem(dataclasses.replace(prob, res=zero), outh=outh)
+    vosb_o, eig0_o = op.diag(bjmax=1000, bjtol=1e-5)
+    ctx.block_diag(1000, 1e-5)
+    vosb_g, eig0_g = ctx.get_precond()
+    assert rel(eig0_g, eig0_o) <= 1e-9
+    for l in range(3):
+        assert np.abs(vosb_g[l] - vosb_o[l]).max() <= 1e-7
+    e0 = float(np.sort(eig0_o)[3]) - 1e-3
+    op.set_e0(e0)
+    ctx.set_shift(e0)
+    ctx.set_precond(vosb_o, eig0_o)   # the oracle's preconditioner: P*X parity at 1e-12
+    x = P.random_vectors(prob.N, 2, 10)
+    yp = ctx.px(ctx.to_device(x)).cpu().numpy()
+    for b in range(2):
+        assert rel(yp[b], op.px(x[b])) <= 1e-12
+    b = P.random_vectors(prob.N, 1, 11)
+    ret, xs, res = ctx.qmr(ctx.to_device(b), ntype=-1, qmrmax=300, qmrtol=1e-9)
+    assert ret[0] > 0
+    xs = xs.cpu().numpy()[0]
+    r = op.hx(xs, O.TA1) - b[0]
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(op.hx(xs, O.TA1))
     ctx.close()

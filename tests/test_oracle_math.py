@@ -178,3 +178,28 @@ def test_mgs_orthogonality_criterion():
         ok, _ = O.mgs_orth(g.standard_normal(n), vj, j)
         assert ok
     assert np.abs(vj.T @ vj - np.eye(m)).max() < 1e-14
+
+
+def test_ndvr_hx_brute_force_and_adjoint():
+    """sNDVR = T operator: RES + T_r (x) 1 + 1 (x) T_R + blockdiag_p(A_p) on the
+    outer index, built densely; the adjoint uses A_p^T."""
+    prob = P.build_problem(4, 3, 5, P.gaussian_potential(4))
+    n1, n2, n3 = prob.sN
+    nin = n1 * n2
+    outh = P.outer_layer(prob, 9, eps=0.05)
+    A = outh.reshape((nin, n3, n3), order="F").copy()
+    A[:, 0, 1] += 0.3        # make one block non-symmetric so the adjoint is tested
+    outh = A.reshape(-1, order="F")
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, outh=outh)
+    N = op.N
+    H = np.diag(prob.res.copy())
+    I1, I2 = np.eye(n1), np.eye(n2)
+    H += np.kron(np.eye(n3), np.kron(I2, prob.h0[0]))
+    H += np.kron(np.eye(n3), np.kron(prob.h0[1], I1))
+    for p in range(nin):
+        for i in range(n3):
+            for j in range(n3):
+                H[p + nin * i, p + nin * j] += A[p, i, j]
+    x = np.random.default_rng(2).standard_normal(N)
+    assert np.allclose(op.hx(x), H @ x, rtol=0, atol=1e-13 * np.abs(H).max() * N)
+    assert np.allclose(op.hx_t(x), H.T @ x, rtol=0, atol=1e-13 * np.abs(H).max() * N)

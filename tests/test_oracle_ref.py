@@ -191,3 +191,39 @@ def test_dvr_builders_match_reference():
         yr = np.zeros((n, n), order="F")
         R.yjpolys_(O._ci(n), O._ci(n - 1), O.dptr(x), O.dptr(yr))
         assert np.array_equal(y, yr)
+
+
+def _ndvr_ops(grid, e0=0.0, seed=21):
+    prob = P.config_problem("pist_fp64", grid=grid)
+    outh = P.outer_layer(prob, seed)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=e0, outh=outh)
+    return prob, op, O.RefOperator(op)
+
+
+def test_h3x_direct():
+    R = O.ref()
+    g = np.random.default_rng(1)
+    nin, nout = 9, 6
+    a = g.standard_normal(nin * nout * nout)
+    x = g.standard_normal(nin * nout)
+    y_ref = np.zeros_like(x)
+    y = np.zeros_like(x)
+    R.h3x_(O._ci(nin), O._ci(nout), O.dptr(a), O.dptr(x), O.dptr(y_ref))
+    O.lib().orc_h3x(nin, nout, O.dptr(a), O.dptr(x), O.dptr(y))
+    assert np.array_equal(y, y_ref)
+
+
+@pytest.mark.parametrize("grid", [(6, 7, 5), (12, 10, 8)])
+def test_ndvr_hx_diag_px_bit_exact(grid):
+    """sNDVR = T: the outermost level is H3X with OUTH (osb.hx.h:93-94) and its
+    OSB block is OUTH itself (osb.diag.h:22-23)."""
+    _, op, rop = _ndvr_ops(grid, e0=-0.01)
+    x = P.random_vectors(op.N, 1, 12)[0]
+    assert np.array_equal(op.hx(x, O.TA0), rop.hx(x, shifted=False))
+    assert np.array_equal(op.hx(x, O.TA1), rop.hx(x, shifted=True))
+    vo, eo = op.diag(bjmax=1000, bjtol=1e-5)
+    vr, er = rop.diag(bjmax=1000, bjtol=1e-5)
+    for a, b in zip(vo, vr):
+        assert np.array_equal(a, b)
+    assert np.array_equal(eo, er)
+    assert np.array_equal(op.px(x), rop.px(x))
