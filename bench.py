@@ -12,7 +12,8 @@ Prints one JSON line: the H*X throughput in GB/s of compulsory bytes
 dominant kernel (its launches timed alone with HIP events on the launch
 stream), the CPU baseline (the oracle's OpenMP restatement of the
 reference's H*X loops on this host), and the PIST eigenpairs/s of the
-24 x 32 x 24 FP64 workload on the GPU next to the oracle's on the CPU.
+24 x 32 x 24 FP64 workload on the GPU (single window in FP64 and with FP32
+inner solves, and as 10 batched shift windows) next to the oracle's on the CPU.
 """
 from __future__ import annotations
 
@@ -141,7 +142,7 @@ def main():
         traffic = tj.get("pass1_bytes" if dom == 1 else "pass2_bytes")
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_hx_modes12 (modes 1+2 + diagonal)" if dom == 1 else "k_hx_mode3 (mode 3 x DEP)",
+                "kernel": "k_hx_modes12 (modes 1+2 + diagonal)" if dom == 1 else "k_outer_mode (mode 3 x DEP, adds to Y)",
                 "kernel_ms": round(t_dom, 5), "flops_per_launch": fl}
     hx_tflops = cnt["flops_batch"] / (ms * 1e-3) / 1e12
 
