@@ -235,7 +235,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     constexpr int NT = 64 * WAVES;
-    constexpr int PFMAX = 16;   // slab values per thread prefetched in registers; the rest load at slab start
+    // slab values per thread prefetched in registers (the rest load at slab
+    // start): 16 x 512 or 8 x 768 threads cover a 64 x 96 slab
+    constexpr int PFMAX = WAVES >= 12 ? 8 : 16;
     extern __shared__ char smem_raw[];
     T *sm = (T *)smem_raw;
     const int n1p = rup16(n1), n2p = rup16(n2);
@@ -432,7 +434,7 @@ bool fast3_supported(const scalit_ctx *c)
 struct VarA {
     int waves, rb, cb;
 };
-static const VarA kVarA[] = {{4, 1, 1}, {4, 3, 2}, {8, 3, 1}, {8, 2, 2}, {8, 1, 2}, {4, 2, 2}, {8, 4, 2}, {4, 4, 2}};
+static const VarA kVarA[] = {{4, 1, 1}, {4, 3, 2}, {8, 3, 1}, {8, 2, 2}, {8, 1, 2}, {4, 2, 2}, {12, 2, 1}, {12, 1, 2}};
 constexpr int NVA = sizeof(kVarA) / sizeof(kVarA[0]);
 constexpr int NVB = 2;   // pass B: 4 or 8 waves per workgroup
 
@@ -468,8 +470,8 @@ static void launch_a(int va, dim3 g, size_t lds, hipStream_t s, int n1, int n2, 
     case 3: LA(8, 2, 2); break;
     case 4: LA(8, 1, 2); break;
     case 5: LA(4, 2, 2); break;
-    case 6: LA(8, 4, 2); break;
-    default: LA(4, 4, 2); break;
+    case 6: LA(12, 2, 1); break;
+    default: LA(12, 1, 2); break;
     }
 #undef LA
 }
