@@ -55,6 +55,52 @@ __device__ __forceinline__ void st_cvt<float, bf16_t>(bf16_t *p, long i, float v
 
 __device__ __forceinline__ bool vec_active(const int *active, int v) { return active == nullptr || active[v] != 0; }
 
+// Raw buffer access: a wave-uniform resource (base, size in bytes) plus a
+// 32-bit per-lane byte offset and a wave-uniform byte offset, so a column of
+// loads at a uniform stride needs one VGPR of address in all.  Callers keep
+// every access inside the resource.  Dword 3 = 0x00020000: 32-bit data
+// format, the value for untyped loads on gfx9-family parts.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
+                                             (int)(bytes > 0x7fffffffL ? 0x7fffffffL : bytes), 0x00020000);
+}
+template <typename T, typename S>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ double buf_ld<double, double>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float buf_ld<float, float>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float buf_ld<float, bf16_t>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return bf16_to_f32((bf16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+}
+template <typename T, typename S>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, int voff, int soff, T v);
+template <>
+__device__ __forceinline__ void buf_st<double, double>(__amdgpu_buffer_rsrc_t r, int voff, int soff, double v)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(__typeof__(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ void buf_st<float, float>(__amdgpu_buffer_rsrc_t r, int voff, int soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ void buf_st<float, bf16_t>(__amdgpu_buffer_rsrc_t r, int voff, int soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)f32_to_bf16(v), r, voff, soff, 0);
+}
+
 __host__ __device__ static inline int rup16(int n) { return (n + 15) & ~15; }
 
 // Copy count contiguous values g[0..count) into LDS, value e to slot(e), with

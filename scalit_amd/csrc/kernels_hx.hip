@@ -436,7 +436,7 @@ struct VarA {
 };
 static const VarA kVarA[] = {{4, 1, 1}, {4, 3, 2}, {8, 3, 1}, {8, 2, 2}, {8, 1, 2}, {4, 2, 2}, {12, 2, 1}, {12, 1, 2}};
 constexpr int NVA = sizeof(kVarA) / sizeof(kVarA[0]);
-constexpr int NVB = 2;   // pass B: 4 or 8 waves per workgroup
+constexpr int NVB = 3;   // pass B: 4, 8 or 12 waves per workgroup
 
 template <typename T, typename S, bool TL, int W, int RB, int CB>
 static void launch_a1(dim3 g, size_t lds, hipStream_t s, int n1, int n2, int n3, int nvec, const T *t1, const T *t2,
@@ -482,7 +482,8 @@ static void launch_b(int vb, int n1, int n2, int n3, int nvec, const T *l2, cons
 {
     const long P = (long)n1 * n2;
     if (vb == 0) launch_outer_mode<T, S, S, 4, true>(P, n3, nvec, l2, dep, x, ldx, y, ldy, act, s);
-    else launch_outer_mode<T, S, S, 8, true>(P, n3, nvec, l2, dep, x, ldx, y, ldy, act, s);
+    else if (vb == 1) launch_outer_mode<T, S, S, 8, true>(P, n3, nvec, l2, dep, x, ldx, y, ldy, act, s);
+    else launch_outer_mode<T, S, S, 12, true>(P, n3, nvec, l2, dep, x, ldx, y, ldy, act, s);
 }
 
 template <typename T, typename S>

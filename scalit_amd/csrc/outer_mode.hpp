@@ -19,7 +19,7 @@
 
 namespace scalit {
 
-template <typename T, typename S, typename SO, int WAVES, int NR, bool ACC>
+template <typename T, typename S, typename SO, int WAVES, int NR, bool ACC, bool EX>
 __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nvec, const T *__restrict__ amat,
                                                           const T *__restrict__ dvec, const S *__restrict__ x,
                                                           long ldx, SO *__restrict__ y, long ldy, const int *active)
@@ -39,9 +39,13 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
     const long wstride = (long)gridDim.x * WAVES;
     long t = (long)blockIdx.x * WAVES + wave;
 
-    // X and d of the current and the next tile; Y (when adding) of the current
-    // tile, loaded at the top of its iteration and used after its MFMAs
-    T bc[KS], dc, bn[KS], dn;
+    // X and d of the current and (PF) the next tile; Y (when adding) of the
+    // current tile, loaded at the top of its iteration and used after its
+    // MFMAs.  Without PF (12-wave workgroups, whose registers do not hold two
+    // X columns) X also loads at the top of its own tile, and the other waves
+    // of the SIMD cover the wait.
+    constexpr bool PF = WAVES < 12;
+    T bc[KS], dc, bn[PF ? KS : 1], dn;
     T yc[ACC ? NR : 1][4];
     // tile tt: vector v and first point p0 (wave-uniform); lane point p0 + lr
     auto tile_of = [&](long tt, int &v, long &p0) {
@@ -49,19 +53,31 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
         p0 = (tt % tpv) * 16;
         return p0 + lr < P && vec_active(active, v);
     };
+    // EX (n == 16 NR): raw buffer loads, one resource per tile, one lane
+    // offset for all of a column (lanes past the last point read point P-1;
+    // their results are not stored).  Otherwise predicated loads.
+    const int ylane = M::row(lk, 0);
     auto load_x = [&](long tt, T(&b)[KS], T &d) {
         int v;
         long p0;
         const bool pin = tile_of(tt, v, p0);
-        const S *xs = x + (long)v * ldx + p0;   // scalar base, 32-bit lane offsets
+        const S *xs = x + (long)v * ldx + p0;
         d = pin ? (dvec ? dvec[p0 + lr] : (T)1) : (T)0;
+        if constexpr (EX) {
+            const long pl = p0 + lr < P ? lr : P - 1 - p0;
+            const auto rs = buf_rsrc(xs, (P * n - p0) * (long)sizeof(S));
+            const int voff = (int)((pl + P * lk) * (long)sizeof(S));
 #pragma unroll
-        for (int s = 0; s < KS; s++) {
-            const int k = 4 * s + lk;
-            b[s] = (pin && k < n) ? ld_cvt<T, S>(xs, (int)(lr + P * k)) : (T)0;
+            for (int s = 0; s < KS; s++) b[s] = buf_ld<T, S>(rs, voff, (int)(P * 4 * s * (long)sizeof(S)));
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                const int k = 4 * s + lk;
+                b[s] = (pin && k < n) ? ld_cvt<T, S>(xs, lr + P * k) : (T)0;
+            }
         }
     };
-    if (t < ntile) load_x(t, bc, dc);
+    if (PF && t < ntile) load_x(t, bc, dc);
 
     if (np != n)
         for (int e = tid; e < np * np; e += 64 * WAVES)
@@ -75,16 +91,27 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
         long p0;
         const bool pin = tile_of(t, v, p0);
         SO *ys = y + (long)v * ldy + p0;
+        const auto ry = buf_rsrc(ys, (P * n - p0) * (long)sizeof(SO));
+        const long plane = p0 + lr < P ? lr : P - 1 - p0;
+        const int yvoff = (int)((plane + P * ylane) * (long)sizeof(SO));
         if constexpr (ACC) {
 #pragma unroll
             for (int r = 0; r < NR; r++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const int i = 16 * r + M::row(lk, q);
-                    yc[r][q] = (pin && i < n) ? ld_cvt<T, SO>(ys, (int)(lr + P * i)) : (T)0;
+                    if constexpr (EX)
+                        yc[r][q] = buf_ld<T, SO>(ry, yvoff,
+                                                 (int)(P * (16 * r + M::row(0, q)) * (long)sizeof(SO)));
+                    else
+                        yc[r][q] = (pin && i < n) ? ld_cvt<T, SO>(ys, lr + P * i) : (T)0;
                 }
         }
-        if (tn < ntile) load_x(tn, bn, dn);
+        if constexpr (PF) {
+            if (tn < ntile) load_x(tn, bn, dn);
+        } else {
+            load_x(t, bc, dc);
+        }
         acc_t acc[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) acc[r] = acc_t{0, 0, 0, 0};
@@ -107,16 +134,19 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const int i = 16 * r + M::row(lk, q);
-                    if (i < n) {
-                        T out = dc * acc[r][q];
-                        if constexpr (ACC) out += yc[r][q];
-                        st_cvt<T, SO>(ys, (int)(lr + P * i), out);
-                    }
+                    T out = dc * acc[r][q];
+                    if constexpr (ACC) out += yc[r][q];
+                    if constexpr (EX)
+                        buf_st<T, SO>(ry, yvoff, (int)(P * (16 * r + M::row(0, q)) * (long)sizeof(SO)), out);
+                    else if (i < n)
+                        st_cvt<T, SO>(ys, lr + P * i, out);
                 }
         }
+        if constexpr (PF) {
 #pragma unroll
-        for (int s = 0; s < KS; s++) bc[s] = bn[s];
-        dc = dn;
+            for (int s = 0; s < KS; s++) bc[s] = bn[s];
+            dc = dn;
+        }
         t = tn;
     }
 }
@@ -127,22 +157,27 @@ template <typename T, typename S, typename SO, int WAVES, int NR, bool ACC>
 static void launch_outer_mode_nr(long P, int n, int nvec, const T *amat, const T *dvec, const S *x, long ldx, SO *y,
                                  long ldy, const int *active, hipStream_t s)
 {
-    static int per_cu = 0;
+    // exact tiles and 32-bit byte offsets within one vector: the buffer path
+    const bool ex = n == 16 * NR && P * n * (long)std::max(sizeof(S), sizeof(SO)) < 0x7fffffffL;
+    static int per_cu[2] = {0, 0};
     const size_t lds = (size_t)(16 * NR) * (16 * NR) * sizeof(T);
-    if (per_cu == 0) {
-        (void)hipFuncSetAttribute((const void *)k_outer_mode<T, S, SO, WAVES, NR, ACC>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    const void *fn = ex ? (const void *)k_outer_mode<T, S, SO, WAVES, NR, ACC, true>
+                        : (const void *)k_outer_mode<T, S, SO, WAVES, NR, ACC, false>;
+    if (per_cu[ex] == 0) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_outer_mode<T, S, SO, WAVES, NR, ACC>, 64 * WAVES,
-                                                         lds) != hipSuccess || nb < 1)
-            nb = 1;
-        per_cu = nb;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64 * WAVES, lds) != hipSuccess || nb < 1) nb = 1;
+        per_cu[ex] = nb;
     }
     const long ntile = ((P + 15) / 16) * nvec;
-    long G = std::min<long>((ntile + WAVES - 1) / WAVES, (long)g_num_cu * per_cu);
+    long G = std::min<long>((ntile + WAVES - 1) / WAVES, (long)g_num_cu * per_cu[ex]);
     if (G < 1) G = 1;
-    hipLaunchKernelGGL((k_outer_mode<T, S, SO, WAVES, NR, ACC>), dim3((unsigned)G), dim3(64 * WAVES), lds, s, P, n, nvec,
-                       amat, dvec, x, ldx, y, ldy, active);
+    if (ex)
+        hipLaunchKernelGGL((k_outer_mode<T, S, SO, WAVES, NR, ACC, true>), dim3((unsigned)G), dim3(64 * WAVES), lds, s,
+                           P, n, nvec, amat, dvec, x, ldx, y, ldy, active);
+    else
+        hipLaunchKernelGGL((k_outer_mode<T, S, SO, WAVES, NR, ACC, false>), dim3((unsigned)G), dim3(64 * WAVES), lds,
+                           s, P, n, nvec, amat, dvec, x, ldx, y, ldy, active);
 }
 
 // n <= 128: the tile count along n rounded up to a compiled one
