@@ -56,6 +56,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pist", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--solver-run", action="store_true",
+                    help="time the solver path on the 64x96x80 operator (OSB set-up, P*X at batch 1 and 16, "
+                         "QMR) and print one line per measurement; prints no JSON")
     ap.add_argument("--pmc-run", action="store_true",
                     help="workload for rocprofv3 --pmc: three 512 MiB device copies (the counter calibration), "
                          "then --steps H*X calls; prints no JSON")
@@ -91,6 +94,9 @@ def main():
         torch.cuda.synchronize()
         ctx.close()
         print(f"pmc run done: 3 copies of 512 MiB, {args.steps} H*X of batch {B}", flush=True)
+        return
+    if args.solver_run:
+        solver_run(args.steps)
         return
     prob = P.config_problem("hx_bench")
     ctx = Context.from_problem(prob)
@@ -173,6 +179,51 @@ def main():
         out["pist"] = pist_bench(args.no_cpu)
     ctx.close()
     print(json.dumps(out))
+
+
+def solver_run(reps):
+    """The solver path on the 64x96x80 operator, timed with HIP events on the
+    current stream (one line per measurement, for profiling runs)."""
+    import torch
+
+    from scalit_amd import problems as P
+    from scalit_amd.context import Context
+
+    prob = P.config_problem("hx_bench")
+    e0 = float(np.sort(prob.res)[50])
+    ctx = Context.from_problem(prob, e0=e0)
+    t0 = time.perf_counter()
+    ctx.block_diag(1000, 1e-5)
+    torch.cuda.synchronize()
+    print(f"osb set-up: {time.perf_counter() - t0:.3f} s", flush=True)
+
+    def timed(f, n):
+        f()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(n):
+            f()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / n
+
+    for nv in (1, 16):
+        x = ctx.to_device(P.random_vectors(prob.N, nv, 3))
+        y = torch.empty_like(x)
+        print(f"batch {nv}: H*X {timed(lambda: ctx.hx(x, out=y), reps):.4f} ms, "
+              f"P*X {timed(lambda: ctx.px(x, out=y), reps):.4f} ms", flush=True)
+        b = ctx.to_device(P.random_vectors(prob.N, nv, 5))
+        for rep in range(2):   # the first solve records the QMR graph
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ret, xs, res = ctx.qmr(b, ntype=-1, qmrmax=1000, qmrtol=1e-8)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        it = int(np.abs(ret).max())
+        print(f"batch {nv}: QMR to 1e-8 in {it} iterations, {dt * 1e3:.2f} ms ({dt * 1e3 / max(it, 1):.3f} ms per "
+              f"iteration), residuals <= {float(np.max(res)):.1e}", flush=True)
+    ctx.close()
 
 
 def cpu_baseline(prob, B, seconds):

@@ -44,28 +44,29 @@ __device__ __forceinline__ double bsum(double v, double *red)
     return r;   // valid on thread 0
 }
 
-__device__ __forceinline__ double sum_parts(const double *p)
-{
-    double s = 0;
-    for (int i = 0; i < NPART; i++) s += p[i];
-    return s;
-}
-
 // partials per vector written by a grid sized for vectors of length n (the
-// host's nparts); the slots past them hold zeros, so this sum is bit-identical
-// to sum_parts
+// host's nparts); the slots past them hold zeros
 __device__ __forceinline__ int nparts_dev(long n)
 {
     const long g = (n + 1023) / 1024;
     return (int)(g < 1 ? 1 : (g > NPART ? NPART : g));
 }
-__device__ __forceinline__ double sum_parts_n(const double *p, long n)
+
+// The sum of one vector's partials, by a whole wave (all 64 lanes call it
+// together): lane l adds partials l, l+64, ... in order, then an xor butterfly
+// combines the lanes.  IEEE addition is commutative, so every lane -- and
+// every wave and workgroup that sums the same partials -- gets the identical
+// value; summing the zero tail as well changes nothing.
+__device__ __forceinline__ double wave_sum_parts(const double *p, int g)
 {
-    const int g = nparts_dev(n);
+    const int lane = threadIdx.x & 63;
     double s = 0;
-    for (int i = 0; i < g; i++) s += p[i];
+    for (int i = lane; i < g; i += 64) s += p[i];
+    for (int o = 1; o < 64; o <<= 1) s += __shfl_xor(s, o);
     return s;
 }
+__device__ __forceinline__ double sum_parts_n(const double *p, long n) { return wave_sum_parts(p, nparts_dev(n)); }
+__device__ __forceinline__ double sum_parts(const double *p) { return wave_sum_parts(p, NPART); }
 
 #define GSTRIDE(i, n) for (long i = blockIdx.x * 256L + threadIdx.x; i < (n); i += (long)gridDim.x * 256L)
 
@@ -97,10 +98,12 @@ __global__ void __launch_bounds__(256) k_dot_part(long n, const double *x, long 
     zero_tail(part + (long)v * NPART);
 }
 
+// one wave per vector
 __global__ void k_reduce_parts(int nvec, const double *part, double *out)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < nvec) out[v] = sum_parts(part + (long)v * NPART);
+    const int v = blockIdx.x;
+    const double r = sum_parts(part + (long)v * NPART);
+    if (threadIdx.x == 0) out[v] = r;
 }
 
 void launch_dot_partials(int nvec, long n, const double *x, long ldx, const double *y, long ldy, double *part,
@@ -111,7 +114,7 @@ void launch_dot_partials(int nvec, long n, const double *x, long ldx, const doub
 
 void launch_reduce_partials(int nvec, const double *part, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce_parts, dim3((nvec + 63) / 64), dim3(64), 0, s, nvec, part, out);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nvec), dim3(64), 0, s, nvec, part, out);
 }
 
 int dot_host(int nvec, long n, const double *x, long ldx, const double *y, long ldy, double *out_host,
@@ -210,16 +213,17 @@ __global__ void __launch_bounds__(256) k_qmr_init_z(long n, QmrVecs<S> w, double
     zero_tail(part + (long)v * NPART);
 }
 
-// init scalars: BNRM, VNRM, ZNRM, GAMAMO=1, ETA=-1, ZVRAT=1
+// init scalars: BNRM, VNRM, ZNRM, GAMAMO=1, ETA=-1, ZVRAT=1 (one wave per vector)
 __global__ void k_qmr_init_scal(int nvec, const double *part_bb, const double *part_zz, QmrScal *q, int *active,
                                 int qmrmax)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvec) return;
+    const int v = blockIdx.x;
+    const double bb = sum_parts(part_bb + (long)v * NPART), zz = sum_parts(part_zz + (long)v * NPART);
+    if (threadIdx.x != 0) return;
     QmrScal &s = q[v];
-    s.bnrm = sqrt(sum_parts(part_bb + (long)v * NPART));
+    s.bnrm = sqrt(bb);
     s.vnrm = s.bnrm;
-    s.znrm = sqrt(sum_parts(part_zz + (long)v * NPART));
+    s.znrm = sqrt(zz);
     s.gamamo = 1.0;
     s.eta = -1.0;
     s.zvrat = 1.0;
@@ -362,6 +366,7 @@ __global__ void __launch_bounds__(256) k_qmr_e(long n, QmrVecs<S> w, QmrScal *q,
     const int v = blockIdx.y;
     if (!running(q, v)) return;
     const double zvrat = q[v].zvrat;
+    const double vnrmpo = blockIdx.x == 0 ? sqrt(sum_parts_n(part_vv + (long)v * NPART, n)) : 0.0;
     S *Z = w.Z + (long)v * w.ld;
     double s = 0;
     GSTRIDE(i, n)
@@ -374,7 +379,7 @@ __global__ void __launch_bounds__(256) k_qmr_e(long n, QmrVecs<S> w, QmrScal *q,
     zero_tail(part + (long)v * NPART);
     if (threadIdx.x == 0) {
         part[(long)v * NPART + blockIdx.x] = s;
-        if (blockIdx.x == 0) q[v].vnrmpo = sqrt(sum_parts_n(part_vv + (long)v * NPART, n));
+        if (blockIdx.x == 0) q[v].vnrmpo = vnrmpo;
     }
 }
 
@@ -483,8 +488,7 @@ __global__ void k_qmr_h(long n, int nvec, int conv, int qmrmax, double tol, QmrS
                         const double *part_rr, const double *part_uxux, const double *part_xx,
                         const double *part_urur)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvec) return;
+    const int v = blockIdx.x;   // one wave per vector: the whole wave sums, lane 0 updates
     QmrScal &s = q[v];
     if (s.status != QMR_RUNNING) return;
     double res;
@@ -501,6 +505,7 @@ __global__ void k_qmr_h(long n, int nvec, int conv, int qmrmax, double tol, QmrS
     } else {
         res = sqrt(sum_parts_n(part_rr + (long)v * NPART, n)) / s.bnrm;
     }
+    if (threadIdx.x != 0) return;
     s.res = res;
     if (fabs(res) < tol) {
         s.ret = s.iter;
@@ -534,7 +539,7 @@ void launch_qmr_init_z(int nvec, long n, const QmrVecs<S> &w, double *part, hipS
 void launch_qmr_init_scal(int nvec, const double *pbb, const double *pzz, QmrScal *q, int *active, int qmrmax,
                           hipStream_t s)
 {
-    hipLaunchKernelGGL(k_qmr_init_scal, dim3((nvec + 63) / 64), dim3(64), 0, s, nvec, pbb, pzz, q, active, qmrmax);
+    hipLaunchKernelGGL(k_qmr_init_scal, dim3(nvec), dim3(64), 0, s, nvec, pbb, pzz, q, active, qmrmax);
 }
 template <typename S>
 void launch_qmr_a(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, double *part, hipStream_t s)
@@ -577,7 +582,7 @@ void launch_qmr_g(int nvec, long n, const QmrVecs<S> &w, const QmrScal *q, doubl
 void launch_qmr_h(long n, int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *phh,
                   const double *prr, const double *pux, const double *pxx, const double *pur, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_qmr_h, dim3((nvec + 63) / 64), dim3(64), 0, s, n, nvec, conv, qmrmax, tol, q, active, phh,
+    hipLaunchKernelGGL(k_qmr_h, dim3(nvec), dim3(64), 0, s, n, nvec, conv, qmrmax, tol, q, active, phh,
                        prr, pux, pxx, pur);
 }
 
@@ -691,11 +696,11 @@ __global__ void __launch_bounds__(256) k_multidot(long n, int i0, int m, const d
 __global__ void k_multidot_fin(int nw, int m, int i0, int j, const double *part, double *hmat, int ldh, long ldhk,
                                const int *active)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nw * m) return;
-    int k = t / m, i = t % m;
+    const int t = blockIdx.x;   // one wave per entry
+    const int k = t / m, i = t % m;
     if (active && !active[k]) return;
-    double s = sum_parts(part + ((long)k * m + i) * NPART);
+    const double s = sum_parts(part + ((long)k * m + i) * NPART);
+    if (threadIdx.x != 0) return;
     double *H = hmat + (long)k * ldhk;
     H[(i0 + i) + (long)ldh * j] = s;
     H[j + (long)ldh * (i0 + i)] = s;
@@ -710,8 +715,7 @@ void launch_multidot(int nw, long n, int i0, int m, const double *vj, long ldv, 
 void launch_multidot_fin(int nw, int m, int i0, int j, const double *part, double *hmat, int ldh, long ldhk,
                          const int *active, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_multidot_fin, dim3((nw * m + 63) / 64), dim3(64), 0, s, nw, m, i0, j, part, hmat, ldh, ldhk,
-                       active);
+    hipLaunchKernelGGL(k_multidot_fin, dim3(nw * m), dim3(64), 0, s, nw, m, i0, j, part, hmat, ldh, ldhk, active);
 }
 
 // Ritz vectors: psi(:, c) = sum_i V(:, i) Z(i, col[c])   (one window)
