@@ -158,8 +158,12 @@ __global__ void __launch_bounds__(64 * WAVES) k_px_rot(int nin, int n, const T *
     }
 }
 
-// Level 1 fused on the VALU: per block b (one workgroup of 4 waves), for
-// every vector: z = V_b^T y_b ;  z /= kernel(EIG0 - E0) ;  y_b = V_b z
+// Level 1 fused on the VALU, persistent: workgroup g of G (4 waves) takes
+// blocks g, g+G, ...; for every vector of a block
+//   z = V_b^T y_b ;  z /= kernel(EIG0 - E0) ;  y_b = V_b z.
+// While one block is computed, the next block's rotation (up to 16 values per
+// thread) and its first vectors (up to 512 values) are loaded into registers,
+// so the VOSB stream never waits behind the arithmetic.
 template <typename T, typename S, typename SO>
 __global__ void __launch_bounds__(256) k_px_level1(int n1, int nblk, int nvec, const T *__restrict__ vosb,
                                                    const T *__restrict__ eig0, const T *__restrict__ shifts, T shift0,
@@ -170,38 +174,79 @@ __global__ void __launch_bounds__(256) k_px_level1(int n1, int nblk, int nvec, c
     T *sV = (T *)smem_raw;              // n1 x (n1+1), V(i,j) at i + (n1+1) j
     const int ld = n1 + 1;
     constexpr int VCH = 8;              // vectors per pass
+    constexpr int PV = 16;              // V values per thread carried to the next block
+    constexpr int PY = 2;               // first-chunk values per thread carried to the next block
     T *sY = sV + (long)n1 * ld;         // n1 x VCH
     T *sZ = sY + n1 * VCH;
-    const int b = blockIdx.x;
     const int tid = threadIdx.x;
-    const T *vb = vosb + (long)b * n1 * n1;
-    stage_lds<256, 16>(vb, n1 * n1, tid, [&](int e) -> T & { return sV[(e % n1) + ld * (e / n1)]; });
-    const T *eb = eig0 + (long)b * n1;
-    for (int v0 = 0; v0 < nvec; v0 += VCH) {
-        const int nv = min(VCH, nvec - v0);
-        __syncthreads();
-        for (int e = tid; e < n1 * nv; e += 256) {
-            int i = e % n1, vv = e / n1;
-            int v = v0 + vv;
-            sY[i + n1 * vv] = vec_active(active, v) ? ld_cvt<T, S>(x + (long)v * ldx + (long)b * n1, i) : (T)0;
+    const int G = gridDim.x;
+    const int nvv = n1 * n1;
+    const int ny0 = n1 * min(VCH, nvec);   // values of the first vector chunk
+    T pv[PV], py[PY];
+    auto y_of = [&](int bb, int e) -> T {   // value e = i + n1 vv of the first chunk of block bb
+        const int i = e % n1, vv = e / n1;
+        return vec_active(active, vv) ? ld_cvt<T, S>(x + (long)vv * ldx + (long)bb * n1, i) : (T)0;
+    };
+    auto load_next = [&](int bb) {
+        const T *vb = vosb + (long)bb * nvv;
+#pragma unroll
+        for (int u = 0; u < PV; u++) {
+            const int e = tid + 256 * u;
+            pv[u] = e < nvv ? vb[e] : (T)0;
         }
-        __syncthreads();
-        for (int e = tid; e < n1 * nv; e += 256) {
-            int i = e % n1, vv = e / n1;
-            T acc = 0;
-            for (int j = 0; j < n1; j++) acc += sV[j + ld * i] * sY[j + n1 * vv];
-            int v = v0 + vv;
-            T e0 = shifts ? shifts[v] : shift0;
-            sZ[i + n1 * vv] = px_kernel<T>(nosb, acc, eb[i], e0, de, beta);
+#pragma unroll
+        for (int u = 0; u < PY; u++) {
+            const int e = tid + 256 * u;
+            py[u] = e < ny0 ? y_of(bb, e) : (T)0;
         }
+    };
+    int b = blockIdx.x;
+    if (b < nblk) load_next(b);
+    for (; b < nblk; b += G) {
+        __syncthreads();   // the previous block is done with sV, sY and sZ
+#pragma unroll
+        for (int u = 0; u < PV; u++) {
+            const int e = tid + 256 * u;
+            if (e < nvv) sV[(e % n1) + ld * (e / n1)] = pv[u];
+        }
+        for (int e = tid + 256 * PV; e < nvv; e += 256) sV[(e % n1) + ld * (e / n1)] = vosb[(long)b * nvv + e];
+#pragma unroll
+        for (int u = 0; u < PY; u++) {
+            const int e = tid + 256 * u;
+            if (e < ny0) sY[e] = py[u];
+        }
+        for (int e = tid + 256 * PY; e < ny0; e += 256) sY[e] = y_of(b, e);
         __syncthreads();
-        for (int e = tid; e < n1 * nv; e += 256) {
-            int i = e % n1, vv = e / n1;
-            int v = v0 + vv;
-            if (!vec_active(active, v)) continue;
-            T acc = 0;
-            for (int j = 0; j < n1; j++) acc += sV[i + ld * j] * sZ[j + n1 * vv];
-            st_cvt<T, SO>(y + (long)v * ldy + (long)b * n1, i, acc);
+        if (b + G < nblk) load_next(b + G);   // in flight while this block is computed
+        const T *eb = eig0 + (long)b * n1;
+        for (int v0 = 0; v0 < nvec; v0 += VCH) {
+            const int nv = min(VCH, nvec - v0);
+            if (v0 > 0) {
+                __syncthreads();
+                for (int e = tid; e < n1 * nv; e += 256) {
+                    int i = e % n1, vv = e / n1;
+                    int v = v0 + vv;
+                    sY[i + n1 * vv] = vec_active(active, v) ? ld_cvt<T, S>(x + (long)v * ldx + (long)b * n1, i) : (T)0;
+                }
+                __syncthreads();
+            }
+            for (int e = tid; e < n1 * nv; e += 256) {
+                int i = e % n1, vv = e / n1;
+                T acc = 0;
+                for (int j = 0; j < n1; j++) acc += sV[j + ld * i] * sY[j + n1 * vv];
+                int v = v0 + vv;
+                T e0 = shifts ? shifts[v] : shift0;
+                sZ[i + n1 * vv] = px_kernel<T>(nosb, acc, eb[i], e0, de, beta);
+            }
+            __syncthreads();
+            for (int e = tid; e < n1 * nv; e += 256) {
+                int i = e % n1, vv = e / n1;
+                int v = v0 + vv;
+                if (!vec_active(active, v)) continue;
+                T acc = 0;
+                for (int j = 0; j < n1; j++) acc += sV[i + ld * j] * sZ[j + n1 * vv];
+                st_cvt<T, SO>(y + (long)v * ldy + (long)b * n1, i, acc);
+            }
         }
     }
 }
@@ -399,9 +444,21 @@ static void level1(const Level &L, const T *vosb, const T *eig0, const T *shifts
     if (nvec >= 8 && n1 <= 112 && l1_mfma_lds<T>(n1) <= PX_LDS_MAX)
         hipLaunchKernelGGL((k_px_l1_mfma<T, SI, SO>), dim3(nb), dim3(256), l1_mfma_lds<T>(n1), s, n1, nvec, vosb, eig0,
                            shifts, shift0, nosb, de, beta, x, ldx, y, ldy, active);
-    else
-        hipLaunchKernelGGL((k_px_level1<T, SI, SO>), dim3(nb), dim3(256), l1_valu_lds<T>(n1), s, n1, nb, nvec, vosb,
+    else {
+        // persistent: as many workgroups as are resident at once
+        static int per_cu = 0;
+        if (per_cu == 0) {
+            int nbk = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, k_px_level1<T, SI, SO>, 256, l1_valu_lds<T>(n1)) !=
+                    hipSuccess ||
+                nbk < 1)
+                nbk = 1;
+            per_cu = nbk;
+        }
+        const int G = std::min(nb, g_num_cu * per_cu);
+        hipLaunchKernelGGL((k_px_level1<T, SI, SO>), dim3(G), dim3(256), l1_valu_lds<T>(n1), s, n1, nb, nvec, vosb,
                            eig0, shifts, shift0, nosb, de, beta, x, ldx, y, ldy, active);
+    }
 }
 
 template <typename T, typename S>
