@@ -5,10 +5,10 @@
 //
 // Level l >= 2 holds one n_l x n_l rotation per block of the outer indices;
 // applying it is a mode product Y(k,i,b) = sum_j A_b(i,j) X(k,j,b) over the
-// contiguous inner index k (NIN = n_1...n_{l-1} values).  k_px_rot computes
-// it on MFMA: the rotation of the block is staged in LDS, each wave takes
-// 16-point tiles of k (coalesced 128-byte runs along k) with all of j in
-// registers, as the mode-3 pass of H*X does.
+// contiguous inner index k (NIN = n_1...n_{l-1} values).  k_outer_mode
+// computes it on MFMA with one grid row per block: the block's rotation is
+// staged in LDS, each wave takes 16-point tiles of k (coalesced runs along k)
+// with all of j in registers, as the mode-3 pass of H*X does.
 //
 // Level 1 (the innermost, NIN = 1) holds one n1 x n1 rotation per block of
 // all outer indices: VOSB level 1 is N*n1 values, the largest stream of the
@@ -66,95 +66,6 @@ __global__ void k_rotate_generic(int nin, int nout, int blk, const T *__restrict
         else
             for (int j = 0; j < nout; j++) acc += vb[i + (long)nout * j] * ld_cvt<T, S>(xb, (long)nin * j);
         st_cvt<T, SO>(yv, idx, acc);
-    }
-}
-
-// MFMA rotation of one level, n <= 128: Y(k,i,b) = sum_j A_b(i,j) X(k,j,b)
-// with A_b column-major in mats (V_b for VX, its stored transpose for VTX).
-// Grid: x = groups of WAVES*PT tiles, y = block b.  A tile is 16 consecutive
-// k of one (b, vector); its lane (lr, lk) holds X(k0 + lr, 4 s + lk) for
-// every s.  LDS: A_b as np x np (np = n rounded up to 16, zero padded).
-template <typename T, typename S, typename SO, int WAVES, int PT>
-__global__ void __launch_bounds__(64 * WAVES) k_px_rot(int nin, int n, const T *__restrict__ mats, int nvec,
-                                                      const S *__restrict__ x, long ldx, SO *__restrict__ y, long ldy,
-                                                      const int *active)
-{
-    using M = Mfma<T>;
-    using acc_t = typename M::acc_t;
-    extern __shared__ char smem_raw[];
-    T *sA = (T *)smem_raw;
-    const int np = rup16(n);
-    const int b = blockIdx.y;
-    const T *mb = mats + (long)b * n * n;
-    const int tid = threadIdx.x;
-    // A(i,j) at sA[i + np j]
-    if (np != n)
-        for (int e = tid; e < np * np; e += 64 * WAVES)
-            if (e % np >= n || e / np >= n) sA[e] = (T)0;
-    stage_lds<64 * WAVES, 16>(mb, n * n, tid, [&](int e) -> T & { return sA[(e % n) + np * (e / n)]; });
-    __syncthreads();
-    const int lane = tid & 63, wave = tid >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const long tpb = (nin + 15) / 16;   // tiles per (block, vector)
-    const long nt = tpb * nvec;
-    const long blkoff = (long)b * nin * n;
-    constexpr int KMAX = 128 / 4;
-    constexpr int RMAX = 128 / 16;
-    const int nk = np / 4, nr = np / 16;
-    long pidx[PT];
-    int vid[PT];
-    bool ok[PT];
-    T bfr[PT][KMAX];
-#pragma unroll
-    for (int u = 0; u < PT; u++) {
-        const long t = ((long)blockIdx.x * WAVES + wave) * PT + u;
-        const int v = (int)(t < nt ? t / tpb : 0);
-        const long p = (t < nt ? t % tpb : 0) * 16 + lr;
-        vid[u] = v;
-        pidx[u] = p;
-        ok[u] = t < nt && vec_active(active, v);
-        const bool pin = ok[u] && p < nin;
-        const S *xs = x + (long)v * ldx + blkoff + p;
-#pragma unroll
-        for (int s = 0; s < KMAX; s++) {
-            if (s < nk) {
-                const int k = 4 * s + lk;
-                bfr[u][s] = (pin && k < n) ? ld_cvt<T, S>(xs, (long)nin * k) : (T)0;
-            }
-        }
-    }
-    acc_t acc[PT][RMAX];
-#pragma unroll
-    for (int u = 0; u < PT; u++)
-#pragma unroll
-        for (int r = 0; r < RMAX; r++) acc[u][r] = acc_t{0, 0, 0, 0};
-#pragma unroll
-    for (int s = 0; s < KMAX; s++) {
-        if (s < nk) {
-#pragma unroll
-            for (int r = 0; r < RMAX; r++) {
-                if (r < nr) {
-                    const T a = sA[(16 * r + lr) + np * (4 * s + lk)];
-#pragma unroll
-                    for (int u = 0; u < PT; u++) acc[u][r] = M::run(a, bfr[u][s], acc[u][r]);
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < PT; u++) {
-        if (!ok[u] || pidx[u] >= nin) continue;
-        SO *ys = y + (long)vid[u] * ldy + blkoff + pidx[u];
-#pragma unroll
-        for (int r = 0; r < RMAX; r++) {
-            if (r < nr) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int i = 16 * r + M::row(lk, q);
-                    if (i < n) st_cvt<T, SO>(ys, (long)nin * i, acc[u][r][q]);
-                }
-            }
-        }
     }
 }
 
@@ -371,30 +282,6 @@ static size_t l1_valu_lds(int n1)
     return ((size_t)n1 * (n1 + 1) + 2 * (size_t)n1 * 8) * sizeof(T);
 }
 
-// rotation variants (waves per workgroup, tiles per wave)
-struct RotVar {
-    int waves, pt;
-};
-static const RotVar kRotVar[] = {{4, 2}, {4, 1}, {2, 1}, {1, 1}};
-constexpr int NROT = sizeof(kRotVar) / sizeof(kRotVar[0]);
-
-template <typename T, typename SI, typename SO, int W, int PT>
-static void launch_rot1(int nin, int n, int blk, const T *mats, int nvec, const SI *x, long ldx, SO *y, long ldy,
-                        const int *active, hipStream_t s)
-{
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_px_rot<T, SI, SO, W, PT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)PX_LDS_MAX);
-        attr = true;
-    }
-    const long nt = (long)((nin + 15) / 16) * nvec;
-    dim3 g((unsigned)((nt + W * PT - 1) / (W * PT)), blk);
-    const size_t lds = (size_t)rup16(n) * rup16(n) * sizeof(T);
-    hipLaunchKernelGGL((k_px_rot<T, SI, SO, W, PT>), g, dim3(64 * W), lds, s, nin, n, mats, nvec, x, ldx, y, ldy,
-                       active);
-}
-
 // One rotation level, VTX (trans) or VX: MFMA when the block fits (n <= 128),
 // else one thread per value.  vosbt holds the per-block transposes.
 template <typename T, typename SI, typename SO>
@@ -408,23 +295,14 @@ static void rotate(const Level &L, const T *vosb, const T *vosbt, bool trans, in
                            active);
         return;
     }
+    // the persistent outer-mode kernel, one grid row per block: 8 waves per
+    // workgroup for batches, 4 for a single vector (fastest measured on the
+    // 64x96x80 workload)
     const T *mats = trans ? vosbt : vosb;
-    if (blk == 1) {
-        // the outermost level: one rotation for all of the inner indices
-        launch_outer_mode<T, SI, SO, 4, false>(nin, n, nvec, mats, nullptr, x, ldx, y, ldy, active, s);
-        return;
-    }
-    int var = 1;   // 4 waves of one tile each: fastest measured from 1 to 16 vectors on the 64x96x80 workload
-    if (const char *e = std::getenv("SCALIT_PX_ROT_VARIANT")) {
-        const int f = std::atoi(e);
-        if (f >= 0 && f < NROT) var = f;
-    }
-    switch (var) {
-    case 0: launch_rot1<T, SI, SO, 4, 2>(nin, n, blk, mats, nvec, x, ldx, y, ldy, active, s); break;
-    case 1: launch_rot1<T, SI, SO, 4, 1>(nin, n, blk, mats, nvec, x, ldx, y, ldy, active, s); break;
-    case 2: launch_rot1<T, SI, SO, 2, 1>(nin, n, blk, mats, nvec, x, ldx, y, ldy, active, s); break;
-    default: launch_rot1<T, SI, SO, 1, 1>(nin, n, blk, mats, nvec, x, ldx, y, ldy, active, s); break;
-    }
+    if (nvec >= 4)
+        launch_outer_mode<T, SI, SO, 8, false>(nin, n, nvec, mats, nullptr, x, ldx, y, ldy, active, s, blk);
+    else
+        launch_outer_mode<T, SI, SO, 4, false>(nin, n, nvec, mats, nullptr, x, ldx, y, ldy, active, s, blk);
 }
 
 // Level 1: MFMA for batches of 8 and more, VALU otherwise.

@@ -1,12 +1,13 @@
-// Product along the outermost index of a P x n array, on MFMA:
-//   Y(p, i) [+]= d(p) * sum_j A(i, j) X(p, j),     p < P (contiguous), i, j < n,
-// for every vector of a batch.  The mode-3 pass of H*X (A = L^2_theta,
-// d = DEP(r, R), added to Y) and the outermost rotation of P*X (A = V or V^T,
-// d = 1, written) are both this shape.
+// Product along the outer index of a P x n array, on MFMA:
+//   Y(p, i, b) [+]= d(p) * sum_j A_b(i, j) X(p, j, b),   p < P (contiguous), i, j < n,
+// for every block b of nblk (A_b at amat + b n^2) and every vector of a batch.
+// The mode-3 pass of H*X (one block, A = L^2_theta, d = DEP(r, R), added to Y)
+// and every rotation level of P*X (A_b = V_b or V_b^T, d = 1, written) are
+// this shape.
 //
-// Persistent grid: each workgroup stages A in LDS once; each wave then walks
-// its own tiles of 16 consecutive p of one vector (no workgroup barrier after
-// the staging) and loads the next tile's X and d into registers while the
+// Persistent grid (x: workgroups per block, y: block): each workgroup stages
+// its block's A in LDS once; each wave then walks its own tiles of 16
+// consecutive p of one vector (no workgroup barrier after the staging) and loads the next tile's X and d into registers while the
 // current tile's MFMAs run (Y, when adding, at the top of its own tile).  Lane (lr, lk) of a tile holds X(p0 + lr, 4 s + lk)
 // for all s (the B fragments); the A fragments come from LDS.  NR = tiles of
 // 16 along n (n <= 16 NR, A zero padded), a compile-time count so the register
@@ -24,6 +25,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
                                                           const T *__restrict__ dvec, const S *__restrict__ x,
                                                           long ldx, SO *__restrict__ y, long ldy, const int *active)
 {
+    // block b of the outer index: its matrix and its P x n slice of every vector
+    const int b = blockIdx.y;
+    amat += (long)b * n * n;
+    x += (long)b * P * n;
+    y += (long)b * P * n;
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     constexpr int np = 16 * NR;
@@ -155,7 +161,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
 // device at once (at most one per 16 tiles' worth of waves).
 template <typename T, typename S, typename SO, int WAVES, int NR, bool ACC>
 static void launch_outer_mode_nr(long P, int n, int nvec, const T *amat, const T *dvec, const S *x, long ldx, SO *y,
-                                 long ldy, const int *active, hipStream_t s)
+                                 long ldy, const int *active, hipStream_t s, int nblk)
 {
     // exact tiles and 32-bit byte offsets within one vector: the buffer path
     const bool ex = n == 16 * NR && P * n * (long)std::max(sizeof(S), sizeof(SO)) < 0x7fffffffL;
@@ -169,33 +175,37 @@ static void launch_outer_mode_nr(long P, int n, int nvec, const T *amat, const T
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64 * WAVES, lds) != hipSuccess || nb < 1) nb = 1;
         per_cu[ex] = nb;
     }
+    // workgroups per block: enough for the block's tiles, and all blocks'
+    // workgroups together at most what is resident at once
     const long ntile = ((P + 15) / 16) * nvec;
-    long G = std::min<long>((ntile + WAVES - 1) / WAVES, (long)g_num_cu * per_cu[ex]);
+    long G = std::min<long>((ntile + WAVES - 1) / WAVES, std::max<long>(1, (long)g_num_cu * per_cu[ex] / nblk));
     if (G < 1) G = 1;
+    const dim3 grid((unsigned)G, (unsigned)nblk);
     if (ex)
-        hipLaunchKernelGGL((k_outer_mode<T, S, SO, WAVES, NR, ACC, true>), dim3((unsigned)G), dim3(64 * WAVES), lds, s,
-                           P, n, nvec, amat, dvec, x, ldx, y, ldy, active);
+        hipLaunchKernelGGL((k_outer_mode<T, S, SO, WAVES, NR, ACC, true>), grid, dim3(64 * WAVES), lds, s, P, n, nvec,
+                           amat, dvec, x, ldx, y, ldy, active);
     else
-        hipLaunchKernelGGL((k_outer_mode<T, S, SO, WAVES, NR, ACC, false>), dim3((unsigned)G), dim3(64 * WAVES), lds,
-                           s, P, n, nvec, amat, dvec, x, ldx, y, ldy, active);
+        hipLaunchKernelGGL((k_outer_mode<T, S, SO, WAVES, NR, ACC, false>), grid, dim3(64 * WAVES), lds, s, P, n,
+                           nvec, amat, dvec, x, ldx, y, ldy, active);
 }
 
-// n <= 128: the tile count along n rounded up to a compiled one
+// n <= 128, nblk <= 65535 blocks (block b's matrix at amat + b n^2, its data
+// at + b P n of each vector): the tile count along n rounded up to a compiled one
 template <typename T, typename S, typename SO, int WAVES, bool ACC>
 static void launch_outer_mode(long P, int n, int nvec, const T *amat, const T *dvec, const S *x, long ldx, SO *y,
-                              long ldy, const int *active, hipStream_t s)
+                              long ldy, const int *active, hipStream_t s, int nblk = 1)
 {
     const int nr = (n + 15) / 16;
     if (nr <= 2)
-        launch_outer_mode_nr<T, S, SO, WAVES, 2, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+        launch_outer_mode_nr<T, S, SO, WAVES, 2, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s, nblk);
     else if (nr <= 4)
-        launch_outer_mode_nr<T, S, SO, WAVES, 4, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+        launch_outer_mode_nr<T, S, SO, WAVES, 4, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s, nblk);
     else if (nr == 5)
-        launch_outer_mode_nr<T, S, SO, WAVES, 5, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+        launch_outer_mode_nr<T, S, SO, WAVES, 5, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s, nblk);
     else if (nr == 6)
-        launch_outer_mode_nr<T, S, SO, WAVES, 6, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+        launch_outer_mode_nr<T, S, SO, WAVES, 6, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s, nblk);
     else
-        launch_outer_mode_nr<T, S, SO, WAVES, 8, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s);
+        launch_outer_mode_nr<T, S, SO, WAVES, 8, ACC>(P, n, nvec, amat, dvec, x, ldx, y, ldy, active, s, nblk);
 }
 
 }  // namespace scalit
