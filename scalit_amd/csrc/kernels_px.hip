@@ -69,195 +69,239 @@ __global__ void k_rotate_generic(int nin, int nout, int blk, const T *__restrict
     }
 }
 
-// Level 1 fused on the VALU, persistent: workgroup g of G (4 waves) takes
-// blocks g, g+G, ...; for every vector of a block
-//   z = V_b^T y_b ;  z /= kernel(EIG0 - E0) ;  y_b = V_b z.
-// While one block is computed, the next block's rotation (up to 16 values per
-// thread) and its first vectors (up to 512 values) are loaded into registers,
-// so the VOSB stream never waits behind the arithmetic.
-template <typename T, typename S, typename SO>
-__global__ void __launch_bounds__(256) k_px_level1(int n1, int nblk, int nvec, const T *__restrict__ vosb,
-                                                   const T *__restrict__ eig0, const T *__restrict__ shifts, T shift0,
-                                                   int nosb, T de, T beta, const S *__restrict__ x, long ldx,
-                                                   SO *__restrict__ y, long ldy, const int *active)
-{
-    extern __shared__ char smem_raw[];
-    T *sV = (T *)smem_raw;              // n1 x (n1+1), V(i,j) at i + (n1+1) j
-    const int ld = n1 + 1;
-    constexpr int VCH = 8;              // vectors per pass
-    constexpr int PV = 16;              // V values per thread carried to the next block
-    constexpr int PY = 2;               // first-chunk values per thread carried to the next block
-    T *sY = sV + (long)n1 * ld;         // n1 x VCH
-    T *sZ = sY + n1 * VCH;
-    const int tid = threadIdx.x;
-    const int G = gridDim.x;
-    const int nvv = n1 * n1;
-    const int ny0 = n1 * min(VCH, nvec);   // values of the first vector chunk
-    T pv[PV], py[PY];
-    auto y_of = [&](int bb, int e) -> T {   // value e = i + n1 vv of the first chunk of block bb
-        const int i = e % n1, vv = e / n1;
-        return vec_active(active, vv) ? ld_cvt<T, S>(x + (long)vv * ldx + (long)bb * n1, i) : (T)0;
-    };
-    auto load_next = [&](int bb) {
-        const T *vb = vosb + (long)bb * nvv;
-#pragma unroll
-        for (int u = 0; u < PV; u++) {
-            const int e = tid + 256 * u;
-            pv[u] = e < nvv ? vb[e] : (T)0;
-        }
-#pragma unroll
-        for (int u = 0; u < PY; u++) {
-            const int e = tid + 256 * u;
-            py[u] = e < ny0 ? y_of(bb, e) : (T)0;
-        }
-    };
-    int b = blockIdx.x;
-    if (b < nblk) load_next(b);
-    for (; b < nblk; b += G) {
-        __syncthreads();   // the previous block is done with sV, sY and sZ
-#pragma unroll
-        for (int u = 0; u < PV; u++) {
-            const int e = tid + 256 * u;
-            if (e < nvv) sV[(e % n1) + ld * (e / n1)] = pv[u];
-        }
-        for (int e = tid + 256 * PV; e < nvv; e += 256) sV[(e % n1) + ld * (e / n1)] = vosb[(long)b * nvv + e];
-#pragma unroll
-        for (int u = 0; u < PY; u++) {
-            const int e = tid + 256 * u;
-            if (e < ny0) sY[e] = py[u];
-        }
-        for (int e = tid + 256 * PY; e < ny0; e += 256) sY[e] = y_of(b, e);
-        __syncthreads();
-        if (b + G < nblk) load_next(b + G);   // in flight while this block is computed
-        const T *eb = eig0 + (long)b * n1;
-        for (int v0 = 0; v0 < nvec; v0 += VCH) {
-            const int nv = min(VCH, nvec - v0);
-            if (v0 > 0) {
-                __syncthreads();
-                for (int e = tid; e < n1 * nv; e += 256) {
-                    int i = e % n1, vv = e / n1;
-                    int v = v0 + vv;
-                    sY[i + n1 * vv] = vec_active(active, v) ? ld_cvt<T, S>(x + (long)v * ldx + (long)b * n1, i) : (T)0;
-                }
-                __syncthreads();
-            }
-            for (int e = tid; e < n1 * nv; e += 256) {
-                int i = e % n1, vv = e / n1;
-                T acc = 0;
-                for (int j = 0; j < n1; j++) acc += sV[j + ld * i] * sY[j + n1 * vv];
-                int v = v0 + vv;
-                T e0 = shifts ? shifts[v] : shift0;
-                sZ[i + n1 * vv] = px_kernel<T>(nosb, acc, eb[i], e0, de, beta);
-            }
-            __syncthreads();
-            for (int e = tid; e < n1 * nv; e += 256) {
-                int i = e % n1, vv = e / n1;
-                int v = v0 + vv;
-                if (!vec_active(active, v)) continue;
-                T acc = 0;
-                for (int j = 0; j < n1; j++) acc += sV[i + ld * j] * sZ[j + n1 * vv];
-                st_cvt<T, SO>(y + (long)v * ldy + (long)b * n1, i, acc);
-            }
-        }
-    }
-}
-
-// Level 1 fused on MFMA, n1 <= 112: per block b (one workgroup of 4 waves),
-// 16 vectors at a time as the columns of the tiles,
-//   Z = V_b^T W (wave r: rows 16r..16r+15), Z /= kernel, Y = V_b Z.
-// LDS: V_b (n1p x (n1p+1)), W / Y and Z ((n1p+1) x 16 each).
-template <typename T, typename S, typename SO>
-__global__ void __launch_bounds__(256) k_px_l1_mfma(int n1, int nvec, const T *__restrict__ vosb,
-                                                    const T *__restrict__ eig0, const T *__restrict__ shifts,
-                                                    T shift0, int nosb, T de, T beta, const S *__restrict__ x,
-                                                    long ldx, SO *__restrict__ y, long ldy, const int *active)
+// Level 1 fused, persistent: workgroup g of G (4 waves) takes blocks g, g+G,
+// ...; for every vector of a block
+//   z = V_b^T x_b ;  z /= kernel(EIG0 - E0) ;  y_b = V_b z,
+// on MFMA with 16 vectors as the columns of the tiles (MF: wave r computes
+// rows 16r..16r+15) or on the VALU.
+//
+// The VOSB stream is read at the rate of a plain copy only if nothing on the
+// critical path of a block waits for memory, so: the next block's rotation,
+// EIG0 values and first vectors are loaded into registers right after this
+// block is staged, with unconditional loads (indices clamped into the block,
+// the extra values dropped when staged), and the computation reads LDS only
+// (shifts and activity of the vectors are staged once per chunk).  A global
+// load in the computation would wait for the whole prefetch (the memory
+// counter drains in order).  On the VALU, small batches split every dot
+// product over up to 8 threads (partial sums through LDS), so that a block
+// costs a few short passes rather than two n1-long chains on one wave.
+//
+// LDS: V_b (ns x (ns+1), V(i,j) at i + (ns+1) j, zero padded for MF with
+// ns = n1 rounded up to 16, else ns = n1), W and Z ((ns+1) x WC each; W holds
+// Y after the MF product), partial sums (256), EIG0 of the block (ns), and
+// the shifts and activity of the first chunk of vectors and of the current
+// later one (2 WC each).  BIG (n1 > 64): the part of a block beyond the
+// registers is loaded while it is staged.
+template <typename T, typename S, typename SO, bool MF, bool BIG>
+__global__ void __launch_bounds__(256) k_px_l1(int n1, int nblk, int nvec, const T *__restrict__ vosb,
+                                               const T *__restrict__ eig0, const T *__restrict__ shifts, T shift0,
+                                               int nosb, T de, T beta, const S *__restrict__ x, long ldx,
+                                               SO *__restrict__ y, long ldy, const int *active)
 {
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
+    constexpr int WC = MF ? 16 : 8;   // vectors per chunk
+    constexpr int PV = 16;            // rotation values per thread
+    constexpr int PY = MF ? 4 : 2;    // first-chunk values per thread
     extern __shared__ char smem_raw[];
-    const int n1p = rup16(n1);
-    const int ldv = n1p + 1, ldw = n1p + 1;
-    T *sV = (T *)smem_raw;          // V(i,j) at i + ldv j
-    T *sW = sV + (long)n1p * ldv;   // W(j,c) at j + ldw c; later Y(i,c)
-    T *sZ = sW + 16 * ldw;          // Z(j,c)
-    const int b = blockIdx.x;
+    const int ns = MF ? rup16(n1) : n1;
+    const int ldv = ns + 1;
+    T *sV = (T *)smem_raw;             // V(i,j) at i + ldv j
+    T *sW = sV + (long)ns * ldv;       // W(j,c) at j + ldv c
+    T *sZ = sW + (long)WC * ldv;       // Z(j,c) at j + ldv c
+    T *sP = sZ + (long)WC * ldv;       // partial sums
+    T *sE = sP + 256;                  // EIG0(i) of the block
+    T *sSh = sE + ns;                  // shifts: first chunk, current later chunk
+    int *sAct = (int *)(sSh + 2 * WC);   // activity (0 / 1): first chunk, current later chunk
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lk = lane >> 4;
-    const int nr = n1p / 16, nk = n1p / 4;
-    const T *vb = vosb + (long)b * n1 * n1;
-    const T *eb = eig0 + (long)b * n1;
-    if (n1p != n1)
-        for (int e = tid; e < n1p * n1p; e += 256)
-            if (e % n1p >= n1 || e / n1p >= n1) sV[(e % n1p) + ldv * (e / n1p)] = (T)0;
-    stage_lds<256, 16>(vb, n1 * n1, tid, [&](int e) -> T & { return sV[(e % n1) + ldv * (e / n1)]; });
-    for (int v0 = 0; v0 < nvec; v0 += 16) {
-        __syncthreads();   // the previous chunk's Y has been stored
-        {
-            // 16 vector columns, n1p / 16 <= 7 values per thread, all loads in flight
-            constexpr int UW = 7;
-            T r[UW];
+    const int G = gridDim.x;
+    const int nvv = n1 * n1;
+    const int nw = ns * WC;   // values of one chunk of vectors
+    auto vec_info = [&](int v0, int slot) {
+        if (tid < WC) {
+            const int v = v0 + tid;
+            sSh[slot * WC + tid] = (shifts && v < nvec) ? shifts[v] : shift0;
+            sAct[slot * WC + tid] = (v < nvec && vec_active(active, v)) ? 1 : 0;
+        }
+    };
+    vec_info(0, 0);
+    if (MF && ns != n1)   // the padding is never overwritten
+        for (int e = tid; e < ns * ns; e += 256)
+            if (e % ns >= n1 || e / ns >= n1) sV[(e % ns) + ldv * (e / ns)] = (T)0;
+    // register u holds element tid + 256 u of the block's rotation.  A block
+    // past the last loads one value of the last block on every lane, so the
+    // loads are the same on every path.
+    T pv[PV], py[PY], pe;
+    auto load = [&](int bb) {
+        const bool live = bb < nblk;
+        const long b0 = min(bb, nblk - 1);
+        const T *vb = vosb + b0 * nvv;
 #pragma unroll
-            for (int u = 0; u < UW; u++) {
-                const int e = tid + 256 * u;
-                const int j = e % n1p, c = e / n1p, v = v0 + c;
-                r[u] = (e < n1p * 16 && j < n1 && v < nvec && vec_active(active, v))
-                           ? ld_cvt<T, S>(x + (long)v * ldx + (long)b * n1, j)
-                           : (T)0;
-            }
+        for (int u = 0; u < PV; u++) pv[u] = vb[live ? min(tid + 256 * u, nvv - 1) : 0];
 #pragma unroll
-            for (int u = 0; u < UW; u++) {
-                const int e = tid + 256 * u;
-                if (e < n1p * 16) sW[(e % n1p) + ldw * (e / n1p)] = r[u];
+        for (int u = 0; u < PY; u++) {
+            const int e = tid + 256 * u;
+            const int j = live ? min(e % ns, n1 - 1) : 0, v = live ? min(e / ns, nvec - 1) : 0;
+            py[u] = ld_cvt<T, S>(x + (long)v * ldx + b0 * n1, j);
+        }
+        pe = eig0[b0 * n1 + (live ? min(tid, n1 - 1) : 0)];
+    };
+    auto stage = [&](int bb) {
+#pragma unroll
+        for (int u = 0; u < PV; u++) {
+            const int e = tid + 256 * u;
+            if (e < nvv) sV[(e % n1) + ldv * (e / n1)] = pv[u];
+        }
+        if constexpr (BIG)
+            for (int e = tid + 256 * PV; e < nvv; e += 256) sV[(e % n1) + ldv * (e / n1)] = vosb[(long)bb * nvv + e];
+#pragma unroll
+        for (int u = 0; u < PY; u++) {
+            const int e = tid + 256 * u;
+            if (e < nw) {
+                const int j = e % ns, c = e / ns;
+                sW[j + ldv * c] = (j < n1 && sAct[c]) ? py[u] : (T)0;
             }
         }
-        __syncthreads();
-        // Z = V^T W, divided by the kernel; zero outside the block and the batch
-        const int vc = v0 + lr;
-        const bool vin = vc < nvec && vec_active(active, vc);
-        const T e0 = (shifts && vc < nvec) ? shifts[vc] : shift0;
-        for (int r = wave; r < nr; r += 4) {
-            acc_t acc = acc_t{0, 0, 0, 0};
-            for (int s = 0; s < nk; s += 4) {
-                T a[4], bb[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int k = 4 * (s + u) + lk;
-                    a[u] = sV[k + ldv * (16 * r + lr)];   // A(i,k) = V(k,i)
-                    bb[u] = sW[k + ldw * lr];
+        if constexpr (BIG)
+            for (int e = tid + 256 * PY; e < nw; e += 256) {
+                const int j = e % ns, c = e / ns;
+                sW[j + ldv * c] = (j < n1 && sAct[c]) ? ld_cvt<T, S>(x + (long)c * ldx + (long)bb * n1, j) : (T)0;
+            }
+        if (tid < ns) sE[tid] = tid < n1 ? pe : (T)0;
+    };
+    // all chunks of vectors of block bb; V, EIG0 and the first chunk are staged
+    auto compute = [&](int bb) {
+        for (int v0 = 0; v0 < nvec; v0 += WC) {
+            const int slot = v0 == 0 ? 0 : 1;
+            if (v0 > 0) {
+                __syncthreads();   // the previous chunk is done with sW, sZ and slot 1
+                vec_info(v0, 1);
+                __syncthreads();
+                for (int e = tid; e < nw; e += 256) {
+                    const int j = e % ns, c = e / ns;
+                    sW[j + ldv * c] = (j < n1 && sAct[WC + c])
+                                          ? ld_cvt<T, S>(x + (long)(v0 + c) * ldx + (long)bb * n1, j)
+                                          : (T)0;
                 }
-#pragma unroll
-                for (int u = 0; u < 4; u++) acc = M::run(a[u], bb[u], acc);
+                __syncthreads();
             }
+            const T *sh = sSh + slot * WC;
+            const int *act = sAct + slot * WC;
+            if constexpr (MF) {
+                const int nr = ns / 16, nk = ns / 4;
+                const bool vin = act[lr] != 0;
+                const T e0 = sh[lr];
+                // Z = V^T W, divided by the kernel; zero outside the block and the batch
+                for (int r = wave; r < nr; r += 4) {
+                    acc_t acc = acc_t{0, 0, 0, 0};
+                    for (int s = 0; s < nk; s += 4) {
+                        T a[4], bv[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int i = 16 * r + M::row(lk, q);
-                sZ[i + ldw * lr] = (vin && i < n1) ? px_kernel<T>(nosb, acc[q], eb[i], e0, de, beta) : (T)0;
-            }
-        }
-        __syncthreads();
-        // Y = V Z into sW
-        for (int r = wave; r < nr; r += 4) {
-            acc_t acc = acc_t{0, 0, 0, 0};
-            for (int s = 0; s < nk; s += 4) {
-                T a[4], bb[4];
+                        for (int u = 0; u < 4; u++) {
+                            const int k = 4 * (s + u) + lk;
+                            a[u] = sV[k + ldv * (16 * r + lr)];   // A(i,k) = V(k,i)
+                            bv[u] = sW[k + ldv * lr];
+                        }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int k = 4 * (s + u) + lk;
-                    a[u] = sV[(16 * r + lr) + ldv * k];   // A(i,k) = V(i,k)
-                    bb[u] = sZ[k + ldw * lr];
+                        for (int u = 0; u < 4; u++) acc = M::run(a[u], bv[u], acc);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int i = 16 * r + M::row(lk, q);
+                        sZ[i + ldv * lr] = (vin && i < n1) ? px_kernel<T>(nosb, acc[q], sE[i], e0, de, beta) : (T)0;
+                    }
                 }
+                __syncthreads();
+                // Y = V Z into sW
+                for (int r = wave; r < nr; r += 4) {
+                    acc_t acc = acc_t{0, 0, 0, 0};
+                    for (int s = 0; s < nk; s += 4) {
+                        T a[4], bv[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++) acc = M::run(a[u], bb[u], acc);
+                        for (int u = 0; u < 4; u++) {
+                            const int k = 4 * (s + u) + lk;
+                            a[u] = sV[(16 * r + lr) + ldv * k];   // A(i,k) = V(i,k)
+                            bv[u] = sZ[k + ldv * lr];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; u++) acc = M::run(a[u], bv[u], acc);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) sW[16 * r + M::row(lk, q) + ldv * lr] = acc[q];
+                }
+                __syncthreads();
+                for (int e = tid; e < n1 * WC; e += 256) {
+                    const int i = e % n1, c = e / n1;
+                    if (act[c]) st_cvt<T, SO>(y + (long)(v0 + c) * ldy + (long)bb * n1, i, sW[i + ldv * c]);
+                }
+            } else {
+                const int nv = min(WC, nvec - v0);
+                const int no = n1 * nv;   // outputs of a pass
+                // np threads per output (a power of two <= 8 with no * np <= 256), each
+                // summing a contiguous slice of j
+                int np = 1;
+                while (np < 8 && 2 * np * no <= 256) np *= 2;
+                const int sl = (n1 + np - 1) / np;
+                if (np == 1) {
+                    for (int e = tid; e < no; e += 256) {
+                        const int i = e % n1, c = e / n1;
+                        T acc = 0;
+#pragma unroll 8
+                        for (int j = 0; j < n1; j++) acc += sV[j + ldv * i] * sW[j + ldv * c];
+                        sZ[i + ldv * c] = px_kernel<T>(nosb, acc, sE[i], sh[c], de, beta);
+                    }
+                    __syncthreads();
+                    for (int e = tid; e < no; e += 256) {
+                        const int i = e % n1, c = e / n1;
+                        if (!act[c]) continue;
+                        T acc = 0;
+#pragma unroll 8
+                        for (int j = 0; j < n1; j++) acc += sV[i + ldv * j] * sZ[j + ldv * c];
+                        st_cvt<T, SO>(y + (long)(v0 + c) * ldy + (long)bb * n1, i, acc);
+                    }
+                } else {
+                    const int o = tid % no, q = tid / no;   // output, slice
+                    const int i = o % n1, c = o / n1;
+                    const int j0 = q * sl, j1 = min(n1, j0 + sl);
+                    if (q < np) {
+                        T acc = 0;
+#pragma unroll 8
+                        for (int j = j0; j < j1; j++) acc += sV[j + ldv * i] * sW[j + ldv * c];
+                        sP[o + no * q] = acc;
+                    }
+                    __syncthreads();
+                    if (tid < no) {
+                        T z = 0;
+                        for (int k = 0; k < np; k++) z += sP[tid + no * k];
+                        sZ[i + ldv * c] = px_kernel<T>(nosb, z, sE[i], sh[c], de, beta);
+                    }
+                    __syncthreads();
+                    if (q < np) {
+                        T acc = 0;
+#pragma unroll 8
+                        for (int j = j0; j < j1; j++) acc += sV[i + ldv * j] * sZ[j + ldv * c];
+                        sP[o + no * q] = acc;
+                    }
+                    __syncthreads();
+                    if (tid < no && act[c]) {
+                        T yv = 0;
+                        for (int k = 0; k < np; k++) yv += sP[tid + no * k];
+                        st_cvt<T, SO>(y + (long)(v0 + c) * ldy + (long)bb * n1, i, yv);
+                    }
+                }
             }
-#pragma unroll
-            for (int q = 0; q < 4; q++) sW[16 * r + M::row(lk, q) + ldw * lr] = acc[q];
         }
+    };
+    int b = blockIdx.x;
+    load(b);
+    for (; b < nblk; b += G) {
+        __syncthreads();   // the previous block is done with the LDS (and vec_info is visible)
+        stage(b);
         __syncthreads();
-        for (int e = tid; e < n1 * 16; e += 256) {
-            const int i = e % n1, c = e / n1, v = v0 + c;
-            if (v < nvec && vec_active(active, v)) st_cvt<T, SO>(y + (long)v * ldy + (long)b * n1, i, sW[i + ldw * c]);
-        }
+        load(b + G);       // in flight while this block is computed
+        compute(b);
     }
 }
 
@@ -270,16 +314,12 @@ static int pgrid(long n)
 
 static constexpr size_t PX_LDS_MAX = 160 * 1024;
 
+// LDS of k_px_l1 for level-1 blocks of n1 (MF: on MFMA)
 template <typename T>
-static size_t l1_mfma_lds(int n1)
+static size_t l1_lds(int n1, bool mf)
 {
-    const size_t n1p = rup16(n1);
-    return (n1p * (n1p + 1) + 2 * 16 * (n1p + 1)) * sizeof(T);
-}
-template <typename T>
-static size_t l1_valu_lds(int n1)
-{
-    return ((size_t)n1 * (n1 + 1) + 2 * (size_t)n1 * 8) * sizeof(T);
+    const size_t ns = mf ? rup16(n1) : n1, wc = mf ? 16 : 8;
+    return (ns * (ns + 1) + 2 * wc * (ns + 1) + 256 + ns + 2 * wc) * sizeof(T) + 2 * wc * sizeof(int);
 }
 
 // One rotation level, VTX (trans) or VX: MFMA when the block fits (n <= 128),
@@ -305,38 +345,46 @@ static void rotate(const Level &L, const T *vosb, const T *vosbt, bool trans, in
         launch_outer_mode<T, SI, SO, 4, false>(nin, n, nvec, mats, nullptr, x, ldx, y, ldy, active, s, blk);
 }
 
+template <typename T, typename SI, typename SO, bool MF, bool BIG>
+static void launch_l1(int n1, int nb, const T *vosb, const T *eig0, const T *shifts, T shift0, int nosb, T de, T beta,
+                      int nvec, const SI *x, long ldx, SO *y, long ldy, const int *active, hipStream_t s)
+{
+    // persistent: as many workgroups as are resident at once (per LDS size)
+    static size_t lds_seen = 0;
+    static int per_cu = 1;
+    const size_t lds = l1_lds<T>(n1, MF);
+    const void *fn = (const void *)k_px_l1<T, SI, SO, MF, BIG>;
+    if (lds != lds_seen) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)PX_LDS_MAX);
+        int nbk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, fn, 256, lds) != hipSuccess || nbk < 1) nbk = 1;
+        per_cu = nbk;
+        lds_seen = lds;
+    }
+    const int G = std::min(nb, g_num_cu * per_cu);
+    hipLaunchKernelGGL((k_px_l1<T, SI, SO, MF, BIG>), dim3(G), dim3(256), lds, s, n1, nb, nvec, vosb, eig0, shifts, shift0,
+                       nosb, de, beta, x, ldx, y, ldy, active);
+}
+
 // Level 1: MFMA for batches of 8 and more, VALU otherwise.
 template <typename T, typename SI, typename SO>
 static void level1(const Level &L, const T *vosb, const T *eig0, const T *shifts, T shift0, int nosb, T de, T beta,
                    int nvec, const SI *x, long ldx, SO *y, long ldy, const int *active, hipStream_t s)
 {
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_px_level1<T, SI, SO>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)PX_LDS_MAX);
-        (void)hipFuncSetAttribute((const void *)k_px_l1_mfma<T, SI, SO>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)PX_LDS_MAX);
-        attr = true;
-    }
     const int n1 = L.n, nb = L.blk;
-    if (nvec >= 8 && n1 <= 112 && l1_mfma_lds<T>(n1) <= PX_LDS_MAX)
-        hipLaunchKernelGGL((k_px_l1_mfma<T, SI, SO>), dim3(nb), dim3(256), l1_mfma_lds<T>(n1), s, n1, nvec, vosb, eig0,
-                           shifts, shift0, nosb, de, beta, x, ldx, y, ldy, active);
-    else {
-        // persistent: as many workgroups as are resident at once
-        static int per_cu = 0;
-        if (per_cu == 0) {
-            int nbk = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, k_px_level1<T, SI, SO>, 256, l1_valu_lds<T>(n1)) !=
-                    hipSuccess ||
-                nbk < 1)
-                nbk = 1;
-            per_cu = nbk;
-        }
-        const int G = std::min(nb, g_num_cu * per_cu);
-        hipLaunchKernelGGL((k_px_level1<T, SI, SO>), dim3(G), dim3(256), l1_valu_lds<T>(n1), s, n1, nb, nvec, vosb,
-                           eig0, shifts, shift0, nosb, de, beta, x, ldx, y, ldy, active);
-    }
+    const bool mf = nvec >= 8 && n1 <= 112 && l1_lds<T>(n1, true) <= PX_LDS_MAX;
+    if (mf && n1 <= 64)
+        launch_l1<T, SI, SO, true, false>(n1, nb, vosb, eig0, shifts, shift0, nosb, de, beta, nvec, x, ldx, y, ldy,
+                                          active, s);
+    else if (mf)
+        launch_l1<T, SI, SO, true, true>(n1, nb, vosb, eig0, shifts, shift0, nosb, de, beta, nvec, x, ldx, y, ldy,
+                                         active, s);
+    else if (n1 <= 64)
+        launch_l1<T, SI, SO, false, false>(n1, nb, vosb, eig0, shifts, shift0, nosb, de, beta, nvec, x, ldx, y, ldy,
+                                           active, s);
+    else
+        launch_l1<T, SI, SO, false, true>(n1, nb, vosb, eig0, shifts, shift0, nosb, de, beta, nvec, x, ldx, y, ldy,
+                                          active, s);
 }
 
 template <typename T, typename S>
@@ -346,7 +394,7 @@ static int px_impl(scalit_ctx *c, int nosb, const T *const *vosb, const T *const
 {
     const long N = c->N;
     const int sF = c->sF;
-    if (l1_valu_lds<T>(c->lv[0].n) > PX_LDS_MAX) return SCALIT_EINVAL;   // level-1 block too large for LDS
+    if (c->lv[0].n > 256 || l1_lds<T>(c->lv[0].n, false) > PX_LDS_MAX) return SCALIT_EINVAL;   // level-1 block too large for LDS
     if (sF == 1) {
         // a single level: input and output are the caller's vectors
         level1<T, S, S>(c->lv[0], vosb[0], eig0, shifts, shift0, nosb, de, beta, nvec, x, ldx, y, ldy, active, s);

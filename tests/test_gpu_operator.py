@@ -143,6 +143,33 @@ def test_px_with_oracle_preconditioner(grid):
     ctx.close()
 
 
+@pytest.mark.parametrize("grid,nv", [((24, 32, 24), 19), ((17, 33, 19), 8), ((64, 96, 80), 16)])
+def test_px_batch_per_vector_shifts(grid, nv):
+    """Batches of 8 and more take the MFMA level-1 path (16 vectors per chunk;
+    19 = one full chunk and a partial one); each vector has its own E0."""
+    print(f"px batch grid={grid} nv={nv}", flush=True)
+    prob, op, ctx = setup(grid, e0=-0.02)
+    if prob.N < 100_000:
+        vosb, eig0 = op.diag(bjmax=200, bjtol=1e-5)
+        ctx.set_precond(vosb, eig0)
+    else:
+        ctx.block_diag(200, 1e-5)
+        vosb, eig0 = ctx.get_precond()
+        op.set_precond(vosb, eig0)
+    x = P.random_vectors(prob.N, nv, 12)
+    shifts = -0.02 + 0.003 * np.arange(nv)
+    xd = ctx.to_device(x)
+    for kind in (O.TOSB, O.TOSBD2):
+        op.s.DE = 0.004
+        op.s.beta = 0.3
+        ctx.set_shift(-0.02, 0.004, 0.3)
+        y = ctx.px(xd, nosb=kind, shifts=shifts).cpu().numpy()
+        for b in range(nv):
+            op.s.E0 = shifts[b]
+            assert rel(y[b], op.px(x[b], kind=kind)) <= 1e-12, (kind, b)
+    ctx.close()
+
+
 @pytest.mark.parametrize("grid", [(24, 32, 24), (12, 10, 8), (32, 40, 24)])
 def test_block_diag_matches_oracle(grid):
     print(f"block_diag grid={grid}", flush=True)
