@@ -55,6 +55,20 @@ __device__ __forceinline__ void st_cvt<float, bf16_t>(bf16_t *p, long i, float v
 
 __device__ __forceinline__ bool vec_active(const int *active, int v) { return active == nullptr || active[v] != 0; }
 
+// Targets of an address select that makes an optional load unconditional:
+// an activity flag for "every vector active" and the unit factor.
+static __device__ const int k_all_on = 1;
+static __device__ const double k_unit_d = 1.0;
+static __device__ const float k_unit_f = 1.0f;
+template <typename T>
+__device__ __forceinline__ const T *unit_ptr()
+{
+    if constexpr (sizeof(T) == sizeof(double))
+        return (const T *)&k_unit_d;
+    else
+        return (const T *)&k_unit_f;
+}
+
 // Raw buffer access: a wave-uniform resource (base, size in bytes) plus a
 // 32-bit per-lane byte offset and a wave-uniform byte offset, so a column of
 // loads at a uniform stride needs one VGPR of address in all.  Callers keep
@@ -81,6 +95,10 @@ template <>
 __device__ __forceinline__ float buf_ld<float, bf16_t>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
 {
     return bf16_to_f32((bf16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+}
+__device__ __forceinline__ int buf_ld_i32(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
 }
 template <typename T, typename S>
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, int voff, int soff, T v);

@@ -45,37 +45,44 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
     const long wstride = (long)gridDim.x * WAVES;
     long t = (long)blockIdx.x * WAVES + wave;
 
-    // X and d of the current and (PF) the next tile; Y (when adding) of the
-    // current tile, loaded at the top of its iteration and used after its
-    // MFMAs.  Without PF (12-wave workgroups, whose registers do not hold two
-    // X columns) X also loads at the top of its own tile, and the other waves
-    // of the SIMD cover the wait.
+    // X, d and the vector's activity of the current and (PF) the next tile;
+    // Y (when adding) of the current tile, loaded at the top of its iteration
+    // and used after its MFMAs.  Without PF (12-wave workgroups, whose
+    // registers do not hold two X columns) X also loads at the top of its own
+    // tile, and the other waves of the SIMD cover the wait.
+    //
+    // EX (n == 16 NR): raw buffer loads, one resource per tile, one lane
+    // offset for all of a column (lanes past the last point read point P-1),
+    // and every load and store is issued on every path: the prefetch of a
+    // tile past the last reads the last tile again (never used), d and the
+    // activity are buffer loads from the caller's arrays or from a constant
+    // (no flat access, which would also count against the LDS counter), and
+    // a lane that must not store gets an offset past the resource, which the
+    // hardware drops.  The wait for one tile's X then counts exactly the
+    // memory operations issued after it and never drains the prefetch.
+    // Otherwise predicated accesses.
     constexpr bool PF = WAVES < 12;
     T bc[KS], dc, bn[PF ? KS : 1], dn;
+    int ac = 0, an = 0;
     T yc[ACC ? NR : 1][4];
-    // tile tt: vector v and first point p0 (wave-uniform); lane point p0 + lr
-    auto tile_of = [&](long tt, int &v, long &p0) {
-        v = (int)(tt / tpv);
-        p0 = (tt % tpv) * 16;
-        return p0 + lr < P && vec_active(active, v);
-    };
-    // EX (n == 16 NR): raw buffer loads, one resource per tile, one lane
-    // offset for all of a column (lanes past the last point read point P-1;
-    // their results are not stored).  Otherwise predicated loads.
     const int ylane = M::row(lk, 0);
-    auto load_x = [&](long tt, T(&b)[KS], T &d) {
-        int v;
-        long p0;
-        const bool pin = tile_of(tt, v, p0);
+    const auto rd = dvec ? buf_rsrc(dvec, P * (long)sizeof(T)) : buf_rsrc(unit_ptr<T>(), sizeof(T));
+    const auto ra = active ? buf_rsrc(active, (long)nvec * sizeof(int)) : buf_rsrc(&k_all_on, sizeof(int));
+    auto load_x = [&](long tt, T(&b)[KS], T &d, int &a) {
+        tt = tt < ntile ? tt : ntile - 1;
+        const int v = (int)(tt / tpv);
+        const long p0 = (tt % tpv) * 16;
+        const long pc = p0 + lr < P ? p0 + lr : P - 1;
+        a = buf_ld_i32(ra, 0, active ? v * (int)sizeof(int) : 0);
+        d = buf_ld<T, T>(rd, dvec ? (int)(pc * (long)sizeof(T)) : 0, 0);
         const S *xs = x + (long)v * ldx + p0;
-        d = pin ? (dvec ? dvec[p0 + lr] : (T)1) : (T)0;
         if constexpr (EX) {
-            const long pl = p0 + lr < P ? lr : P - 1 - p0;
             const auto rs = buf_rsrc(xs, (P * n - p0) * (long)sizeof(S));
-            const int voff = (int)((pl + P * lk) * (long)sizeof(S));
+            const int voff = (int)((pc - p0 + P * lk) * (long)sizeof(S));
 #pragma unroll
             for (int s = 0; s < KS; s++) b[s] = buf_ld<T, S>(rs, voff, (int)(P * 4 * s * (long)sizeof(S)));
         } else {
+            const bool pin = p0 + lr < P && a != 0;
 #pragma unroll
             for (int s = 0; s < KS; s++) {
                 const int k = 4 * s + lk;
@@ -83,7 +90,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
             }
         }
     };
-    if (PF && t < ntile) load_x(t, bc, dc);
+    if (PF) load_x(t, bc, dc, ac);
 
     if (np != n)
         for (int e = tid; e < np * np; e += 64 * WAVES)
@@ -93,9 +100,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
 
     while (t < ntile) {
         const long tn = t + wstride;
-        int v;
-        long p0;
-        const bool pin = tile_of(t, v, p0);
+        const int v = (int)(t / tpv);
+        const long p0 = (t % tpv) * 16;
         SO *ys = y + (long)v * ldy + p0;
         const auto ry = buf_rsrc(ys, (P * n - p0) * (long)sizeof(SO));
         const long plane = p0 + lr < P ? lr : P - 1 - p0;
@@ -110,14 +116,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
                         yc[r][q] = buf_ld<T, SO>(ry, yvoff,
                                                  (int)(P * (16 * r + M::row(0, q)) * (long)sizeof(SO)));
                     else
-                        yc[r][q] = (pin && i < n) ? ld_cvt<T, SO>(ys, lr + P * i) : (T)0;
+                        yc[r][q] = (p0 + lr < P && i < n) ? ld_cvt<T, SO>(ys, lr + P * i) : (T)0;
                 }
         }
-        if constexpr (PF) {
-            if (tn < ntile) load_x(tn, bn, dn);
-        } else {
-            load_x(t, bc, dc);
-        }
+        if constexpr (PF) load_x(tn, bn, dn, an);
+        else load_x(t, bc, dc, ac);
         acc_t acc[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) acc[r] = acc_t{0, 0, 0, 0};
@@ -134,7 +137,18 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
                 for (int r = 0; r < NR; r++) acc[r] = M::run(sA[abase + 16 * r + 4 * np * s], bc[s], acc[r]);
             __builtin_amdgcn_sched_barrier(0);
         }
-        if (pin) {
+        const bool pin = p0 + lr < P && ac != 0;
+        if constexpr (EX) {
+            const int so = pin ? yvoff : (int)0x80000000;   // past the resource: dropped
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    T out = dc * acc[r][q];
+                    if constexpr (ACC) out += yc[r][q];
+                    buf_st<T, SO>(ry, so, (int)(P * (16 * r + M::row(0, q)) * (long)sizeof(SO)), out);
+                }
+        } else if (pin) {
 #pragma unroll
             for (int r = 0; r < NR; r++)
 #pragma unroll
@@ -142,16 +156,14 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
                     const int i = 16 * r + M::row(lk, q);
                     T out = dc * acc[r][q];
                     if constexpr (ACC) out += yc[r][q];
-                    if constexpr (EX)
-                        buf_st<T, SO>(ry, yvoff, (int)(P * (16 * r + M::row(0, q)) * (long)sizeof(SO)), out);
-                    else if (i < n)
-                        st_cvt<T, SO>(ys, lr + P * i, out);
+                    if (i < n) st_cvt<T, SO>(ys, lr + P * i, out);
                 }
         }
         if constexpr (PF) {
 #pragma unroll
             for (int s = 0; s < KS; s++) bc[s] = bn[s];
             dc = dn;
+            ac = an;
         }
         t = tn;
     }
