@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
                                                           const T *__restrict__ t2, const T *__restrict__ res,
                                                           const T *__restrict__ shifts, T shift0,
                                                           const S *__restrict__ x, long ldx, S *__restrict__ y,
-                                                          long ldy, const int *active)
+                                                          long ldy, const int *__restrict__ active)
 {
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
@@ -268,47 +268,56 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
     // first slab of this workgroup that is active
     int k = 0, s = slab_of(g, 0, G, nslab);
     while (s >= 0 && !vec_active(active, s % nvec)) s = slab_of(g, ++k, G, nslab);
+    // The prefetches load unconditionally (indices clamped into the slab; the
+    // values outside it are replaced by zeros when staged, or never stored), so
+    // that the waits for them count exactly and never serialize the loads.
     T pf[PFMAX];
+    // (sl < 0, no next slab: one value of slab 0 on every lane)
+    // (the thread index goes through an opaque copy so that the per-value
+    // addresses are recomputed per slab rather than held in registers)
     auto prefetch = [&](int sl) {
-        const S *xs = x + (long)(sl % nvec) * ldx + P * (sl / nvec);
+        const bool live = sl >= 0;
+        const S *xs = live ? x + (long)(sl % nvec) * ldx + P * (sl / nvec) : x;
+        int t0 = tid;
+        asm volatile("" : "+v"(t0));
 #pragma unroll
         for (int j = 0; j < PFMAX; j++) {
-            const int e = tid + j * NT;
-            if (j < npf) {
-                const int i1 = e % n1p, i2 = e / n1p;
-                pf[j] = (i1 < n1 && i2 < n2 && e < n1p * n2p) ? ld_cvt<T, S>(xs, i1 + (long)n1 * i2) : (T)0;
-            }
+            const int e = t0 + j * NT;
+            const int i1 = live ? min(e % n1p, n1 - 1) : 0, i2 = live ? min(e / n1p, n2 - 1) : 0;
+            pf[j] = ld_cvt<T, S>(xs, i1 + (long)n1 * i2);
         }
     };
     // V of the wave's first output block (block index = wave)
     T pv[RB][CB][4];
     auto prefetch_v = [&](int sl) {
-        const T *rsn = res + P * (sl / nvec);
-        const int rg = wave / ncg, cg = wave % ncg;
-        const bool own = wave < nrg * ncg;
+        const bool live = sl >= 0;
+        const T *rsn = res + (live ? P * (sl / nvec) : 0);
+        const int ob = min(wave, nrg * ncg - 1);
+        const int rg = ob / ncg, cg = ob % ncg;
 #pragma unroll
         for (int r = 0; r < RB; r++)
 #pragma unroll
             for (int c = 0; c < CB; c++) {
-                const int i1 = (cg * CB + c) * 16 + lr;
+                const int i1 = live ? min((cg * CB + c) * 16 + lr, n1 - 1) : 0;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const int i2 = (rg * RB + r) * 16 + M::row(lk, q);
-                    pv[r][c][q] = (own && i1 < n1 && i2 < n2) ? rsn[i1 + (long)n1 * i2] : (T)0;
+                    const int i2 = live ? min((rg * RB + r) * 16 + M::row(lk, q), n2 - 1) : 0;
+                    pv[r][c][q] = rsn[i1 + (long)n1 * i2];
                 }
             }
     };
-    if (s >= 0) {
-        prefetch(s);
-        prefetch_v(s);
-    }
+    prefetch(s);
+    prefetch_v(s);
 
     while (s >= 0) {
         // the prefetched slab into LDS (padding zero)
+        int t0 = tid;
+        asm volatile("" : "+v"(t0));
 #pragma unroll
         for (int j = 0; j < PFMAX; j++) {
-            const int e = tid + j * NT;
-            if (j < npf && e < n1p * n2p) sS[(e % n1p) + ldS * (e / n1p)] = pf[j];
+            const int e = t0 + j * NT;
+            const int i1 = e % n1p, i2 = e / n1p;
+            if (j < npf && e < n1p * n2p) sS[i1 + ldS * i2] = (i1 < n1 && i2 < n2) ? pf[j] : (T)0;
         }
         if (npf > PFMAX) {
             const S *xs = x + (long)(s % nvec) * ldx + P * (s / nvec);
@@ -321,11 +330,13 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
         const int v = s % nvec, i3 = s / nvec;
         int sn = slab_of(g, ++k, G, nslab);
         while (sn >= 0 && !vec_active(active, sn % nvec)) sn = slab_of(g, ++k, G, nslab);
-        if (sn >= 0) prefetch(sn);
+        prefetch(sn);
         S *ys = y + (long)v * ldy + P * i3;
+        const auto ry = buf_rsrc(ys, P * (long)sizeof(S));
         const T *rs = res + P * i3;
         const T sh = shifts ? shifts[v] : shift0;
-        for (int ob = wave; ob < nrg * ncg; ob += WAVES) {
+        // block ob of RB x CB output tiles, V already in pv
+        auto block = [&](int ob, bool own) {
             const int rg = ob / ncg, cg = ob % ncg;
             acc_t acc[RB][CB];
 #pragma unroll
@@ -383,8 +394,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
 #pragma unroll
                         for (int c = 0; c < CB; c++) acc[r][c] = M::run(a[u][r], b[u][c], acc[r][c]);
             }
-            // epilogue: + (V - s) S, store
-            const bool first = ob == wave;
+            // epilogue: + (V - s) S, store (a lane outside the slab, or a wave
+            // past the last block, stores to an offset past the resource,
+            // which the hardware drops: the store count is the same on every path)
 #pragma unroll
             for (int r = 0; r < RB; r++)
 #pragma unroll
@@ -393,16 +405,34 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
                         const int i2 = (rg * RB + r) * 16 + M::row(lk, q);
-                        if (i1 < n1 && i2 < n2) {
-                            const long o = i1 + (long)n1 * i2;
-                            const T sv = sS[i1 + ldS * i2];
-                            const T vv = first ? pv[r][c][q] : rs[o];
-                            st_cvt<T, S>(ys, o, acc[r][c][q] + (vv - sh) * sv);
-                        }
+                        const bool in = own && i1 < n1 && i2 < n2;
+                        const long o = i1 + (long)n1 * i2;
+                        const T sv = sS[min(i1, n1p - 1) + ldS * min(i2, n2p - 1)];
+                        buf_st<T, S>(ry, in ? (int)(o * (long)sizeof(S)) : (int)0x80000000, 0,
+                                     acc[r][c][q] + (pv[r][c][q] - sh) * sv);
                     }
                 }
+        };
+        // the first block of every wave with the prefetched V (a wave past the
+        // last block computes the last one and stores nothing), then the rest,
+        // each loading its V first
+        block(min(wave, nrg * ncg - 1), wave < nrg * ncg);
+        for (int ob = wave + WAVES; ob < nrg * ncg; ob += WAVES) {
+            const int rg = ob / ncg, cg = ob % ncg;
+#pragma unroll
+            for (int r = 0; r < RB; r++)
+#pragma unroll
+                for (int c = 0; c < CB; c++) {
+                    const int i1 = min((cg * CB + c) * 16 + lr, n1 - 1);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int i2 = min((rg * RB + r) * 16 + M::row(lk, q), n2 - 1);
+                        pv[r][c][q] = rs[i1 + (long)n1 * i2];
+                    }
+                }
+            block(ob, true);
         }
-        if (sn >= 0) prefetch_v(sn);
+        prefetch_v(sn);
         __syncthreads();   // every wave is done with sS before the next slab lands
         s = sn;
     }
