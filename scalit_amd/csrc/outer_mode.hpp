@@ -82,7 +82,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_outer_mode(long P, int n, int nv
 #pragma unroll
             for (int s = 0; s < KS; s++) b[s] = buf_ld<T, S>(rs, voff, (int)(P * 4 * s * (long)sizeof(S)));
         } else {
-            const bool pin = p0 + lr < P && a != 0;
+            // predicated on the point only: the activity is applied at the store,
+            // so the X loads do not wait for the flag
+            const bool pin = p0 + lr < P;
 #pragma unroll
             for (int s = 0; s < KS; s++) {
                 const int k = 4 * s + lk;
