@@ -282,21 +282,35 @@ def pist_bench(no_cpu):
     # (one QMR launch per iteration for all windows); window centres are the
     # means of consecutive pairs of the reference levels, and every reference
     # level must be found by some window
+    # Lock-step windows wait for the slowest one, whose Krylov count moves with
+    # rounding-level differences in the inner solves, so the batched figure is
+    # the mean over several sets of start vectors (each timed after a warm-up run
+    # that records the batched QMR graph).
     centres = [float(ref[2 * k:2 * k + 2].mean()) for k in range(10)]
+    seeds = [g["start_seed"] + i for i in range(4)]
     for name, prec in (("batched_fp64", L.FP64), ("batched_fp32", L.FP32)):
-        x0b = ctx.to_device(P.random_vectors(prob.N, 10, g["start_seed"]))
-        for rep in range(2):   # the first run records the batched QMR graph
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            ret, eig, rr, stats = ctx.pist_conv_batched(x0b, centres, 1e-11, 2, 200, 12, 4, 1, qmrtol=1e-6, prec=prec)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-        found = np.sort(eig.reshape(-1))
-        err = max(float(np.min(np.abs(found - r)) / abs(r)) for r in ref)
-        res[name] = {"windows": 10, "levels_per_window": 2, "seconds": round(dt, 4),
-                     "eigenpairs_per_s": round(20 / dt, 3), "max_rel_err": err, "qmr_iters": int(stats[0]),
-                     "speedup_vs_fp64": round((20 / dt) / res["fp64"]["eigenpairs_per_s"], 3),
-                     "converged": bool(all(int(v) > 0 for v in ret))}
+        times, errs, iters, conv, krylov = [], [], [], True, []
+        for i, seed in enumerate(seeds):
+            x0b = ctx.to_device(P.random_vectors(prob.N, 10, seed))
+            for rep in range(2 if i == 0 else 1):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ret, eig, rr, stats = ctx.pist_conv_batched(x0b, centres, 1e-11, 2, 200, 12, 4, 1, qmrtol=1e-6,
+                                                            prec=prec)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+            found = np.sort(eig.reshape(-1))
+            errs.append(max(float(np.min(np.abs(found - r)) / abs(r)) for r in ref))
+            times.append(dt)
+            iters.append(int(stats[0]))
+            krylov.append(int(stats[2]))
+            conv = conv and all(int(v) > 0 for v in ret)
+        dt = float(np.mean(times))
+        res[name] = {"windows": 10, "levels_per_window": 2, "start_seeds": seeds,
+                     "seconds_per_seed": [round(t, 4) for t in times], "seconds": round(dt, 4),
+                     "eigenpairs_per_s": round(20 / dt, 3), "max_rel_err": max(errs), "qmr_iters": iters,
+                     "krylov": krylov, "speedup_vs_fp64": round((20 / dt) / res["fp64"]["eigenpairs_per_s"], 3),
+                     "converged": bool(conv)}
     if not no_cpu:
         from oracle import oracle as O
 

@@ -489,7 +489,10 @@ int mgs_host(scalit_ctx *c, long n, const double *wj, int m, double *vj, int *ok
 // refinement x += d, r = b - (H - E0) x, until the reference's type-1
 // criterion ||(H-E0)x - b|| / ||(H-E0)x|| < tol (or ||r||/||b|| for other
 // types) holds or outer_max refinements are spent.  ret = total inner
-// iterations (negated when not converged).
+// iterations (negated when not converged).  The inner solves stop on the
+// recurrence residual (type 0): a true residual formed in fp32 stalls near
+// eps32 * cond, far above the inner tolerance for the near-singular shifts
+// PIST uses, while the outer fp64 residual decides convergence anyway.
 int qmr_mixed(scalit_ctx *c, int prec, int ntype, int qmrmax, double tol, int inner_max, double inner_tol, int nvec,
               const double *b, long ldb, double *x, long ldx, const double *shifts, int *ret, double *res,
               hipStream_t s)
@@ -523,12 +526,12 @@ int qmr_mixed(scalit_ctx *c, int prec, int ntype, int qmrmax, double tol, int in
         if (prec == SCALIT_BF16) {
             SC_TRY(P.q16.ensure(nvec, N));
             launch_cast_d2b(nvec, N, P.r64.p, N, P.q16.B(), P.q16.ld, s);
-            SC_TRY(qmr_solve<bf16_t>(c, P.q16, -1, inner_max, inner_tol, sh.data(), rin.data(), nullptr, s));
+            SC_TRY(qmr_solve<bf16_t>(c, P.q16, 0, inner_max, inner_tol, sh.data(), rin.data(), nullptr, s));
             launch_cast_b2d(nvec, N, P.q16.X(), P.q16.ld, P.tmp.p, N, s);
         } else {
             SC_TRY(P.q32.ensure(nvec, N));
             launch_cast_d2f(nvec, N, P.r64.p, N, P.q32.B(), P.q32.ld, s);
-            SC_TRY(qmr_solve<float>(c, P.q32, -1, inner_max, inner_tol, sh.data(), rin.data(), nullptr, s));
+            SC_TRY(qmr_solve<float>(c, P.q32, 0, inner_max, inner_tol, sh.data(), rin.data(), nullptr, s));
             launch_cast_f2d(nvec, N, P.q32.X(), P.q32.ld, P.tmp.p, N, s);
         }
         for (int v = 0; v < nvec; v++) {
