@@ -11,11 +11,12 @@
 //
 // Level 1 (NDK = 1): one wave per block runs the whole Jacobi in LDS.
 // Levels with NDK > 1: one launch per rotation; each workgroup owns a slice of
-// the inner points d, reduces the A/B/C sums of its pair from the per-slice
-// partials written by the previous rotation (every workgroup in the same fixed
-// order, so all agree on the angle), applies the rotation to its slice, and
-// writes the partial sums of the next pair.  A sweep of launches is recorded
-// once in a HIP graph and replayed.
+// 64 inner points d (its 4 waves split the rotated rows and columns), reduces
+// the A/B/C sums of its pair from the per-slice partials written by the
+// previous rotation (every workgroup in the same fixed order, so all agree on
+// the angle), applies the rotation to its slice, and writes the partial sums
+// of the next pair.  A sweep of launches is recorded once in a HIP graph and
+// replayed.
 #include "scalit_impl.hpp"
 
 #include <cmath>
@@ -164,51 +165,49 @@ __device__ __forceinline__ double block_sum_256(double v, double *red)
     return r;
 }
 
-// Partial A/B/C sums of pair (p,q) over each 256-point slice of d.
-__global__ void __launch_bounds__(256) k_jpart(int ndk, int mak, int p, int q, const double *__restrict__ hosb,
-                                               JPart *__restrict__ part, const int *active)
+// The rotation kernels give a workgroup 64 values of d (JD) and split the
+// other index over its 4 waves; a pair's A/B/C partials are one per 64-point
+// slice of d (nrc = ceil(ndk / 64) per block).
+constexpr int JD = 64;
+
+// sum over the workgroup of three values, a fixed tree (deterministic)
+__device__ __forceinline__ void block_sum3_256(double &a, double &b, double &c, double *red)
 {
-    __shared__ double red[256];
-    const int b = blockIdx.y;
-    if (!active[b]) return;
-    const int d = blockIdx.x * 256 + threadIdx.x;
-    const double *hb = hosb + (long)ndk * mak * mak * b;
-    double odg = 0, ddg = 0;
-    if (d < ndk) {
-        odg = hb[d + (long)ndk * (p + (long)mak * q)];
-        ddg = 0.5 * hb[d + (long)ndk * (p + (long)mak * p)] - 0.5 * hb[d + (long)ndk * (q + (long)mak * q)];
+    const int tid = threadIdx.x;
+    red[tid] = a;
+    red[256 + tid] = b;
+    red[512 + tid] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[tid] += red[tid + s];
+            red[256 + tid] += red[256 + tid + s];
+            red[512 + tid] += red[512 + tid + s];
+        }
+        __syncthreads();
     }
-    double a = block_sum_256(odg * odg, red);
-    double bb = block_sum_256(odg * ddg, red);
-    double c = block_sum_256(ddg * ddg, red);
-    if (threadIdx.x == 0) part[(long)b * gridDim.x + blockIdx.x] = {a, bb, c};
+    a = red[0];
+    b = red[256];
+    c = red[512];
+    __syncthreads();
 }
 
-// One rotation (p,q) of every active block, then the partials of (pn,qn).
-__global__ void __launch_bounds__(256) k_jrot(int ndk, int mak, int nch, int p, int q, int pn, int qn,
-                                              double *__restrict__ hosb, double *__restrict__ vosb,
-                                              const JPart *__restrict__ part_in, JPart *__restrict__ part_out,
-                                              const int *active)
+// Rotation (p,q) of the 64-point slice of d at d0 of block hb (mak x mak per
+// d) and of the block's vectors V (do_v).  Wave w rotates rows/columns
+// r = w, w+4, ... in groups of RG whose loads are all issued before any of
+// their stores (the row and column writes cannot alias the column reads of the
+// same group, but the compiler cannot know that, and would otherwise make
+// every element a full memory round trip).
+__device__ __forceinline__ void jrot_slice(double *hb, double *V, int ndk, int mak, int d0, int p, int q, double CS,
+                                           double SN, bool do_v)
 {
-    __shared__ double red[256];
-    const int b = blockIdx.y;
-    if (!active[b]) return;
-    double A = 0, B = 0, C = 0;
-    const JPart *pb = part_in + (long)b * nch;
-    for (int i = 0; i < nch; i++) {
-        A += pb[i].a;
-        B += pb[i].b;
-        C += pb[i].c;
-    }
-    B = 2.0 * B;
-    const double X = jacobi_angle(A, B, C);
-    const double SN = sin(X), CS = cos(X);
+    constexpr int RG = 8;
     const double C2 = CS * CS, S2 = SN * SN, SC = SN * CS;
-    const long m2 = (long)mak * mak;
-    double *hb = hosb + (long)ndk * m2 * b;
-    const int d = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & (JD - 1), w = threadIdx.x / JD;
+    const int d = d0 + lane;
+    const bool din = d < ndk;
 #define HO(r, c) hb[d + (long)ndk * ((r) + (long)mak * (c))]
-    if (d < ndk) {
+    if (din && w == 0) {
         const double hpp = HO(p, p), hqq = HO(q, q), odg = HO(p, q);
         const double ddg = 0.5 * hpp - 0.5 * hqq;
         HO(p, p) = C2 * hpp + S2 * hqq - 2 * SC * odg;
@@ -216,37 +215,116 @@ __global__ void __launch_bounds__(256) k_jrot(int ndk, int mak, int nch, int p, 
         const double nqp = (C2 - S2) * odg + 2 * SC * ddg;
         HO(p, q) = nqp;
         HO(q, p) = nqp;
-        for (int r = 0; r < mak; r++) {
-            if (r == p || r == q) continue;
-            const double hrp = HO(r, p), hrq = HO(r, q);
-            const double np = CS * hrp - SN * hrq;
-            const double nq = CS * hrq + SN * hrp;
-            HO(r, p) = np;
-            HO(p, r) = np;
-            HO(r, q) = nq;
-            HO(q, r) = nq;
-        }
     }
-    if (blockIdx.x == 0) {
-        double *V = vosb + m2 * b;
+    if (din)
+        for (int r0 = w; r0 < mak; r0 += 4 * RG) {
+            double hp[RG], hq[RG];
+#pragma unroll
+            for (int k = 0; k < RG; k++) {
+                const int r = r0 + 4 * k;
+                const int rc = r < mak ? r : p;
+                hp[k] = HO(rc, p);
+                hq[k] = HO(rc, q);
+            }
+#pragma unroll
+            for (int k = 0; k < RG; k++) {
+                const int r = r0 + 4 * k;
+                if (r >= mak || r == p || r == q) continue;
+                const double np = CS * hp[k] - SN * hq[k];
+                const double nq = CS * hq[k] + SN * hp[k];
+                HO(r, p) = np;
+                HO(p, r) = np;
+                HO(r, q) = nq;
+                HO(q, r) = nq;
+            }
+        }
+#undef HO
+    if (do_v)
         for (int r = threadIdx.x; r < mak; r += 256) {
             const double vp = V[r + (long)mak * p], vq = V[r + (long)mak * q];
             V[r + (long)mak * p] = CS * vp - SN * vq;
             V[r + (long)mak * q] = CS * vq + SN * vp;
         }
-    }
-    // partial sums of the next pair from this slice's updated values
+}
+
+// A/B/C sums of pair (p,q) over the slice of d at d0 (wave 0 reads, the whole
+// workgroup reduces)
+__device__ __forceinline__ void jpair_sums(const double *hb, int ndk, int mak, int d0, int p, int q, double &a,
+                                           double &bb, double &c, double *red)
+{
+    const int d = d0 + threadIdx.x;
     double odg = 0, ddg = 0;
-    if (d < ndk && pn >= 0) {
-        odg = HO(pn, qn);
-        ddg = 0.5 * HO(pn, pn) - 0.5 * HO(qn, qn);
+    if (threadIdx.x < JD && d < ndk) {
+        odg = hb[d + (long)ndk * (p + (long)mak * q)];
+        ddg = 0.5 * hb[d + (long)ndk * (p + (long)mak * p)] - 0.5 * hb[d + (long)ndk * (q + (long)mak * q)];
     }
-#undef HO
+    a = odg * odg;
+    bb = odg * ddg;
+    c = ddg * ddg;
+    block_sum3_256(a, bb, c, red);
+}
+
+// Partial A/B/C sums of pair (p,q) over each 64-point slice of d.
+__global__ void __launch_bounds__(256) k_jpart(int ndk, int mak, int p, int q, const double *__restrict__ hosb,
+                                               JPart *__restrict__ part, const int *active)
+{
+    __shared__ double red[3 * 256];
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    double a, bb, c;
+    jpair_sums(hosb + (long)ndk * mak * mak * b, ndk, mak, blockIdx.x * JD, p, q, a, bb, c, red);
+    if (threadIdx.x == 0) part[(long)b * gridDim.x + blockIdx.x] = {a, bb, c};
+}
+
+// One rotation (p,q) of every active block, then the partials of (pn,qn).
+__global__ void __launch_bounds__(256) k_jrot(int ndk, int mak, int nrc, int p, int q, int pn, int qn,
+                                              double *__restrict__ hosb, double *__restrict__ vosb,
+                                              const JPart *__restrict__ part_in, JPart *__restrict__ part_out,
+                                              const int *active)
+{
+    __shared__ double red[3 * 256];
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    double A = 0, B = 0, C = 0;
+    const JPart *pb = part_in + (long)b * nrc;
+    for (int i = 0; i < nrc; i++) {
+        A += pb[i].a;
+        B += pb[i].b;
+        C += pb[i].c;
+    }
+    B = 2.0 * B;
+    const double X = jacobi_angle(A, B, C);
+    const long m2 = (long)mak * mak;
+    double *hb = hosb + (long)ndk * m2 * b;
+    jrot_slice(hb, vosb + m2 * b, ndk, mak, blockIdx.x * JD, p, q, cos(X), sin(X), blockIdx.x == 0);
     if (pn < 0) return;
-    double a = block_sum_256(odg * odg, red);
-    double bb = block_sum_256(odg * ddg, red);
-    double c = block_sum_256(ddg * ddg, red);
-    if (threadIdx.x == 0) part_out[(long)b * nch + blockIdx.x] = {a, bb, c};
+    __syncthreads();   // every wave's updates of this slice are in memory
+    double a, bb, c;
+    jpair_sums(hb, ndk, mak, blockIdx.x * JD, pn, qn, a, bb, c, red);
+    if (threadIdx.x == 0) part_out[(long)b * nrc + blockIdx.x] = {a, bb, c};
+}
+
+// A whole sweep for levels with ndk <= 64 (one slice per block): one
+// workgroup per active block applies every rotation of the pair sequence in
+// turn, the sums of each pair reduced in the workgroup.
+__global__ void __launch_bounds__(256) k_jsweep(int ndk, int mak, int npair, const int *__restrict__ pq,
+                                                double *__restrict__ hosb, double *__restrict__ vosb,
+                                                const int *active)
+{
+    __shared__ double red[3 * 256];
+    const int b = blockIdx.x;
+    if (!active[b]) return;
+    const long m2 = (long)mak * mak;
+    double *hb = hosb + (long)ndk * m2 * b;
+    double *V = vosb + m2 * b;
+    for (int r = 0; r < npair; r++) {
+        const int p = pq[2 * r], q = pq[2 * r + 1];
+        double a, bb, c;
+        jpair_sums(hb, ndk, mak, 0, p, q, a, bb, c, red);
+        const double X = jacobi_angle(a, 2.0 * bb, c);
+        jrot_slice(hb, V, ndk, mak, 0, p, q, cos(X), sin(X), true);
+        __syncthreads();   // this rotation is in memory before the next pair's sums
+    }
 }
 
 // per-slice partial of sum_{i>j} H(d,j,i)^2
@@ -319,16 +397,17 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const d
 {
     Level &L = c->lv[l];
     const int ndk = L.dim, mak = L.n, nblk = L.blk;
-    const int nch = (ndk + 255) / 256;
+    const int nch = (ndk + 255) / 256;    // off-diagonal partials: 256-point slices of d
+    const int nrc = (ndk + JD - 1) / JD;  // rotation partials: 64-point slices of d
     DBuf<double> hosb, part_od, ye, initye;
     DBuf<JPart> pa, pb;
-    DBuf<int> active, kcount;
+    DBuf<int> active, kcount, pq;
     SC_TRY(hosb.alloc((size_t)ndk * mak * mak * nblk));
     SC_TRY(part_od.alloc((size_t)nch * nblk));
     SC_TRY(ye.alloc(nblk));
     SC_TRY(initye.alloc(nblk));
-    SC_TRY(pa.alloc((size_t)nch * nblk));
-    SC_TRY(pb.alloc((size_t)nch * nblk));
+    SC_TRY(pa.alloc((size_t)nrc * nblk));
+    SC_TRY(pb.alloc((size_t)nrc * nblk));
     SC_TRY(active.alloc(nblk));
     SC_TRY(kcount.alloc(nblk));
     int rc = SCALIT_OK;
@@ -352,6 +431,18 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const d
                 qq.push_back(j + i - 1);
             }
         const int npair = (int)pp.size();
+        // ndk <= 64: one workgroup per block runs a whole sweep
+        if (nrc == 1 && npair > 0) {
+            std::vector<int> h(2 * (size_t)npair);
+            for (int r = 0; r < npair; r++) {
+                h[2 * r] = pp[r];
+                h[2 * r + 1] = qq[r];
+            }
+            if (pq.alloc(h.size()) != SCALIT_OK) rc = SCALIT_ENOMEM;
+            else if (hipMemcpyAsync(pq.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+                     hipStreamSynchronize(s) != hipSuccess)
+                rc = SCALIT_EHIP;
+        }
         std::vector<int> hact(nblk);
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
@@ -365,19 +456,28 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const d
             bool any = false;
             for (int b = 0; b < nblk; b++) any = any || hact[b];
             if (!any) break;
+            if (nrc == 1) {
+                hipLaunchKernelGGL(k_jsweep, dim3(nblk), dim3(256), 0, s, ndk, mak, npair, pq.p, hosb.p, L.vosb.p,
+                                   active.p);
+                hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, hosb.p, part_od.p);
+                hipLaunchKernelGGL(k_jconv, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, nch, 0, part_od.p, ye.p,
+                                   initye.p, kcount.p, active.p, bjmax, bjtol);
+                if (hipGetLastError() != hipSuccess) rc = SCALIT_EHIP;
+                continue;
+            }
             if (!captured) {
                 // record one sweep: partials of the first pair, every rotation, convergence
                 if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                     rc = SCALIT_EHIP;
                     break;
                 }
-                hipLaunchKernelGGL(k_jpart, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, pp[0], qq[0], hosb.p, pa.p,
+                hipLaunchKernelGGL(k_jpart, dim3(nrc, nblk), dim3(256), 0, s, ndk, mak, pp[0], qq[0], hosb.p, pa.p,
                                    active.p);
                 JPart *pin = pa.p, *pout = pb.p;
                 for (int r = 0; r < npair; r++) {
                     int pn = (r + 1 < npair) ? pp[r + 1] : -1;
                     int qn = (r + 1 < npair) ? qq[r + 1] : -1;
-                    hipLaunchKernelGGL(k_jrot, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, nch, pp[r], qq[r], pn, qn,
+                    hipLaunchKernelGGL(k_jrot, dim3(nrc, nblk), dim3(256), 0, s, ndk, mak, nrc, pp[r], qq[r], pn, qn,
                                        hosb.p, L.vosb.p, pin, pout, active.p);
                     JPart *t = pin;
                     pin = pout;
@@ -405,6 +505,7 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const d
             if (hipStreamSynchronize(s) != hipSuccess) rc = SCALIT_EHIP;
         }
     }
+    pq.release();
     hosb.release();
     part_od.release();
     ye.release();
