@@ -213,6 +213,10 @@ def solver_run(reps):
         y = torch.empty_like(x)
         print(f"batch {nv}: H*X {timed(lambda: ctx.hx(x, out=y), reps):.4f} ms, "
               f"P*X {timed(lambda: ctx.px(x, out=y), reps):.4f} ms", flush=True)
+        xf = x.float()
+        yf = torch.empty_like(xf)
+        print(f"batch {nv} fp32: H*X {timed(lambda: ctx.hx_lp(xf, out=yf), reps):.4f} ms, "
+              f"P*X {timed(lambda: ctx.px_lp(xf, out=yf), reps):.4f} ms", flush=True)
         b = ctx.to_device(P.random_vectors(prob.N, nv, 5))
         for rep in range(2):   # the first solve records the QMR graph
             torch.cuda.synchronize()
