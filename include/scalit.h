@@ -37,6 +37,7 @@ extern "C" {
 #define SCALIT_EHIP (-1002)     /* HIP runtime error */
 #define SCALIT_ENOMEM (-1003)   /* device allocation failed */
 #define SCALIT_ESTATE (-1004)   /* operator incomplete (e.g. no preconditioner yet) */
+#define SCALIT_EIO (-1005)      /* file missing, short or not in the expected format */
 
 /* H*X shift selector: src/iterate/module/osb.hx.h:82-89 */
 #define SCALIT_TA0 0   /* Y = H X */
@@ -233,6 +234,29 @@ int scalit_debug_hx_variant(scalit_ctx *ctx, int prec, int nvec, int *va, int *v
  * 2: mode 3 accumulate) so that each kernel can be timed by itself. */
 int scalit_profile_hx_pass(scalit_ctx *ctx, int pass, int nvec, const double *x, long ldx, double *y, long ldy,
                            scalit_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* on-disk formats (host only, no device needed)                             */
+/* ------------------------------------------------------------------------ */
+
+/* saveData / loadData (src/libutil/serial/io/io.f90:9-33, 98-126): bin_mode != 0
+ * is Fortran sequential unformatted, one record framed by 4-byte byte counts
+ * (at most 2^31-1 bytes); bin_mode == 0 is list-directed text.  Loading reads
+ * n values from the first record (text: the first n items). */
+int scalit_save_data(const char *path, long n, const double *data, int bin_mode);
+int scalit_load_data(const char *path, long n, double *data, int bin_mode);
+/* saveDataDir / loadDataDir (src/libutil/serial/io/iodirect.f90:10-36, 68-93):
+ * direct access, one record whose length is the array's: the raw n doubles. */
+int scalit_save_data_dir(const char *path, long n, const double *data);
+int scalit_load_data_dir(const char *path, long n, double *data);
+/* calOSB / calDataInfo (src/iterate/module/osb.init.h:2-49): 0-based start of
+ * each level's piece in the concatenated H0 (sN(l)^2 each, none for the sNDVR
+ * layer), DEP (myDim(l) for each dependent level) and VOSB (myBlk(l) sN(l)^2)
+ * arrays, any of which may be NULL, and totals[5] = {myLen, myH0%mLen,
+ * myDEP%mLen, myVOSB%mLen, outLen}: the sizes of the files the reference's
+ * loadH0, loadDEP, loadVOSBEig and loadOutH read (osb.io.h:46-62, 171-235). */
+int scalit_layout(int sF, const int *sN, const int *sDEP, int ndvr, long *h0_start, long *dep_start,
+                  long *vosb_start, long *totals);
 
 #ifdef __cplusplus
 }

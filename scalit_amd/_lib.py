@@ -16,6 +16,7 @@ SCALIT_EINVAL = -1001
 SCALIT_EHIP = -1002
 SCALIT_ENOMEM = -1003
 SCALIT_ESTATE = -1004
+SCALIT_EIO = -1005
 TA0, TA1 = 0, 1
 TOSB, TOSBD1, TOSBD2 = 0, 1, 2
 FP64, FP32, BF16 = 0, 1, 2
@@ -74,6 +75,11 @@ SIGNATURES = {
     "scalit_debug_mfma_peak": (_i, [_i, _i, _i, C.POINTER(C.c_float)]),
     "scalit_debug_hx_variant": (_i, [_vp, _i, _i, _ip, _ip]),
     "scalit_debug_sym_eig": (_i, [_i, _dp, _i, _dp, _dp, _i]),
+    "scalit_save_data": (_i, [C.c_char_p, _l, _dp, _i]),
+    "scalit_load_data": (_i, [C.c_char_p, _l, _dp, _i]),
+    "scalit_save_data_dir": (_i, [C.c_char_p, _l, _dp]),
+    "scalit_load_data_dir": (_i, [C.c_char_p, _l, _dp]),
+    "scalit_layout": (_i, [_i, _ip, _ip, _i, C.POINTER(_l), C.POINTER(_l), C.POINTER(_l), C.POINTER(_l)]),
 }
 
 _lib = None
@@ -100,5 +106,6 @@ def lib():
 
 def check(rc: int, what: str = "scalit"):
     if rc != SCALIT_OK:
-        names = {SCALIT_EINVAL: "EINVAL", SCALIT_EHIP: "EHIP", SCALIT_ENOMEM: "ENOMEM", SCALIT_ESTATE: "ESTATE"}
+        names = {SCALIT_EINVAL: "EINVAL", SCALIT_EHIP: "EHIP", SCALIT_ENOMEM: "ENOMEM", SCALIT_ESTATE: "ESTATE",
+                 SCALIT_EIO: "EIO"}
         raise ScalitError(f"{what} failed: {names.get(rc, rc)}")

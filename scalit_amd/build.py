@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libscalit_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["kernels_hx.hip", "kernels_px.hip", "kernels_diag.hip", "kernels_vec.hip", "solvers.hip", "capi.hip"]
+SOURCES = ["kernels_hx.hip", "kernels_px.hip", "kernels_diag.hip", "kernels_vec.hip", "solvers.hip", "capi.hip",
+           "formats.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

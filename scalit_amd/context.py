@@ -66,6 +66,86 @@ class Context:
         ctx.set_shift(e0)
         return ctx
 
+    # ------------------------------------------------------------ files
+    @staticmethod
+    def layout(sN, sDEP, ndvr: bool = False):
+        """calOSB's offsets and sizes (scalit_layout): where each level's piece
+        starts in the concatenated H0 / DEP / VOSB files, and their lengths."""
+        sF = len(sN)
+        h0s, deps, vs, tot = (C.c_long * sF)(), (C.c_long * sF)(), (C.c_long * sF)(), (C.c_long * 5)()
+        check(L.lib().scalit_layout(sF, (C.c_int * sF)(*sN), (C.c_int * sF)(*[1 if d else 0 for d in sDEP]),
+                                    1 if ndvr else 0, h0s, deps, vs, tot), "scalit_layout")
+        return {"h0_start": list(h0s), "dep_start": list(deps), "vosb_start": list(vs), "N": tot[0],
+                "h0_len": tot[1], "dep_len": tot[2], "vosb_len": tot[3], "outh_len": tot[4]}
+
+    @staticmethod
+    def load_file(path: str, n: int, mode: str = "seq") -> np.ndarray:
+        """n doubles from a file in one of the reference's formats: "seq"
+        (sequential unformatted), "text" (list-directed) or "dir" (direct access)."""
+        a = np.zeros(int(n))
+        if mode == "dir":
+            rc = L.lib().scalit_load_data_dir(path.encode(), a.size, _np_dp(a))
+        else:
+            rc = L.lib().scalit_load_data(path.encode(), a.size, _np_dp(a), 1 if mode == "seq" else 0)
+        check(rc, f"load {path}")
+        return a
+
+    @staticmethod
+    def save_file(path: str, a: np.ndarray, mode: str = "seq"):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        if mode == "dir":
+            rc = L.lib().scalit_save_data_dir(path.encode(), a.size, _np_dp(a))
+        else:
+            rc = L.lib().scalit_save_data(path.encode(), a.size, _np_dp(a), 1 if mode == "seq" else 0)
+        check(rc, f"save {path}")
+
+    @classmethod
+    def from_files(cls, sN, sDEP, h0_file: str, res_file: str, dep_file: str | None = None,
+                   outh_file: str | None = None, bin_mode: bool = True, device: int = 0, e0: float = 0.0):
+        """An operator from the files an iterate job reads (osb.io.h:46-62,
+        211-235): H0 of every level concatenated, sequential (binary when
+        bin_mode, else text); RES, DEP (dependent levels concatenated) and, for
+        the sNDVR = T layer, OUTH as direct-access records."""
+        ndvr = outh_file is not None
+        lay = cls.layout(sN, sDEP, ndvr)
+        h0 = cls.load_file(h0_file, lay["h0_len"], "seq" if bin_mode else "text")
+        res = cls.load_file(res_file, lay["N"], "dir")
+        dep = cls.load_file(dep_file, lay["dep_len"], "dir") if lay["dep_len"] else None
+        ctx = cls(sN, sDEP, device)
+        for l, n in enumerate(ctx.sN):
+            if ndvr and l == ctx.sF - 1:
+                continue
+            st = lay["h0_start"][l]
+            ctx.set_h0(l + 1, h0[st:st + n * n].reshape(n, n, order="F"))
+            if ctx.sDEP[l]:
+                st = lay["dep_start"][l]
+                ctx.set_dep(l + 1, dep[st:st + ctx.dim[l]])
+        if ndvr:
+            ctx.set_outh(cls.load_file(outh_file, lay["outh_len"], "dir"))
+        ctx.set_res(res)
+        ctx.set_shift(e0)
+        return ctx
+
+    def save_precond(self, vosb_file: str, eig_file: str, mode: str = "dir"):
+        """VOSB (levels concatenated) and EIG0, as saveVOSBEig writes them
+        (osb.io.h:192-206): direct access, or sequential ("seq" / "text")."""
+        vosb, eig0 = self.get_precond()
+        self.save_file(vosb_file, np.concatenate(vosb), mode)
+        self.save_file(eig_file, eig0, mode)
+
+    def load_precond(self, vosb_file: str, eig_file: str, mode: str = "dir"):
+        """The preconditioner from files (loadVOSBEig, osb.io.h:171-188)
+        instead of a set-up."""
+        tot = sum(self.blk[l] * self.sN[l] ** 2 for l in range(self.sF))
+        flat = self.load_file(vosb_file, tot, mode)
+        eig0 = self.load_file(eig_file, self.N, mode)
+        parts, st = [], 0
+        for l in range(self.sF):
+            n = self.blk[l] * self.sN[l] ** 2
+            parts.append(flat[st:st + n])
+            st += n
+        self.set_precond(parts, eig0)
+
     def close(self):
         if self._h:
             L.lib().scalit_destroy(self._h)

@@ -1,0 +1,83 @@
+"""Operators and preconditioners through the reference's file formats (GPU).
+
+The files are written in the reference's layouts (H0 of every level
+concatenated as one sequential record, RES / DEP / OUTH as direct-access
+records, VOSB + EIG0 as saveVOSBEig writes them); the format functions
+themselves are pinned against the reference's saveData / loadData in
+tests/test_formats.py.  A context built from the files must give the same
+H*X and P*X, bit for bit, as one built from the arrays.
+"""
+import numpy as np
+import pytest
+
+from scalit_amd import problems as P
+from scalit_amd.context import Context
+
+pytestmark = pytest.mark.gpu
+
+
+def write_operator(tmp_path, prob, bin_mode=True, outh=None):
+    deps = list(prob.dep)
+    if outh is not None:   # the sNDVR layer replaces the level's H0 and DEP
+        deps[-1] = None
+    lay = Context.layout(prob.sN, [d is not None for d in deps], outh is not None)
+    h0 = np.zeros(lay["h0_len"])
+    for l, h in enumerate(prob.h0):
+        if outh is not None and l == len(prob.h0) - 1:
+            continue
+        st = lay["h0_start"][l]
+        h0[st:st + h.size] = np.asfortranarray(h).reshape(-1, order="F")
+    files = {"h0_file": str(tmp_path / "h0.dat"), "res_file": str(tmp_path / "res.dat")}
+    Context.save_file(files["h0_file"], h0, "seq" if bin_mode else "text")
+    Context.save_file(files["res_file"], prob.res, "dir")
+    if lay["dep_len"]:
+        dep = np.zeros(lay["dep_len"])
+        for l, d in enumerate(deps):
+            if d is not None:
+                dep[lay["dep_start"][l]:lay["dep_start"][l] + d.size] = d
+        files["dep_file"] = str(tmp_path / "dep.dat")
+        Context.save_file(files["dep_file"], dep, "dir")
+    if outh is not None:
+        files["outh_file"] = str(tmp_path / "outh.dat")
+        Context.save_file(files["outh_file"], outh, "dir")
+    return files
+
+
+@pytest.mark.parametrize("bin_mode", [True, False])
+def test_operator_from_files(tmp_path, bin_mode):
+    prob = P.config_problem("pist_fp64", grid=(24, 32, 24))
+    files = write_operator(tmp_path, prob, bin_mode)
+    a = Context.from_problem(prob, e0=-0.01)
+    b = Context.from_files(prob.sN, [d is not None for d in prob.dep], bin_mode=bin_mode, e0=-0.01, **files)
+    x = a.to_device(P.random_vectors(prob.N, 2, 4))
+    assert bool((a.hx(x) == b.hx(x)).all())
+    a.close()
+    b.close()
+
+
+def test_ndvr_operator_from_files(tmp_path):
+    prob = P.build_problem(13, 11, 9, P.gaussian_potential(5), seed=5)
+    outh = P.outer_layer(prob, seed=3)
+    files = write_operator(tmp_path, prob, True, outh=outh)
+    dep = [d is not None for d in prob.dep]
+    dep[-1] = False
+    a = Context.from_problem(prob, outh=outh)
+    b = Context.from_files(prob.sN, dep, **files)
+    x = a.to_device(P.random_vectors(prob.N, 3, 7))
+    assert bool((a.hx(x) == b.hx(x)).all())
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("mode", ["dir", "seq", "text"])
+def test_precond_checkpoint(tmp_path, mode):
+    prob = P.config_problem("pist_fp64", grid=(24, 32, 24))
+    a = Context.from_problem(prob, e0=-0.02)
+    a.block_diag(200, 1e-5)
+    a.save_precond(str(tmp_path / "vosb.dat"), str(tmp_path / "eig.dat"), mode)
+    b = Context.from_problem(prob, e0=-0.02)
+    b.load_precond(str(tmp_path / "vosb.dat"), str(tmp_path / "eig.dat"), mode)
+    x = a.to_device(P.random_vectors(prob.N, 2, 9))
+    assert bool((a.px(x) == b.px(x)).all())
+    a.close()
+    b.close()
