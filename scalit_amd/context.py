@@ -100,16 +100,17 @@ class Context:
         check(rc, f"save {path}")
 
     @classmethod
-    def from_files(cls, sN, sDEP, h0_file: str, res_file: str, dep_file: str | None = None,
+    def from_files(cls, sN, sDEP, h0_file: str, res_file: str | None = None, dep_file: str | None = None,
                    outh_file: str | None = None, bin_mode: bool = True, device: int = 0, e0: float = 0.0):
-        """An operator from the files an iterate job reads (osb.io.h:46-62,
-        211-235): H0 of every level concatenated, sequential (binary when
-        bin_mode, else text); RES, DEP (dependent levels concatenated) and, for
-        the sNDVR = T layer, OUTH as direct-access records."""
+        """An operator from the files an iterate job reads (loadInitData,
+        osb.io.h:2-62, 211-235): H0 of every level concatenated, sequential
+        (binary when bin_mode, else text); RES, DEP (dependent levels
+        concatenated) and, for the sNDVR = T layer, OUTH as direct-access
+        records.  With OUTH the job has no RES file and RES is zero."""
         ndvr = outh_file is not None
         lay = cls.layout(sN, sDEP, ndvr)
         h0 = cls.load_file(h0_file, lay["h0_len"], "seq" if bin_mode else "text")
-        res = cls.load_file(res_file, lay["N"], "dir")
+        res = np.zeros(lay["N"]) if (ndvr and res_file is None) else cls.load_file(res_file, lay["N"], "dir")
         dep = cls.load_file(dep_file, lay["dep_len"], "dir") if lay["dep_len"] else None
         ctx = cls(sN, sDEP, device)
         for l, n in enumerate(ctx.sN):
