@@ -235,6 +235,29 @@ int scalit_debug_hx_variant(scalit_ctx *ctx, int prec, int nvec, int *va, int *v
 int scalit_profile_hx_pass(scalit_ctx *ctx, int pass, int nvec, const double *x, long ldx, double *y, long ldy,
                            scalit_stream stream);
 
+/* Several problems (uneven grids) in one call: job j runs
+ * scalit_pist_conv_batched on its own context (its windows, start vectors and
+ * outputs as there), every job in its own host thread on its context's
+ * stream, so the problems' kernels share the device.  No counterpart in the
+ * reference, which solves one problem per run (PIST_CONV,
+ * src/iterate/libiterate/pist/pist.f90:7-98, per window).  status and stats
+ * are per job; the call returns the first job error, else SCALIT_OK.  One job
+ * per context. */
+typedef struct scalit_pist_job {
+    scalit_ctx *ctx;
+    int nwin;
+    const double *e0;   /* nwin window centres */
+    const double *x;    /* device, nwin start vectors, leading dimension ld */
+    long ld;
+    double *eig;        /* host, nwin * m */
+    int *ret;           /* host, nwin */
+    double *res;        /* host, nwin, may be NULL */
+    int stats[4];
+    int status;
+} scalit_pist_job;
+int scalit_pist_conv_multi(int njob, scalit_pist_job *jobs, double etol, int ntype, int startcnt, int stepcnt,
+                           int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int prec);
+
 /* ------------------------------------------------------------------------ */
 /* on-disk formats (host only, no device needed)                             */
 /* ------------------------------------------------------------------------ */

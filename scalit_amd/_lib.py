@@ -29,6 +29,14 @@ _i = C.c_int
 _d = C.c_double
 
 FOP = C.CFUNCTYPE(None, _ip, _dp, _dp)
+
+
+class PistJob(C.Structure):
+    """scalit_pist_job of include/scalit.h."""
+    _fields_ = [("ctx", C.c_void_p), ("nwin", C.c_int), ("e0", _dp), ("x", C.c_void_p), ("ld", C.c_long),
+                ("eig", _dp), ("ret", _ip), ("res", _dp), ("stats", C.c_int * 4), ("status", C.c_int)]
+
+
 FLINSOLV = C.CFUNCTYPE(C.c_int, _ip, _dp, _dp, _dp)
 
 # name -> (restype, argtypes)
@@ -80,6 +88,7 @@ SIGNATURES = {
     "scalit_save_data_dir": (_i, [C.c_char_p, _l, _dp]),
     "scalit_load_data_dir": (_i, [C.c_char_p, _l, _dp]),
     "scalit_layout": (_i, [_i, _ip, _ip, _i, C.POINTER(_l), C.POINTER(_l), C.POINTER(_l), C.POINTER(_l)]),
+    "scalit_pist_conv_multi": (_i, [_i, C.c_void_p, _d, _i, _i, _i, _i, _i, _i, _i, _i, _d, _i]),
 }
 
 _lib = None

@@ -147,6 +147,39 @@ class Context:
             st += n
         self.set_precond(parts, eig0)
 
+    @staticmethod
+    def pist_conv_multi(problems, etol: float, m: int, m_max: int, startcnt: int, stepcnt: int, stepeig: int = 1,
+                        ntype: int = 0, qmr_type: int = -1, qmrmax: int = 1000, qmrtol: float = 1e-3,
+                        prec: int = L.FP64):
+        """Several problems in one call (scalit_pist_conv_multi): `problems` is a
+        list of (context, x0 tensor of shape (nwin, N), window centres); each
+        runs its batched PIST on its own stream.  Returns per problem
+        (ret, eig (nwin, m), res, stats)."""
+        jobs = (L.PistJob * len(problems))()
+        keep = []
+        for j, (ctx, x0, e0s) in enumerate(problems):
+            x0 = x0.contiguous()
+            nw = x0.shape[0]
+            e = (C.c_double * nw)(*[float(v) for v in e0s])
+            eig = (C.c_double * (nw * m))()
+            ret = (C.c_int * nw)()
+            res = (C.c_double * nw)()
+            keep += [x0, e, eig, ret, res]
+            jobs[j].ctx, jobs[j].nwin, jobs[j].ld = ctx._h.value, nw, ctx.N
+            jobs[j].e0 = C.cast(e, _dp)
+            jobs[j].x = x0.data_ptr()
+            jobs[j].eig, jobs[j].ret, jobs[j].res = C.cast(eig, _dp), C.cast(ret, C.POINTER(C.c_int)), C.cast(res, _dp)
+        rc = L.lib().scalit_pist_conv_multi(len(problems), C.cast(jobs, C.c_void_p), etol, ntype, startcnt, stepcnt,
+                                            stepeig, m, m_max, qmr_type, qmrmax, qmrtol, prec)
+        check(rc, "scalit_pist_conv_multi")
+        out = []
+        for j, (ctx, x0, e0s) in enumerate(problems):
+            nw = len(e0s)
+            eig, ret, res = keep[5 * j + 2], keep[5 * j + 3], keep[5 * j + 4]
+            out.append((np.array(ret[:]), np.array(eig[:]).reshape(nw, m), np.array(res[:]),
+                        np.array(jobs[j].stats[:])))
+        return out
+
     def close(self):
         if self._h:
             L.lib().scalit_destroy(self._h)

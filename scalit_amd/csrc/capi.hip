@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "scalit_impl.hpp"
@@ -443,6 +444,44 @@ int scalit_pist_conv_batched(scalit_ctx *c, int nwin, const double *e0, double e
     PistParams pp{etol, ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, prec, qmrtol};
     HIP_TRY(hipDeviceSynchronize());
     return pist_conv(c, nwin, e0, pp, x, ld, eig, ret, res, stats, c->stream);
+}
+
+int scalit_pist_conv_multi(int njob, scalit_pist_job *jobs, double etol, int ntype, int startcnt, int stepcnt,
+                           int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int prec)
+{
+    if (njob < 1 || !jobs || m < 1 || m_max < 1) return SCALIT_EINVAL;
+    if (prec != SCALIT_FP64 && prec != SCALIT_FP32 && prec != SCALIT_BF16) return SCALIT_EINVAL;
+    for (int j = 0; j < njob; j++) {
+        scalit_pist_job &J = jobs[j];
+        J.status = SCALIT_EINVAL;
+        if (!J.ctx || J.nwin < 1 || !J.e0 || !J.x || !J.eig || !J.ret || J.ld < J.ctx->N) return SCALIT_EINVAL;
+        for (int i = 0; i < j; i++)
+            if (jobs[i].ctx == J.ctx) return SCALIT_EINVAL;   // one job per context: each owns its stream and state
+        if (hipSetDevice(J.ctx->device) != hipSuccess) return SCALIT_EHIP;
+        SC_TRY(operator_ready(J.ctx));
+    }
+    PistParams pp{etol, ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, prec, qmrtol};
+    for (int j = 0; j < njob; j++) {
+        HIP_TRY(hipSetDevice(jobs[j].ctx->device));
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    // one host thread per problem, each on its context's stream: the problems'
+    // kernels share the device while every solve keeps its own control flow
+    std::vector<std::thread> th;
+    for (int j = 0; j < njob; j++)
+        th.emplace_back([&jobs, j, &pp]() {
+            scalit_pist_job &J = jobs[j];
+            scalit_ctx *c = J.ctx;
+            if (hipSetDevice(c->device) != hipSuccess) {
+                J.status = SCALIT_EHIP;
+                return;
+            }
+            J.status = pist_conv(c, J.nwin, J.e0, pp, J.x, J.ld, J.eig, J.ret, J.res, J.stats, c->stream);
+        });
+    for (auto &t : th) t.join();
+    for (int j = 0; j < njob; j++)
+        if (jobs[j].status != SCALIT_OK) return jobs[j].status;
+    return SCALIT_OK;
 }
 
 int scalit_pist_vectors(scalit_ctx *c, int win, double *psi, long ld)

@@ -359,7 +359,7 @@ struct PistState {
     QmrWS<double> q64;
     QmrWS<float> q32;
     QmrWS<bf16_t> q16;
-    DBuf<double> vj, hmat, w, eigw, zw, part, part2, tmp, r64, x64, psi_z;
+    DBuf<double> vj, hmat, w, eigw, zw, part, part2, tmp, r64, x64, psi_z, zsh, bb, shd;
     DBuf<int> ok, active, cols;
     int nwin = 0, m_max = 0, m_last = 0;
     std::vector<int> last_m;
@@ -385,6 +385,9 @@ struct PistState {
         r64.release();
         x64.release();
         psi_z.release();
+        zsh.release();
+        bb.release();
+        shd.release();
         ok.release();
         active.release();
         cols.release();
@@ -507,7 +510,7 @@ int qmr_mixed(scalit_ctx *c, int prec, int ntype, int qmrmax, double tol, int in
     std::vector<double> sh(nvec, c->e0);
     if (shifts)
         for (int i = 0; i < nvec; i++) sh[i] = shifts[i];
-    DBuf<double> shd;
+    DBuf<double> &shd = P.shd;
     SC_TRY(shd.alloc(nvec));
     HIP_TRY(hipMemcpyAsync(shd.p, sh.data(), 8 * nvec, hipMemcpyHostToDevice, s));
     std::vector<int> total(nvec, 0), conv(nvec, 0), rin(nvec);
@@ -559,7 +562,6 @@ int qmr_mixed(scalit_ctx *c, int prec, int ntype, int qmrmax, double tol, int in
         ret[v] = conv[v] ? std::max(total[v], 1) : -std::max(total[v], 1);
         if (res) res[v] = rsn[v];
     }
-    shd.release();
     return SCALIT_OK;
 }
 
@@ -598,7 +600,7 @@ static int pist_linsolv(scalit_ctx *c, PistState &P, const PistParams &pp, int n
         SC_TRY(P.x64.alloc((size_t)na * N));
         SC_TRY(P.tmp.alloc((size_t)na * N));
         // gather the right-hand sides
-        DBuf<double> bb;
+        DBuf<double> &bb = P.bb;
         SC_TRY(bb.alloc((size_t)na * N));
         for (int i = 0; i < na; i++)
             HIP_TRY(hipMemcpyAsync(bb.p + (long)i * N, P.vj.p + (long)idx[i] * P.ldv * pp.m_max + (long)jcol * P.ldv,
@@ -615,7 +617,6 @@ static int pist_linsolv(scalit_ctx *c, PistState &P, const PistParams &pp, int n
         for (int i = 0; i < na; i++)
             HIP_TRY(hipMemcpyAsync(P.w.p + (long)idx[i] * P.ldv, P.x64.p + (long)i * N, 8 * N,
                                    hipMemcpyDeviceToDevice, s));
-        bb.release();
     }
     for (int i = 0; i < na; i++) {
         num[idx[i]] = r[i];
@@ -686,7 +687,7 @@ static int pist_h0(scalit_ctx *c, PistState &P, const PistParams &pp, int nw, co
     // Rayleigh-Ritz columns j = m_low..m_high: H0X = H (no shift)
     SC_TRY(P.part.alloc((size_t)nw * m_high * NPART));
     std::vector<double> zero(nw, 0.0);
-    DBuf<double> z;
+    DBuf<double> &z = P.zsh;   // kept: a device free would synchronise the whole device
     SC_TRY(z.alloc(nw));
     HIP_TRY(hipMemcpyAsync(z.p, zero.data(), 8 * nw, hipMemcpyHostToDevice, s));
     for (int j = m_low; j <= m_high; j++) {
@@ -696,7 +697,6 @@ static int pist_h0(scalit_ctx *c, PistState &P, const PistParams &pp, int nw, co
         stats[3] += 1;
     }
     HIP_TRY(hipStreamSynchronize(s));
-    z.release();
     SC_TRY(P.part.alloc((size_t)2 * nw * NPART));
     return SCALIT_OK;
 }
