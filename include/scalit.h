@@ -227,6 +227,11 @@ int scalit_debug_mfma_peak(int prec, int iters, int blocks, float *ms);
  * (host arrays, column-major, upper triangle of a read): w = eigenvalues
  * ascending; z (n x n, ldz) = eigenvectors when not NULL. */
 int scalit_debug_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz);
+/* Eigenvalues of several small symmetric matrices in one launch, as the
+ * Rayleigh-Ritz steps of all PIST windows are solved: matrix j is the leading
+ * ns[j] x ns[j] block of a + j lda lda (host, column-major, leading dimension
+ * lda, upper triangle read); its eigenvalues, ascending, go to w + j lda. */
+int scalit_debug_sym_eigvals_batch(int njobs, const int *ns, const double *a, int lda, double *w);
 /* The fused H*X kernel variants (pass A block shape, pass B width) the
  * autotuner picked for this precision and batch size (tuning them if needed). */
 int scalit_debug_hx_variant(scalit_ctx *ctx, int prec, int nvec, int *va, int *vb);

@@ -83,6 +83,7 @@ SIGNATURES = {
     "scalit_debug_mfma_peak": (_i, [_i, _i, _i, C.POINTER(C.c_float)]),
     "scalit_debug_hx_variant": (_i, [_vp, _i, _i, _ip, _ip]),
     "scalit_debug_sym_eig": (_i, [_i, _dp, _i, _dp, _dp, _i]),
+    "scalit_debug_sym_eigvals_batch": (_i, [_i, _ip, _dp, _i, _dp]),
     "scalit_save_data": (_i, [C.c_char_p, _l, _dp, _i]),
     "scalit_load_data": (_i, [C.c_char_p, _l, _dp, _i]),
     "scalit_save_data_dir": (_i, [C.c_char_p, _l, _dp]),

@@ -710,6 +710,32 @@ int scalit_debug_sym_eig(int n, const double *a, int lda, double *w, double *z, 
     return rc;
 }
 
+int scalit_debug_sym_eigvals_batch(int njobs, const int *ns, const double *a, int lda, double *w)
+{
+    if (njobs < 1 || !ns || !a || !w || lda < 1) return SCALIT_EINVAL;
+    for (int j = 0; j < njobs; j++)
+        if (ns[j] < 1 || ns[j] > lda) return SCALIT_EINVAL;
+    const size_t per = (size_t)lda * lda;
+    DBuf<double> da, dw, work;
+    DBuf<EigvalJob> djobs;
+    auto run = [&]() -> int {
+        SC_TRY(da.alloc(per * njobs));
+        SC_TRY(dw.alloc((size_t)lda * njobs));
+        HIP_TRY(hipMemcpy(da.p, a, sizeof(double) * per * njobs, hipMemcpyHostToDevice));
+        std::vector<EigvalJob> jobs(njobs);
+        for (int j = 0; j < njobs; j++) jobs[j] = {da.p + per * j, dw.p + (size_t)lda * j, ns[j], lda};
+        SC_TRY(small_sym_eigvals_batch(njobs, jobs.data(), djobs, work, nullptr));
+        HIP_TRY(hipMemcpy(w, dw.p, sizeof(double) * lda * njobs, hipMemcpyDeviceToHost));
+        return SCALIT_OK;
+    };
+    const int rc = run();
+    da.release();
+    dw.release();
+    work.release();
+    djobs.release();
+    return rc;
+}
+
 int scalit_debug_mfma_probe(int prec, double *out)
 {
     if (!out) return SCALIT_EINVAL;

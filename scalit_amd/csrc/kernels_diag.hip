@@ -19,6 +19,7 @@
 // replayed.
 #include "scalit_impl.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -677,10 +678,14 @@ __global__ void __launch_bounds__(1024) k_small_sym_eig(int n, const double *__r
 // ---------------------------------------------------------------------------
 // Eigenvalues only (DSYEV 'N'): Householder reduction to tridiagonal form
 // (Golub & Van Loan, Matrix Computations, alg. 8.3.1), then every eigenvalue
-// by bisection on the Sturm sequence of the tridiagonal matrix (one thread per
-// eigenvalue index, which also gives the ascending order).  One workgroup of
-// 1024 threads; the matrix lives in LDS when it fits (LDS_A) and in the work
-// buffer otherwise.
+// by multisection on the Sturm sequence of the tridiagonal matrix (Demmel,
+// Applied Numerical Linear Algebra, sec. 5.3.4: the count of negative pivots
+// of T - xI is the number of eigenvalues below x).  T threads share one
+// eigenvalue index and each round count at T points of its interval, which
+// shrinks the interval (T+1)-fold per round instead of 2-fold; the index also
+// gives the ascending order.  One workgroup of 1024 threads per problem
+// (blockIdx.x picks the job); the matrix lives in LDS when it fits (LDS_A) and
+// in the work buffer (wstride doubles per job) otherwise.
 // ---------------------------------------------------------------------------
 constexpr int EIGV_NT = 1024;
 constexpr int EIGV_LDS_N = 112;   // largest n whose n x (n+1) matrix sits in LDS with the vectors
@@ -699,13 +704,17 @@ __device__ double block_sum_1024(double v, double *red)
 }
 
 template <bool LDS_A>
-__global__ void __launch_bounds__(EIGV_NT) k_sym_eigvals(int n, const double *__restrict__ a_in, int lda,
-                                                         double *__restrict__ w, double *__restrict__ gwork)
+__global__ void __launch_bounds__(EIGV_NT) k_sym_eigvals(const EigvalJob *__restrict__ jobs, EigvalJob one,
+                                                         double *__restrict__ gwork, long wstride)
 {
     extern __shared__ double sm_eig[];
+    const EigvalJob jb = jobs ? jobs[blockIdx.x] : one;
+    const int n = jb.n, lda = jb.lda;
+    const double *__restrict__ a_in = jb.a;
+    double *__restrict__ w = jb.w;
     const int tid = threadIdx.x;
     const int ld = n + 1;
-    double *A = LDS_A ? sm_eig : gwork;                    // A(i,j) at i + ld j
+    double *A = LDS_A ? sm_eig : gwork + (long)blockIdx.x * wstride;   // A(i,j) at i + ld j
     double *vec = LDS_A ? sm_eig + (long)n * ld : sm_eig;  // v | p | d | e (n each) | red (16)
     double *v = vec, *pw = vec + n, *d = vec + 2 * n, *e = vec + 3 * n, *red = vec + 4 * n;
     for (int x = tid; x < n * n; x += EIGV_NT) {
@@ -773,25 +782,39 @@ __global__ void __launch_bounds__(EIGV_NT) k_sym_eigvals(int n, const double *__
     lo -= 1e-12 * span;
     hi += 1e-12 * span;
     const double pivmin = 1e-290;
-    for (int idx = tid; idx < n; idx += EIGV_NT) {
-        // smallest x with count(eigenvalues < x) > idx
-        double a = lo, b = hi;
+    // T = threads per eigenvalue: a power of two <= 64 (a group never spans
+    // two wavefronts, so the group reductions are lane shuffles)
+    int T = 1;
+    while (T < 64 && 2 * T * n <= EIGV_NT) T *= 2;
+    const int sub = tid & (T - 1);
+    const double frac = (double)(sub + 1) / (double)(T + 1);
+    for (int base = 0; base < n; base += EIGV_NT / T) {
+        // the index this group brackets: the smallest x with count(x) > idx
+        const int idx = base + tid / T;
+        const int ie = idx < n ? idx : n - 1;
+        double a = lo, b = hi;   // count(a) <= ie < count(b), identical across the group
         for (int it = 0; it < 200; it++) {
             const double mid = 0.5 * (a + b);
             if (mid <= a || mid >= b) break;   // interval at one ulp
+            const double x = T == 1 ? mid : a + (b - a) * frac;
             int cnt = 0;
-            double q = d[0] - mid;
+            double q = d[0] - x;
             if (fabs(q) < pivmin) q = -pivmin;
             cnt += q < 0;
             for (int i = 1; i < n; i++) {
-                q = d[i] - mid - e[i - 1] * e[i - 1] / q;
+                q = d[i] - x - e[i - 1] * e[i - 1] / q;
                 if (fabs(q) < pivmin) q = -pivmin;
                 cnt += q < 0;
             }
-            if (cnt > idx) b = mid;
-            else a = mid;
+            double bn = cnt > ie ? x : b;
+            for (int o = 1; o < T; o <<= 1) bn = fmin(bn, __shfl_xor(bn, o));
+            double an = (cnt <= ie && x < bn) ? x : a;
+            for (int o = 1; o < T; o <<= 1) an = fmax(an, __shfl_xor(an, o));
+            if (an == a && bn == b) break;   // every point fell on an end: a few ulps apart
+            a = an;
+            b = bn;
         }
-        w[idx] = 0.5 * (a + b);
+        if (idx < n && sub == 0) w[idx] = 0.5 * (a + b);
     }
 }
 
@@ -805,16 +828,50 @@ int small_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz
             attr = true;
         }
         const size_t vec_bytes = (4 * (size_t)n + 16) * sizeof(double);
+        const EigvalJob one{a, w, n, lda};
         if (n <= EIGV_LDS_N)
-            hipLaunchKernelGGL(k_sym_eigvals<true>, dim3(1), dim3(EIGV_NT), (size_t)n * (n + 1) * 8 + vec_bytes, s, n,
-                               a, lda, w, work);
+            hipLaunchKernelGGL(k_sym_eigvals<true>, dim3(1), dim3(EIGV_NT), (size_t)n * (n + 1) * 8 + vec_bytes, s,
+                               nullptr, one, work, 0L);
         else
-            hipLaunchKernelGGL(k_sym_eigvals<false>, dim3(1), dim3(EIGV_NT), vec_bytes, s, n, a, lda, w, work);
+            hipLaunchKernelGGL(k_sym_eigvals<false>, dim3(1), dim3(EIGV_NT), vec_bytes, s, nullptr, one, work, 0L);
         HIP_TRY(hipGetLastError());
         return SCALIT_OK;
     }
     if (n < 1 || n > 1024) return SCALIT_EINVAL;
     hipLaunchKernelGGL(k_small_sym_eig, dim3(1), dim3(1024), 0, s, n, a, lda, w, z, ldz, work);
+    HIP_TRY(hipGetLastError());
+    return SCALIT_OK;
+}
+
+int small_sym_eigvals_batch(int njobs, const EigvalJob *jobs, DBuf<EigvalJob> &djobs, DBuf<double> &work,
+                            hipStream_t s)
+{
+    if (njobs < 0) return SCALIT_EINVAL;
+    if (njobs == 0) return SCALIT_OK;
+    int nmax = 0;
+    for (int j = 0; j < njobs; j++) {
+        if (jobs[j].n < 1 || jobs[j].lda < jobs[j].n || !jobs[j].a || !jobs[j].w) return SCALIT_EINVAL;
+        nmax = std::max(nmax, jobs[j].n);
+    }
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_sym_eigvals<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr = true;
+    }
+    SC_TRY(djobs.alloc(njobs));
+    HIP_TRY(hipMemcpyAsync(djobs.p, jobs, sizeof(EigvalJob) * njobs, hipMemcpyHostToDevice, s));
+    const size_t vec_bytes = (4 * (size_t)nmax + 16) * sizeof(double);
+    const EigvalJob none{nullptr, nullptr, 0, 0};
+    if (nmax <= EIGV_LDS_N) {
+        hipLaunchKernelGGL(k_sym_eigvals<true>, dim3(njobs), dim3(EIGV_NT), (size_t)nmax * (nmax + 1) * 8 + vec_bytes,
+                           s, djobs.p, none, nullptr, 0L);
+    } else {
+        const long stride = (long)nmax * (nmax + 1);
+        SC_TRY(work.alloc((size_t)njobs * stride));
+        hipLaunchKernelGGL(k_sym_eigvals<false>, dim3(njobs), dim3(EIGV_NT), vec_bytes, s, djobs.p, none, work.p,
+                           stride);
+    }
     HIP_TRY(hipGetLastError());
     return SCALIT_OK;
 }

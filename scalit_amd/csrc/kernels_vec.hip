@@ -168,6 +168,16 @@ void launch_axpby(int nvec, long n, double a, const double *x, long ldx, double 
 // Vectors of one QMR: nvec columns each of R, UX, UR, Q, S, V, Z, HX0 with
 // leading dimension ld (storage type S); B and X are the caller's (leading
 // dimension ldb / ldx, storage type S).
+//
+// V and Z are stored as the operators leave them: the reference divides V by
+// VNRM and Z by ZVRAT and then by ZNRM in place (qmr.f90: the VNRM / ZNRM
+// scaling at the top of the loop and Z = Z / ZVRAT after the preconditioner);
+// here the kernels that read them apply those divisions to the loaded values
+// instead, which saves the two passes that only rescaled them.  V.Z is summed
+// while Z is read for Z.Z, as (V/VNRMPO).(Z/ZVRAT), and divided by ZNRM once
+// ZNRM is known.  ZVRAT_NEXT holds the ratio of the running iteration (the
+// reference's ZVRAT after its update); the end of the iteration rolls it into
+// ZVRAT.
 
 __device__ __forceinline__ bool running(const QmrScal *q, int v) { return q[v].status == QMR_RUNNING; }
 
@@ -195,22 +205,28 @@ __global__ void __launch_bounds__(256) k_qmr_init(long n, int zero_x, QmrVecs<S>
     zero_tail(part + (long)v * NPART);
 }
 
-// init part 2 (after Z = P B): partial Z.Z
+// init part 2 (after Z = P B): partials of Z.Z and (V/VNRM).Z (ZVRAT = 1)
 template <typename S>
-__global__ void __launch_bounds__(256) k_qmr_init_z(long n, QmrVecs<S> w, double *part)
+__global__ void __launch_bounds__(256) k_qmr_init_z(long n, QmrVecs<S> w, const double *part_bb, double *part,
+                                                    double *part_vz)
 {
     __shared__ double red[4];
     const int v = blockIdx.y;
-    const S *Z = w.Z + (long)v * w.ld;
-    double s = 0;
+    const S *Z = w.Z + (long)v * w.ld, *V = w.V + (long)v * w.ld;
+    const double vnrm = sqrt(sum_parts(part_bb + (long)v * NPART));
+    double s = 0, svz = 0;
     GSTRIDE(i, n)
     {
         double d = lv<S>(Z, i);
         s += d * d;
+        svz += (lv<S>(V, i) / vnrm) * d;
     }
     s = bsum(s, red);
     if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
     zero_tail(part + (long)v * NPART);
+    svz = bsum(svz, red);
+    if (threadIdx.x == 0) part_vz[(long)v * NPART + blockIdx.x] = svz;
+    zero_tail(part_vz + (long)v * NPART);
 }
 
 // init scalars: BNRM, VNRM, ZNRM, GAMAMO=1, ETA=-1, ZVRAT=1 (one wave per vector)
@@ -236,47 +252,23 @@ __global__ void k_qmr_init_scal(int nvec, const double *part_bb, const double *p
     active[v] = s.status == QMR_RUNNING;
 }
 
-// k2: exit check on VNRM/ZNRM, ZVRAT *= ZNRM/VNRM, V /= VNRM, Z /= ZNRM, partial V.Z
+// k2 + k3: exit check on VNRM / ZNRM, ZVRAT_NEXT = ZVRAT ZNRM / VNRM,
+// VZDOT = (V/VNRM).(Z/ZVRAT) / ZNRM from the partials of the previous step,
+// breakdown check, Q = Z/ZVRAT/ZNRM - (VNRM VZDOT / QSDOT) Q
 template <typename S>
-__global__ void __launch_bounds__(256) k_qmr_a(long n, QmrVecs<S> w, QmrScal *q, int *active, double *part)
+__global__ void __launch_bounds__(256) k_qmr_b(long n, QmrVecs<S> w, QmrScal *q, int *active, const double *part_vz)
 {
-    __shared__ double red[4];
     const int v = blockIdx.y;
     if (!running(q, v)) return;
-    const double vnrm = q[v].vnrm, znrm = q[v].znrm;
+    const double vnrm = q[v].vnrm, znrm = q[v].znrm, zvrat = q[v].zvrat;
     if (vnrm == 0.0 || znrm == 0.0) {
-        __syncthreads();
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             q[v].status = QMR_EXIT;
             active[v] = 0;
         }
         return;
     }
-    S *V = w.V + (long)v * w.ld, *Z = w.Z + (long)v * w.ld;
-    double s = 0;
-    GSTRIDE(i, n)
-    {
-        double a = lv<S>(V, i) / vnrm;
-        double z = lv<S>(Z, i) / znrm;
-        sv<S>(V, i, a);
-        sv<S>(Z, i, z);
-        s += lv<S>(V, i) * lv<S>(Z, i);
-    }
-    s = bsum(s, red);
-    zero_tail(part + (long)v * NPART);
-    if (threadIdx.x == 0) {
-        part[(long)v * NPART + blockIdx.x] = s;
-        if (blockIdx.x == 0) q[v].zvrat_next = q[v].zvrat * znrm / vnrm;
-    }
-}
-
-// k3: VZDOT, breakdown check, Q = Z - (VNRM VZDOT / QSDOT) Q
-template <typename S>
-__global__ void __launch_bounds__(256) k_qmr_b(long n, QmrVecs<S> w, QmrScal *q, int *active, const double *part)
-{
-    const int v = blockIdx.y;
-    if (!running(q, v)) return;
-    const double vzdot = sum_parts_n(part + (long)v * NPART, n);
+    const double vzdot = sum_parts_n(part_vz + (long)v * NPART, n) / znrm;
     if (vzdot == 0.0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             q[v].status = QMR_EXIT;
@@ -287,15 +279,15 @@ __global__ void __launch_bounds__(256) k_qmr_b(long n, QmrVecs<S> w, QmrScal *q,
     S *Q = w.Q + (long)v * w.ld;
     const S *Z = w.Z + (long)v * w.ld;
     const bool first = q[v].iter == 1;
-    const double c = first ? 0.0 : q[v].vnrm * vzdot / q[v].qsdot;
+    const double c = first ? 0.0 : vnrm * vzdot / q[v].qsdot;
     if (first) {
-        GSTRIDE(i, n) Q[i] = Z[i];
+        GSTRIDE(i, n) sv<S>(Q, i, lv<S>(Z, i) / zvrat / znrm);
     } else {
-        GSTRIDE(i, n) sv<S>(Q, i, lv<S>(Z, i) - c * lv<S>(Q, i));
+        GSTRIDE(i, n) sv<S>(Q, i, lv<S>(Z, i) / zvrat / znrm - c * lv<S>(Q, i));
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         q[v].vzdot = vzdot;
-        q[v].zvrat = q[v].zvrat_next;
+        q[v].zvrat_next = zvrat * znrm / vnrm;
     }
 }
 
@@ -306,7 +298,7 @@ __global__ void __launch_bounds__(256) k_qmr_c(long n, QmrVecs<S> w, const QmrSc
     __shared__ double red[4];
     const int v = blockIdx.y;
     if (!running(q, v)) return;
-    const double zvrat = q[v].zvrat;
+    const double zvrat = q[v].zvrat_next;
     S *Sv = w.Sv + (long)v * w.ld;
     const S *Q = w.Q + (long)v * w.ld;
     double s = 0;
@@ -320,7 +312,7 @@ __global__ void __launch_bounds__(256) k_qmr_c(long n, QmrVecs<S> w, const QmrSc
     zero_tail(part + (long)v * NPART);
 }
 
-// k5: QSDOT, BETA, breakdown, V = S - BETA V, partial V.V
+// k5: QSDOT, BETA, breakdown, V = S - BETA V/VNRM, partial V.V
 template <typename S>
 __global__ void __launch_bounds__(256) k_qmr_d(long n, QmrVecs<S> w, QmrScal *q, int *active, const double *part_in,
                                                double *part)
@@ -338,12 +330,13 @@ __global__ void __launch_bounds__(256) k_qmr_d(long n, QmrVecs<S> w, QmrScal *q,
         }
         return;
     }
+    const double vnrm = q[v].vnrm;
     S *V = w.V + (long)v * w.ld;
     const S *Sv = w.Sv + (long)v * w.ld;
     double s = 0;
     GSTRIDE(i, n)
     {
-        sv<S>(V, i, lv<S>(Sv, i) - beta * lv<S>(V, i));
+        sv<S>(V, i, lv<S>(Sv, i) - beta * (lv<S>(V, i) / vnrm));
         double a = lv<S>(V, i);
         s += a * a;
     }
@@ -358,28 +351,86 @@ __global__ void __launch_bounds__(256) k_qmr_d(long n, QmrVecs<S> w, QmrScal *q,
     }
 }
 
-// k6 (after Z = P V): VNRMPO, Z = Z / ZVRAT, partial Z.Z
+// k6 (after Z = P V): VNRMPO; partials of (Z/ZVRAT).(Z/ZVRAT) and
+// (V/VNRMPO).(Z/ZVRAT), V and Z left as they are
 template <typename S>
-__global__ void __launch_bounds__(256) k_qmr_e(long n, QmrVecs<S> w, QmrScal *q, const double *part_vv, double *part)
+__global__ void __launch_bounds__(256) k_qmr_e(long n, QmrVecs<S> w, QmrScal *q, const double *part_vv, double *part,
+                                               double *part_vz)
 {
     __shared__ double red[4];
     const int v = blockIdx.y;
     if (!running(q, v)) return;
-    const double zvrat = q[v].zvrat;
-    const double vnrmpo = blockIdx.x == 0 ? sqrt(sum_parts_n(part_vv + (long)v * NPART, n)) : 0.0;
-    S *Z = w.Z + (long)v * w.ld;
-    double s = 0;
+    const double zvrat = q[v].zvrat_next;
+    const double vnrmpo = sqrt(sum_parts_n(part_vv + (long)v * NPART, n));
+    const S *Z = w.Z + (long)v * w.ld, *V = w.V + (long)v * w.ld;
+    double s = 0, svz = 0;
     GSTRIDE(i, n)
     {
-        sv<S>(Z, i, lv<S>(Z, i) / zvrat);
-        double a = lv<S>(Z, i);
-        s += a * a;
+        const double z = lv<S>(Z, i) / zvrat;
+        s += z * z;
+        svz += (lv<S>(V, i) / vnrmpo) * z;
     }
     s = bsum(s, red);
     zero_tail(part + (long)v * NPART);
+    if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
+    svz = bsum(svz, red);
+    zero_tail(part_vz + (long)v * NPART);
     if (threadIdx.x == 0) {
-        part[(long)v * NPART + blockIdx.x] = s;
+        part_vz[(long)v * NPART + blockIdx.x] = svz;
         if (blockIdx.x == 0) q[v].vnrmpo = vnrmpo;
+    }
+}
+
+// The end of an iteration (residual, convergence, roll of the two-level
+// variables), one wave per vector.  It stays a kernel of its own: ending the
+// iteration in the last workgroup of the kernel before it would need a
+// device-scope release and acquire per workgroup, which on this GPU write back
+// and invalidate the XCD's L2 each time (measured: +0.5 ms per iteration at
+// 16 vectors of 491,520 points).
+struct QmrEnd {
+    int conv, qmrmax;
+    double tol;
+    const double *hh, *rr, *uxux, *xx, *urur;
+};
+
+// one wave; gama, theta, eta, znrm: this iteration's values (k_qmr_f)
+__device__ void qmr_end_vector(long n, const QmrEnd &e, QmrScal *q, int *active, int v, double gama, double theta,
+                               double eta, double znrm)
+{
+    QmrScal &s = q[v];
+    double res;
+    if (e.conv == 1) {
+        double xnrm = sqrt(sum_parts_n(e.hh + (long)v * NPART, n));
+        res = sqrt(sum_parts_n(e.rr + (long)v * NPART, n)) / xnrm;
+    } else if (e.conv == 2) {
+        double rr = sum_parts_n(e.rr + (long)v * NPART, n);
+        double r1 = sqrt(rr) / s.bnrm;
+        double r2 = sqrt(sum_parts_n(e.uxux + (long)v * NPART, n) / sum_parts_n(e.xx + (long)v * NPART, n));
+        double r3 = sqrt(sum_parts_n(e.urur + (long)v * NPART, n) / rr);
+        res = r1 > r2 ? r1 : r2;
+        res = res > r3 ? res : r3;
+    } else {
+        res = sqrt(sum_parts_n(e.rr + (long)v * NPART, n)) / s.bnrm;
+    }
+    if (threadIdx.x != 0) return;
+    s.res = res;
+    if (fabs(res) < e.tol) {
+        s.ret = s.iter;
+        s.status = QMR_CONVERGED;
+        active[v] = 0;
+        return;
+    }
+    s.ret = -s.iter;
+    s.zvrat = s.zvrat_next;
+    s.vnrm = s.vnrmpo;
+    s.gamamo = gama;
+    s.thetamo = theta;
+    s.eta = eta;
+    s.znrm = znrm;
+    s.iter += 1;
+    if (s.iter > e.qmrmax) {
+        s.status = QMR_EXIT;
+        active[v] = 0;
     }
 }
 
@@ -407,7 +458,7 @@ __global__ void __launch_bounds__(256) k_qmr_f(long n, int conv, QmrVecs<S> w, Q
         return;
     }
     const bool first = o.iter == 1;
-    const double a = eta * o.zvrat;
+    const double a = eta * o.zvrat_next;
     const double tg = (o.thetamo * gama) * (o.thetamo * gama);
     S *UX = w.UX + (long)v * w.ld, *UR = w.UR + (long)v * w.ld, *R = w.R + (long)v * w.ld;
     const S *Q = w.Q + (long)v * w.ld, *Sv = w.Sv + (long)v * w.ld;
@@ -482,48 +533,12 @@ __global__ void __launch_bounds__(256) k_qmr_g(long n, QmrVecs<S> w, const QmrSc
     zero_tail(part_rr + (long)v * NPART);
 }
 
-// k9: residual, convergence, roll the two-level variables, next iteration
-__global__ void k_qmr_h(long n, int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active,
-                        const double *part_hh,
-                        const double *part_rr, const double *part_uxux, const double *part_xx,
-                        const double *part_urur)
+// k9: the end of the iteration
+__global__ void k_qmr_h(long n, QmrEnd e, QmrScal *q, int *active)
 {
-    const int v = blockIdx.x;   // one wave per vector: the whole wave sums, lane 0 updates
-    QmrScal &s = q[v];
-    if (s.status != QMR_RUNNING) return;
-    double res;
-    if (conv == 1) {
-        double xnrm = sqrt(sum_parts_n(part_hh + (long)v * NPART, n));
-        res = sqrt(sum_parts_n(part_rr + (long)v * NPART, n)) / xnrm;
-    } else if (conv == 2) {
-        double rr = sum_parts_n(part_rr + (long)v * NPART, n);
-        double r1 = sqrt(rr) / s.bnrm;
-        double r2 = sqrt(sum_parts_n(part_uxux + (long)v * NPART, n) / sum_parts_n(part_xx + (long)v * NPART, n));
-        double r3 = sqrt(sum_parts_n(part_urur + (long)v * NPART, n) / rr);
-        res = r1 > r2 ? r1 : r2;
-        res = res > r3 ? res : r3;
-    } else {
-        res = sqrt(sum_parts_n(part_rr + (long)v * NPART, n)) / s.bnrm;
-    }
-    if (threadIdx.x != 0) return;
-    s.res = res;
-    if (fabs(res) < tol) {
-        s.ret = s.iter;
-        s.status = QMR_CONVERGED;
-        active[v] = 0;
-        return;
-    }
-    s.ret = -s.iter;
-    s.vnrm = s.vnrmpo;
-    s.gamamo = s.gama;
-    s.thetamo = s.theta;
-    s.eta = s.eta_next;
-    s.znrm = s.znrm_next;
-    s.iter += 1;
-    if (s.iter > qmrmax) {
-        s.status = QMR_EXIT;
-        active[v] = 0;
-    }
+    const int v = blockIdx.x;
+    if (q[v].status != QMR_RUNNING) return;
+    qmr_end_vector(n, e, q, active, v, q[v].gama, q[v].theta, q[v].eta_next, q[v].znrm_next);
 }
 
 template <typename S>
@@ -532,19 +547,15 @@ void launch_qmr_init(int nvec, long n, int zero_x, const QmrVecs<S> &w, double *
     hipLaunchKernelGGL((k_qmr_init<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, zero_x, w, part);
 }
 template <typename S>
-void launch_qmr_init_z(int nvec, long n, const QmrVecs<S> &w, double *part, hipStream_t s)
+void launch_qmr_init_z(int nvec, long n, const QmrVecs<S> &w, const double *pbb, double *part, double *pvz,
+                       hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_init_z<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, part);
+    hipLaunchKernelGGL((k_qmr_init_z<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, pbb, part, pvz);
 }
 void launch_qmr_init_scal(int nvec, const double *pbb, const double *pzz, QmrScal *q, int *active, int qmrmax,
                           hipStream_t s)
 {
     hipLaunchKernelGGL(k_qmr_init_scal, dim3(nvec), dim3(64), 0, s, nvec, pbb, pzz, q, active, qmrmax);
-}
-template <typename S>
-void launch_qmr_a(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, double *part, hipStream_t s)
-{
-    hipLaunchKernelGGL((k_qmr_a<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, active, part);
 }
 template <typename S>
 void launch_qmr_b(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, const double *part, hipStream_t s)
@@ -563,9 +574,10 @@ void launch_qmr_d(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active
     hipLaunchKernelGGL((k_qmr_d<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, active, pin, part);
 }
 template <typename S>
-void launch_qmr_e(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, const double *pvv, double *part, hipStream_t s)
+void launch_qmr_e(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, const double *pvv, double *part, double *pvz,
+                  hipStream_t s)
 {
-    hipLaunchKernelGGL((k_qmr_e<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, pvv, part);
+    hipLaunchKernelGGL((k_qmr_e<S>), dim3(nparts(n), nvec), dim3(256), 0, s, n, w, q, pvv, part, pvz);
 }
 template <typename S>
 void launch_qmr_f(int nvec, long n, int conv, const QmrVecs<S> &w, QmrScal *q, int *active, const double *pzz,
@@ -582,19 +594,20 @@ void launch_qmr_g(int nvec, long n, const QmrVecs<S> &w, const QmrScal *q, doubl
 void launch_qmr_h(long n, int nvec, int conv, int qmrmax, double tol, QmrScal *q, int *active, const double *phh,
                   const double *prr, const double *pux, const double *pxx, const double *pur, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_qmr_h, dim3(nvec), dim3(64), 0, s, n, nvec, conv, qmrmax, tol, q, active, phh,
-                       prr, pux, pxx, pur);
+    hipLaunchKernelGGL(k_qmr_h, dim3(nvec), dim3(64), 0, s, n, QmrEnd{conv, qmrmax, tol, phh, prr, pux, pxx, pur}, q,
+                       active);
 }
 
 #define QMR_INST(S)                                                                                               \
     template void launch_qmr_init<S>(int, long, int, const QmrVecs<S> &, double *, hipStream_t);                  \
-    template void launch_qmr_init_z<S>(int, long, const QmrVecs<S> &, double *, hipStream_t);                     \
-    template void launch_qmr_a<S>(int, long, const QmrVecs<S> &, QmrScal *, int *, double *, hipStream_t);        \
+    template void launch_qmr_init_z<S>(int, long, const QmrVecs<S> &, const double *, double *, double *,          \
+                                       hipStream_t);                                                             \
     template void launch_qmr_b<S>(int, long, const QmrVecs<S> &, QmrScal *, int *, const double *, hipStream_t);  \
     template void launch_qmr_c<S>(int, long, const QmrVecs<S> &, const QmrScal *, double *, hipStream_t);         \
     template void launch_qmr_d<S>(int, long, const QmrVecs<S> &, QmrScal *, int *, const double *, double *,      \
                                   hipStream_t);                                                                  \
-    template void launch_qmr_e<S>(int, long, const QmrVecs<S> &, QmrScal *, const double *, double *, hipStream_t); \
+    template void launch_qmr_e<S>(int, long, const QmrVecs<S> &, QmrScal *, const double *, double *, double *,  \
+                                  hipStream_t);                                                                  \
     template void launch_qmr_f<S>(int, long, int, const QmrVecs<S> &, QmrScal *, int *, const double *, double *, \
                                   double *, double *, double *, hipStream_t);                                    \
     template void launch_qmr_g<S>(int, long, const QmrVecs<S> &, const QmrScal *, double *, double *, hipStream_t);

@@ -140,6 +140,19 @@ int block_diag(scalit_ctx *c, int bjmax, double bjtol);
 // a: n x n device, lda; w: n device (ascending); z: eigenvectors (device, ldz) or nullptr
 int small_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz, double *work, hipStream_t s);
 size_t small_sym_eig_work(int n);
+// eigenvalues only of several small symmetric matrices in one launch (one
+// workgroup each): the leading n x n block of a (device, leading dimension
+// lda, upper triangle read), eigenvalues ascending into w (device).  The jobs
+// are copied to djobs on stream s; the host array must stay valid until the
+// stream has passed the launch.  work grows to njobs n_max (n_max + 1) doubles
+// when n_max exceeds the in-LDS size.
+struct EigvalJob {
+    const double *a;
+    double *w;
+    int n, lda;
+};
+int small_sym_eigvals_batch(int njobs, const EigvalJob *jobs, DBuf<EigvalJob> &djobs, DBuf<double> &work,
+                            hipStream_t s);
 
 // vector helpers (kernels_vec.hip)
 constexpr int NPART = 512;   // partial sums per vector in the deterministic reductions

@@ -27,11 +27,10 @@ struct QmrVecs {
 template <typename S>
 void launch_qmr_init(int nvec, long n, int zero_x, const QmrVecs<S> &w, double *part, hipStream_t s);
 template <typename S>
-void launch_qmr_init_z(int nvec, long n, const QmrVecs<S> &w, double *part, hipStream_t s);
+void launch_qmr_init_z(int nvec, long n, const QmrVecs<S> &w, const double *pbb, double *part, double *pvz,
+                       hipStream_t s);
 void launch_qmr_init_scal(int nvec, const double *pbb, const double *pzz, QmrScal *q, int *active, int qmrmax,
                           hipStream_t s);
-template <typename S>
-void launch_qmr_a(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, double *part, hipStream_t s);
 template <typename S>
 void launch_qmr_b(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, const double *part, hipStream_t s);
 template <typename S>
@@ -40,7 +39,8 @@ template <typename S>
 void launch_qmr_d(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, int *active, const double *pin, double *part,
                   hipStream_t s);
 template <typename S>
-void launch_qmr_e(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, const double *pvv, double *part, hipStream_t s);
+void launch_qmr_e(int nvec, long n, const QmrVecs<S> &w, QmrScal *q, const double *pvv, double *part, double *pvz,
+                  hipStream_t s);
 template <typename S>
 void launch_qmr_f(int nvec, long n, int conv, const QmrVecs<S> &w, QmrScal *q, int *active, const double *pzz,
                   double *prr, double *pux, double *pxx, double *pur, hipStream_t s);

@@ -136,7 +136,7 @@ struct QmrWS {
     int nvec = 0;
     long n = 0, ld = 0;
     DBuf<S> vecs;        // B, X, R, UX, UR, Q, S, V, Z, HX0
-    DBuf<double> parts;  // 9 partial arrays
+    DBuf<double> parts;  // 10 partial arrays
     DBuf<QmrScal> scal;
     DBuf<int> active;
     DBuf<typename ShiftT<S>::type> shifts;
@@ -186,7 +186,7 @@ struct QmrWS {
         n = nn;
         ld = (nn + 63) & ~63L;
         SC_TRY(vecs.alloc((size_t)10 * nv * ld));
-        SC_TRY(parts.alloc((size_t)9 * nv * NPART));
+        SC_TRY(parts.alloc((size_t)10 * nv * NPART));
         SC_TRY(scal.alloc(nv));
         SC_TRY(active.alloc(nv));
         SC_TRY(shifts.alloc(nv));
@@ -256,13 +256,12 @@ static int qmr_iteration(scalit_ctx *c, QmrWS<S> &w, int conv, int qmrmax, doubl
     QmrVecs<S> v = w.view();
     QmrScal *q = w.scal.p;
     int *act = w.active.p;
-    launch_qmr_a<S>(nv, n, v, q, act, w.part(0), s);
-    launch_qmr_b<S>(nv, n, v, q, act, w.part(0), s);
+    launch_qmr_b<S>(nv, n, v, q, act, w.part(9), s);
     SC_TRY(op_hx<S>(c, w, cb, v.Q, v.Sv, act, s));
     launch_qmr_c<S>(nv, n, v, q, w.part(1), s);
     launch_qmr_d<S>(nv, n, v, q, act, w.part(1), w.part(2), s);
     SC_TRY(op_px<S>(c, w, cb, v.V, v.Z, act, s));
-    launch_qmr_e<S>(nv, n, v, q, w.part(2), w.part(3), s);
+    launch_qmr_e<S>(nv, n, v, q, w.part(2), w.part(3), w.part(9), s);
     launch_qmr_f<S>(nv, n, conv, v, q, act, w.part(3), w.part(4), w.part(5), w.part(6), w.part(7), s);
     if (conv == 1) {
         SC_TRY(op_hx<S>(c, w, cb, v.x, v.HX0, act, s));
@@ -306,7 +305,7 @@ static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double t
     // initialisation (qmr.f90:36-50)
     launch_qmr_init<S>(nv, n, ntype <= 0 ? 1 : 0, v, w.part(0), s);
     SC_TRY(op_px<S>(c, w, cb, v.b, v.Z, w.active.p, s));
-    launch_qmr_init_z<S>(nv, n, v, w.part(1), s);
+    launch_qmr_init_z<S>(nv, n, v, w.part(0), w.part(1), w.part(9), s);
     launch_qmr_init_scal(nv, w.part(0), w.part(1), w.scal.p, w.active.p, qmrmax, s);
     HIP_TRY(hipGetLastError());
     const bool use_graph = g_qmr_use_graph != 0 && cb == nullptr;
@@ -361,6 +360,7 @@ struct PistState {
     QmrWS<bf16_t> q16;
     DBuf<double> vj, hmat, w, eigw, zw, part, part2, tmp, r64, x64, psi_z, zsh, bb, shd;
     DBuf<int> ok, active, cols;
+    DBuf<EigvalJob> ejobs;
     int nwin = 0, m_max = 0, m_last = 0;
     std::vector<int> last_m;
     std::vector<std::vector<int>> last_ind;   // window index of each returned eigenvalue
@@ -722,7 +722,7 @@ int pist_conv(scalit_ctx *c, int nw, const double *e0, const PistParams &pp, con
     SC_TRY(P.tmp.alloc((size_t)std::max<long>(nw, 64)));
     SC_TRY(P.ok.alloc(nw));
     SC_TRY(P.active.alloc(nw));
-    SC_TRY(P.eigw.alloc((size_t)m_max));
+    SC_TRY(P.eigw.alloc((size_t)2 * nw * m_max));
     SC_TRY(P.zw.alloc(small_sym_eig_work(m_max)));
     if (pp.prec != SCALIT_FP64) SC_TRY(ensure_f32(c));
     HIP_TRY(hipMemsetAsync(P.hmat.p, 0, 8 * (size_t)nw * m_max * m_max, s));
@@ -761,6 +761,10 @@ int pist_conv(scalit_ctx *c, int nw, const double *e0, const PistParams &pp, con
         std::vector<int> hret(nw, 0);
         SC_TRY(pist_h0(c, P, pp, nw, e0, cnt_low, cnt_high, act, hret, stats, s));
         bool any = false;
+        // DSYEV on the leading (cnt_high - stepeig) and cnt_high blocks of
+        // every window still running, all in one launch
+        std::vector<int> wins;
+        std::vector<EigvalJob> jobs;
         for (int k = 0; k < nw; k++) {
             if (done[k]) continue;
             if (hret[k] < 0 || !act[k]) {
@@ -769,17 +773,24 @@ int pist_conv(scalit_ctx *c, int nw, const double *e0, const PistParams &pp, con
                 act[k] = 0;
                 continue;
             }
-            // DSYEV on the leading (cnt_high - stepeig) and cnt_high blocks
             const double *hk = P.hmat.p + (long)k * m_max * m_max;
+            double *wk = P.eigw.p + (long)2 * wins.size() * m_max;
+            jobs.push_back({hk, wk, cnt_high - pp.stepeig, m_max});
+            jobs.push_back({hk, wk + m_max, cnt_high, m_max});
+            wins.push_back(k);
+        }
+        std::vector<double> weig((size_t)2 * wins.size() * m_max);
+        if (!wins.empty()) {
+            SC_TRY(small_sym_eigvals_batch((int)jobs.size(), jobs.data(), P.ejobs, P.zw, s));
+            HIP_TRY(hipMemcpyAsync(weig.data(), P.eigw.p, 8 * weig.size(), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        for (size_t wi = 0; wi < wins.size(); wi++) {
+            const int k = wins[wi];
             int num = cnt_high - pp.stepeig;
-            SC_TRY(small_sym_eig(num, hk, m_max, P.eigw.p, nullptr, 0, P.zw.p, s));
-            HIP_TRY(hipMemcpyAsync(tmpeig.data(), P.eigw.p, 8 * num, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
-            get_window0(e0[k], num, tmpeig.data(), m, ind.data(), oldeig.data());
+            get_window0(e0[k], num, weig.data() + 2 * wi * m_max, m, ind.data(), oldeig.data());
             num = cnt_high;
-            SC_TRY(small_sym_eig(num, hk, m_max, P.eigw.p, nullptr, 0, P.zw.p, s));
-            HIP_TRY(hipMemcpyAsync(tmpeig.data(), P.eigw.p, 8 * num, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
+            std::memcpy(tmpeig.data(), weig.data() + (2 * wi + 1) * m_max, 8 * (size_t)num);
             get_window0(e0[k], num, tmpeig.data(), m, ind.data(), eigk.data());
             double r = pp.ntype <= 0 ? get_diff_max(m, eigk.data(), oldeig.data())
                                      : get_sep_max(m, oldeig.data(), cnt_high, tmpeig.data());
@@ -922,13 +933,14 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
     SC_TRY(part2.alloc(NPART));
     SC_TRY(nrm.alloc(4));
     SC_TRY(tri.alloc((size_t)m_max * m_max));
-    SC_TRY(ev.alloc(m_max));
+    SC_TRY(ev.alloc(2 * (size_t)m_max));
     SC_TRY(work.alloc(small_sym_eig_work(m_max)));
+    DBuf<EigvalJob> ejobs;
     (void)P;
     HIP_TRY(hipMemsetAsync(alpha.p, 0, 8 * m_max, s));
     HIP_TRY(hipMemsetAsync(beta.p, 0, 8 * m_max, s));
     int ret = 0;
-    std::vector<double> w_h(m_max), oldeig(m), eigk(m);
+    std::vector<double> w_h(2 * (size_t)m_max), oldeig(m), eigk(m);
     std::vector<int> ind(m);
     // VJ(:,1) = X/||X||
     launch_dot_partials(1, N, x, ldv, x, ldv, part.p, s);
@@ -954,6 +966,7 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
         tri.release();
         ev.release();
         work.release();
+        ejobs.release();
         zero.release();
         return r;
     };
@@ -988,17 +1001,16 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
             if (nn[0] == 0.0 || bj == 0.0) bad = true;
         }
         if (bad) return fail(-(cnt_low - 1));
-        int num = cnt_high - stepeig;
-        hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(1024), 0, s, num, alpha.p, beta.p, tri.p, m_max);
-        SC_TRY(small_sym_eig(num, tri.p, m_max, ev.p, nullptr, 0, work.p, s));
-        HIP_TRY(hipMemcpyAsync(w_h.data(), ev.p, 8 * num, hipMemcpyDeviceToHost, s));
+        // eigenvalues of the leading (cnt_high - stepeig) and cnt_high blocks
+        // of the tridiagonal matrix, in one launch
+        hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(1024), 0, s, cnt_high, alpha.p, beta.p, tri.p, m_max);
+        const EigvalJob jobs[2] = {{tri.p, ev.p, cnt_high - stepeig, m_max}, {tri.p, ev.p + m_max, cnt_high, m_max}};
+        SC_TRY(small_sym_eigvals_batch(2, jobs, ejobs, work, s));
+        HIP_TRY(hipMemcpyAsync(w_h.data(), ev.p, 8 * 2 * (size_t)m_max, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        get_window0(e0, num, w_h.data(), m, ind.data(), oldeig.data());
-        num = cnt_high;
-        hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(1024), 0, s, num, alpha.p, beta.p, tri.p, m_max);
-        SC_TRY(small_sym_eig(num, tri.p, m_max, ev.p, nullptr, 0, work.p, s));
-        HIP_TRY(hipMemcpyAsync(w_h.data(), ev.p, 8 * num, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        get_window0(e0, cnt_high - stepeig, w_h.data(), m, ind.data(), oldeig.data());
+        const int num = cnt_high;
+        std::memmove(w_h.data(), w_h.data() + m_max, 8 * (size_t)num);
         get_window0(e0, num, w_h.data(), m, ind.data(), eigk.data());
         double r = ntype <= 0 ? get_diff_max(m, eigk.data(), oldeig.data())
                               : get_sep_max(m, oldeig.data(), cnt_high, w_h.data());

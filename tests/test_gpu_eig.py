@@ -2,8 +2,9 @@
 reference calls LAPACK DSYEV / DSTEV there: src/iterate/libiterate/pist/pist.f90,
 src/iterate/libiterate/lanczos/lan.f90) against numpy's eigvalsh / eigh.
 
-Eigenvalues only: Householder tridiagonalisation + bisection (matrix in LDS up
-to n = 112, in device memory above); with eigenvectors: cyclic Jacobi."""
+Eigenvalues only: Householder tridiagonalisation + Sturm-count multisection
+(matrix in LDS up to n = 112, in device memory above), one problem or several
+in one launch; with eigenvectors: cyclic Jacobi."""
 import ctypes as C
 
 import numpy as np
@@ -77,3 +78,38 @@ def test_eig_arg_errors():
     a = np.eye(2)
     assert L.lib().scalit_debug_sym_eig(0, a.ctypes.data_as(_dp), 2, w.ctypes.data_as(_dp), None, 2) == L.SCALIT_EINVAL
     assert L.lib().scalit_debug_sym_eig(2, a.ctypes.data_as(_dp), 1, w.ctypes.data_as(_dp), None, 2) == L.SCALIT_EINVAL
+
+
+@pytest.mark.parametrize("ns", [[12, 16, 11, 24, 1, 2], [96, 112, 40, 7], [100, 150, 200, 3]])
+def test_eigvals_batch(ns):
+    """Several problems in one launch (as the PIST windows' Rayleigh-Ritz
+    steps run): each against eigvalsh, and bit for bit against the one-problem
+    launch, which runs the same arithmetic."""
+    lda = max(ns)
+    a = np.zeros((len(ns), lda, lda))
+    mats = []
+    for j, n in enumerate(ns):
+        m = _sym(n, 300 + 7 * j + n)
+        mats.append(m)
+        a[j, :n, :n] = m.T   # column-major blocks
+        a[j, n:, :] = 9.9e30   # outside the leading block: never read
+        a[j, :, n:] = 9.9e30
+    w = np.zeros((len(ns), lda))
+    nsa = (C.c_int * len(ns))(*ns)
+    L.check(L.lib().scalit_debug_sym_eigvals_batch(len(ns), nsa, np.ascontiguousarray(a).ctypes.data_as(_dp), lda,
+                                                   w.ctypes.data_as(_dp)), "sym_eigvals_batch")
+    for j, n in enumerate(ns):
+        ref = np.linalg.eigvalsh(mats[j])
+        assert np.max(np.abs(w[j, :n] - ref)) <= 1e-12 * np.abs(ref).max() * max(1, n / 16)
+        assert np.all(np.diff(w[j, :n]) >= 0)
+        assert np.array_equal(w[j, :n], _eig(mats[j])[0])
+
+
+def test_eigvals_batch_arg_errors():
+    w = np.zeros(4)
+    a = np.eye(2).reshape(-1)
+    ns = (C.c_int * 2)(2, 3)
+    assert L.lib().scalit_debug_sym_eigvals_batch(2, ns, a.ctypes.data_as(_dp), 2, w.ctypes.data_as(_dp)) \
+        == L.SCALIT_EINVAL
+    assert L.lib().scalit_debug_sym_eigvals_batch(0, ns, a.ctypes.data_as(_dp), 2, w.ctypes.data_as(_dp)) \
+        == L.SCALIT_EINVAL
