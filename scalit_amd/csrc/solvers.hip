@@ -146,7 +146,8 @@ struct QmrWS {
     double g_tol = -1;
     long g_gen = -1;   // operator buffer generation the graph was recorded with
     std::vector<QmrScal> hscal;
-    std::vector<int> hact;
+    std::vector<int> hact, hones;
+    std::vector<typename ShiftT<S>::type> hshift;
 
     S *col(int which) { return vecs.p + (long)which * nvec * ld; }
     S *B() { return col(0); }
@@ -293,14 +294,13 @@ static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double t
     const int nv = w.nvec;
     const long n = w.n;
     const int conv = std::abs(ntype) == 1 ? 1 : (std::abs(ntype) == 2 ? 2 : 0);
-    {
-        std::vector<typename ShiftT<S>::type> hs(nv);
-        for (int i = 0; i < nv; i++) hs[i] = (typename ShiftT<S>::type)shifts[i];
-        HIP_TRY(hipMemcpyAsync(w.shifts.p, hs.data(), sizeof(hs[0]) * nv, hipMemcpyHostToDevice, s));
-        std::vector<int> ones(nv, 1);
-        HIP_TRY(hipMemcpyAsync(w.active.p, ones.data(), sizeof(int) * nv, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
-    }
+    // the host sources live in the workspace, so the copies need no wait here:
+    // they are rewritten only by the next solve, after this one's last wait
+    w.hshift.resize(nv);
+    for (int i = 0; i < nv; i++) w.hshift[i] = (typename ShiftT<S>::type)shifts[i];
+    HIP_TRY(hipMemcpyAsync(w.shifts.p, w.hshift.data(), sizeof(w.hshift[0]) * nv, hipMemcpyHostToDevice, s));
+    w.hones.assign(nv, 1);
+    HIP_TRY(hipMemcpyAsync(w.active.p, w.hones.data(), sizeof(int) * nv, hipMemcpyHostToDevice, s));
     QmrVecs<S> v = w.view();
     // initialisation (qmr.f90:36-50)
     launch_qmr_init<S>(nv, n, ntype <= 0 ? 1 : 0, v, w.part(0), s);
@@ -338,14 +338,19 @@ static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double t
             else SC_TRY(qmr_iteration<S>(c, w, conv, qmrmax, tol, s, cb));
         }
         done_iters += k;
+        // the flags and the scalars together: after the last check the
+        // results are already on the host
         HIP_TRY(hipMemcpyAsync(w.hact.data(), w.active.p, sizeof(int) * nv, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(w.hscal.data(), w.scal.p, sizeof(QmrScal) * nv, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         bool any = false;
         for (int i = 0; i < nv; i++) any = any || w.hact[i];
         if (!any) break;
     }
-    HIP_TRY(hipMemcpyAsync(w.hscal.data(), w.scal.p, sizeof(QmrScal) * nv, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    if (done_iters == 0) {
+        HIP_TRY(hipMemcpyAsync(w.hscal.data(), w.scal.p, sizeof(QmrScal) * nv, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
     for (int i = 0; i < nv; i++) {
         if (ret) ret[i] = w.hscal[i].ret;
         if (res) res[i] = w.hscal[i].res;
@@ -488,6 +493,29 @@ int mgs_host(scalit_ctx *c, long n, const double *wj, int m, double *vj, int *ok
     return SCALIT_OK;
 }
 
+// Copy na columns of n doubles, column i from src + sidx[i] sld to
+// dst + didx[i] dld (a null index list means i); runs of consecutive columns
+// go as one 2-D copy (one launch) while both pitches stay below 2 GiB.
+static int copy_cols(long n, int na, double *dst, long dld, const int *didx, const double *src, long sld,
+                     const int *sidx, hipStream_t s)
+{
+    auto si = [&](int k) { return sidx ? sidx[k] : k; };
+    auto di = [&](int k) { return didx ? didx[k] : k; };
+    const bool rect = 8 * sld < 0x7fffffffL && 8 * dld < 0x7fffffffL;
+    for (int i = 0; i < na;) {
+        int j = i + 1;
+        while (rect && j < na && si(j) == si(j - 1) + 1 && di(j) == di(j - 1) + 1) j++;
+        if (j - i == 1)
+            HIP_TRY(hipMemcpyAsync(dst + (long)di(i) * dld, src + (long)si(i) * sld, 8 * n, hipMemcpyDeviceToDevice,
+                                   s));
+        else
+            HIP_TRY(hipMemcpy2DAsync(dst + (long)di(i) * dld, 8 * dld, src + (long)si(i) * sld, 8 * sld, 8 * n, j - i,
+                                     hipMemcpyDeviceToDevice, s));
+        i = j;
+    }
+    return SCALIT_OK;
+}
+
 // Mixed precision: inner QMR in fp32 / bf16 on the residual, outer fp64
 // refinement x += d, r = b - (H - E0) x, until the reference's type-1
 // criterion ||(H-E0)x - b|| / ||(H-E0)x|| < tol (or ||r||/||b|| for other
@@ -518,10 +546,8 @@ int qmr_mixed(scalit_ctx *c, int prec, int ntype, int qmrmax, double tol, int in
     // ||b||
     SC_TRY(dot_host(nvec, N, b, ldb, b, ldb, bn.data(), P.part.p, P.part2.p, s));
     // x = 0, r = b
-    for (int v = 0; v < nvec; v++) {
-        HIP_TRY(hipMemsetAsync(x + (long)v * ldx, 0, 8 * N, s));
-        HIP_TRY(hipMemcpyAsync(P.r64.p + (long)v * N, b + (long)v * ldb, 8 * N, hipMemcpyDeviceToDevice, s));
-    }
+    HIP_TRY(hipMemset2DAsync(x, 8 * ldx, 0, 8 * N, nvec, s));
+    SC_TRY(copy_cols(N, nvec, P.r64.p, N, nullptr, b, ldb, nullptr, s));
     const int outer_max = std::max(1, qmrmax);
     int outer = 0;
     for (; outer < outer_max; outer++) {
@@ -544,10 +570,8 @@ int qmr_mixed(scalit_ctx *c, int prec, int ntype, int qmrmax, double tol, int in
         // fp64 residual: hx = (H - E0) x ;  r = b - hx
         SC_TRY(hx_apply_f64(c, shd.p, c->e0, nvec, x, ldx, P.tmp.p, N, false, nullptr, s));
         SC_TRY(dot_host(nvec, N, P.tmp.p, N, P.tmp.p, N, hh.data(), P.part.p, P.part2.p, s));
-        for (int v = 0; v < nvec; v++) {
-            HIP_TRY(hipMemcpyAsync(P.r64.p + (long)v * N, b + (long)v * ldb, 8 * N, hipMemcpyDeviceToDevice, s));
-            launch_axpby(1, N, -1.0, P.tmp.p + (long)v * N, N, 1.0, P.r64.p + (long)v * N, N, s);
-        }
+        SC_TRY(copy_cols(N, nvec, P.r64.p, N, nullptr, b, ldb, nullptr, s));
+        launch_axpby(nvec, N, -1.0, P.tmp.p, N, 1.0, P.r64.p, N, s);
         SC_TRY(dot_host(nvec, N, P.r64.p, N, P.r64.p, N, rr.data(), P.part.p, P.part2.p, s));
         bool all = true;
         for (int v = 0; v < nvec; v++) {
@@ -573,6 +597,7 @@ struct PistParams {
 };
 
 // LinSolv for the active windows: W_k = (H - E0_k)^-1 V_k(:, j) ; returns per-window codes
+
 static int pist_linsolv(scalit_ctx *c, PistState &P, const PistParams &pp, int nw, const double *e0, int jcol,
                         const std::vector<int> &act, std::vector<int> &num, std::vector<int> &stats,
                         hipStream_t s)
@@ -588,23 +613,17 @@ static int pist_linsolv(scalit_ctx *c, PistState &P, const PistParams &pp, int n
     std::vector<int> r(na);
     if (pp.prec == SCALIT_FP64) {
         SC_TRY(P.q64.ensure(na, N));
-        for (int i = 0; i < na; i++)
-            HIP_TRY(hipMemcpyAsync(P.q64.B() + (long)i * P.q64.ld,
-                                   P.vj.p + (long)idx[i] * P.ldv * pp.m_max + (long)jcol * P.ldv, 8 * N,
-                                   hipMemcpyDeviceToDevice, s));
+        SC_TRY(copy_cols(N, na, P.q64.B(), P.q64.ld, nullptr, P.vj.p + (long)jcol * P.ldv, P.ldv * pp.m_max,
+                         idx.data(), s));
         SC_TRY(qmr_solve<double>(c, P.q64, pp.qmr_type, pp.qmrmax, pp.qmrtol, sh.data(), r.data(), nullptr, s));
-        for (int i = 0; i < na; i++)
-            HIP_TRY(hipMemcpyAsync(P.w.p + (long)idx[i] * P.ldv, P.q64.X() + (long)i * P.q64.ld, 8 * N,
-                                   hipMemcpyDeviceToDevice, s));
+        SC_TRY(copy_cols(N, na, P.w.p, P.ldv, idx.data(), P.q64.X(), P.q64.ld, nullptr, s));
     } else {
         SC_TRY(P.x64.alloc((size_t)na * N));
         SC_TRY(P.tmp.alloc((size_t)na * N));
         // gather the right-hand sides
         DBuf<double> &bb = P.bb;
         SC_TRY(bb.alloc((size_t)na * N));
-        for (int i = 0; i < na; i++)
-            HIP_TRY(hipMemcpyAsync(bb.p + (long)i * N, P.vj.p + (long)idx[i] * P.ldv * pp.m_max + (long)jcol * P.ldv,
-                                   8 * N, hipMemcpyDeviceToDevice, s));
+        SC_TRY(copy_cols(N, na, bb.p, N, nullptr, P.vj.p + (long)jcol * P.ldv, P.ldv * pp.m_max, idx.data(), s));
         std::vector<double> rs(na);
         // inner tolerance: the requested one, floored at 1e-3 (2e-3 for BF16
         // storage).  Each FP64 refinement step then gains about three digits,
@@ -614,9 +633,7 @@ static int pist_linsolv(scalit_ctx *c, PistState &P, const PistParams &pp, int n
         double itol = std::max(pp.qmrtol, pp.prec == SCALIT_BF16 ? 2e-3 : 1e-3);
         SC_TRY(qmr_mixed(c, pp.prec, pp.qmr_type, 8, pp.qmrtol, pp.qmrmax, itol, na, bb.p, N, P.x64.p, N, sh.data(),
                          r.data(), rs.data(), s));
-        for (int i = 0; i < na; i++)
-            HIP_TRY(hipMemcpyAsync(P.w.p + (long)idx[i] * P.ldv, P.x64.p + (long)i * N, 8 * N,
-                                   hipMemcpyDeviceToDevice, s));
+        SC_TRY(copy_cols(N, na, P.w.p, P.ldv, idx.data(), P.x64.p, N, nullptr, s));
     }
     for (int i = 0; i < na; i++) {
         num[idx[i]] = r[i];
@@ -635,8 +652,7 @@ static int pist_mgs(scalit_ctx *c, PistState &P, int nw, int m_max, int jnew, co
     const long ldw = P.ldv * m_max;
     double *col = P.vj.p + (long)jnew * P.ldv;   // window 0's column jnew; window k at + k*ldw
     // VJ(:, M0) = WJ ; normw = ||WJ||
-    for (int k = 0; k < nw; k++)
-        HIP_TRY(hipMemcpyAsync(col + (long)k * ldw, P.w.p + (long)k * P.ldv, 8 * N, hipMemcpyDeviceToDevice, s));
+    SC_TRY(copy_cols(N, nw, col, ldw, nullptr, P.w.p, P.ldv, nullptr, s));
     launch_dot_partials(nw, N, P.w.p, P.ldv, P.w.p, P.ldv, P.part.p, s);
     launch_reduce_partials(nw, P.part.p, P.tmp.p, s);
     std::vector<double> nrm(nw);
