@@ -290,17 +290,27 @@ def pist_bench(no_cpu):
     # rounding-level differences in the inner solves, so the batched figure is
     # the mean over several sets of start vectors (each timed after a warm-up run
     # that records the batched QMR graph).
-    centres = [float(ref[2 * k:2 * k + 2].mean()) for k in range(10)]
+    # A second layout: 20 windows of one level each, every centre a quarter of
+    # the smaller neighbouring gap inside its level's interval (so that its
+    # nearest level is that level), as when approximate levels from an earlier
+    # run are refined.  The one-level windows converge in fewer Krylov steps.
+    centres2 = [float(ref[2 * k:2 * k + 2].mean()) for k in range(10)]
+    gaps = np.diff(ref)
+    centres1 = [float(ref[k] + 0.25 * min(gaps[max(k - 1, 0)], gaps[k])) for k in range(19)]
+    centres1.append(float(ref[19] - 0.25 * gaps[18]))
     seeds = [g["start_seed"] + i for i in range(4)]
-    for name, prec in (("batched_fp64", L.FP64), ("batched_fp32", L.FP32)):
+    for name, prec, centres, m_win, start in (("batched_fp64", L.FP64, centres2, 2, 12),
+                                              ("batched_fp32", L.FP32, centres2, 2, 12),
+                                              ("batched_fp64_1level", L.FP64, centres1, 1, 4)):
+        nwin = len(centres)
         times, errs, iters, conv, krylov = [], [], [], True, []
         for i, seed in enumerate(seeds):
-            x0b = ctx.to_device(P.random_vectors(prob.N, 10, seed))
+            x0b = ctx.to_device(P.random_vectors(prob.N, nwin, seed))
             for rep in range(2 if i == 0 else 1):
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                ret, eig, rr, stats = ctx.pist_conv_batched(x0b, centres, 1e-11, 2, 200, 12, 4, 1, qmrtol=1e-6,
-                                                            prec=prec)
+                ret, eig, rr, stats = ctx.pist_conv_batched(x0b, centres, 1e-11, m_win, 200, start, 4, 1,
+                                                            qmrtol=1e-6, prec=prec)
                 torch.cuda.synchronize()
                 dt = time.perf_counter() - t0
             found = np.sort(eig.reshape(-1))
@@ -310,7 +320,7 @@ def pist_bench(no_cpu):
             krylov.append(int(stats[2]))
             conv = conv and all(int(v) > 0 for v in ret)
         dt = float(np.mean(times))
-        res[name] = {"windows": 10, "levels_per_window": 2, "start_seeds": seeds,
+        res[name] = {"windows": nwin, "levels_per_window": m_win, "start_seeds": seeds,
                      "seconds_per_seed": [round(t, 4) for t in times], "seconds": round(dt, 4),
                      "eigenpairs_per_s": round(20 / dt, 3), "max_rel_err": max(errs), "qmr_iters": iters,
                      "krylov": krylov, "speedup_vs_fp64": round((20 / dt) / res["fp64"]["eigenpairs_per_s"], 3),

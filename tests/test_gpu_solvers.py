@@ -140,6 +140,25 @@ def test_pist_batched_windows():
     ctx.close()
 
 
+def test_pist_batched_one_level_windows():
+    """20 windows of one level each (the batched layout the throughput figure
+    uses): every golden level is the one its window finds, to 1e-10."""
+    g = gold()
+    prob = P.config_problem("pist_fp64")
+    ctx = Context.from_problem(prob, e0=g["e0"])
+    ctx.block_diag(1000, 1e-5)
+    ref = np.array(g["eigsh"])
+    gaps = np.diff(ref)
+    e0s = [float(ref[k] + 0.25 * min(gaps[max(k - 1, 0)], gaps[k])) for k in range(19)]
+    e0s.append(float(ref[19] - 0.25 * gaps[18]))
+    x0 = ctx.to_device(P.random_vectors(prob.N, 20, 77))
+    ret, eig, res, stats = ctx.pist_conv_batched(x0, e0s, 1e-11, 1, 200, 4, 4, 1, qmrtol=1e-6)
+    print("one-level windows", ret, stats, flush=True)
+    assert (ret > 0).all()
+    assert (np.abs(eig.reshape(-1) - ref) / np.abs(ref)).max() <= 1e-10
+    ctx.close()
+
+
 def test_lanczos_small_grid():
     prob = P.config_problem("pist_fp64", grid=(12, 14, 10))
     op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
