@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--solver-run", action="store_true",
                     help="time the solver path on the 64x96x80 operator (OSB set-up, P*X at batch 1 and 16, "
                          "QMR) and print one line per measurement; prints no JSON")
+    ap.add_argument("--pist-run", action="store_true",
+                    help="run only the PIST measurements on the 24x32x24 workload (no H*X bench, no host "
+                         "baseline) and print them as one JSON line, for profiling that path")
     ap.add_argument("--pmc-run", action="store_true",
                     help="workload for rocprofv3 --pmc: three 512 MiB device copies (the counter calibration), "
                          "then --steps H*X calls; prints no JSON")
@@ -97,6 +100,9 @@ def main():
         return
     if args.solver_run:
         solver_run(args.steps)
+        return
+    if args.pist_run:
+        print(json.dumps({"pist": pist_bench(True)}), flush=True)
         return
     prob = P.config_problem("hx_bench")
     ctx = Context.from_problem(prob)
