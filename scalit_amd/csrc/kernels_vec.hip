@@ -685,6 +685,95 @@ void launch_normalize(int nw, long n, const double *w, long ldwk, double *dst, l
     hipLaunchKernelGGL(k_normalize, dim3(nparts(n), nw), dim3(256), 0, s, n, w, ldwk, dst, ldd, part, ok, active);
 }
 
+// MGS_ORTH (gsorth.f90:42-80) of a short vector in one workgroup per window:
+// W is copied to C, its norm checked, the projections on V(:,0..jnew-1) taken
+// one after the other (each dot product reduced in the workgroup, then the
+// update), and C normalised; ok = 0 on a zero norm before or after.  The
+// vector lives in registers (PER values per thread), so a projection is one
+// read of V(:,i) and one workgroup reduction instead of a launch of its own.
+// W may be C itself: all of W is read before C is written.
+constexpr int MGS_NT = 1024;
+template <int PER>
+__global__ void __launch_bounds__(MGS_NT) k_mgs_one(long n, int jnew, const double *__restrict__ vj, long ldv,
+                                                    long ldw, const double *w, long ldwk, double *col,
+                                                    const int *__restrict__ active,
+                                                    int *__restrict__ ok)
+{
+    __shared__ double red[2][MGS_NT / 64];
+    const int k = blockIdx.x;
+    if (active && !active[k]) return;
+    const double *V = vj + (long)k * ldw;
+    const double *W = w + (long)k * ldwk;
+    double *C = col + (long)k * ldw;
+    const int tid = threadIdx.x;
+    int par = 0;
+    // the sum over the workgroup, the same on every thread; two buffers, so
+    // one barrier per reduction suffices
+    auto wg_sum = [&](double v) -> double {
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((tid & 63) == 0) red[par][tid >> 6] = v;
+        __syncthreads();
+        double r = 0;
+#pragma unroll
+        for (int q = 0; q < MGS_NT / 64; q++) r += red[par][q];
+        par ^= 1;
+        return r;
+    };
+    double a[PER];
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+        const long t = tid + (long)e * MGS_NT;
+        a[e] = t < n ? W[t] : 0.0;
+    }
+    double s = 0;
+#pragma unroll
+    for (int e = 0; e < PER; e++) s += a[e] * a[e];
+    const double normw = sqrt(wg_sum(s));
+    if (normw != 0.0) {
+        for (int i = 0; i < jnew; i++) {
+            const double *vi = V + (long)i * ldv;
+            double b[PER];
+#pragma unroll
+            for (int e = 0; e < PER; e++) {
+                const long t = tid + (long)e * MGS_NT;
+                b[e] = t < n ? vi[t] : 0.0;
+            }
+            double d = 0;
+#pragma unroll
+            for (int e = 0; e < PER; e++) d += a[e] * b[e];
+            d = wg_sum(d);
+#pragma unroll
+            for (int e = 0; e < PER; e++) a[e] -= d * b[e];
+        }
+        s = 0;
+#pragma unroll
+        for (int e = 0; e < PER; e++) s += a[e] * a[e];
+    }
+    const double nrm = normw != 0.0 ? sqrt(wg_sum(s)) : 0.0;
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+        const long t = tid + (long)e * MGS_NT;
+        if (t < n) C[t] = nrm != 0.0 ? a[e] / nrm : a[e];
+    }
+    if (tid == 0) ok[k] = nrm != 0.0;
+}
+
+bool launch_mgs_one(int nw, long n, int jnew, const double *vj, long ldv, long ldw, const double *w, long ldwk,
+                    double *col, const int *active, int *ok, hipStream_t s)
+{
+    if (n <= 8L * MGS_NT)
+        hipLaunchKernelGGL(k_mgs_one<8>, dim3(nw), dim3(MGS_NT), 0, s, n, jnew, vj, ldv, ldw, w, ldwk, col, active, ok);
+    else if (n <= 16L * MGS_NT)
+        hipLaunchKernelGGL(k_mgs_one<16>, dim3(nw), dim3(MGS_NT), 0, s, n, jnew, vj, ldv, ldw, w, ldwk, col, active,
+                           ok);
+    else if (n <= 20L * MGS_NT)
+        hipLaunchKernelGGL(k_mgs_one<20>, dim3(nw), dim3(MGS_NT), 0, s, n, jnew, vj, ldv, ldw, w, ldwk, col, active,
+                           ok);
+    else
+        return false;
+    return true;
+}
+
 // Multi-dot: part[k][i][blk] = partial of <V_k(:, i0+i), w_k> for i < m
 // (the Rayleigh-Ritz column of pistH0, pistHij.f90:375-381).
 __global__ void __launch_bounds__(256) k_multidot(long n, int i0, int m, const double *vj, long ldv, long ldw,

@@ -467,20 +467,24 @@ int mgs_host(scalit_ctx *c, long n, const double *wj, int m, double *vj, int *ok
     HIP_TRY(hipMemcpyAsync(v.p, vj, 8 * (size_t)m * n, hipMemcpyHostToDevice, s));
     double *col = v.p + (long)m * ld;
     HIP_TRY(hipMemcpyAsync(col, wj, 8 * n, hipMemcpyHostToDevice, s));
-    launch_dot_partials(1, n, col, ld, col, ld, pa.p, s);
-    launch_reduce_partials(1, pa.p, nr.p, s);
     double nw = 0;
-    HIP_TRY(hipMemcpyAsync(&nw, nr.p, 8, hipMemcpyDeviceToHost, s));
     int okh = 0;
-    if (m > 0) launch_mgs_step(1, n, -1, 0, v.p, ld, 0, col, 0, nullptr, pa.p, nullptr, s);
-    double *p0 = pa.p, *p1 = pb.p;
-    for (int i = 0; i < m; i++) {
-        int inext = (i + 1 < m) ? i + 1 : -1;
-        launch_mgs_step(1, n, i, inext, v.p, ld, 0, col, 0, p0, p1, nullptr, s);
-        std::swap(p0, p1);
+    if (launch_mgs_one(1, n, m, v.p, ld, 0, col, 0, col, nullptr, okd.p, s)) {
+        nw = 1.0;   // a zero norm of WJ shows in ok
+    } else {
+        launch_dot_partials(1, n, col, ld, col, ld, pa.p, s);
+        launch_reduce_partials(1, pa.p, nr.p, s);
+        HIP_TRY(hipMemcpyAsync(&nw, nr.p, 8, hipMemcpyDeviceToHost, s));
+        if (m > 0) launch_mgs_step(1, n, -1, 0, v.p, ld, 0, col, 0, nullptr, pa.p, nullptr, s);
+        double *p0 = pa.p, *p1 = pb.p;
+        for (int i = 0; i < m; i++) {
+            int inext = (i + 1 < m) ? i + 1 : -1;
+            launch_mgs_step(1, n, i, inext, v.p, ld, 0, col, 0, p0, p1, nullptr, s);
+            std::swap(p0, p1);
+        }
+        if (m == 0) launch_dot_partials(1, n, col, ld, col, ld, p0, s);
+        launch_normalize(1, n, col, ld, col, ld, p0, okd.p, nullptr, s);
     }
-    if (m == 0) launch_dot_partials(1, n, col, ld, col, ld, p0, s);
-    launch_normalize(1, n, col, ld, col, ld, p0, okd.p, nullptr, s);
     HIP_TRY(hipMemcpyAsync(&okh, okd.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(vj + (long)m * n, col, 8 * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -651,6 +655,11 @@ static int pist_mgs(scalit_ctx *c, PistState &P, int nw, int m_max, int jnew, co
     const long N = c->N;
     const long ldw = P.ldv * m_max;
     double *col = P.vj.p + (long)jnew * P.ldv;   // window 0's column jnew; window k at + k*ldw
+    if (launch_mgs_one(nw, N, jnew, P.vj.p, P.ldv, ldw, P.w.p, P.ldv, col, act_dev, P.ok.p, s)) {
+        HIP_TRY(hipMemcpyAsync(ok.data(), P.ok.p, sizeof(int) * nw, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return SCALIT_OK;
+    }
     // VJ(:, M0) = WJ ; normw = ||WJ||
     SC_TRY(copy_cols(N, nw, col, ldw, nullptr, P.w.p, P.ldv, nullptr, s));
     launch_dot_partials(nw, N, P.w.p, P.ldv, P.w.p, P.ldv, P.part.p, s);

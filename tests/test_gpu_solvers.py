@@ -205,6 +205,31 @@ def test_hellmann_feynman_gradients():
     ctx.close()
 
 
+@pytest.mark.parametrize("n", [700, 16385, 20480, 20481])
+def test_mgs_orth_lengths(n):
+    """MGS_ORTH through the Fortran-layer entry at lengths either side of the
+    one-workgroup kernel's limits (8, 16 and 20 values per thread; 20481 takes
+    the multi-launch path), against the oracle's restatement; a zero W fails."""
+    prob, op, ctx = op_ctx((8, 9, 6), e0=0.0)
+    ctx.bind()
+    lib = L.lib()
+    ci = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    m = 7
+    q, _ = np.linalg.qr(np.random.default_rng(n).standard_normal((n, m)))
+    vj = np.zeros((n, m + 1), order="F")
+    vj[:, :m] = q
+    w = q @ np.arange(1.0, m + 1) + 0.1 * np.random.default_rng(n + 1).standard_normal(n)
+    vo = vj.copy(order="F")
+    assert lib.mgs_orth_(ci(n), O.dptr(w), ci(m), O.dptr(vj)) == 1
+    ok_o, _ = O.mgs_orth(w, vo, m)
+    assert ok_o
+    assert np.abs(vj - vo).max() <= 1e-11
+    assert np.abs(vj.T @ vj - np.eye(m + 1)).max() <= 1e-12
+    z = np.zeros(n)
+    assert lib.mgs_orth_(ci(n), O.dptr(z), ci(m), O.dptr(vj)) == 0
+    ctx.close()
+
+
 def test_fortran_layer_drop_in():
     # the reference's call pattern: QMR(nType,QMRMAX,QMRTOL,N,B,HX,PX,X,RES) on host arrays
     prob, op, ctx = op_ctx((12, 14, 10), e0=-0.002)
