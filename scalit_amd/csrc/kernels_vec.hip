@@ -775,23 +775,43 @@ bool launch_mgs_one(int nw, long n, int jnew, const double *vj, long ldv, long l
 }
 
 // Multi-dot: part[k][i][blk] = partial of <V_k(:, i0+i), w_k> for i < m
-// (the Rayleigh-Ritz column of pistH0, pistHij.f90:375-381).
+// (the Rayleigh-Ritz column of pistH0, pistHij.f90:375-381).  blockIdx.z picks
+// a group of MD_CG columns: a thread keeps one running sum per column of its
+// group while it walks its points once, and the group's sums share one
+// workgroup reduction.
+constexpr int MD_CG = 8;
 __global__ void __launch_bounds__(256) k_multidot(long n, int i0, int m, const double *vj, long ldv, long ldw,
                                                   const double *w, long ldwk, double *part, const int *active)
 {
-    __shared__ double red[4];
+    __shared__ double red[4][MD_CG];
     const int k = blockIdx.y;
     if (active && !active[k]) return;
-    const double *V = vj + (long)k * ldw;
+    const int c0 = blockIdx.z * MD_CG;
+    const int nc = m - c0 < MD_CG ? m - c0 : MD_CG;
+    const double *V = vj + (long)k * ldw + (long)(i0 + c0) * ldv;
     const double *W = w + (long)k * ldwk;
-    for (int i = 0; i < m; i++) {
-        const double *vi = V + (long)(i0 + i) * ldv;
-        double s = 0;
-        GSTRIDE(t, n) s += vi[t] * W[t];
-        s = bsum(s, red);
-        if (threadIdx.x == 0) part[((long)k * m + i) * NPART + blockIdx.x] = s;
-        zero_tail(part + ((long)k * m + i) * NPART);
+    double s[MD_CG];
+#pragma unroll
+    for (int c = 0; c < MD_CG; c++) s[c] = 0.0;
+    GSTRIDE(t, n)
+    {
+        const double x = W[t];
+#pragma unroll
+        for (int c = 0; c < MD_CG; c++)
+            if (c < nc) s[c] += V[(long)c * ldv + t] * x;
     }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < MD_CG; c++) {
+        for (int o = 32; o > 0; o >>= 1) s[c] += __shfl_down(s[c], o, 64);
+        if (lane == 0) red[wv][c] = s[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < nc) {
+        const int c = threadIdx.x;
+        part[((long)k * m + c0 + c) * NPART + blockIdx.x] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+    }
+    for (int c = 0; c < nc; c++) zero_tail(part + ((long)k * m + c0 + c) * NPART);
 }
 
 // hmat_k(i0+i, j) = hmat_k(j, i0+i) = sum of partials   (upper and lower)
@@ -811,7 +831,9 @@ __global__ void k_multidot_fin(int nw, int m, int i0, int j, const double *part,
 void launch_multidot(int nw, long n, int i0, int m, const double *vj, long ldv, long ldw, const double *w, long ldwk,
                      double *part, const int *active, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_multidot, dim3(nparts(n), nw), dim3(256), 0, s, n, i0, m, vj, ldv, ldw, w, ldwk, part, active);
+    if (m < 1) return;
+    hipLaunchKernelGGL(k_multidot, dim3(nparts(n), nw, (m + MD_CG - 1) / MD_CG), dim3(256), 0, s, n, i0, m, vj, ldv, ldw,
+                       w, ldwk, part, active);
 }
 
 void launch_multidot_fin(int nw, int m, int i0, int j, const double *part, double *hmat, int ldh, long ldhk,
