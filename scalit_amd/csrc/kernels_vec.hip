@@ -70,6 +70,26 @@ __device__ __forceinline__ double sum_parts(const double *p) { return wave_sum_p
 
 #define GSTRIDE(i, n) for (long i = blockIdx.x * 256L + threadIdx.x; i < (n); i += (long)gridDim.x * 256L)
 
+// The QMR kernels walk the same elements in the same order as GSTRIDE, in
+// chunks of QU per thread: every input of a chunk is loaded (from an index
+// clamped into the vector) before any output of it is stored, so a thread has
+// QU loads of each vector in flight.  Written element by element, each load
+// would wait for the store before it (the compiler cannot tell that the
+// vectors do not overlap), which serialises a thread's memory round trips.
+constexpr int QU = 4;
+#define QCHUNK(base, n) for (long base = blockIdx.x * 256L + threadIdx.x; base < (n); base += (long)gridDim.x * 256L * QU)
+#define QELEM(t, tc, base, e, n)                                                                                  \
+    const long t = (base) + (long)(e) * gridDim.x * 256L;                                                         \
+    const long tc = t < (n) ? t : (n) - 1
+// the value a store to S reads back as
+template <typename S>
+__device__ __forceinline__ double rt(double v)
+{
+    S tmp;
+    sv<S>(&tmp, 0, v);
+    return lv<S>(&tmp, 0);
+}
+
 // A grid of nparts(n) <= NPART workgroups writes partials 0..gridDim.x-1;
 // workgroup 0 zeroes the rest, so every consumer still adds all NPART slots in
 // index order (deterministic) while small vectors launch few workgroups.
@@ -280,10 +300,21 @@ __global__ void __launch_bounds__(256) k_qmr_b(long n, QmrVecs<S> w, QmrScal *q,
     const S *Z = w.Z + (long)v * w.ld;
     const bool first = q[v].iter == 1;
     const double c = first ? 0.0 : vnrm * vzdot / q[v].qsdot;
-    if (first) {
-        GSTRIDE(i, n) sv<S>(Q, i, lv<S>(Z, i) / zvrat / znrm);
-    } else {
-        GSTRIDE(i, n) sv<S>(Q, i, lv<S>(Z, i) / zvrat / znrm - c * lv<S>(Q, i));
+    QCHUNK(base, n)
+    {
+        double z[QU], qo[QU];
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            z[e] = lv<S>(Z, tc);
+            qo[e] = first ? 0.0 : lv<S>(Q, tc);
+        }
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            (void)tc;
+            if (t < n) sv<S>(Q, t, first ? z[e] / zvrat / znrm : z[e] / zvrat / znrm - c * qo[e]);
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         q[v].vzdot = vzdot;
@@ -302,10 +333,25 @@ __global__ void __launch_bounds__(256) k_qmr_c(long n, QmrVecs<S> w, const QmrSc
     S *Sv = w.Sv + (long)v * w.ld;
     const S *Q = w.Q + (long)v * w.ld;
     double s = 0;
-    GSTRIDE(i, n)
+    QCHUNK(base, n)
     {
-        sv<S>(Sv, i, zvrat * lv<S>(Sv, i));
-        s += lv<S>(Q, i) * lv<S>(Sv, i);
+        double sv0[QU], qv[QU];
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            sv0[e] = lv<S>(Sv, tc);
+            qv[e] = lv<S>(Q, tc);
+        }
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            (void)tc;
+            if (t < n) {
+                const double a = zvrat * sv0[e];
+                sv<S>(Sv, t, a);
+                s += qv[e] * rt<S>(a);
+            }
+        }
     }
     s = bsum(s, red);
     if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
@@ -334,11 +380,26 @@ __global__ void __launch_bounds__(256) k_qmr_d(long n, QmrVecs<S> w, QmrScal *q,
     S *V = w.V + (long)v * w.ld;
     const S *Sv = w.Sv + (long)v * w.ld;
     double s = 0;
-    GSTRIDE(i, n)
+    QCHUNK(base, n)
     {
-        sv<S>(V, i, lv<S>(Sv, i) - beta * (lv<S>(V, i) / vnrm));
-        double a = lv<S>(V, i);
-        s += a * a;
+        double sv0[QU], vv[QU];
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            sv0[e] = lv<S>(Sv, tc);
+            vv[e] = lv<S>(V, tc);
+        }
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            (void)tc;
+            if (t < n) {
+                const double a = sv0[e] - beta * (vv[e] / vnrm);
+                sv<S>(V, t, a);
+                const double ar = rt<S>(a);
+                s += ar * ar;
+            }
+        }
     }
     s = bsum(s, red);
     zero_tail(part + (long)v * NPART);
@@ -364,11 +425,25 @@ __global__ void __launch_bounds__(256) k_qmr_e(long n, QmrVecs<S> w, QmrScal *q,
     const double vnrmpo = sqrt(sum_parts_n(part_vv + (long)v * NPART, n));
     const S *Z = w.Z + (long)v * w.ld, *V = w.V + (long)v * w.ld;
     double s = 0, svz = 0;
-    GSTRIDE(i, n)
+    QCHUNK(base, n)
     {
-        const double z = lv<S>(Z, i) / zvrat;
-        s += z * z;
-        svz += (lv<S>(V, i) / vnrmpo) * z;
+        double zv[QU], vv[QU];
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            zv[e] = lv<S>(Z, tc);
+            vv[e] = lv<S>(V, tc);
+        }
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            (void)tc;
+            if (t < n) {
+                const double z = zv[e] / zvrat;
+                s += z * z;
+                svz += (vv[e] / vnrmpo) * z;
+            }
+        }
     }
     s = bsum(s, red);
     zero_tail(part + (long)v * NPART);
@@ -464,22 +539,42 @@ __global__ void __launch_bounds__(256) k_qmr_f(long n, int conv, QmrVecs<S> w, Q
     const S *Q = w.Q + (long)v * w.ld, *Sv = w.Sv + (long)v * w.ld;
     S *X = w.x + (long)v * w.ldx;
     double srr = 0, sux = 0, sxx = 0, sur = 0;
-    GSTRIDE(i, n)
+    QCHUNK(base, n)
     {
-        double ux = first ? a * lv<S>(Q, i) : a * lv<S>(Q, i) + tg * lv<S>(UX, i);
-        double ur = first ? eta * lv<S>(Sv, i) : eta * lv<S>(Sv, i) + tg * lv<S>(UR, i);
-        sv<S>(UX, i, ux);
-        sv<S>(UR, i, ur);
-        sv<S>(X, i, lv<S>(X, i) + lv<S>(UX, i));
-        sv<S>(R, i, lv<S>(R, i) - lv<S>(UR, i));
-        if (conv != 1) {
-            double r = lv<S>(R, i);
-            srr += r * r;
-            if (conv == 2) {
-                double u = lv<S>(UX, i), xx = lv<S>(X, i), uu = lv<S>(UR, i);
-                sux += u * u;
-                sxx += xx * xx;
-                sur += uu * uu;
+        double qv[QU], sv0[QU], uxo[QU], uro[QU], xo[QU], ro[QU];
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            qv[e] = lv<S>(Q, tc);
+            sv0[e] = lv<S>(Sv, tc);
+            uxo[e] = first ? 0.0 : lv<S>(UX, tc);
+            uro[e] = first ? 0.0 : lv<S>(UR, tc);
+            xo[e] = lv<S>(X, tc);
+            ro[e] = lv<S>(R, tc);
+        }
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            (void)tc;
+            if (t < n) {
+                const double ux = first ? a * qv[e] : a * qv[e] + tg * uxo[e];
+                const double ur = first ? eta * sv0[e] : eta * sv0[e] + tg * uro[e];
+                sv<S>(UX, t, ux);
+                sv<S>(UR, t, ur);
+                const double uxr = rt<S>(ux), urr = rt<S>(ur);
+                const double xn = xo[e] + uxr, rn = ro[e] - urr;
+                sv<S>(X, t, xn);
+                sv<S>(R, t, rn);
+                if (conv != 1) {
+                    const double r = rt<S>(rn);
+                    srr += r * r;
+                    if (conv == 2) {
+                        const double xx = rt<S>(xn);
+                        sux += uxr * uxr;
+                        sxx += xx * xx;
+                        sur += urr * urr;
+                    }
+                }
             }
         }
     }
@@ -518,12 +613,25 @@ __global__ void __launch_bounds__(256) k_qmr_g(long n, QmrVecs<S> w, const QmrSc
     const S *H = w.HX0 + (long)v * w.ld;
     const S *b = w.b + (long)v * w.ldb;
     double s1 = 0, s2 = 0;
-    GSTRIDE(i, n)
+    QCHUNK(base, n)
     {
-        double h = lv<S>(H, i);
-        double r = h - lv<S>(b, i);
-        s1 += h * h;
-        s2 += r * r;
+        double hv[QU], bv[QU];
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            hv[e] = lv<S>(H, tc);
+            bv[e] = lv<S>(b, tc);
+        }
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            (void)tc;
+            if (t < n) {
+                const double r = hv[e] - bv[e];
+                s1 += hv[e] * hv[e];
+                s2 += r * r;
+            }
+        }
     }
     s1 = bsum(s1, red);
     if (threadIdx.x == 0) part_hh[(long)v * NPART + blockIdx.x] = s1;
