@@ -211,14 +211,26 @@ __global__ void __launch_bounds__(256) k_qmr_init(long n, int zero_x, QmrVecs<S>
     S *x = w.x + (long)v * w.ldx;
     S *R = w.R + (long)v * w.ld, *V = w.V + (long)v * w.ld;
     double s = 0;
-    GSTRIDE(i, n)
+    QCHUNK(base, n)
     {
-        S bi = b[i];
-        R[i] = bi;
-        V[i] = bi;
-        if (zero_x) x[i] = (S)0;
-        double d = lv<S>(b, i);
-        s += d * d;
+        S bv[QU];
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            bv[e] = b[tc];
+        }
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            (void)tc;
+            if (t < n) {
+                R[t] = bv[e];
+                V[t] = bv[e];
+                if (zero_x) x[t] = (S)0;
+                const double d = lv<S>(bv, e);
+                s += d * d;
+            }
+        }
     }
     s = bsum(s, red);
     if (threadIdx.x == 0) part[(long)v * NPART + blockIdx.x] = s;
@@ -741,20 +753,27 @@ __global__ void __launch_bounds__(256) k_mgs_step(long n, int i, int inext, cons
     if (i >= 0) {
         const double d = sum_parts_n(part_in + (long)k * NPART, n);
         const double *vi = V + (long)i * ldv;
-        if (inext >= 0) {
-            const double *vn = V + (long)inext * ldv;
-            GSTRIDE(t, n)
-            {
-                double a = W[t] - d * vi[t];
-                W[t] = a;
-                s += a * vn[t];
+        // with no next column the partial is of the updated W with itself
+        const double *vn = inext >= 0 ? V + (long)inext * ldv : nullptr;
+        QCHUNK(base, n)
+        {
+            double wv[QU], iv[QU], nv[QU];
+#pragma unroll
+            for (int e = 0; e < QU; e++) {
+                QELEM(t, tc, base, e, n);
+                wv[e] = W[tc];
+                iv[e] = vi[tc];
+                nv[e] = vn ? vn[tc] : 0.0;
             }
-        } else {
-            GSTRIDE(t, n)
-            {
-                double a = W[t] - d * vi[t];
-                W[t] = a;
-                s += a * a;
+#pragma unroll
+            for (int e = 0; e < QU; e++) {
+                QELEM(t, tc, base, e, n);
+                (void)tc;
+                if (t < n) {
+                    const double a = wv[e] - d * iv[e];
+                    W[t] = a;
+                    s += a * (vn ? nv[e] : a);
+                }
             }
         }
     } else {
@@ -777,7 +796,21 @@ __global__ void __launch_bounds__(256) k_normalize(long n, const double *w, long
     if (nrm == 0.0) return;
     const double *W = w + (long)k * ldwk;
     double *D = dst + (long)k * ldd;
-    GSTRIDE(t, n) D[t] = W[t] / nrm;
+    QCHUNK(base, n)
+    {
+        double wv[QU];
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            wv[e] = W[tc];
+        }
+#pragma unroll
+        for (int e = 0; e < QU; e++) {
+            QELEM(t, tc, base, e, n);
+            (void)tc;
+            if (t < n) D[t] = wv[e] / nrm;
+        }
+    }
 }
 
 void launch_mgs_step(int nw, long n, int i, int inext, const double *vj, long ldv, long ldw, double *w, long ldwk,
