@@ -331,6 +331,30 @@ def pist_bench(no_cpu):
                      "eigenpairs_per_s": round(20 / dt, 3), "max_rel_err": max(errs), "qmr_iters": iters,
                      "krylov": krylov, "speedup_vs_fp64": round((20 / dt) / res["fp64"]["eigenpairs_per_s"], 3),
                      "converged": bool(conv)}
+    # the one-level layout again with the Hellmann-Feynman gradients of every
+    # level found (one Ritz vector per window, dE/dp for the potential's 12
+    # parameters) inside the timed region
+    dvdp = ctx.to_device(prob.potential.gradient(prob.r, prob.R, prob.cos_theta))
+    times = []
+    for i, seed in enumerate(seeds):
+        x0b = ctx.to_device(P.random_vectors(prob.N, len(centres1), seed))
+        for rep in range(2 if i == 0 else 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ret, eig, rr, stats = ctx.pist_conv_batched(x0b, centres1, 1e-11, 1, 200, 4, 4, 1, qmrtol=1e-6,
+                                                        prec=L.FP64)
+            psi = torch.cat([ctx.pist_vectors(k, 1) for k in range(len(centres1))])
+            grad = ctx.eig_gradients(psi, dvdp)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        times.append(dt)
+    dt = float(np.mean(times))
+    res["batched_fp64_1level_gradients"] = {
+        "windows": len(centres1), "levels_per_window": 1, "parameters": int(dvdp.shape[0]), "start_seeds": seeds,
+        "seconds_per_seed": [round(t, 4) for t in times], "seconds": round(dt, 4),
+        "eigenpairs_per_s": round(20 / dt, 3),
+        "speedup_vs_fp64": round((20 / dt) / res["fp64"]["eigenpairs_per_s"], 3),
+        "gradients_finite": bool(np.isfinite(grad).all())}
     if not no_cpu:
         from oracle import oracle as O
 

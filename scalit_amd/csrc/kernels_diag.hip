@@ -606,7 +606,10 @@ __global__ void __launch_bounds__(1024) k_small_sym_eig(int n, const double *__r
         }
         tot = red[0];
         __syncthreads();
-        if (off <= 1e-34 * tot || off == 0.0) break;
+        // stop once the off-diagonal part is below 1e-15 of the whole (in
+        // Frobenius norm; off and tot are sums of squares): a tighter test
+        // sits under the rounding floor and would run every sweep
+        if (off <= 1e-30 * tot || off == 0.0) break;
         for (int round = 0; round < m - 1; round++) {
             // tournament pairing: position 0 fixed, others rotate
             for (int k = tid; k < m / 2; k += nt) {
@@ -838,7 +841,11 @@ int small_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz
         return SCALIT_OK;
     }
     if (n < 1 || n > 1024) return SCALIT_EINVAL;
-    hipLaunchKernelGGL(k_small_sym_eig, dim3(1), dim3(1024), 0, s, n, a, lda, w, z, ldz, work);
+    // a sweep is a few barriers per rotation round, so small matrices run on
+    // fewer threads, where a barrier costs less (the loops and the tree
+    // reduction take any power of two)
+    const int nt = n <= 32 ? 64 : (n <= 96 ? 256 : 1024);
+    hipLaunchKernelGGL(k_small_sym_eig, dim3(1), dim3(nt), 0, s, n, a, lda, w, z, ldz, work);
     HIP_TRY(hipGetLastError());
     return SCALIT_OK;
 }
