@@ -343,7 +343,7 @@ def pist_bench(no_cpu):
             t0 = time.perf_counter()
             ret, eig, rr, stats = ctx.pist_conv_batched(x0b, centres1, 1e-11, 1, 200, 4, 4, 1, qmrtol=1e-6,
                                                         prec=L.FP64)
-            psi = torch.cat([ctx.pist_vectors(k, 1) for k in range(len(centres1))])
+            psi = ctx.pist_vectors_all(len(centres1), 1)
             grad = ctx.eig_gradients(psi, dvdp)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0

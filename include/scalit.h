@@ -156,6 +156,10 @@ int scalit_pist_conv_batched(scalit_ctx *ctx, int nwin, const double *e0, double
 /* Ritz vectors of the last PIST solve of window `win` for its m eigenvalues:
  * psi (device, N x m, leading dimension ld), unit norm. */
 int scalit_pist_vectors(scalit_ctx *ctx, int win, double *psi, long ld);
+/* The same for every window of the last (batched) PIST solve in one call:
+ * window k's m vectors at psi + (k m + i) ld (device, ncols columns of N,
+ * ncols >= nwin m, else SCALIT_EINVAL). */
+int scalit_pist_vectors_all(scalit_ctx *ctx, double *psi, long ld, int ncols);
 
 /* Lan_Conv (src/iterate/libiterate/lanczos/lan.f90:7-105) with lanH0
  * (lanHij.f90:6-45): FP64 Lanczos with full re-orthogonalisation. */

@@ -63,6 +63,7 @@ SIGNATURES = {
     "scalit_pist_conv_batched": (_i, [_vp, _i, _dp, _d, _i, _vp, _l, _i, _i, _i, _i, _i, _i, _i, _d, _i, _dp, _ip,
                                       _dp, _ip]),
     "scalit_pist_vectors": (_i, [_vp, _i, _vp, _l]),
+    "scalit_pist_vectors_all": (_i, [_vp, _vp, _l, _i]),
     "scalit_lan_conv": (_i, [_vp, _d, _d, _i, _vp, _i, _i, _i, _i, _i, _dp, _dp]),
     "scalit_eig_gradients": (_i, [_vp, _i, _vp, _l, _i, _vp, _l, _dp]),
     "scalit_bind": (_i, [_vp]),

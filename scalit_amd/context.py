@@ -347,6 +347,12 @@ class Context:
         check(L.lib().scalit_pist_vectors(self._h, win, _ptr(psi), self.N), "scalit_pist_vectors")
         return psi
 
+    def pist_vectors_all(self, nwin: int, m: int):
+        """Ritz vectors of all nwin windows of the last batched solve, (nwin m, N)."""
+        psi = torch.empty((nwin * m, self.N), dtype=torch.float64, device=self.dev)
+        check(L.lib().scalit_pist_vectors_all(self._h, _ptr(psi), self.N, nwin * m), "scalit_pist_vectors_all")
+        return psi
+
     def lan_conv(self, x0: torch.Tensor, e0: float, etol: float, m: int, m_max: int, startcnt: int, stepcnt: int,
                  stepeig: int = 1, ntype: int = 0):
         x0 = x0.contiguous().reshape(-1)

@@ -24,6 +24,7 @@ struct PistParams {
 int pist_conv(scalit_ctx *c, int nw, const double *e0, const PistParams &pp, const double *x, long ldx, double *eig,
               int *ret_out, double *res_out, int *stats_out, hipStream_t s);
 int pist_vectors(scalit_ctx *c, int k, double *psi, long ldp, hipStream_t s);
+int pist_vectors_all(scalit_ctx *c, double *psi, long ldp, int ncols, hipStream_t s);
 int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, int startcnt, int stepcnt,
              int stepeig, int m, int m_max, double *eig, double *res, hipStream_t s);
 int eig_gradients(scalit_ctx *c, int nk, const double *psi, long ld, int np, const double *dvdp, long lddv,
@@ -489,6 +490,13 @@ int scalit_pist_vectors(scalit_ctx *c, int win, double *psi, long ld)
     CTX_GUARD(c);
     if (!psi || ld < c->N) return SCALIT_EINVAL;
     return pist_vectors(c, win, psi, ld, c->stream);
+}
+
+int scalit_pist_vectors_all(scalit_ctx *c, double *psi, long ld, int ncols)
+{
+    CTX_GUARD(c);
+    if (!psi || ld < c->N || ncols < 1) return SCALIT_EINVAL;
+    return pist_vectors_all(c, psi, ld, ncols, c->stream);
 }
 
 int scalit_lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, int startcnt, int stepcnt,

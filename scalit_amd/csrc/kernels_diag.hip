@@ -563,9 +563,8 @@ int block_diag(scalit_ctx *c, int bjmax, double bjtol)
 // ---------------------------------------------------------------------------
 size_t small_sym_eig_work(int n) { return (size_t)n * n * 2 + 4 * (size_t)n + 8; }
 
-__global__ void __launch_bounds__(1024) k_small_sym_eig(int n, const double *__restrict__ a_in, int lda,
-                                                        double *__restrict__ w, double *__restrict__ z_out, int ldz,
-                                                        double *__restrict__ work)
+__device__ void small_sym_eig_body(int n, const double *__restrict__ a_in, int lda, double *__restrict__ w,
+                                   double *__restrict__ z_out, int ldz, double *__restrict__ work)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
     double *A = work;
@@ -676,6 +675,21 @@ __global__ void __launch_bounds__(1024) k_small_sym_eig(int n, const double *__r
         if (z_out)
             for (int r = 0; r < n; r++) z_out[r + (long)ldz * rank] = Z[r + (long)n * i];
     }
+}
+
+__global__ void __launch_bounds__(1024) k_small_sym_eig(int n, const double *__restrict__ a_in, int lda,
+                                                        double *__restrict__ w, double *__restrict__ z_out, int ldz,
+                                                        double *__restrict__ work)
+{
+    small_sym_eig_body(n, a_in, lda, w, z_out, ldz, work);
+}
+
+// one problem per workgroup (blockIdx.x picks the job; work: wstride doubles each)
+__global__ void __launch_bounds__(1024) k_small_sym_eig_batch(const EigvecJob *__restrict__ jobs,
+                                                              double *__restrict__ work, long wstride)
+{
+    const EigvecJob jb = jobs[blockIdx.x];
+    small_sym_eig_body(jb.n, jb.a, jb.lda, jb.w, jb.z, jb.ldz, work + (long)blockIdx.x * wstride);
 }
 
 // ---------------------------------------------------------------------------
@@ -846,6 +860,28 @@ int small_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz
     // reduction take any power of two)
     const int nt = n <= 32 ? 64 : (n <= 96 ? 256 : 1024);
     hipLaunchKernelGGL(k_small_sym_eig, dim3(1), dim3(nt), 0, s, n, a, lda, w, z, ldz, work);
+    HIP_TRY(hipGetLastError());
+    return SCALIT_OK;
+}
+
+int small_sym_eig_batch(int njobs, const EigvecJob *jobs, DBuf<EigvecJob> &djobs, DBuf<double> &work,
+                        hipStream_t s)
+{
+    if (njobs < 0) return SCALIT_EINVAL;
+    if (njobs == 0) return SCALIT_OK;
+    int nmax = 0;
+    for (int j = 0; j < njobs; j++) {
+        if (jobs[j].n < 1 || jobs[j].n > 1024 || jobs[j].lda < jobs[j].n || jobs[j].ldz < jobs[j].n || !jobs[j].a ||
+            !jobs[j].w || !jobs[j].z)
+            return SCALIT_EINVAL;
+        nmax = std::max(nmax, jobs[j].n);
+    }
+    const long stride = (long)small_sym_eig_work(nmax);
+    SC_TRY(djobs.alloc(njobs));
+    SC_TRY(work.alloc((size_t)njobs * stride));
+    HIP_TRY(hipMemcpyAsync(djobs.p, jobs, sizeof(EigvecJob) * njobs, hipMemcpyHostToDevice, s));
+    const int nt = nmax <= 32 ? 64 : (nmax <= 96 ? 256 : 1024);
+    hipLaunchKernelGGL(k_small_sym_eig_batch, dim3(njobs), dim3(nt), 0, s, djobs.p, work.p, stride);
     HIP_TRY(hipGetLastError());
     return SCALIT_OK;
 }

@@ -1003,6 +1003,24 @@ void launch_ritz(long n, int m, int nc, const double *vj, long ldv, const double
     hipLaunchKernelGGL(k_ritz, dim3(nparts(n), nc), dim3(256), 0, s, n, m, nc, vj, ldv, z, ldz, cols, psi, ldp);
 }
 
+// Ritz vectors of several windows in one launch: blockIdx.y picks the job
+// psi = sum_i vj(:, i) z(i, col) for i < m
+__global__ void k_ritz_batch(long n, long ldv, const RitzJob *jobs)
+{
+    const RitzJob jb = jobs[blockIdx.y];
+    GSTRIDE(t, n)
+    {
+        double s = 0;
+        for (int i = 0; i < jb.m; i++) s += jb.vj[t + ldv * i] * jb.z[i + (long)jb.ldz * jb.col];
+        jb.psi[t] = s;
+    }
+}
+
+void launch_ritz_batch(long n, long ldv, int njobs, const RitzJob *jobs_dev, hipStream_t s)
+{
+    if (njobs > 0) hipLaunchKernelGGL(k_ritz_batch, dim3(nparts(n), njobs), dim3(256), 0, s, n, ldv, jobs_dev);
+}
+
 // Hellmann-Feynman: part[(k*np + j)][blk] = partial of sum psi_k^2 dvdp_j
 __global__ void __launch_bounds__(256) k_hf_part(long n, int np, const double *psi, long ld, const double *dvdp,
                                                  long lddv, double *part)

@@ -153,6 +153,17 @@ struct EigvalJob {
 };
 int small_sym_eigvals_batch(int njobs, const EigvalJob *jobs, DBuf<EigvalJob> &djobs, DBuf<double> &work,
                             hipStream_t s);
+// eigenvalues and eigenvectors of several small symmetric matrices in one
+// launch (cyclic Jacobi, one workgroup each): z gets the eigenvectors (n x n,
+// ldz) in the order of w.  Same lifetime rule for the host jobs; work grows to
+// njobs small_sym_eig_work(n_max) doubles.
+struct EigvecJob {
+    const double *a;
+    double *w, *z;
+    int n, lda, ldz;
+};
+int small_sym_eig_batch(int njobs, const EigvecJob *jobs, DBuf<EigvecJob> &djobs, DBuf<double> &work,
+                        hipStream_t s);
 
 // vector helpers (kernels_vec.hip)
 constexpr int NPART = 512;   // partial sums per vector in the deterministic reductions

@@ -71,6 +71,15 @@ void launch_multidot_fin(int nw, int m, int i0, int j, const double *part, doubl
                          const int *active, hipStream_t s);
 void launch_ritz(long n, int m, int nc, const double *vj, long ldv, const double *z, int ldz, const int *cols,
                  double *psi, long ldp, hipStream_t s);
+// one Ritz vector per job (window's Krylov basis vj, eigenvector column col
+// of its z), all in one launch; jobs_dev: device array of njobs
+struct RitzJob {
+    const double *vj;
+    const double *z;
+    double *psi;
+    int m, ldz, col;
+};
+void launch_ritz_batch(long n, long ldv, int njobs, const RitzJob *jobs_dev, hipStream_t s);
 void launch_hf_part(int nk, long n, int np, const double *psi, long ld, const double *dvdp, long lddv, double *part,
                     hipStream_t s);
 void launch_dv_accum(long n, int nvec, const double *x, const double *g, long ld, double *dv, hipStream_t s);

@@ -366,6 +366,8 @@ struct PistState {
     DBuf<double> vj, hmat, w, eigw, zw, part, part2, tmp, r64, x64, psi_z, zsh, bb, shd;
     DBuf<int> ok, active, cols;
     DBuf<EigvalJob> ejobs;
+    DBuf<EigvecJob> vjobs;
+    DBuf<RitzJob> rjobs;
     int nwin = 0, m_max = 0, m_last = 0;
     std::vector<int> last_m;
     std::vector<std::vector<int>> last_ind;   // window index of each returned eigenvalue
@@ -879,6 +881,40 @@ int pist_vectors(scalit_ctx *c, int k, double *psi, long ldp, hipStream_t s)
     HIP_TRY(hipMemcpyAsync(P.cols.p, P.last_ind[k].data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
     launch_ritz(c->N, cnt, m, P.vj.p + (long)k * P.ldv * m_max, P.ldv, P.psi_z.p, cnt, P.cols.p, psi, ldp, s);
     HIP_TRY(hipStreamSynchronize(s));
+    return SCALIT_OK;
+}
+
+// Ritz vectors of every window for its m returned eigenvalues, window k's at
+// psi + (k m + i) ldp: one eigenvector launch and one Ritz launch for all
+int pist_vectors_all(scalit_ctx *c, double *psi, long ldp, int ncols, hipStream_t s)
+{
+    if (!c->pist || c->pist->nwin < 1) return SCALIT_EINVAL;
+    PistState &P = *c->pist;
+    const int nw = P.nwin, m = P.m_last, m_max = P.m_max;
+    if ((long)nw * m > ncols) return SCALIT_EINVAL;   // psi must hold every window's m vectors
+    int cmax = 0;
+    for (int k = 0; k < nw; k++) {
+        if (P.last_cnt[k] <= 0) return SCALIT_ESTATE;
+        cmax = std::max(cmax, P.last_cnt[k]);
+    }
+    SC_TRY(P.psi_z.alloc((size_t)nw * cmax * cmax));
+    SC_TRY(P.eigw.alloc((size_t)nw * cmax));
+    std::vector<EigvecJob> ej(nw);
+    std::vector<RitzJob> rj((size_t)nw * m);
+    for (int k = 0; k < nw; k++) {
+        const int cnt = P.last_cnt[k];
+        double *z = P.psi_z.p + (long)k * cmax * cmax;
+        ej[k] = {P.hmat.p + (long)k * m_max * m_max, P.eigw.p + (long)k * cmax, z, cnt, m_max, cnt};
+        for (int i = 0; i < m; i++)
+            rj[(size_t)k * m + i] = {P.vj.p + (long)k * P.ldv * m_max, z, psi + ((long)k * m + i) * ldp, cnt, cnt,
+                                     P.last_ind[k][i]};
+    }
+    SC_TRY(small_sym_eig_batch(nw, ej.data(), P.vjobs, P.zw, s));
+    SC_TRY(P.rjobs.alloc(rj.size()));
+    HIP_TRY(hipMemcpyAsync(P.rjobs.p, rj.data(), sizeof(RitzJob) * rj.size(), hipMemcpyHostToDevice, s));
+    launch_ritz_batch(c->N, P.ldv, (int)rj.size(), P.rjobs.p, s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));   // the host job arrays die here
     return SCALIT_OK;
 }
 

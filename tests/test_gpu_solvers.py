@@ -156,6 +156,15 @@ def test_pist_batched_one_level_windows():
     print("one-level windows", ret, stats, flush=True)
     assert (ret > 0).all()
     assert (np.abs(eig.reshape(-1) - ref) / np.abs(ref)).max() <= 1e-10
+    # the Ritz vectors of all windows in one call agree with one window at a
+    # time, and their Rayleigh quotients give the eigenvalues
+    psi = ctx.pist_vectors_all(20, 1)
+    one = torch.cat([ctx.pist_vectors(k, 1) for k in range(20)])
+    assert float((psi - one).abs().max()) <= 1e-12
+    rq = (psi * ctx.hx(psi)).sum(dim=1).cpu().numpy()
+    assert np.abs(rq - eig.reshape(-1)).max() <= 1e-9 * np.abs(ref).max()
+    with pytest.raises(L.ScalitError):
+        ctx.pist_vectors_all(10, 1)   # too few columns for 20 windows: refused
     ctx.close()
 
 
