@@ -837,8 +837,8 @@ constexpr int MGS_NT = 1024;
 template <int PER>
 __global__ void __launch_bounds__(MGS_NT) k_mgs_one(long n, int jnew, const double *__restrict__ vj, long ldv,
                                                     long ldw, const double *w, long ldwk, double *col,
-                                                    const int *__restrict__ active,
-                                                    int *__restrict__ ok)
+                                                    const int *__restrict__ active, int *__restrict__ ok,
+                                                    double *__restrict__ nrm_out)
 {
     __shared__ double red[2][MGS_NT / 64];
     const int k = blockIdx.x;
@@ -896,20 +896,24 @@ __global__ void __launch_bounds__(MGS_NT) k_mgs_one(long n, int jnew, const doub
         const long t = tid + (long)e * MGS_NT;
         if (t < n) C[t] = nrm != 0.0 ? a[e] / nrm : a[e];
     }
-    if (tid == 0) ok[k] = nrm != 0.0;
+    if (tid == 0) {
+        ok[k] = nrm != 0.0;
+        if (nrm_out) nrm_out[k] = nrm;
+    }
 }
 
 bool launch_mgs_one(int nw, long n, int jnew, const double *vj, long ldv, long ldw, const double *w, long ldwk,
-                    double *col, const int *active, int *ok, hipStream_t s)
+                    double *col, const int *active, int *ok, double *nrm_out, hipStream_t s)
 {
     if (n <= 8L * MGS_NT)
-        hipLaunchKernelGGL(k_mgs_one<8>, dim3(nw), dim3(MGS_NT), 0, s, n, jnew, vj, ldv, ldw, w, ldwk, col, active, ok);
+        hipLaunchKernelGGL(k_mgs_one<8>, dim3(nw), dim3(MGS_NT), 0, s, n, jnew, vj, ldv, ldw, w, ldwk, col, active, ok,
+                           nrm_out);
     else if (n <= 16L * MGS_NT)
         hipLaunchKernelGGL(k_mgs_one<16>, dim3(nw), dim3(MGS_NT), 0, s, n, jnew, vj, ldv, ldw, w, ldwk, col, active,
-                           ok);
+                           ok, nrm_out);
     else if (n <= 20L * MGS_NT)
         hipLaunchKernelGGL(k_mgs_one<20>, dim3(nw), dim3(MGS_NT), 0, s, n, jnew, vj, ldv, ldw, w, ldwk, col, active,
-                           ok);
+                           ok, nrm_out);
     else
         return false;
     return true;

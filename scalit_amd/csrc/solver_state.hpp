@@ -57,10 +57,11 @@ void launch_axpby(int nvec, long n, double a, const double *x, long ldx, double 
                   hipStream_t s);
 
 // MGS_ORTH of W (ldwk apart per window) into column col (ldw apart) in one
-// launch, one workgroup per active window; false (nothing launched) when n is
-// too long for a workgroup's registers
+// launch, one workgroup per active window; ok[k] = 0 on a zero norm before or
+// after the projections, nrm_out[k] (when not null) = the norm after them;
+// false (nothing launched) when n is too long for a workgroup's registers
 bool launch_mgs_one(int nw, long n, int jnew, const double *vj, long ldv, long ldw, const double *w, long ldwk,
-                    double *col, const int *active, int *ok, hipStream_t s);
+                    double *col, const int *active, int *ok, double *nrm_out, hipStream_t s);
 void launch_mgs_step(int nw, long n, int i, int inext, const double *vj, long ldv, long ldw, double *w, long ldwk,
                      const double *pin, double *pout, const int *active, hipStream_t s);
 void launch_normalize(int nw, long n, const double *w, long ldwk, double *dst, long ldd, const double *part, int *ok,

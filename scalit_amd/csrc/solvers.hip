@@ -471,7 +471,7 @@ int mgs_host(scalit_ctx *c, long n, const double *wj, int m, double *vj, int *ok
     HIP_TRY(hipMemcpyAsync(col, wj, 8 * n, hipMemcpyHostToDevice, s));
     double nw = 0;
     int okh = 0;
-    if (launch_mgs_one(1, n, m, v.p, ld, 0, col, 0, col, nullptr, okd.p, s)) {
+    if (launch_mgs_one(1, n, m, v.p, ld, 0, col, 0, col, nullptr, okd.p, nullptr, s)) {
         nw = 1.0;   // a zero norm of WJ shows in ok
     } else {
         launch_dot_partials(1, n, col, ld, col, ld, pa.p, s);
@@ -657,7 +657,7 @@ static int pist_mgs(scalit_ctx *c, PistState &P, int nw, int m_max, int jnew, co
     const long N = c->N;
     const long ldw = P.ldv * m_max;
     double *col = P.vj.p + (long)jnew * P.ldv;   // window 0's column jnew; window k at + k*ldw
-    if (launch_mgs_one(nw, N, jnew, P.vj.p, P.ldv, ldw, P.w.p, P.ldv, col, act_dev, P.ok.p, s)) {
+    if (launch_mgs_one(nw, N, jnew, P.vj.p, P.ldv, ldw, P.w.p, P.ldv, col, act_dev, P.ok.p, nullptr, s)) {
         HIP_TRY(hipMemcpyAsync(ok.data(), P.ok.p, sizeof(int) * nw, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         return SCALIT_OK;
@@ -997,6 +997,8 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
     SC_TRY(ev.alloc(2 * (size_t)m_max));
     SC_TRY(work.alloc(small_sym_eig_work(m_max)));
     DBuf<EigvalJob> ejobs;
+    DBuf<int> okv;   // per Lanczos step: the re-orthogonalised vector has a nonzero norm
+    SC_TRY(okv.alloc((size_t)m_max + 1));
     (void)P;
     HIP_TRY(hipMemsetAsync(alpha.p, 0, 8 * m_max, s));
     HIP_TRY(hipMemsetAsync(beta.p, 0, 8 * m_max, s));
@@ -1028,13 +1030,14 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
         ev.release();
         work.release();
         ejobs.release();
+        okv.release();
         zero.release();
         return r;
     };
     if (xn == 0.0) return fail(0);
     while (cnt_high <= m_max) {
         // lanH0 (lanHij.f90:6-45)
-        bool bad = false;
+        bool bad = false, deferred = false;
         for (int j = cnt_low; j <= cnt_high && !bad; j++) {
             double *vjj = vj.p + (long)(j - 1) * ldv;
             double *vnext = vj.p + (long)j * ldv;
@@ -1043,6 +1046,14 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
                                j > 1 ? vj.p + (long)(j - 2) * ldv : vjj, vjj, w.p, part.p);
             hipLaunchKernelGGL(k_lan_sub_alpha, dim3(NPART), dim3(256), 0, s, N, j - 1, part.p, alpha.p, vjj, w.p, vnext,
                                part2.p);
+            // LAN_MGS_ORTH in one workgroup when the vector fits it: BETA(J) and
+            // the flag of step j land on the device and are read once after the
+            // steps (a failed step makes this block fail with the same code, so
+            // the steps after it only compute values that are never used)
+            if (launch_mgs_one(1, N, j, vj.p, ldv, 0, vnext, 0, vnext, nullptr, okv.p + j, beta.p + (j - 1), s)) {
+                deferred = true;
+                continue;
+            }
             hipLaunchKernelGGL(k_store_norm, dim3(1), dim3(1), 0, s, part2.p, nrm.p, 0);
             // LAN_MGS_ORTH: against columns 1..j
             launch_mgs_step(1, N, -1, 0, vj.p, ldv, 0, vnext, 0, nullptr, part.p, nullptr, s);
@@ -1060,6 +1071,13 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
             HIP_TRY(hipMemcpyAsync(&bj, beta.p + (j - 1), 8, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             if (nn[0] == 0.0 || bj == 0.0) bad = true;
+        }
+        if (deferred) {
+            std::vector<int> okh(cnt_high - cnt_low + 1);
+            HIP_TRY(hipMemcpyAsync(okh.data(), okv.p + cnt_low, sizeof(int) * okh.size(), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            for (int v : okh)
+                if (!v) bad = true;
         }
         if (bad) return fail(-(cnt_low - 1));
         // eigenvalues of the leading (cnt_high - stepeig) and cnt_high blocks
