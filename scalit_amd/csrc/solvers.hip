@@ -764,16 +764,13 @@ int pist_conv(scalit_ctx *c, int nw, const double *e0, const PistParams &pp, con
     P.m_last = m;
     // VJ(:,1) = X / ||X||  (pistHij.f90:345-352)
     {
-        DBuf<double> xs;
-        SC_TRY(xs.alloc((size_t)nw * P.ldv));
-        for (int k = 0; k < nw; k++)
-            HIP_TRY(hipMemcpyAsync(xs.p + (long)k * P.ldv, x + (long)k * ldx, 8 * N, hipMemcpyDeviceToDevice, s));
-        launch_dot_partials(nw, N, xs.p, P.ldv, xs.p, P.ldv, P.part.p, s);
-        launch_normalize(nw, N, xs.p, P.ldv, P.vj.p, ldw, P.part.p, P.ok.p, nullptr, s);
+        // read straight from the caller's start vectors (no scratch copy: a
+        // device free would synchronise the whole device)
+        launch_dot_partials(nw, N, x, ldx, x, ldx, P.part.p, s);
+        launch_normalize(nw, N, x, ldx, P.vj.p, ldw, P.part.p, P.ok.p, nullptr, s);
         std::vector<int> ok(nw);
         HIP_TRY(hipMemcpyAsync(ok.data(), P.ok.p, sizeof(int) * nw, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        xs.release();
         for (int k = 0; k < nw; k++)
             if (!ok[k]) {
                 act[k] = 0;
