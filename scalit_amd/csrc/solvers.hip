@@ -368,6 +368,7 @@ struct PistState {
     DBuf<EigvalJob> ejobs;
     DBuf<EigvecJob> vjobs;
     DBuf<RitzJob> rjobs;
+    DBuf<double> gpart, gred;   // Hellmann-Feynman partial sums and results
     int nwin = 0, m_max = 0, m_last = 0;
     std::vector<int> last_m;
     std::vector<std::vector<int>> last_ind;   // window index of each returned eigenvalue
@@ -398,6 +399,11 @@ struct PistState {
         ok.release();
         active.release();
         cols.release();
+        ejobs.release();
+        vjobs.release();
+        rjobs.release();
+        gpart.release();
+        gred.release();
     }
 };
 
@@ -1112,15 +1118,14 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
 int eig_gradients(scalit_ctx *c, int nk, const double *psi, long ld, int np, const double *dvdp, long lddv,
                   double *out, hipStream_t s)
 {
-    DBuf<double> part, red;
-    SC_TRY(part.alloc((size_t)nk * np * NPART));
-    SC_TRY(red.alloc((size_t)nk * np));
-    launch_hf_part(nk, c->N, np, psi, ld, dvdp, lddv, part.p, s);
-    launch_reduce_partials(nk * np, part.p, red.p, s);
-    HIP_TRY(hipMemcpyAsync(out, red.p, 8 * (size_t)nk * np, hipMemcpyDeviceToHost, s));
+    // kept in the context: a device free would synchronise the whole device
+    PistState &P = pstate(c);
+    SC_TRY(P.gpart.alloc((size_t)nk * np * NPART));
+    SC_TRY(P.gred.alloc((size_t)nk * np));
+    launch_hf_part(nk, c->N, np, psi, ld, dvdp, lddv, P.gpart.p, s);
+    launch_reduce_partials(nk * np, P.gpart.p, P.gred.p, s);
+    HIP_TRY(hipMemcpyAsync(out, P.gred.p, 8 * (size_t)nk * np, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    part.release();
-    red.release();
     return SCALIT_OK;
 }
 
