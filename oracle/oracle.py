@@ -253,6 +253,12 @@ class Operator:
     def set_e0(self, e0: float):
         self.s.E0 = e0
 
+    def set_shift(self, e0: float, de: float = 1.0, beta: float = 0.0):
+        """sOSBW%mE0 / mDE / mBeta: the shift and the OSBD window."""
+        self.s.E0 = e0
+        self.s.DE = de
+        self.s.beta = beta
+
     def hx(self, x: np.ndarray, ntype: int = TA0) -> np.ndarray:
         x = f64(x)
         y = np.zeros(self.N)
@@ -396,10 +402,20 @@ class RefOperator:
         ref().refdrv_set_precond((_dp * FMAX)(*[dptr(v) for v in self.vosb]), dptr(self.eig0))
         return self.vosb, self.eig0
 
-    def px(self, x):
+    def set_shift(self, e0, de=1.0, beta=0.0):
+        """sOSBW%mE0 / mDE / mBeta of module osb (and the oracle's E0)."""
+        self.op.set_e0(e0)
+        ref().refdrv_set_shift(C.c_double(e0), C.c_double(de), C.c_double(beta))
+
+    def px(self, x, kind=None, npc=TA1):
+        """Module osb's MYPX: the TOSB / TA1 callback, or kernel `kind`
+        (TOSB / TOSBD1 / TOSBD2) with preconditioner shift `npc`."""
         x = f64(x)
         y = np.zeros_like(x)
-        ref().refdrv_px(_ci(len(x)), dptr(x), dptr(y))
+        if kind is None:
+            ref().refdrv_px(_ci(len(x)), dptr(x), dptr(y))
+        else:
+            ref().refdrv_px_kind(C.c_int(kind), C.c_int(npc), dptr(x), dptr(y))
         return y
 
     def qmr(self, b, qmr_type=-1, qmr_max=1000, qmr_tol=1e-3):

@@ -1,23 +1,17 @@
 /*
  * TEST INFRASTRUCTURE ONLY.  Glue for the reference checker library
- * (_ref/libscalit_ref.so).  The reference's operator callbacks HX/PX and the
- * linear-solver callback live in Fortran modules that pull in its whole I/O
- * and input layer; this file restates those module loops in C
- * (src/iterate/module/osb.hx.h:71-110, osb.px.h:125-219, osbw.qmr.h:13-23,
- * osb.diag.h:111-137) and calls the reference's own compiled library routines
- * (H1X_XYZ, H1X_DEP, H3X, VX, VTX, HOSBINIT_*, HosbDiag, QMR) for all arithmetic.
+ * (_ref/libscalit_ref.so).  H*X and P*X are the reference's own module
+ * routines MYHX / MYPX (src/iterate/module/osb.hx.h:71-110, osb.px.h:125-219),
+ * called through ref_osbop.f90 on the state this file hands over; the
+ * linear-solver callback (osbw.qmr.h:13-23) and the OSB set-up loop
+ * (osb.diag.h:111-137) are restated here around the reference's compiled
+ * QMR, HOSBINIT_* and HosbDiag, which do all the arithmetic.
  */
 #include <stdlib.h>
 #include <string.h>
 
 #define FMAX 8
 
-extern void h1x_xyz_(const int *nin, const int *nout, const int *blk, const double *h1, const double *x, double *y);
-extern void h1x_dep_(const int *nin, const int *nout, const int *blk, const double *h1, const double *corx,
-                     const double *x, double *y);
-extern void h3x_(const int *nin, const int *nout, const double *a, const double *x, double *xout);
-extern void vx_(const int *nin, const int *nout, const int *blk, const double *vout, double *x);
-extern void vtx_(const int *nin, const int *nout, const int *blk, const double *vout, double *x);
 extern void hosbinit_xyz_(const int *blk, const int *mak, const int *ndk, const double *h1, const double *res,
                           double *hosb);
 extern void hosbinit_dep_(const int *blk, const int *mak, const int *ndk, const double *h1, const double *dep,
@@ -25,6 +19,13 @@ extern void hosbinit_dep_(const int *blk, const int *mak, const int *ndk, const 
 extern void hosbdiag_(const int *bjmax, const double *bjtol, const int *blk, const int *mak, const int *ndk,
                       double *hosb, double *vosb, double *eigval);
 typedef void (*fop)(const int *, const double *, double *);
+/* ref_osbop.f90: module osb's state and its MYHX / MYPX */
+extern void refop_set(int f, const int *n, const int *dep, int ndvr, const double *h0, const double *dep_v,
+                      const double *res, const double *outh);
+extern void refop_set_shift(double e0, double de, double beta);
+extern void refop_set_precond(const double *vosb, const double *eig0);
+extern void refop_hx(int ntype, const double *x, double *y);
+extern void refop_px(int nosb, int npc, const double *x, double *y);
 extern int qmr_(const int *ntype, const int *qmrmax, const double *qmrtol, const int *n, const double *b, fop hx,
                 fop px, double *x, double *res);
 
@@ -60,12 +61,58 @@ void refdrv_setup(int sF, const int *sN, const int *sDEP, const double **H0, con
     S.E0 = E0;
     free(S.tmp);
     S.tmp = (double *)malloc(sizeof(double) * S.myLen);
+    /* module osb's concatenated H0 (sN(l)^2 per level, none for the sNDVR
+     * layer) and DEP (myDim(l) per dependent level), osb.init.h:22-46 */
+    long nh = 0, nd = 0;
+    for (int i = 0; i < sF; i++) {
+        if (!(OUTH && i == sF - 1)) nh += (long)sN[i] * sN[i];
+        if (sDEP[i] && !(OUTH && i == sF - 1)) nd += S.myDim[i];
+    }
+    double *h = (double *)malloc(sizeof(double) * (nh > 0 ? nh : 1));
+    double *d = (double *)malloc(sizeof(double) * (nd > 0 ? nd : 1));
+    long oh = 0, od = 0;
+    int dep_eff[FMAX];
+    for (int i = 0; i < sF; i++) {
+        const int layer = OUTH && i == sF - 1;
+        dep_eff[i] = layer ? 0 : sDEP[i];
+        if (!layer) {
+            memcpy(h + oh, H0[i], sizeof(double) * sN[i] * sN[i]);
+            oh += (long)sN[i] * sN[i];
+        }
+        if (dep_eff[i]) {
+            memcpy(d + od, DEP[i], sizeof(double) * S.myDim[i]);
+            od += S.myDim[i];
+        }
+    }
+    refop_set(sF, sN, dep_eff, OUTH ? 1 : 0, h, d, RES, OUTH);
+    refop_set_shift(E0, 1.0, 0.0);
+    free(h);
+    free(d);
+}
+
+void refdrv_set_shift(double e0, double de, double beta)
+{
+    S.E0 = e0;
+    refop_set_shift(e0, de, beta);
 }
 
 void refdrv_set_precond(const double **VOSB, const double *EIG0)
 {
-    for (int i = 0; i < S.sF; i++) S.VOSB[i] = VOSB[i];
+    long nv = 0;
+    for (int i = 0; i < S.sF; i++) {
+        S.VOSB[i] = VOSB[i];
+        nv += (long)S.myBlk[i] * S.sN[i] * S.sN[i];
+    }
     S.EIG0 = EIG0;
+    double *v = (double *)malloc(sizeof(double) * nv);
+    long o = 0;
+    for (int i = 0; i < S.sF; i++) {
+        const long len = (long)S.myBlk[i] * S.sN[i] * S.sN[i];
+        memcpy(v + o, VOSB[i], sizeof(double) * len);
+        o += len;
+    }
+    refop_set_precond(v, EIG0);
+    free(v);
 }
 
 void refdrv_set_qmr(int ntype, int qmrmax, double qmrtol)
@@ -75,45 +122,28 @@ void refdrv_set_qmr(int ntype, int qmrmax, double qmrtol)
     S.qmr_tol = qmrtol;
 }
 
-static void myhx(double c, const double *x, double *y)
+/* HX0 (TA0): Y = H X, module osb's MYHX */
+void refdrv_hx0(const int *n, const double *x, double *y)
 {
-    int n = S.myLen;
-    for (int k = 0; k < n; k++) y[k] = (S.RES[k] - c) * x[k];
-    for (int l = 0; l < S.sF; l++) {
-        if (S.OUTH && l == S.sF - 1) h3x_(&S.myDim[l], &S.sN[l], S.OUTH, x, S.tmp);   /* osb.hx.h:93-94 */
-        else if (S.sDEP[l]) h1x_dep_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], S.DEP[l], x, S.tmp);
-        else h1x_xyz_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], x, S.tmp);
-        for (int k = 0; k < n; k++) y[k] = y[k] + S.tmp[k];
-    }
+    (void)n;
+    refop_hx(0, x, y);
 }
-
-/* HX0 (TA0): Y = H X */
-void refdrv_hx0(const int *n, const double *x, double *y) { (void)n; myhx(0.0, x, y); }
-/* EHX / HX with sHC=TA1: Y = (H - E0) X.  TA0 subtracts nothing, TA1 subtracts E0;
- * y = (RES - E0) x computed as in osb.hx.h:86. */
+/* EHX / HX with sHC=TA1: Y = (H - E0) X, module osb's MYHX */
 void refdrv_hx(const int *n, const double *x, double *y)
 {
     (void)n;
-    int nn = S.myLen;
-    for (int k = 0; k < nn; k++) y[k] = (S.RES[k] - S.E0) * x[k];
-    for (int l = 0; l < S.sF; l++) {
-        if (S.OUTH && l == S.sF - 1) h3x_(&S.myDim[l], &S.sN[l], S.OUTH, x, S.tmp);   /* osb.hx.h:93-94 */
-        else if (S.sDEP[l]) h1x_dep_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], S.DEP[l], x, S.tmp);
-        else h1x_xyz_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.H0[l], x, S.tmp);
-        for (int k = 0; k < nn; k++) y[k] = y[k] + S.tmp[k];
-    }
+    refop_hx(1, x, y);
 }
 
-/* PX with sOSB=TOSB, sPC=TA1 */
+/* PX with sOSB=TOSB, sPC=TA1: module osb's MYPX */
 void refdrv_px(const int *n, const double *x, double *y)
 {
     (void)n;
-    int nn = S.myLen;
-    memcpy(y, x, sizeof(double) * nn);
-    for (int l = S.sF - 1; l >= 0; l--) vtx_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.VOSB[l], y);
-    for (int k = 0; k < nn; k++) y[k] = y[k] / (S.EIG0[k] - S.E0);
-    for (int l = 0; l < S.sF; l++) vx_(&S.myDim[l], &S.sN[l], &S.myBlk[l], S.VOSB[l], y);
+    refop_px(1, 1, x, y);
 }
+/* MYPX with kernel nosb (0 TOSB, 1 TOSBD1, 2 TOSBD2, this library's numbering)
+ * and npc (0 TA0, 1 TA1) */
+void refdrv_px_kind(int nosb, int npc, const double *x, double *y) { refop_px(nosb + 1, npc, x, y); }
 
 /* OSB_QMR: the LinSolv callback of PIST */
 int refdrv_linsolv(const int *n, const double *b, double *x, double *res)

@@ -1,0 +1,79 @@
+! TEST INFRASTRUCTURE ONLY (our own glue, not reference code).  Fills the
+! operator state of the reference's serial module osb (sF, sN, sDEP, sNDVR,
+! H0, DEP, RES, OUTH, VOSB, EIG0, sOSBW) from caller arrays and calls the
+! module's own MYHX (src/iterate/module/osb.hx.h:71-110) and MYPX
+! (src/iterate/module/osb.px.h:125-219), so that the oracle's level loops and
+! every preconditioner kernel are checked against the reference as compiled.
+! Arrays are the module's concatenated layouts (calOSB, osb.init.h:2-49).
+
+subroutine refop_set(f, an, aflag, ndvr, ah0, adep, ares, aouth) bind(C, name='refop_set')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: f, ndvr
+  integer(c_int), intent(in) :: an(f), aflag(f)
+  real(c_double), intent(in) :: ah0(*), adep(*), ares(*), aouth(*)
+  sF = f
+  sN(1:f) = an(1:f)
+  sDEP(1:f) = aflag(1:f) /= 0
+  sNDVR = ndvr /= 0
+  sST = .false.
+  call calOSB()
+  if (allocated(H0)) deallocate(H0)
+  allocate(H0(max(1, myH0%mLen)))
+  if (myH0%mLen > 0) H0(1:myH0%mLen) = ah0(1:myH0%mLen)
+  if (allocated(DEP)) deallocate(DEP)
+  allocate(DEP(max(1, myDep%mLen)))
+  if (myDep%mLen > 0) DEP(1:myDep%mLen) = adep(1:myDep%mLen)
+  if (allocated(RES)) deallocate(RES)
+  allocate(RES(myLen))
+  RES(1:myLen) = ares(1:myLen)
+  if (allocated(OUTH)) deallocate(OUTH)
+  allocate(OUTH(max(1, outLen)))
+  if (outLen > 0) OUTH(1:outLen) = aouth(1:outLen)
+end subroutine
+
+subroutine refop_set_shift(e0, de, beta) bind(C, name='refop_set_shift')
+  use iso_c_binding
+  use osb
+  implicit none
+  real(c_double), value :: e0, de, beta
+  sOSBW%mE0 = e0
+  sOSBW%mDE = de
+  sOSBW%mBeta = beta
+end subroutine
+
+subroutine refop_set_precond(avosb, aeig0) bind(C, name='refop_set_precond')
+  use iso_c_binding
+  use osb
+  implicit none
+  real(c_double), intent(in) :: avosb(*), aeig0(*)
+  if (allocated(VOSB)) deallocate(VOSB)
+  allocate(VOSB(myVOSB%mLen))
+  VOSB(1:myVOSB%mLen) = avosb(1:myVOSB%mLen)
+  if (allocated(EIG0)) deallocate(EIG0)
+  allocate(EIG0(myLen))
+  EIG0(1:myLen) = aeig0(1:myLen)
+end subroutine
+
+! nType as osb's TA0 = 0 / TA1 = 1
+subroutine refop_hx(ntype, x, y) bind(C, name='refop_hx')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: ntype
+  real(c_double), intent(in) :: x(*)
+  real(c_double), intent(out) :: y(*)
+  call MYHX(ntype, myLen, x, y)
+end subroutine
+
+! nOSB as osb's TOSB = 1 / TOSBD1 = 2 / TOSBD2 = 3; nPC TA0 = 0 / TA1 = 1
+subroutine refop_px(nosbk, npc, x, y) bind(C, name='refop_px')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: nosbk, npc
+  real(c_double), intent(in) :: x(*)
+  real(c_double), intent(out) :: y(*)
+  call MYPX(nosbk, npc, myLen, x, y)
+end subroutine

@@ -564,21 +564,6 @@ int orc_mgs_orth(long n, const double *wj, int m, double *vj)
     return 0;
 }
 
-/* src/iterate/libiterate/orth/gsorth.f90:6-33 (classical GS) */
-int orc_gs_orth(long n, const double *wj, int m, double *vj)
-{
-    double *v0 = vj + n * m, d;
-    memcpy(v0, wj, sizeof(double) * n);
-    for (int i = 0; i < m; i++) {
-        d = orc_dot(n, wj, vj + n * i);
-        for (long k = 0; k < n; k++) v0[k] = v0[k] - d * vj[n * i + k];
-    }
-    d = sqrt(orc_dot(n, v0, v0));
-    if (d == 0.0) return 0;
-    for (long k = 0; k < n; k++) v0[k] = v0[k] / d;
-    return 1;
-}
-
 /* src/iterate/libiterate/orth/gsorth.f90:122-152 */
 int orc_lan_mgs_orth(long n, const double *wj, int m, double *vj, double *beta)
 {
@@ -820,23 +805,6 @@ int orc_sym_eigvals(int n, const double *a, int lda, double *w)
     sym_jacobi(n, b, NULL, w);
     sort_pairs(n, w, NULL, 0);
     free(b);
-    return 0;
-}
-
-int orc_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz)
-{
-    double *b = (double *)malloc(sizeof(double) * n * n);
-    double *v = (double *)malloc(sizeof(double) * n * n);
-    for (int j = 0; j < n; j++)
-        for (int i = 0; i <= j; i++) {
-            b[i + (long)n * j] = a[i + (long)lda * j];
-            b[j + (long)n * i] = a[i + (long)lda * j];
-        }
-    sym_jacobi(n, b, v, w);
-    for (int j = 0; j < n; j++)
-        for (int i = 0; i < n; i++) z[i + (long)ldz * j] = v[i + (long)n * j];
-    sort_pairs(n, w, z, ldz);
-    free(b); free(v);
     return 0;
 }
 

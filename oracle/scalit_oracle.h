@@ -105,7 +105,6 @@ int  orc_cb_linsolv(void *ctx, long n, const double *b, double *x, double *res);
 int orc_qmr(int ntype, int qmrmax, double qmrtol, long n, const double *b,
             orc_op_fn hx, orc_op_fn px, void *ctx, double *x, double *res);
 int orc_mgs_orth(long n, const double *wj, int m, double *vj);
-int orc_gs_orth(long n, const double *wj, int m, double *vj);
 int orc_lan_mgs_orth(long n, const double *wj, int m, double *vj, double *beta);
 int orc_pist_h0(long n, const double *x, int m_max, int m_low, int m_high,
                 orc_op_fn h0x, orc_linsolv_fn linsolv, void *ctx, double *vj, double *hmat, int *qmr_iters);
@@ -119,7 +118,6 @@ int orc_lan_conv(double e0, double etol, int ntype, long n, const double *x, int
 
 /* ---- small dense eigensolvers standing in for DSYEV('N') / DSTEV('N') ---- */
 int orc_sym_eigvals(int n, const double *a, int lda, double *w);      /* ascending */
-int orc_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz); /* vectors, ascending */
 int orc_tridiag_eigvals(int n, const double *d, const double *e, double *w);
 
 /* ---- window helpers: src/libutil/serial/base/util.f90 ---- */

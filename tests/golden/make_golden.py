@@ -1,11 +1,13 @@
 """Writes the golden fixtures under tests/golden/ (run from the repo root).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [ref_small] [pist_config1]   (default: both)
 
 * ref_small.npz -- input/output vectors of the reference's own compiled
   routines (oracle/_ref, built from the reference sources by
-  `make -C oracle ref`) on a 6 x 7 x 5 operator: H*X (TA0, TA1), the OSB
-  set-up (VOSB, EIG0), P*X, and a QMR solve with its return code.
+  `make -C oracle ref`) on a 6 x 7 x 5 operator: H*X (TA0, TA1) and P*X
+  (TOSB, and TOSBD1 / TOSBD2 with a window DE around E0) from module osb's
+  own MYHX / MYPX, the OSB set-up (VOSB, EIG0), and a QMR solve with its
+  return code.
 * pist_fp64_config1.json -- the lowest 20 levels of the 24 x 32 x 24
   jittered Morse+P2 triatomic (seed 1234): the oracle's PIST_CONV at tight
   tolerances and an independent scipy Lanczos (eigsh) of the same operator.
@@ -37,9 +39,17 @@ def ref_small():
     px = rop.px(x)
     b = P.random_vectors(op.N, 1, 12)[0]
     ret, xq, res = rop.qmr(b, qmr_type=-1, qmr_max=200, qmr_tol=1e-9)
+    # the OSBD kernels: DE reaches the 31st eigenvalue of EIG0 above E0
+    srt = np.sort(eig0)
+    de = float(srt[30] - e0)
+    beta = 0.3
+    rop.set_shift(e0, de, beta)
+    px_d1 = rop.px(x, kind=O.TOSBD1, npc=O.TA1)
+    px_d2 = rop.px(x, kind=O.TOSBD2, npc=O.TA1)
     np.savez_compressed(
         os.path.join(HERE, "ref_small.npz"), grid=np.array(prob.sN), seed=3, e0=e0, x=x, hx0=hx0, hx1=hx1,
-        vosb1=vosb[0], vosb2=vosb[1], vosb3=vosb[2], eig0=eig0, px=px, qmr_b=b, qmr_x=xq, qmr_ret=ret, qmr_res=res)
+        vosb1=vosb[0], vosb2=vosb[1], vosb3=vosb[2], eig0=eig0, px=px, qmr_b=b, qmr_x=xq, qmr_ret=ret, qmr_res=res,
+        de=de, beta=beta, px_d1=px_d1, px_d2=px_d2)
 
 
 def pist_config1():
@@ -62,7 +72,11 @@ def pist_config1():
 
 
 if __name__ == "__main__":
-    if O.ref_available():
+    which = sys.argv[1:] or ["ref_small", "pist_config1"]
+    if "ref_small" in which:
+        if not O.ref_available():
+            raise SystemExit("ref_small needs the reference checker library (make -C oracle ref)")
         ref_small()
-    pist_config1()
+    if "pist_config1" in which:
+        pist_config1()
     print("golden fixtures written")

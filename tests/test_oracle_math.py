@@ -203,3 +203,42 @@ def test_ndvr_hx_brute_force_and_adjoint():
     x = np.random.default_rng(2).standard_normal(N)
     assert np.allclose(op.hx(x), H @ x, rtol=0, atol=1e-13 * np.abs(H).max() * N)
     assert np.allclose(op.hx_t(x), H.T @ x, rtol=0, atol=1e-13 * np.abs(H).max() * N)
+
+
+@pytest.mark.parametrize("n", [8, 24, 80, 96, 128])
+def test_legendre_dvr_l2_spectrum(n):
+    """The Legendre-DVR L^2 every workload uses: Gauss-Legendre nodes and
+    weights equal numpy's leggauss, the DVR<-FBR transform is orthogonal
+    (Gauss quadrature is exact for degree <= 2n-1), and the eigenvalues of
+    L^2 are j(j+1), j = 0..n-1.  The oracle's own L^2 (built from the
+    reference-pinned Gau_legendre and YjPolys) agrees with it."""
+    x, w = P.gauss_legendre(n)
+    xn, wn = np.polynomial.legendre.leggauss(n)
+    # (numpy's own weights carry errors of ~1e-14 at n = 128)
+    assert np.abs(x - xn).max() <= 1e-14 and np.abs(w - wn).max() <= 5e-14
+    t = np.sqrt(w)[:, None] * P.normalized_legendre(x, n - 1)
+    assert np.abs(t.T @ t - np.eye(n)).max() <= 1e-12
+    xl, l2 = P.legendre_dvr_l2(n)
+    assert np.array_equal(xl, x) and np.array_equal(l2, l2.T)
+    j = np.arange(n, dtype=np.float64)
+    ev = np.linalg.eigvalsh(l2)
+    assert np.abs(ev - j * (j + 1)).max() <= 1e-11 * n * (n - 1)
+    xo, l2o = O.legendre_l2(n)
+    assert np.abs(xo - x).max() <= 1e-15
+    assert np.abs(l2o - l2).max() <= 1e-11 * n * (n - 1)
+    evo = np.linalg.eigvalsh(0.5 * (l2o + l2o.T))
+    assert np.abs(evo - j * (j + 1)).max() <= 1e-11 * n * (n - 1)
+
+
+@pytest.mark.parametrize("grid", [(6, 7, 5), (24, 32, 24), (17, 9, 13)])
+def test_hx_omp_equals_serial(grid):
+    """orc_myhx_omp (the CPU baseline: the reference's per-block loops split
+    over host threads) equals the serial orc_myhx bit for bit, TA0 and TA1."""
+    prob = P.build_problem(*grid, P.gaussian_potential(5), seed=5)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=-0.012)
+    x = P.random_vectors(op.N, 1, 3)[0]
+    for ntype in (O.TA0, O.TA1):
+        y = np.zeros(op.N)
+        tmp = np.zeros(op.N)
+        op.hx_omp(x, y, tmp, ntype)
+        assert np.array_equal(y, op.hx(x, ntype))

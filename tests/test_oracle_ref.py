@@ -77,6 +77,30 @@ def test_px_bit_exact():
     assert np.array_equal(op.px(x), rop.px(x))
 
 
+@pytest.mark.parametrize("kind", [O.TOSB, O.TOSBD1, O.TOSBD2])
+@pytest.mark.parametrize("npc", [O.TA0, O.TA1])
+def test_px_kernels_bit_exact(kind, npc):
+    """Every PX_KERNEL case of the reference's own MYPX (osb.px.h:156-188),
+    called from module osb, against the oracle's orc_mypx: E0 sits inside
+    the EIG0 spectrum and DE covers eigenvalues on both sides of it, so the
+    clamped branches (ADE < 0 and >= 0) and the plain divide all run."""
+    prob = P.config_problem("pist_fp64", grid=(12, 10, 8))
+    op, rop = make_ops(prob, e0=0.0)
+    _, eig0 = op.diag()
+    rop.diag()
+    srt = np.sort(eig0)
+    e0 = float(0.5 * (srt[40] + srt[41]))
+    # TA0 measures EIG0 from 0, TA1 from E0: DE reaches the 61st eigenvalue either way
+    de = float(srt[60] - (e0 if npc == O.TA1 else 0.0))
+    op.set_shift(e0, de, 0.3)
+    rop.set_shift(e0, de, 0.3)
+    ade = (eig0 - e0) if npc == O.TA1 else eig0
+    assert ((np.abs(ade) <= de) & (ade < 0)).any() or npc == O.TA0
+    assert ((np.abs(ade) <= de) & (ade >= 0)).any() and (np.abs(ade) > de).any()
+    x = P.random_vectors(op.N, 1, 6)[0]
+    assert np.array_equal(op.px(x, npc=npc, kind=kind), rop.px(x, kind=kind, npc=npc))
+
+
 @pytest.mark.parametrize("qmr_type", [-1, 0, -2])
 def test_qmr_bit_exact(qmr_type):
     prob = P.config_problem("pist_fp64", grid=(12, 10, 8))
@@ -227,3 +251,22 @@ def test_ndvr_hx_diag_px_bit_exact(grid):
         assert np.array_equal(a, b)
     assert np.array_equal(eo, er)
     assert np.array_equal(op.px(x), rop.px(x))
+
+
+def test_problem_legendre_builders_match_reference():
+    """scalit_amd.problems' Gauss-Legendre nodes / weights and normalised
+    Legendre polynomials (the inputs of every workload's L^2) against the
+    reference's Gau_legendre (gauLege.f90:63-103) and YjPolys (yj.f90:16-40).
+    Not bit-exact: problems.py iterates all nodes to 1e-15 together (the
+    reference each node to 1e-14) and uses exact sqrt(1/2), sqrt(3/2) (the
+    reference 14-digit constants)."""
+    R = O.ref()
+    for n in (24, 40, 80, 96, 128):
+        x, w = P.gauss_legendre(n)
+        xr, wr = np.zeros(n), np.zeros(n)
+        R.gau_legendre_(O._cd(-1), O._cd(1), O._ci(n), O.dptr(xr), O.dptr(wr))
+        assert np.abs(x - xr).max() <= 1e-15 and np.abs(w - wr).max() <= 1e-14
+        y = P.normalized_legendre(xr, n - 1)
+        yr = np.zeros((n, n), order="F")
+        R.yjpolys_(O._ci(n), O._ci(n - 1), O.dptr(xr), O.dptr(yr))
+        assert np.abs(y - yr).max() <= 1e-12 * np.abs(yr).max()
