@@ -264,6 +264,14 @@ class Context:
                                    _stream()), "scalit_hx_lp")
         return y
 
+    def hx_variant(self, prec: int, nvec: int):
+        """(pass-A, pass-B) variants of the fused H*X for this precision and
+        batch size, as the autotuner (or SCALIT_HX_VARIANT_A/B) chose them."""
+        va, vb = C.c_int(), C.c_int()
+        check(L.lib().scalit_debug_hx_variant(self._h, prec, nvec, C.byref(va), C.byref(vb)),
+              "scalit_debug_hx_variant")
+        return va.value, vb.value
+
     def hx_adjoint(self, x: torch.Tensor, g: torch.Tensor, ntype: int = L.TA0, with_dv: bool = True):
         g = g.contiguous()
         x = x.contiguous()

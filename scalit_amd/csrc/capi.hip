@@ -4,7 +4,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <set>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "scalit_impl.hpp"
@@ -89,6 +93,41 @@ int precond_to_f32(scalit_ctx *c)
     SC_TRY(to_f32(c->eig0, c->eig0_f, c->N, c->stream));
     c->gen++;
     return SCALIT_OK;
+}
+
+// Launch-shape caches shared by every context and host thread (several
+// problems may run on their own threads at once): one mutex guards both.
+static std::mutex g_launch_mu;
+static std::set<std::pair<int, const void *>> g_lds_raised;
+static std::map<std::tuple<int, const void *, int, size_t>, int> g_resident;
+
+static void allow_full_lds_locked(int dev, const void *fn)
+{
+    if (g_lds_raised.insert({dev, fn}).second)
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+void allow_full_lds(const void *fn)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_launch_mu);
+    allow_full_lds_locked(dev, fn);
+}
+
+int resident_per_cu(const void *fn, int threads, size_t lds)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_launch_mu);
+    allow_full_lds_locked(dev, fn);
+    const auto key = std::make_tuple(dev, fn, threads, lds);
+    auto it = g_resident.find(key);
+    if (it != g_resident.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, lds) != hipSuccess || nb < 1) nb = 1;
+    g_resident.emplace(key, nb);
+    return nb;
 }
 
 static void drop_f32(scalit_ctx *c)

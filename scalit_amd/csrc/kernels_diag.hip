@@ -534,8 +534,7 @@ int block_diag(scalit_ctx *c, int bjmax, double bjtol)
         if (L.dim == 1) {
             size_t lds = sizeof(double) * (size_t)L.n * L.n;
             if (lds > 160 * 1024) return SCALIT_EINVAL;
-            (void)hipFuncSetAttribute((const void *)k_osb_jacobi_ndk1, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      160 * 1024);
+            allow_full_lds((const void *)k_osb_jacobi_ndk1);
             hipLaunchKernelGGL(k_osb_jacobi_ndk1, dim3(L.blk), dim3(64), lds, s, L.n, L.blk, L.h0.p, c->eig0.p,
                                L.vosb.p, bjmax, bjtol, (int *)nullptr);
             HIP_TRY(hipGetLastError());
@@ -838,12 +837,7 @@ __global__ void __launch_bounds__(EIGV_NT) k_sym_eigvals(const EigvalJob *__rest
 int small_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz, double *work, hipStream_t s)
 {
     if (z == nullptr && n > 0) {
-        static bool attr = false;
-        if (!attr) {
-            (void)hipFuncSetAttribute((const void *)k_sym_eigvals<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      160 * 1024);
-            attr = true;
-        }
+        allow_full_lds((const void *)k_sym_eigvals<true>);
         const size_t vec_bytes = (4 * (size_t)n + 16) * sizeof(double);
         const EigvalJob one{a, w, n, lda};
         if (n <= EIGV_LDS_N)
@@ -896,12 +890,7 @@ int small_sym_eigvals_batch(int njobs, const EigvalJob *jobs, DBuf<EigvalJob> &d
         if (jobs[j].n < 1 || jobs[j].lda < jobs[j].n || !jobs[j].a || !jobs[j].w) return SCALIT_EINVAL;
         nmax = std::max(nmax, jobs[j].n);
     }
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_sym_eigvals<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        attr = true;
-    }
+    allow_full_lds((const void *)k_sym_eigvals<true>);
     SC_TRY(djobs.alloc(njobs));
     HIP_TRY(hipMemcpyAsync(djobs.p, jobs, sizeof(EigvalJob) * njobs, hipMemcpyHostToDevice, s));
     const size_t vec_bytes = (4 * (size_t)nmax + 16) * sizeof(double);

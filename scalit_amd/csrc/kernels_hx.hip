@@ -28,7 +28,7 @@
 namespace scalit {
 
 int g_force_generic = 0;
-int g_num_cu = 256;   // compute units of the device, set when a context is created
+std::atomic<int> g_num_cu{256};   // compute units of the device, set when a context is created
 
 
 // ---------------------------------------------------------------------------
@@ -472,12 +472,7 @@ template <typename T, typename S, bool TL, int W, int RB, int CB>
 static void launch_a1(dim3 g, size_t lds, hipStream_t s, int n1, int n2, int n3, int nvec, const T *t1, const T *t2,
                       const T *res, const T *sh, T sh0, const S *x, long ldx, S *y, long ldy, const int *act)
 {
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_hx_modes12<T, S, TL, W, RB, CB>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX);
-        attr = true;
-    }
+    allow_full_lds((const void *)k_hx_modes12<T, S, TL, W, RB, CB>);
     // persistent grid: a multiple of the 8 XCDs, at most what fits on the CUs
     const int per_cu = std::max(1, std::min((int)(LDS_MAX / std::max<size_t>(lds, 1)), 32 / W));
     int G = std::min((int)g.x, g_num_cu * per_cu);

@@ -350,17 +350,8 @@ static void launch_l1(int n1, int nb, const T *vosb, const T *eig0, const T *shi
                       int nvec, const SI *x, long ldx, SO *y, long ldy, const int *active, hipStream_t s)
 {
     // persistent: as many workgroups as are resident at once (per LDS size)
-    static size_t lds_seen = 0;
-    static int per_cu = 1;
     const size_t lds = l1_lds<T>(n1, MF);
-    const void *fn = (const void *)k_px_l1<T, SI, SO, MF, BIG>;
-    if (lds != lds_seen) {
-        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)PX_LDS_MAX);
-        int nbk = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, fn, 256, lds) != hipSuccess || nbk < 1) nbk = 1;
-        per_cu = nbk;
-        lds_seen = lds;
-    }
+    const int per_cu = resident_per_cu((const void *)k_px_l1<T, SI, SO, MF, BIG>, 256, lds);
     const int G = std::min(nb, g_num_cu * per_cu);
     hipLaunchKernelGGL((k_px_l1<T, SI, SO, MF, BIG>), dim3(G), dim3(256), lds, s, n1, nb, nvec, vosb, eig0, shifts, shift0,
                        nosb, de, beta, x, ldx, y, ldy, active);

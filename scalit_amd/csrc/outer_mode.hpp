@@ -179,20 +179,14 @@ static void launch_outer_mode_nr(long P, int n, int nvec, const T *amat, const T
 {
     // exact tiles and 32-bit byte offsets within one vector: the buffer path
     const bool ex = n == 16 * NR && P * n * (long)std::max(sizeof(S), sizeof(SO)) < 0x7fffffffL;
-    static int per_cu[2] = {0, 0};
     const size_t lds = (size_t)(16 * NR) * (16 * NR) * sizeof(T);
     const void *fn = ex ? (const void *)k_outer_mode<T, S, SO, WAVES, NR, ACC, true>
                         : (const void *)k_outer_mode<T, S, SO, WAVES, NR, ACC, false>;
-    if (per_cu[ex] == 0) {
-        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64 * WAVES, lds) != hipSuccess || nb < 1) nb = 1;
-        per_cu[ex] = nb;
-    }
+    const int per_cu = resident_per_cu(fn, 64 * WAVES, lds);
     // workgroups per block: enough for the block's tiles, and all blocks'
     // workgroups together at most what is resident at once
     const long ntile = ((P + 15) / 16) * nvec;
-    long G = std::min<long>((ntile + WAVES - 1) / WAVES, std::max<long>(1, (long)g_num_cu * per_cu[ex] / nblk));
+    long G = std::min<long>((ntile + WAVES - 1) / WAVES, std::max<long>(1, (long)g_num_cu * per_cu / nblk));
     if (G < 1) G = 1;
     const dim3 grid((unsigned)G, (unsigned)nblk);
     if (ex)

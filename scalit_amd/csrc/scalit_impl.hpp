@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -31,11 +32,32 @@ namespace scalit {
 
 using bf16_t = uint16_t;
 
-// A device buffer owned by the context.
+// A device buffer owned by the context (or by a scope: it is freed when the
+// owner goes, on every return path).  Movable, not copyable.
 template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DBuf &operator=(DBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
     int alloc(size_t cnt) {
         if (cnt <= n && p) return SCALIT_OK;
         if (p) (void)hipFree(p);
@@ -120,7 +142,13 @@ int hx_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec
                   bf16_t *y, long ldy, const int *active, hipStream_t s);
 // force the generic (any sF) path, for testing
 extern int g_force_generic;
-extern int g_num_cu;
+extern std::atomic<int> g_num_cu;
+// Raise kernel fn's dynamic-LDS limit to the CU's 160 KB on the current device
+// (once per kernel and device), and return how many workgroups of `threads`
+// threads using `lds` bytes are resident per CU (cached per kernel, device,
+// threads and lds; at least 1).  Safe to call from several host threads.
+int resident_per_cu(const void *fn, int threads, size_t lds);
+void allow_full_lds(const void *fn);
 // choose the fused-path kernel variants for (prec, nvec) before graph capture
 int hx_prepare(scalit_ctx *c, int prec, int nvec, hipStream_t s);
 int hx_variants(const scalit_ctx *c, int prec, int nvec, int *va, int *vb);

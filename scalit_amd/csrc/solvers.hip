@@ -369,6 +369,10 @@ struct PistState {
     DBuf<EigvecJob> vjobs;
     DBuf<RitzJob> rjobs;
     DBuf<double> gpart, gred;   // Hellmann-Feynman partial sums and results
+    // Lanczos (kept between calls: a device free would synchronise the whole device)
+    DBuf<double> lvj, lalpha, lbeta, lw, lpart, lpart2, lnrm, ltri, lev, lwork, lzero;
+    DBuf<EigvalJob> lejobs;
+    DBuf<int> lokv;
     int nwin = 0, m_max = 0, m_last = 0;
     std::vector<int> last_m;
     std::vector<std::vector<int>> last_ind;   // window index of each returned eigenvalue
@@ -404,6 +408,10 @@ struct PistState {
         rjobs.release();
         gpart.release();
         gred.release();
+        for (DBuf<double> *b : {&lvj, &lalpha, &lbeta, &lw, &lpart, &lpart2, &lnrm, &ltri, &lev, &lwork, &lzero})
+            b->release();
+        lejobs.release();
+        lokv.release();
     }
 };
 
@@ -988,7 +996,8 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
     if (stepeig >= m_max || startcnt > m_max || m > m_max) return 0;
     PistState &P = pstate(c);
     const long ldv = (N + 63) & ~63L;
-    DBuf<double> vj, alpha, beta, w, part, part2, nrm, tri, ev, work;
+    DBuf<double> &vj = P.lvj, &alpha = P.lalpha, &beta = P.lbeta, &w = P.lw, &part = P.lpart, &part2 = P.lpart2,
+                 &nrm = P.lnrm, &tri = P.ltri, &ev = P.lev, &work = P.lwork, &zero = P.lzero;
     SC_TRY(vj.alloc((size_t)(m_max + 1) * ldv));
     SC_TRY(alpha.alloc(m_max));
     SC_TRY(beta.alloc(m_max));
@@ -999,10 +1008,9 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
     SC_TRY(tri.alloc((size_t)m_max * m_max));
     SC_TRY(ev.alloc(2 * (size_t)m_max));
     SC_TRY(work.alloc(small_sym_eig_work(m_max)));
-    DBuf<EigvalJob> ejobs;
-    DBuf<int> okv;   // per Lanczos step: the re-orthogonalised vector has a nonzero norm
+    DBuf<EigvalJob> &ejobs = P.lejobs;
+    DBuf<int> &okv = P.lokv;   // per Lanczos step: the re-orthogonalised vector has a nonzero norm
     SC_TRY(okv.alloc((size_t)m_max + 1));
-    (void)P;
     HIP_TRY(hipMemsetAsync(alpha.p, 0, 8 * m_max, s));
     HIP_TRY(hipMemsetAsync(beta.p, 0, 8 * m_max, s));
     int ret = 0;
@@ -1018,28 +1026,16 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
         HIP_TRY(hipStreamSynchronize(s));
     }
     int cnt_low = 1, cnt_high = std::max({startcnt, m, stepeig + 1});
-    DBuf<double> zero;
     SC_TRY(zero.alloc(1));
     HIP_TRY(hipMemsetAsync(zero.p, 0, 8, s));
-    auto fail = [&](int r) {
-        vj.release();
-        alpha.release();
-        beta.release();
-        w.release();
-        part.release();
-        part2.release();
-        nrm.release();
-        tri.release();
-        ev.release();
-        work.release();
-        ejobs.release();
-        okv.release();
-        zero.release();
-        return r;
-    };
-    if (xn == 0.0) return fail(0);
+    // the buffers stay with the context for the next call
+    auto fail = [](int r) { return r; };
     while (cnt_high <= m_max) {
-        // lanH0 (lanHij.f90:6-45)
+        // lanH0 (lanHij.f90:6-45): no steps unless 1 <= CNT_LOW < CNT_HIGH, and
+        // then Lan_Conv returns -(CNT_LOW-1) (lan.f90:47-50); a zero start
+        // vector makes the first block fail the same way
+        if (cnt_low < 1 || cnt_low >= cnt_high) return fail(-(cnt_low - 1));
+        if (cnt_low == 1 && xn == 0.0) return fail(0);
         bool bad = false, deferred = false;
         for (int j = cnt_low; j <= cnt_high && !bad; j++) {
             double *vjj = vj.p + (long)(j - 1) * ldv;

@@ -259,3 +259,61 @@ def test_ndvr_block_diag_px_qmr():
     r = op.hx(xs, O.TA1) - b[0]
     assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(op.hx(xs, O.TA1))
     ctx.close()
+
+
+def _precond_from_gpu(ctx, op):
+    ctx.block_diag(200, 1e-5)
+    vosb, eig0 = ctx.get_precond()
+    op.set_precond(vosb, eig0)
+
+
+@pytest.mark.parametrize("grid,nv", [((80, 96, 64), 1), ((80, 96, 64), 16), ((128, 128, 96), 1)])
+def test_px_large_level1_blocks(grid, nv):
+    """Level-1 rotations larger than 64: the BIG forms of k_px_l1, on the MFMA
+    path for n1 = 80 at batch 16 and on the VALU path for n1 = 80 at batch 1
+    and n1 = 128, TOSB and TOSBD2, FP64 (1e-12) and FP32 (1e-4)."""
+    print(f"px large blocks grid={grid} nv={nv}", flush=True)
+    prob, op, ctx = setup(grid, e0=-0.02)
+    _precond_from_gpu(ctx, op)
+    x = P.random_vectors(prob.N, nv, 14)
+    xd = ctx.to_device(x)
+    check = sorted({0, nv // 2, nv - 1})
+    for kind in (O.TOSB, O.TOSBD2):
+        op.set_shift(-0.02, 0.004, 0.3)
+        ctx.set_shift(-0.02, 0.004, 0.3)
+        y = ctx.px(xd, nosb=kind).cpu().numpy()
+        for b in check:
+            assert rel(y[b], op.px(x[b], kind=kind)) <= 1e-12, (kind, b)
+    ref = [op.px(x[b]) for b in check]
+    y32 = ctx.px_lp(ctx.to_device(x, torch.float32)).double().cpu().numpy()
+    for i, b in enumerate(check):
+        assert rel(y32[b], ref[i]) <= 1e-4, b
+    ctx.close()
+
+
+def test_hx_bench_shape_every_variant(monkeypatch):
+    """The bench workload's exact shape, 64 x 96 x 80 at batch 16, through
+    every pass-A block shape and pass-B width the autotuner can pick (forced
+    one pair at a time), FP64 within 1e-12 and FP32 within 2e-6 of the oracle
+    on the first, a middle and the last vector of the batch."""
+    grid, nv = (64, 96, 80), 16
+    prob = P.build_problem(*grid, P.gaussian_potential(5), seed=5)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    x = P.random_vectors(prob.N, nv, 21)
+    check = (0, 7, 15)
+    ref = {b: op.hx(x[b]) for b in check}
+    for va in range(8):
+        for vb in range(3):
+            print(f"hx variant A={va} B={vb}", flush=True)
+            monkeypatch.setenv("SCALIT_HX_VARIANT_A", str(va))
+            monkeypatch.setenv("SCALIT_HX_VARIANT_B", str(vb))
+            ctx = Context.from_problem(prob)
+            y = ctx.hx(ctx.to_device(x)).cpu().numpy()
+            got = ctx.hx_variant(L.FP64, nv)
+            assert got == (va, vb), got
+            for b in check:
+                assert rel(y[b], ref[b]) <= 1e-12, (va, vb, b)
+            y32 = ctx.hx_lp(ctx.to_device(x, torch.float32)).double().cpu().numpy()
+            for b in check:
+                assert rel(y32[b], ref[b]) <= 2e-6, (va, vb, b)
+            ctx.close()
