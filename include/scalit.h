@@ -172,6 +172,16 @@ int scalit_lan_conv(scalit_ctx *ctx, double e0, double etol, int ntype, const do
 int scalit_eig_gradients(scalit_ctx *ctx, int nk, const double *psi, long ld, int np, const double *dvdp, long lddv,
                          double *out);
 
+/* scalit_eig_gradients without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as scalit_eig_gradients, with nk * np <= 2048 (else
+ * SCALIT_EINVAL): its partial sums live in a fixed 8.4 MB device array per
+ * GPU, held until the process ends.  Calls that overlap on one GPU share that
+ * array and race: the caller serialises them. */
+int scalit_eig_gradients_noalloc(scalit_ctx *ctx, int nk, const double *psi, long ld, int np, const double *dvdp,
+                                 long lddv, double *out);
+
 /* ------------------------------------------------------------------------ */
 /* Fortran drop-in layer (host arrays, arguments by reference)               */
 /* ------------------------------------------------------------------------ */

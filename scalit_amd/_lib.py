@@ -66,6 +66,7 @@ SIGNATURES = {
     "scalit_pist_vectors_all": (_i, [_vp, _vp, _l, _i]),
     "scalit_lan_conv": (_i, [_vp, _d, _d, _i, _vp, _i, _i, _i, _i, _i, _dp, _dp]),
     "scalit_eig_gradients": (_i, [_vp, _i, _vp, _l, _i, _vp, _l, _dp]),
+    "scalit_eig_gradients_noalloc": (_i, [_vp, _i, _vp, _l, _i, _vp, _l, _dp]),
     "scalit_bind": (_i, [_vp]),
     "scalit_set_qmr_params": (None, [_i, _i, _d]),
     "hx_": (None, [_ip, _dp, _dp]),

@@ -370,13 +370,15 @@ class Context:
                                     C.byref(res))
         return r, np.array(eig[:]), res.value
 
-    def eig_gradients(self, psi: torch.Tensor, dvdp: torch.Tensor):
+    def eig_gradients(self, psi: torch.Tensor, dvdp: torch.Tensor, noalloc: bool = False):
+        """dE_k/dp_j for the nk rows of psi and the np rows of dvdp; with
+        noalloc, through the entry point that allocates nothing."""
         psi = psi.contiguous()
         dvdp = dvdp.contiguous()
         nk, np_ = psi.shape[0], dvdp.shape[0]
         out = np.zeros((nk, np_))
-        check(L.lib().scalit_eig_gradients(self._h, nk, _ptr(psi), self.N, np_, _ptr(dvdp), self.N, _np_dp(out)),
-              "scalit_eig_gradients")
+        name = "scalit_eig_gradients_noalloc" if noalloc else "scalit_eig_gradients"
+        check(getattr(L.lib(), name)(self._h, nk, _ptr(psi), self.N, np_, _ptr(dvdp), self.N, _np_dp(out)), name)
         return out
 
     def bind(self):

@@ -195,6 +195,13 @@ int scalit_create(scalit_ctx **out, int sF, const int *sN, const int *sDEP, int 
         return SCALIT_EHIP;
     }
     c->fast3 = fast3_ok(c);
+    // the fixed scratch of the no-allocation entry point: its address on this device
+    double *scratch_dev = nullptr;
+    if (int r = noalloc_scratch(device, &scratch_dev); r != SCALIT_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return r;
+    }
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
         g_num_cu = ncu;

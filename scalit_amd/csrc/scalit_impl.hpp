@@ -198,6 +198,10 @@ constexpr int NPART = 512;   // partial sums per vector in the deterministic red
 void launch_dot_partials(int nvec, long n, const double *x, long ldx, const double *y, long ldy, double *part,
                          hipStream_t s);
 void launch_reduce_partials(int nvec, const double *part, double *out, hipStream_t s);   // out[v] = sum
+// device address of the no-allocation entry point's fixed scratch on `device`
+// (current), looked up once per device (noalloc.hip)
+int noalloc_scratch(int device, double **ptr);
+
 int dot_host(int nvec, long n, const double *x, long ldx, const double *y, long ldy, double *out_host,
              double *part_dev, double *red_dev, hipStream_t s);
 
