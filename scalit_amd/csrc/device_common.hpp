@@ -121,6 +121,12 @@ __device__ __forceinline__ void buf_st<float, bf16_t>(__amdgpu_buffer_rsrc_t r, 
 
 __host__ __device__ static inline int rup16(int n) { return (n + 15) & ~15; }
 
+// e / d for 0 <= e < 2^22 and d >= 1, with inv = 1.0f / d: (e + 1/2) / d lies
+// at least 1/(2d) from an integer, and the fp32 product is off by at most
+// (e / d) 2^-23 < 1/(2d), so truncation gives the exact quotient in four
+// VALU operations instead of an integer division's twenty-odd.
+__device__ __forceinline__ int fdiv_small(int e, float inv) { return (int)(((float)e + 0.5f) * inv); }
+
 // Copy count contiguous values g[0..count) into LDS, value e to slot(e), with
 // U independent loads in flight per thread (NT threads): a block of tens of
 // KB then arrives in a few memory latencies instead of one latency per

@@ -719,6 +719,60 @@ __device__ double block_sum_1024(double v, double *red)
     return t;
 }
 
+// Eigenvalues, ascending, of the symmetric tridiagonal matrix with diagonal
+// d[0..n) and off-diagonal e[0..n-1) into w, by the whole workgroup
+// (EIGV_NT threads): multisection on the Sturm count, T threads per
+// eigenvalue (a power of two <= 64, so a group never spans two wavefronts).
+__device__ void tridiag_bisect(int n, const double *d, const double *e, double *__restrict__ w)
+{
+    const int tid = threadIdx.x;
+    // Gershgorin interval of the tridiagonal matrix
+    double lo = 1e300, hi = -1e300;
+    for (int i = 0; i < n; i++) {
+        const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
+        lo = fmin(lo, d[i] - r);
+        hi = fmax(hi, d[i] + r);
+    }
+    const double span = fmax(hi - lo, 1e-300);
+    lo -= 1e-12 * span;
+    hi += 1e-12 * span;
+    const double pivmin = 1e-290;
+    // T = threads per eigenvalue: a power of two <= 64 (a group never spans
+    // two wavefronts, so the group reductions are lane shuffles)
+    int T = 1;
+    while (T < 64 && 2 * T * n <= EIGV_NT) T *= 2;
+    const int sub = tid & (T - 1);
+    const double frac = (double)(sub + 1) / (double)(T + 1);
+    for (int base = 0; base < n; base += EIGV_NT / T) {
+        // the index this group brackets: the smallest x with count(x) > idx
+        const int idx = base + tid / T;
+        const int ie = idx < n ? idx : n - 1;
+        double a = lo, b = hi;   // count(a) <= ie < count(b), identical across the group
+        for (int it = 0; it < 200; it++) {
+            const double mid = 0.5 * (a + b);
+            if (mid <= a || mid >= b) break;   // interval at one ulp
+            const double x = T == 1 ? mid : a + (b - a) * frac;
+            int cnt = 0;
+            double q = d[0] - x;
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q < 0;
+            for (int i = 1; i < n; i++) {
+                q = d[i] - x - e[i - 1] * e[i - 1] / q;
+                if (fabs(q) < pivmin) q = -pivmin;
+                cnt += q < 0;
+            }
+            double bn = cnt > ie ? x : b;
+            for (int o = 1; o < T; o <<= 1) bn = fmin(bn, __shfl_xor(bn, o));
+            double an = (cnt <= ie && x < bn) ? x : a;
+            for (int o = 1; o < T; o <<= 1) an = fmax(an, __shfl_xor(an, o));
+            if (an == a && bn == b) break;   // every point fell on an end: a few ulps apart
+            a = an;
+            b = bn;
+        }
+        if (idx < n && sub == 0) w[idx] = 0.5 * (a + b);
+    }
+}
+
 template <bool LDS_A>
 __global__ void __launch_bounds__(EIGV_NT) k_sym_eigvals(const EigvalJob *__restrict__ jobs, EigvalJob one,
                                                          double *__restrict__ gwork, long wstride)
@@ -787,51 +841,7 @@ __global__ void __launch_bounds__(EIGV_NT) k_sym_eigvals(const EigvalJob *__rest
         d[n - 1] = A[(n - 1) + ld * (n - 1)];
     }
     __syncthreads();
-    // Gershgorin interval of the tridiagonal matrix
-    double lo = 1e300, hi = -1e300;
-    for (int i = 0; i < n; i++) {
-        const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
-        lo = fmin(lo, d[i] - r);
-        hi = fmax(hi, d[i] + r);
-    }
-    const double span = fmax(hi - lo, 1e-300);
-    lo -= 1e-12 * span;
-    hi += 1e-12 * span;
-    const double pivmin = 1e-290;
-    // T = threads per eigenvalue: a power of two <= 64 (a group never spans
-    // two wavefronts, so the group reductions are lane shuffles)
-    int T = 1;
-    while (T < 64 && 2 * T * n <= EIGV_NT) T *= 2;
-    const int sub = tid & (T - 1);
-    const double frac = (double)(sub + 1) / (double)(T + 1);
-    for (int base = 0; base < n; base += EIGV_NT / T) {
-        // the index this group brackets: the smallest x with count(x) > idx
-        const int idx = base + tid / T;
-        const int ie = idx < n ? idx : n - 1;
-        double a = lo, b = hi;   // count(a) <= ie < count(b), identical across the group
-        for (int it = 0; it < 200; it++) {
-            const double mid = 0.5 * (a + b);
-            if (mid <= a || mid >= b) break;   // interval at one ulp
-            const double x = T == 1 ? mid : a + (b - a) * frac;
-            int cnt = 0;
-            double q = d[0] - x;
-            if (fabs(q) < pivmin) q = -pivmin;
-            cnt += q < 0;
-            for (int i = 1; i < n; i++) {
-                q = d[i] - x - e[i - 1] * e[i - 1] / q;
-                if (fabs(q) < pivmin) q = -pivmin;
-                cnt += q < 0;
-            }
-            double bn = cnt > ie ? x : b;
-            for (int o = 1; o < T; o <<= 1) bn = fmin(bn, __shfl_xor(bn, o));
-            double an = (cnt <= ie && x < bn) ? x : a;
-            for (int o = 1; o < T; o <<= 1) an = fmax(an, __shfl_xor(an, o));
-            if (an == a && bn == b) break;   // every point fell on an end: a few ulps apart
-            a = an;
-            b = bn;
-        }
-        if (idx < n && sub == 0) w[idx] = 0.5 * (a + b);
-    }
+    tridiag_bisect(n, d, e, w);
 }
 
 int small_sym_eig(int n, const double *a, int lda, double *w, double *z, int ldz, double *work, hipStream_t s)
@@ -904,6 +914,46 @@ int small_sym_eigvals_batch(int njobs, const EigvalJob *jobs, DBuf<EigvalJob> &d
         hipLaunchKernelGGL(k_sym_eigvals<false>, dim3(njobs), dim3(EIGV_NT), vec_bytes, s, djobs.p, none, work.p,
                            stride);
     }
+    HIP_TRY(hipGetLastError());
+    return SCALIT_OK;
+}
+
+// Eigenvalues of symmetric tridiagonal matrices (the Lanczos DSTEV steps),
+// one workgroup per job: the diagonal and off-diagonal are staged in LDS
+// when they fit, then the same Sturm multisection as k_sym_eigvals.
+constexpr int TRI_LDS_N = 9000;
+__global__ void __launch_bounds__(EIGV_NT) k_tri_eigvals(const TriEigJob *__restrict__ jobs)
+{
+    extern __shared__ double sm_tri[];
+    const TriEigJob jb = jobs[blockIdx.x];
+    const int n = jb.n;
+    if (n <= TRI_LDS_N) {
+        double *d = sm_tri, *e = sm_tri + n;
+        for (int i = threadIdx.x; i < n; i += EIGV_NT) {
+            d[i] = jb.d[i];
+            e[i] = i + 1 < n ? jb.e[i] : 0.0;
+        }
+        __syncthreads();
+        tridiag_bisect(n, d, e, jb.w);
+    } else {
+        tridiag_bisect(n, jb.d, jb.e, jb.w);
+    }
+}
+
+int tridiag_eigvals_batch(int njobs, const TriEigJob *jobs, DBuf<TriEigJob> &djobs, hipStream_t s)
+{
+    if (njobs < 0) return SCALIT_EINVAL;
+    if (njobs == 0) return SCALIT_OK;
+    int nmax = 0;
+    for (int j = 0; j < njobs; j++) {
+        if (jobs[j].n < 1 || !jobs[j].d || !jobs[j].e || !jobs[j].w) return SCALIT_EINVAL;
+        nmax = std::max(nmax, jobs[j].n);
+    }
+    allow_full_lds((const void *)k_tri_eigvals);
+    SC_TRY(djobs.alloc(njobs));
+    HIP_TRY(hipMemcpyAsync(djobs.p, jobs, sizeof(TriEigJob) * njobs, hipMemcpyHostToDevice, s));
+    const size_t lds = nmax <= TRI_LDS_N ? 2 * (size_t)nmax * sizeof(double) : 0;
+    hipLaunchKernelGGL(k_tri_eigvals, dim3(njobs), dim3(EIGV_NT), lds, s, djobs.p);
     HIP_TRY(hipGetLastError());
     return SCALIT_OK;
 }

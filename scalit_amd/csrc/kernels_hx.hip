@@ -262,8 +262,15 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
             for (int e = tid; e < n2p * n2p; e += NT)
                 if (e % n2p >= n2 || e / n2p >= n2) sT2[e] = (T)0;
         }
-        stage_lds<NT, 16>(t1, n1 * n1, tid, [&](int e) -> T & { return sT1[(e % n1) + n1p * (e / n1)]; });
-        stage_lds<NT, 16>(t2, n2 * n2, tid, [&](int e) -> T & { return sT2[(e % n2) + n2p * (e / n2)]; });
+        const float r1 = 1.0f / (float)n1, r2 = 1.0f / (float)n2;
+        stage_lds<NT, 16>(t1, n1 * n1, tid, [&](int e) -> T & {
+            const int q = fdiv_small(e, r1);
+            return sT1[(e - q * n1) + n1p * q];
+        });
+        stage_lds<NT, 16>(t2, n2 * n2, tid, [&](int e) -> T & {
+            const int q = fdiv_small(e, r2);
+            return sT2[(e - q * n2) + n2p * q];
+        });
     }
     // first slab of this workgroup that is active
     int k = 0, s = slab_of(g, 0, G, nslab);
@@ -271,20 +278,54 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
     // The prefetches load unconditionally (indices clamped into the slab; the
     // values outside it are replaced by zeros when staged, or never stored), so
     // that the waits for them count exactly and never serialize the loads.
-    T pf[PFMAX];
-    // (sl < 0, no next slab: one value of slab 0 on every lane)
-    // (the thread index goes through an opaque copy so that the per-value
-    // addresses are recomputed per slab rather than held in registers)
-    auto prefetch = [&](int sl) {
-        const bool live = sl >= 0;
-        const S *xs = live ? x + (long)(sl % nvec) * ldx + P * (sl / nvec) : x;
-        int t0 = tid;
-        asm volatile("" : "+v"(t0));
+    // Slot j of a thread is element e = tid + j NT of the padded slab
+    // (i1 = e % n1p, i2 = e / n1p); its index in the X slab (clamped into the
+    // grid, low 16 bits of slot[j]), its LDS index (high 16 bits: the slab
+    // holds at most 20,480 values) and two flags (bit j of `inside`: the value
+    // lies in the grid, else a zero is staged; bit j of `owned`: the slot lies
+    // in the padded slab, else nothing is stored) are the same for every slab,
+    // so with 8 slots they are worked out once here.  With 16 (fewer waves)
+    // a table would cost 18 registers the 8-wave shapes do not have: there
+    // each slot is recomputed per slab, from an opaque copy of the thread
+    // index so that the compiler cannot hoist the table back into registers.
+    const float rp = 1.0f / (float)n1p;
+    auto slot_calc = [&](int t0, int j, unsigned &pk, bool &in, bool &own) {
+        const int e = t0 + j * NT;
+        const int i2 = fdiv_small(e, rp), i1 = e - i2 * n1p;
+        const unsigned g = (unsigned)(min(i1, n1 - 1) + n1 * min(i2, n2 - 1));
+        pk = (g & 0xffffu) | ((unsigned)(i1 + ldS * i2) << 16);
+        in = i1 < n1 && i2 < n2;
+        own = j < npf && e < n1p * n2p;
+    };
+    constexpr bool TABLE = PFMAX <= 8;
+    unsigned slot[TABLE ? PFMAX : 1];
+    unsigned inside = 0, owned = 0;
+    if constexpr (TABLE) {
 #pragma unroll
         for (int j = 0; j < PFMAX; j++) {
-            const int e = t0 + j * NT;
-            const int i1 = live ? min(e % n1p, n1 - 1) : 0, i2 = live ? min(e / n1p, n2 - 1) : 0;
-            pf[j] = ld_cvt<T, S>(xs, i1 + (long)n1 * i2);
+            bool in, own;
+            slot_calc(tid, j, slot[j], in, own);
+            inside |= (unsigned)in << j;
+            owned |= (unsigned)own << j;
+        }
+    }
+    T pf[PFMAX];
+    // (sl < 0, no next slab: the slots of slab 0 again)
+    auto prefetch = [&](int sl) {
+        const S *xs = sl >= 0 ? x + (long)(sl % nvec) * ldx + P * (sl / nvec) : x;
+        const auto rx = buf_rsrc(xs, P * (long)sizeof(S));
+        int t0 = tid;
+        if constexpr (!TABLE) asm volatile("" : "+v"(t0));
+#pragma unroll
+        for (int j = 0; j < PFMAX; j++) {
+            unsigned pk;
+            if constexpr (TABLE) {
+                pk = slot[j];
+            } else {
+                bool in, own;
+                slot_calc(t0, j, pk, in, own);
+            }
+            pf[j] = buf_ld<T, S>(rx, (int)((pk & 0xffffu) * sizeof(S)), 0);
         }
     };
     // V of the wave's first output block (block index = wave)
@@ -311,13 +352,22 @@ __global__ void __launch_bounds__(64 * WAVES) k_hx_modes12(int n1, int n2, int n
 
     while (s >= 0) {
         // the prefetched slab into LDS (padding zero)
-        int t0 = tid;
-        asm volatile("" : "+v"(t0));
+        {
+            int t0 = tid;
+            if constexpr (!TABLE) asm volatile("" : "+v"(t0));
 #pragma unroll
-        for (int j = 0; j < PFMAX; j++) {
-            const int e = t0 + j * NT;
-            const int i1 = e % n1p, i2 = e / n1p;
-            if (j < npf && e < n1p * n2p) sS[i1 + ldS * i2] = (i1 < n1 && i2 < n2) ? pf[j] : (T)0;
+            for (int j = 0; j < PFMAX; j++) {
+                unsigned pk;
+                bool in, own;
+                if constexpr (TABLE) {
+                    pk = slot[j];
+                    in = (inside >> j) & 1u;
+                    own = (owned >> j) & 1u;
+                } else {
+                    slot_calc(t0, j, pk, in, own);
+                }
+                if (own) sS[pk >> 16] = in ? pf[j] : (T)0;
+            }
         }
         if (npf > PFMAX) {
             const S *xs = x + (long)(s % nvec) * ldx + P * (s / nvec);

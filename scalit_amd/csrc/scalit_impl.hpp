@@ -181,6 +181,15 @@ struct EigvalJob {
 };
 int small_sym_eigvals_batch(int njobs, const EigvalJob *jobs, DBuf<EigvalJob> &djobs, DBuf<double> &work,
                             hipStream_t s);
+// eigenvalues, ascending, of several symmetric tridiagonal matrices in one
+// launch (one workgroup each): diagonal d[0..n), off-diagonal e[0..n-1), into
+// w (device).  Same lifetime rule for the host jobs.
+struct TriEigJob {
+    const double *d, *e;
+    double *w;
+    int n;
+};
+int tridiag_eigvals_batch(int njobs, const TriEigJob *jobs, DBuf<TriEigJob> &djobs, hipStream_t s);
 // eigenvalues and eigenvectors of several small symmetric matrices in one
 // launch (cyclic Jacobi, one workgroup each): z gets the eigenvectors (n x n,
 // ldz) in the order of w.  Same lifetime rule for the host jobs; work grows to

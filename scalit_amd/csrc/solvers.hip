@@ -370,8 +370,8 @@ struct PistState {
     DBuf<RitzJob> rjobs;
     DBuf<double> gpart, gred;   // Hellmann-Feynman partial sums and results
     // Lanczos (kept between calls: a device free would synchronise the whole device)
-    DBuf<double> lvj, lalpha, lbeta, lw, lpart, lpart2, lnrm, ltri, lev, lwork, lzero;
-    DBuf<EigvalJob> lejobs;
+    DBuf<double> lvj, lalpha, lbeta, lw, lpart, lpart2, lnrm, lev, lzero;
+    DBuf<TriEigJob> lejobs;
     DBuf<int> lokv;
     int nwin = 0, m_max = 0, m_last = 0;
     std::vector<int> last_m;
@@ -408,7 +408,7 @@ struct PistState {
         rjobs.release();
         gpart.release();
         gred.release();
-        for (DBuf<double> *b : {&lvj, &lalpha, &lbeta, &lw, &lpart, &lpart2, &lnrm, &ltri, &lev, &lwork, &lzero})
+        for (DBuf<double> *b : {&lvj, &lalpha, &lbeta, &lw, &lpart, &lpart2, &lnrm, &lev, &lzero})
             b->release();
         lejobs.release();
         lokv.release();
@@ -977,18 +977,6 @@ __global__ void k_store_norm(const double *part, double *out, int idx)
     out[idx] = sqrt(a);
 }
 
-__global__ void k_tridiag(int n, const double *alpha, const double *beta, double *t, int ldt)
-{
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-        int i = e % n, j = e / n;
-        double v = 0;
-        if (i == j) v = alpha[i];
-        else if (i + 1 == j) v = beta[i];
-        else if (j + 1 == i) v = beta[j];
-        t[i + (long)ldt * j] = v;
-    }
-}
-
 int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, int startcnt, int stepcnt,
              int stepeig, int m, int m_max, double *eig, double *res, hipStream_t s)
 {
@@ -997,7 +985,7 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
     PistState &P = pstate(c);
     const long ldv = (N + 63) & ~63L;
     DBuf<double> &vj = P.lvj, &alpha = P.lalpha, &beta = P.lbeta, &w = P.lw, &part = P.lpart, &part2 = P.lpart2,
-                 &nrm = P.lnrm, &tri = P.ltri, &ev = P.lev, &work = P.lwork, &zero = P.lzero;
+                 &nrm = P.lnrm, &ev = P.lev, &zero = P.lzero;
     SC_TRY(vj.alloc((size_t)(m_max + 1) * ldv));
     SC_TRY(alpha.alloc(m_max));
     SC_TRY(beta.alloc(m_max));
@@ -1005,10 +993,8 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
     SC_TRY(part.alloc(NPART));
     SC_TRY(part2.alloc(NPART));
     SC_TRY(nrm.alloc(4));
-    SC_TRY(tri.alloc((size_t)m_max * m_max));
     SC_TRY(ev.alloc(2 * (size_t)m_max));
-    SC_TRY(work.alloc(small_sym_eig_work(m_max)));
-    DBuf<EigvalJob> &ejobs = P.lejobs;
+    DBuf<TriEigJob> &ejobs = P.lejobs;
     DBuf<int> &okv = P.lokv;   // per Lanczos step: the re-orthogonalised vector has a nonzero norm
     SC_TRY(okv.alloc((size_t)m_max + 1));
     HIP_TRY(hipMemsetAsync(alpha.p, 0, 8 * m_max, s));
@@ -1079,11 +1065,11 @@ int lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const double *x, 
                 if (!v) bad = true;
         }
         if (bad) return fail(-(cnt_low - 1));
-        // eigenvalues of the leading (cnt_high - stepeig) and cnt_high blocks
-        // of the tridiagonal matrix, in one launch
-        hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(1024), 0, s, cnt_high, alpha.p, beta.p, tri.p, m_max);
-        const EigvalJob jobs[2] = {{tri.p, ev.p, cnt_high - stepeig, m_max}, {tri.p, ev.p + m_max, cnt_high, m_max}};
-        SC_TRY(small_sym_eigvals_batch(2, jobs, ejobs, work, s));
+        // DSTEV of the leading (cnt_high - stepeig) and cnt_high blocks of the
+        // tridiagonal (ALPHA, BETA), lan.f90:57-69, in one launch
+        const TriEigJob jobs[2] = {{alpha.p, beta.p, ev.p, cnt_high - stepeig},
+                                   {alpha.p, beta.p, ev.p + m_max, cnt_high}};
+        SC_TRY(tridiag_eigvals_batch(2, jobs, ejobs, s));
         HIP_TRY(hipMemcpyAsync(w_h.data(), ev.p, 8 * 2 * (size_t)m_max, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         get_window0(e0, cnt_high - stepeig, w_h.data(), m, ind.data(), oldeig.data());

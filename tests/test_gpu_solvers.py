@@ -84,8 +84,7 @@ def test_pist_fp64_config1_against_golden():
     print("pist", r, res, stats, flush=True)
     assert r > 0
     ref = np.array(g["eigsh"])
-    assert np.abs(eig - ref).max() / np.abs(ref).max() <= 1e-10
-    assert (np.abs(eig - ref) / np.abs(ref)).max() <= 1e-9
+    assert (np.abs(eig - ref) / np.abs(ref)).max() <= 1e-10
     # Ritz vectors: unit norm, Rayleigh quotients reproduce the eigenvalues
     psi = ctx.pist_vectors(0, g["m"])
     hp = ctx.hx(psi)
@@ -136,7 +135,7 @@ def test_pist_batched_windows():
     for k, e0 in enumerate(e0s):
         # the 4 eigenvalues nearest E0 of the golden set
         want = np.sort(ref[np.argsort(np.abs(ref - e0), kind="stable")[:4]])
-        assert (np.abs(eig[k] - want) / np.abs(want)).max() <= 1e-9
+        assert (np.abs(eig[k] - want) / np.abs(want)).max() <= 1e-10
     ctx.close()
 
 
@@ -165,6 +164,90 @@ def test_pist_batched_one_level_windows():
     assert np.abs(rq - eig.reshape(-1)).max() <= 1e-9 * np.abs(ref).max()
     with pytest.raises(L.ScalitError):
         ctx.pist_vectors_all(10, 1)   # too few columns for 20 windows: refused
+    ctx.close()
+
+
+def _lan_abi(ctx, x, e0, etol, m, m_max, startcnt, stepcnt, stepeig, ntype=0):
+    """Lan_Conv through the Fortran drop-in layer with the library's own H*X
+    callback (hx0_), host arrays by reference."""
+    lib = L.lib()
+    ctx.bind()
+    ci = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    cd = lambda v: C.byref(C.c_double(v))  # noqa: E731
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    eig = np.zeros(m)
+    res = C.c_double()
+    r = lib.lan_conv_(cd(e0), cd(etol), ci(ntype), ci(x.size), O.dptr(x), ci(startcnt), ci(stepcnt), ci(stepeig),
+                      ci(m), ci(m_max), C.cast(lib.hx0_, L.FOP), O.dptr(eig), C.byref(res))
+    return r, eig, res.value
+
+
+def test_lanczos_config1_against_golden():
+    """GPU Lanczos on the 24 x 32 x 24 workload: the lowest 20 levels within
+    1e-10 of the golden set, on the handle API and through lan_conv_."""
+    g = gold()
+    ref = np.array(g["eigsh"])
+    prob = P.config_problem("pist_fp64")
+    ctx = Context.from_problem(prob)
+    x = P.random_vectors(prob.N, 1, 3)
+    e0 = float(ref[0]) - 0.002
+    # (the 20 lowest levels need about 1,300 Lanczos steps on this operator)
+    r, eig, res = ctx.lan_conv(ctx.to_device(x), e0, 2e-12, 20, 1600, 400, 100, 10)
+    print("lanczos config1", r, res, flush=True)
+    assert r > 0, r
+    assert (np.abs(eig - ref) / np.abs(ref)).max() <= 1e-10
+    r2, eig2, _ = _lan_abi(ctx, x[0], e0, 2e-12, 20, 1600, 400, 100, 10)
+    assert r2 == r and np.array_equal(eig2, eig)
+    ctx.close()
+
+
+def test_lanczos_per_step_path():
+    """A grid above 20,480 points (24 x 32 x 28 = 21,504) takes the
+    per-step re-orthogonalisation path (launch per projection, BETA through
+    k_store_norm, a host check per step): converged, against scipy's eigsh
+    of the oracle's H*X (1e-10) and through lan_conv_; and a short run that
+    stops at M_MAX unconverged, step for step against the oracle's Lan_Conv
+    (same return code, Ritz values within 1e-9)."""
+    import scipy.sparse.linalg as sla
+
+    prob = P.config_problem("pist_fp64", grid=(24, 32, 28))
+    assert prob.N > 20480
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    a = sla.LinearOperator((op.N, op.N), matvec=lambda v: op.hx(v), dtype=np.float64)
+    ev = np.sort(sla.eigsh(a, k=8, which="SA", tol=1e-13, ncv=60)[0])[:5]
+    ctx = Context.from_problem(prob)
+    x = P.random_vectors(prob.N, 1, 4)
+    e0 = float(ev[0]) - 0.002
+    # (the 5 lowest levels need about 650 steps here)
+    r, eig, res = ctx.lan_conv(ctx.to_device(x), e0, 1e-12, 5, 800, 200, 100, 5)
+    print("lanczos per-step", r, res, flush=True)
+    assert r > 0, r
+    assert (np.abs(eig - ev) / np.abs(ev)).max() <= 1e-10
+    r2, eig2, _ = _lan_abi(ctx, x[0], e0, 1e-12, 5, 800, 200, 100, 5)
+    assert r2 == r and np.array_equal(eig2, eig)
+    r_o, eig_o, _ = op.lan_conv(x[0], e0, 1e-30, 3, 150, 60, 30, 5)
+    r_s, eig_s, _ = ctx.lan_conv(ctx.to_device(x), e0, 1e-30, 3, 150, 60, 30, 5)
+    print("lanczos per-step short", r_s, r_o, flush=True)
+    assert r_s == r_o == -150
+    assert np.abs(eig_s - eig_o).max() <= 1e-9 * np.abs(eig_o).max()
+    ctx.close()
+
+
+def test_lanczos_block_without_steps():
+    """lanH0 returns 0 when a block would have no step (CNT_LOW >= CNT_HIGH,
+    lanHij.f90:20) and Lan_Conv then returns -(CNT_LOW-1) (lan.f90:47-50):
+    stepCnt 0 or 1 after an unconverged first block, and a first block of one
+    step, give the oracle's return codes."""
+    prob = P.config_problem("pist_fp64", grid=(8, 9, 6))
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    ctx = Context.from_problem(prob)
+    x = P.random_vectors(prob.N, 1, 6)
+    cases = [(1e-30, 3, 60, 20, 0, 2), (1e-30, 3, 60, 20, 1, 2), (1e-8, 1, 60, 1, 5, 0)]
+    for etol, m, m_max, start, step, stepeig in cases:
+        r_o, _, _ = op.lan_conv(x[0], 0.0, etol, m, m_max, start, step, stepeig)
+        r, _, _ = ctx.lan_conv(ctx.to_device(x), 0.0, etol, m, m_max, start, step, stepeig)
+        print("lanczos no-step block", (etol, m, m_max, start, step, stepeig), r, r_o, flush=True)
+        assert r == r_o and r <= 0
     ctx.close()
 
 
