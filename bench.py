@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -183,6 +184,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline(prob, B, args.cpu_seconds)
     if not args.no_pist:
         out["pist"] = pist_bench(args.no_cpu)
+        out["gradients_64x96x80"] = gradients_bench(ctx, prob)
     ctx.close()
     print(json.dumps(out))
 
@@ -258,12 +260,75 @@ def cpu_baseline(prob, B, seconds):
     bytes_vec = 8 * (2 * prob.N + prob.N / B)
     return {"value": round(done * bytes_vec / dt / 1e9, 3), "unit": "GB/s", "cores": int(O.lib().orc_omp_threads()),
             "kind": "port", "sample": f"{done} single-vector H*X applications of the 64x96x80 operator "
-                                      f"({dt:.1f} s), same byte accounting as the GPU value"}
+                                      f"({dt:.1f} s), same byte accounting as the GPU value",
+            "host": host_info()}
+
+
+def host_info():
+    """Host cores and CPU model of the machine the baseline runs on."""
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:   # lscpu missing: leave the model empty
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except Exception:
+        nproc = os.cpu_count()
+    return {"nproc": nproc, "cpu_model": model}
+
+
+def gradients_bench(ctx, prob):
+    """The adjoint-gradient workload: the lowest 50 levels of the 64x96x80
+    operator by one FP64 PIST window (ETOL 1e-10, QMR 1e-6), their Ritz
+    vectors and dE_k/dp_j for the 12 potential parameters; timed with and
+    without the vectors and gradients (OSB set-up outside the timed region)."""
+    import torch
+
+    from scalit_amd import problems as P
+
+    ctx.block_diag(1000, 1e-5)
+    e0 = float(prob.res.min())
+    ctx.set_shift(e0)
+    x0 = ctx.to_device(P.random_vectors(prob.N, 1, 91))
+    dvdp = ctx.to_device(prob.potential.gradient(prob.r, prob.R, prob.cos_theta))
+    out = {}
+    for with_grad in (False, True):
+        for rep in range(2):   # the first run records the QMR graph
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r, eig, rr, stats = ctx.pist_conv(x0, e0, 1e-10, 50, 400, 150, 20, 10, qmrtol=1e-6)
+            if with_grad:
+                grad = ctx.eig_gradients(ctx.pist_vectors(0, 50), dvdp)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        key = "pist_and_gradients" if with_grad else "pist_only"
+        out[key] = {"ret": int(r), "seconds": round(dt, 3), "eigenpairs_per_s": round(50 / dt, 3),
+                    "qmr_iters": int(stats[0]), "krylov": int(stats[2])}
+        if with_grad:
+            out[key]["parameters"] = int(grad.shape[1])
+            out[key]["gradients_finite"] = bool(np.isfinite(grad).all())
+    return out
+
+
+def window_centres(levels):
+    """One window per level, centred a quarter of the smaller neighbouring gap
+    inside the level's interval, so that its nearest level is that level."""
+    lv = np.sort(np.asarray(levels))
+    gaps = np.diff(lv)
+    c = [float(lv[k] + 0.25 * min(gaps[max(k - 1, 0)], gaps[k])) for k in range(len(lv) - 1)]
+    c.append(float(lv[-1] - 0.25 * gaps[-1]))
+    return c
 
 
 def pist_bench(no_cpu):
-    """PIST on the 24x32x24 workload (lowest 20 levels): plain FP64 and FP32-inner
-    single-window runs, and the same levels as 10 batched shift windows."""
+    """PIST on the 24x32x24 workload (lowest 20 levels, ETOL 1e-11): plain FP64
+    and FP32-inner single windows; and one step of a potential-parameter scan
+    solved as batched one-level windows placed from the previous step's
+    levels, with every level's Ritz vector and Hellmann-Feynman gradients."""
     import torch
 
     from scalit_amd import _lib as L
@@ -285,76 +350,66 @@ def pist_bench(no_cpu):
             r, eig, rr, stats = ctx.pist_conv(x0, g["e0"], etol, 20, 200, 40, 10, 5, qmrtol=qtol, prec=prec)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
+        if name == "fp64":
+            eig_fp64 = eig.copy()
         err = float((np.abs(eig - ref) / np.abs(ref)).max())
         res[name] = {"ret": int(r), "seconds": round(dt, 3), "eigenpairs_per_s": round(20 / dt, 3),
                      "max_rel_err": err, "qmr_iters": int(stats[0]), "krylov": int(stats[2])}
-    # the same 20 levels as 10 shift windows of 2 levels, solved in lock-step
-    # (one QMR launch per iteration for all windows); window centres are the
-    # means of consecutive pairs of the reference levels, and every reference
-    # level must be found by some window
-    # Lock-step windows wait for the slowest one, whose Krylov count moves with
-    # rounding-level differences in the inner solves, so the batched figure is
-    # the mean over several sets of start vectors (each timed after a warm-up run
-    # that records the batched QMR graph).
-    # A second layout: 20 windows of one level each, every centre a quarter of
-    # the smaller neighbouring gap inside its level's interval (so that its
-    # nearest level is that level), as when approximate levels from an earlier
-    # run are refined.  The one-level windows converge in fewer Krylov steps.
-    centres2 = [float(ref[2 * k:2 * k + 2].mean()) for k in range(10)]
-    gaps = np.diff(ref)
-    centres1 = [float(ref[k] + 0.25 * min(gaps[max(k - 1, 0)], gaps[k])) for k in range(19)]
-    centres1.append(float(ref[19] - 0.25 * gaps[18]))
+    # Continuation along a potential-parameter scan: the levels of the plain
+    # FP64 solve above (this potential) place one window per level for the
+    # next scan point, the potential with its 6 Morse parameters moved by a
+    # seeded 0.2%.  There the batched one-level windows (20 windows in
+    # lock-step), every level's Ritz vector and the Hellmann-Feynman gradients
+    # for the 12 parameters are timed against a plain FP64 PIST of the same
+    # new operator, which is also the reference for the errors.
+    prev = np.sort(eig_fp64)
+    pot = prob.potential
+    u = P.rng(4321).uniform(-1.0, 1.0, 12)
+    p_new = pot.params() * (1.0 + 0.002 * u)
+    prob2 = prob.with_potential(pot.with_params(p_new))
+    ctx.set_res(prob2.res)
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r2, ref2, _, st_plain = ctx.pist_conv(x0, g["e0"], 1e-11, 20, 200, 40, 10, 5, qmrtol=1e-6)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+    ref2 = np.sort(ref2)
+    centres = window_centres(prev)
+    dvdp = ctx.to_device(prob2.potential.gradient(prob.r, prob.R, prob.cos_theta))
     seeds = [g["start_seed"] + i for i in range(4)]
-    for name, prec, centres, m_win, start in (("batched_fp64", L.FP64, centres2, 2, 12),
-                                              ("batched_fp32", L.FP32, centres2, 2, 12),
-                                              ("batched_fp64_1level", L.FP64, centres1, 1, 4)):
-        nwin = len(centres)
-        times, errs, iters, conv, krylov = [], [], [], True, []
-        for i, seed in enumerate(seeds):
-            x0b = ctx.to_device(P.random_vectors(prob.N, nwin, seed))
-            for rep in range(2 if i == 0 else 1):
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                ret, eig, rr, stats = ctx.pist_conv_batched(x0b, centres, 1e-11, m_win, 200, start, 4, 1,
-                                                            qmrtol=1e-6, prec=prec)
-                torch.cuda.synchronize()
-                dt = time.perf_counter() - t0
-            found = np.sort(eig.reshape(-1))
-            errs.append(max(float(np.min(np.abs(found - r)) / abs(r)) for r in ref))
-            times.append(dt)
-            iters.append(int(stats[0]))
-            krylov.append(int(stats[2]))
-            conv = conv and all(int(v) > 0 for v in ret)
-        dt = float(np.mean(times))
-        res[name] = {"windows": nwin, "levels_per_window": m_win, "start_seeds": seeds,
-                     "seconds_per_seed": [round(t, 4) for t in times], "seconds": round(dt, 4),
-                     "eigenpairs_per_s": round(20 / dt, 3), "max_rel_err": max(errs), "qmr_iters": iters,
-                     "krylov": krylov, "speedup_vs_fp64": round((20 / dt) / res["fp64"]["eigenpairs_per_s"], 3),
-                     "converged": bool(conv)}
-    # the one-level layout again with the Hellmann-Feynman gradients of every
-    # level found (one Ritz vector per window, dE/dp for the potential's 12
-    # parameters) inside the timed region
-    dvdp = ctx.to_device(prob.potential.gradient(prob.r, prob.R, prob.cos_theta))
-    times = []
+    times, errs, iters, krylov = [], [], [], []
+    converged, grad_equal = True, True
     for i, seed in enumerate(seeds):
-        x0b = ctx.to_device(P.random_vectors(prob.N, len(centres1), seed))
-        for rep in range(2 if i == 0 else 1):
+        xs = ctx.to_device(P.random_vectors(prob.N, 20, seed))
+        for rep in range(2 if i == 0 else 1):   # the first run records the graphs
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            ret, eig, rr, stats = ctx.pist_conv_batched(x0b, centres1, 1e-11, 1, 200, 4, 4, 1, qmrtol=1e-6,
-                                                        prec=L.FP64)
-            psi = ctx.pist_vectors_all(len(centres1), 1)
+            ret, eig, rr, st2 = ctx.pist_conv_batched(xs, centres, 1e-11, 1, 200, 4, 4, 1, qmrtol=1e-6)
+            psi = ctx.pist_vectors_all(20, 1)
             grad = ctx.eig_gradients(psi, dvdp)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
+        converged = converged and r2 > 0 and bool((ret > 0).all())
+        # one window per level: the sorted levels found against the plain
+        # solve's, so a level found twice (and another missed) shows as an error
+        errs.append(float((np.abs(np.sort(eig.reshape(-1)) - ref2) / np.abs(ref2)).max()))
+        # the gradients of the batched call against one window at a time
+        one = np.concatenate([ctx.eig_gradients(ctx.pist_vectors(k, 1), dvdp) for k in range(20)])
+        grad_equal = grad_equal and bool(np.array_equal(one, grad))
         times.append(dt)
+        iters.append(int(st2[0]))
+        krylov.append(int(st2[2]))
+    ctx.set_res(prob.res)
     dt = float(np.mean(times))
-    res["batched_fp64_1level_gradients"] = {
-        "windows": len(centres1), "levels_per_window": 1, "parameters": int(dvdp.shape[0]), "start_seeds": seeds,
-        "seconds_per_seed": [round(t, 4) for t in times], "seconds": round(dt, 4),
-        "eigenpairs_per_s": round(20 / dt, 3),
-        "speedup_vs_fp64": round((20 / dt) / res["fp64"]["eigenpairs_per_s"], 3),
-        "gradients_finite": bool(np.isfinite(grad).all())}
+    res["scan_step_batched_gradients"] = {
+        "windows": 20, "levels_per_window": 1, "parameters": int(dvdp.shape[0]),
+        "placement": "one window per level of the previous scan point (Morse parameters moved by 0.2%)",
+        "plain_fp64_seconds": round(t_plain, 4), "plain_fp64_ret": int(r2),
+        "start_seeds": seeds, "seconds_per_seed": [round(t, 4) for t in times], "seconds": round(dt, 4),
+        "eigenpairs_per_s": round(20 / dt, 3), "max_rel_err_vs_plain_fp64": max(errs), "qmr_iters": iters,
+        "krylov": krylov, "speedup_vs_plain_fp64": round(t_plain / dt, 3), "converged": converged,
+        "gradients_equal_one_window_at_a_time": grad_equal}
     if not no_cpu:
         from oracle import oracle as O
 
@@ -362,11 +417,12 @@ def pist_bench(no_cpu):
         op.diag(1000, 1e-5)
         t0 = time.perf_counter()
         r, eig, rr, c = op.pist_conv(P.random_vectors(prob.N, 1, g["start_seed"])[0], g["e0"], 1e-11, 20, 200, 40,
-                                     10, 5, qmr_tol=1e-6)
+                                     10, 5, qmr_tol=1e-6, threaded=True)
         dt = time.perf_counter() - t0
         res["cpu_fp64"] = {"ret": int(r), "seconds": round(dt, 3), "eigenpairs_per_s": round(20 / dt, 3),
-                           "max_rel_err": float((np.abs(eig - ref) / np.abs(ref)).max()), "cores": 1,
-                           "kind": "port"}
+                           "max_rel_err": float((np.abs(eig - ref) / np.abs(ref)).max()),
+                           "cores": int(O.lib().orc_omp_threads()), "kind": "port",
+                           "note": "the oracle's PIST_CONV with H*X and P*X split over the host threads"}
     ctx.close()
     return res
 

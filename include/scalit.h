@@ -154,11 +154,13 @@ int scalit_pist_conv_batched(scalit_ctx *ctx, int nwin, const double *e0, double
                              long ld, int startcnt, int stepcnt, int stepeig, int m, int m_max, int qmr_type,
                              int qmrmax, double qmrtol, int prec, double *eig, int *ret, double *res, int *stats);
 /* Ritz vectors of the last PIST solve of window `win` for its m eigenvalues:
- * psi (device, N x m, leading dimension ld), unit norm. */
+ * psi (device, N x m, leading dimension ld), unit norm.  SCALIT_ESTATE when
+ * that window's solve did not converge (its return code was <= 0). */
 int scalit_pist_vectors(scalit_ctx *ctx, int win, double *psi, long ld);
 /* The same for every window of the last (batched) PIST solve in one call:
  * window k's m vectors at psi + (k m + i) ld (device, ncols columns of N,
- * ncols >= nwin m, else SCALIT_EINVAL). */
+ * ncols >= nwin m, else SCALIT_EINVAL); SCALIT_ESTATE when any window's
+ * solve did not converge. */
 int scalit_pist_vectors_all(scalit_ctx *ctx, double *psi, long ld, int ncols);
 
 /* Lan_Conv (src/iterate/libiterate/lanczos/lan.f90:7-105) with lanH0
@@ -276,6 +278,29 @@ typedef struct scalit_pist_job {
 } scalit_pist_job;
 int scalit_pist_conv_multi(int njob, scalit_pist_job *jobs, double etol, int ntype, int startcnt, int stepcnt,
                            int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int prec);
+
+/* ------------------------------------------------------------------------ */
+/* sequential truncation of a DVR basis (host arrays; eigensolves on the     */
+/* current GPU)                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* DVR_Step2 + DVR_Step3 (src/ham/libham/dvr/dvr.f90:84-160): the
+ * potential-optimised DVR of one coordinate.  x: the n primitive grid points,
+ * t: their n x n kinetic matrix (column-major), v: the 1-D reference
+ * potential at x.  Keeps the m lowest eigenvectors of T + diag(v) and returns
+ * the m new grid points x0 (ascending: the eigenvalues of the position
+ * operator in that basis), the m x m Hamiltonian h0 = W^T diag(E) W on them
+ * (W with DIAG's sign convention, src/libutil/serial/base/mm.f90:66-84) and,
+ * when e != NULL, the m eigenvalues E.  n <= 1024. */
+int scalit_podvr(int n, const double *x, const double *t, const double *v, int m, double *x0, double *h0, double *e);
+/* saveH0CBALL (src/ham/ja3/module/ja3.h0:154-199): the energy-cut combined
+ * layer of two coordinates.  Keeps the nkeep product points (i, j) with the
+ * lowest v1[i] + v2[j] (ties in the reference's loop order, j outer), lists
+ * them in that loop order (idx1[a] = i, idx2[a] = j, 0-based) and builds the
+ * nkeep x nkeep layer Hamiltonian h0c(a,b) = [j_a == j_b] h1(i_a,i_b)
+ * + [i_a == i_b] h2(j_a,j_b).  Host only. */
+int scalit_combine_2d(int n1, const double *h1, const double *v1, int n2, const double *h2, const double *v2,
+                      int nkeep, int *idx1, int *idx2, double *h0c);
 
 /* ------------------------------------------------------------------------ */
 /* on-disk formats (host only, no device needed)                             */

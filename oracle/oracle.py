@@ -63,6 +63,8 @@ class OrcSolverCtx(C.Structure):
         ("px_calls", C.c_long),
         ("solves", C.c_long),
         ("qmr_iters", C.c_long),
+        ("threaded", C.c_int),
+        ("tmp", _dp),
     ]
 
 
@@ -298,10 +300,21 @@ class Operator:
         lib().orc_mypx(C.byref(self.s), C.c_int(kind), C.c_int(npc), dptr(x), dptr(y))
         return y
 
-    def _ctx(self, qmr_type, qmr_max, qmr_tol):
+    def px_omp(self, x: np.ndarray, npc: int = TA1, kind: int = TOSB) -> np.ndarray:
+        assert self.VOSB is not None
+        x = f64(x)
+        y = np.zeros(self.N)
+        lib().orc_mypx_omp(C.byref(self.s), C.c_int(kind), C.c_int(npc), dptr(x), dptr(y))
+        return y
+
+    def _ctx(self, qmr_type, qmr_max, qmr_tol, threaded=False):
         c = OrcSolverCtx()
         c.o = C.pointer(self.s)
         c.qmr_type, c.qmr_max, c.qmr_tol = qmr_type, qmr_max, qmr_tol
+        if threaded:
+            self._tmp = np.zeros(self.N)
+            c.threaded = 1
+            c.tmp = dptr(self._tmp)
         return c
 
     def qmr(self, b, qmr_type: int = -1, qmr_max: int = 1000, qmr_tol: float = 1e-3, x0=None):
@@ -317,13 +330,15 @@ class Operator:
         return r, x, res.value, c
 
     def pist_conv(self, x, e0, etol, m, m_max, start, step, stepeig=1, ntype=0, qmr_type=-1, qmr_max=1000,
-                  qmr_tol=1e-3):
+                  qmr_tol=1e-3, threaded=False):
+        """PIST_CONV; `threaded` runs its H*X and P*X on the host threads
+        (same results), as the reference's MPI path spreads them over ranks."""
         L = lib()
         self.set_e0(e0)
         x = f64(x)
         eig = np.zeros(m)
         res = C.c_double(0.0)
-        c = self._ctx(qmr_type, qmr_max, qmr_tol)
+        c = self._ctx(qmr_type, qmr_max, qmr_tol, threaded)
         iters = C.c_int(0)
         r = L.orc_pist_conv(C.c_double(e0), C.c_double(etol), C.c_int(ntype), C.c_long(self.N), dptr(x),
                             C.c_int(start), C.c_int(step), C.c_int(stepeig), C.c_int(m), C.c_int(m_max),
@@ -443,3 +458,49 @@ class RefOperator:
         x = f64(x)
         return R.lanh0_(_ci(len(x)), dptr(x), dptr(vj), _ci(m_low), _ci(m_high), C.cast(R.refdrv_hx0, FOP),
                         dptr(alpha), dptr(beta))
+
+
+# --------------------------------------------------------------------------
+# sequential truncation (the checker for scalit_podvr / scalit_combine_2d)
+# --------------------------------------------------------------------------
+def diag_signs(z: np.ndarray) -> np.ndarray:
+    """DIAG's sign convention (src/libutil/serial/base/mm.f90:66-84): in the
+    first row with no zero entry, every eigenvector column is made >= 0."""
+    z = np.array(z, dtype=np.float64, order="F")
+    for i in range(z.shape[0]):
+        if z[i, 0] != 0.0 and np.all(z[i, :] != 0.0):
+            z[:, z[i, :] < 0] *= -1.0
+            break
+    return z
+
+
+def podvr(x, t, v, m):
+    """DVR_Step2 + DVR_Step3 (src/ham/libham/dvr/dvr.f90:84-160) with numpy's
+    LAPACK eigensolver: h = T + diag(v), U its m lowest eigenvectors,
+    X(I,J) = sum_k U(k,I) x(k) U(k,J) = W diag(x0) W^T, H0 = W^T diag(E) W.
+    Returns x0, H0, E (the m lowest eigenvalues of h)."""
+    x = f64(x)
+    h = np.array(t, dtype=np.float64) + np.diag(f64(v))
+    e, u = np.linalg.eigh(0.5 * (h + h.T))
+    um = u[:, :m]
+    xm = um.T @ (x[:, None] * um)
+    x0, w = np.linalg.eigh(0.5 * (xm + xm.T))
+    w = diag_signs(w)
+    h0 = w.T @ (e[:m, None] * w)
+    return x0, 0.5 * (h0 + h0.T), e[:m]
+
+
+def combine_2d(h1, v1, h2, v2, nkeep):
+    """saveH0CBALL (src/ham/ja3/module/ja3.h0:154-199) with the cut placed so
+    that exactly nkeep points stay: the product points in the reference's
+    loop order (level 2 outer), the nkeep lowest v1[i] + v2[j] kept (stable in
+    that order), H0c(a,b) = [j_a == j_b] h1(i_a,i_b) + [i_a == i_b] h2(j_a,j_b).
+    Returns idx1, idx2 (0-based) and H0c."""
+    v1, v2 = f64(v1), f64(v2)
+    n1, n2 = len(v1), len(v2)
+    e = (v1[None, :] + v2[:, None]).reshape(-1)       # loop order: a = i + n1 j
+    keep = np.sort(np.argsort(e, kind="stable")[:nkeep])
+    i, j = keep % n1, keep // n1
+    h1, h2 = np.asarray(h1), np.asarray(h2)
+    hc = (j[:, None] == j[None, :]) * h1[i[:, None], i[None, :]] + (i[:, None] == i[None, :]) * h2[j[:, None], j[None, :]]
+    return i.astype(np.int32), j.astype(np.int32), np.asfortranarray(hc)

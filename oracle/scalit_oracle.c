@@ -427,7 +427,8 @@ void orc_cb_hx_e(void *ctx, long n, const double *x, double *y)
     orc_solver_ctx *c = (orc_solver_ctx *)ctx;
     (void)n;
     c->hx_calls++;
-    orc_myhx(c->o, ORC_TA1, x, y);
+    if (c->threaded) orc_myhx_omp(c->o, ORC_TA1, x, y, c->tmp);
+    else orc_myhx(c->o, ORC_TA1, x, y);
 }
 
 void orc_cb_hx0(void *ctx, long n, const double *x, double *y)
@@ -435,7 +436,8 @@ void orc_cb_hx0(void *ctx, long n, const double *x, double *y)
     orc_solver_ctx *c = (orc_solver_ctx *)ctx;
     (void)n;
     c->hx_calls++;
-    orc_myhx(c->o, ORC_TA0, x, y);
+    if (c->threaded) orc_myhx_omp(c->o, ORC_TA0, x, y, c->tmp);
+    else orc_myhx(c->o, ORC_TA0, x, y);
 }
 
 void orc_cb_px(void *ctx, long n, const double *x, double *y)
@@ -443,7 +445,8 @@ void orc_cb_px(void *ctx, long n, const double *x, double *y)
     orc_solver_ctx *c = (orc_solver_ctx *)ctx;
     (void)n;
     c->px_calls++;
-    orc_mypx(c->o, c->o->osb_kind, ORC_TA1, x, y);
+    if (c->threaded) orc_mypx_omp(c->o, c->o->osb_kind, ORC_TA1, x, y);
+    else orc_mypx(c->o, c->o->osb_kind, ORC_TA1, x, y);
 }
 
 int orc_cb_linsolv(void *ctx, long n, const double *b, double *x, double *res)
@@ -1032,4 +1035,62 @@ void orc_myhx_omp(const orc_osb *o, int ntype, const double *x, double *y, doubl
 #pragma omp parallel for schedule(static)
         for (long k = 0; k < n; k++) y[k] = y[k] + tmp[k];
     }
+}
+
+/* ---- threaded P*X: MYPX's per-block rotations with the blocks (or, for
+ *      the few large blocks of outer levels, chunks of 64 inner points)
+ *      split over the host threads, as MPI ranks split them; every value is
+ *      summed in orc_h1x / orc_h1tx order, so the result is orc_mypx's ---- */
+static void vx_omp(int nin, int nout, int blk, const double *v, double *x, int trans)
+{
+    const long nch = (nin + 63) / 64, bs = (long)nin * nout;
+#pragma omp parallel
+    {
+        double *t = (double *)malloc(sizeof(double) * 64 * nout);
+#pragma omp for schedule(static)
+        for (long w = 0; w < (long)blk * nch; w++) {
+            const long b = w / nch;
+            const int p0 = (int)(w % nch) * 64, p1 = p0 + 64 < nin ? p0 + 64 : nin, np = p1 - p0;
+            const double *vb = v + (long)nout * nout * b;
+            double *xb = x + bs * b;
+            for (int i = 0; i < nout; i++) {
+                double *to = t + (long)64 * i;
+                double h = trans ? vb[IDX2(0, i, nout)] : vb[IDX2(i, 0, nout)];
+                for (int k = 0; k < np; k++) to[k] = h * xb[p0 + k];
+                for (int j = 1; j < nout; j++) {
+                    h = trans ? vb[IDX2(j, i, nout)] : vb[IDX2(i, j, nout)];
+                    const double *xj = xb + (long)nin * j + p0;
+                    for (int k = 0; k < np; k++) to[k] = to[k] + h * xj[k];
+                }
+            }
+            for (int i = 0; i < nout; i++)
+                for (int k = 0; k < np; k++) xb[(long)nin * i + p0 + k] = t[(long)64 * i + k];
+        }
+        free(t);
+    }
+}
+
+void orc_mypx_omp(const orc_osb *o, int nosb, int npc, const double *x, double *y)
+{
+    long n = o->myLen;
+    double de = fabs(o->DE), rde = 1.0 / de, e0 = (npc == ORC_TA0) ? 0.0 : o->E0;
+    memcpy(y, x, sizeof(double) * n);
+    for (int l = o->sF - 1; l >= 0; l--) vx_omp(o->myDim[l], o->sN[l], o->myBlk[l], o->VOSB[l], y, 1);
+    const double beta = 1.0 + o->beta, alpha = -o->beta * de;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        const double ade = o->EIG0[i] - e0;
+        if (nosb == ORC_TOSBD1) {
+            if (fabs(ade) <= de) y[i] = (ade < 0.0) ? -y[i] * rde : y[i] * rde;
+            else y[i] = y[i] / ade;
+        } else if (nosb == ORC_TOSBD2) {
+            double t;
+            if (fabs(ade) <= de) t = (ade < 0.0) ? (-alpha + beta * ade) : (alpha + beta * ade);
+            else t = ade;
+            y[i] = y[i] / t;
+        } else {
+            y[i] = y[i] / ade;
+        }
+    }
+    for (int l = 0; l < o->sF; l++) vx_omp(o->myDim[l], o->sN[l], o->myBlk[l], o->VOSB[l], y, 0);
 }

@@ -96,6 +96,11 @@ typedef struct {
     int qmr_type, qmr_max;
     double qmr_tol;
     long hx_calls, px_calls, solves, qmr_iters;
+    /* nonzero: H*X and P*X run on the host threads (orc_myhx_omp /
+     * orc_mypx_omp, results identical to the serial ones) with tmp as their
+     * N-value work vector */
+    int threaded;
+    double *tmp;
 } orc_solver_ctx;
 void orc_cb_hx_e(void *ctx, long n, const double *x, double *y);   /* (H - E0) x */
 void orc_cb_hx0(void *ctx, long n, const double *x, double *y);    /* H x */
@@ -135,6 +140,7 @@ void orc_legendre_l2(int n, double *x, double *l2);   /* j(j+1) in the Gauss-Leg
 
 /* ---- OpenMP-threaded H*X for the CPU baseline (same loop nests, blocks split over threads) ---- */
 void orc_myhx_omp(const orc_osb *o, int ntype, const double *x, double *y, double *tmp);
+void orc_mypx_omp(const orc_osb *o, int nosb, int npc, const double *x, double *y);
 int  orc_omp_threads(void);
 
 #ifdef __cplusplus

@@ -67,6 +67,8 @@ SIGNATURES = {
     "scalit_lan_conv": (_i, [_vp, _d, _d, _i, _vp, _i, _i, _i, _i, _i, _dp, _dp]),
     "scalit_eig_gradients": (_i, [_vp, _i, _vp, _l, _i, _vp, _l, _dp]),
     "scalit_eig_gradients_noalloc": (_i, [_vp, _i, _vp, _l, _i, _vp, _l, _dp]),
+    "scalit_podvr": (_i, [_i, _dp, _dp, _dp, _i, _dp, _dp, _dp]),
+    "scalit_combine_2d": (_i, [_i, _dp, _dp, _i, _dp, _dp, _i, _ip, _ip, _dp]),
     "scalit_bind": (_i, [_vp]),
     "scalit_set_qmr_params": (None, [_i, _i, _d]),
     "hx_": (None, [_ip, _dp, _dp]),

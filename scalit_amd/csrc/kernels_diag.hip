@@ -518,6 +518,31 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const d
     return rc;
 }
 
+// Level 1 with blocks too large for the one-wave LDS Jacobi (n1 > 143 does
+// not fit 160 KB: the combined (r, R) layer of an energy-cut basis): every
+// block's HOSB = H0 + diag(EIG0) (hosbdiag.f90 with NDK = 1) is
+// diagonalised by the batched cyclic Jacobi eigensolver, one workgroup per
+// block, to an off-diagonal norm of 1e-15 of the whole.  Its pair order is
+// not MJACOBI's, so VOSB and EIG0 equal the reference's up to the order and
+// signs of the eigenpairs: the same preconditioner.
+static int diag_level_dense(scalit_ctx *c, int l, hipStream_t s)
+{
+    Level &L = c->lv[l];
+    const int n = L.n, nb = L.blk;
+    if (n > 1024) return SCALIT_EINVAL;
+    DBuf<double> hosb, work;
+    DBuf<EigvecJob> djobs;
+    SC_TRY(hosb.alloc((size_t)nb * n * n));
+    hipLaunchKernelGGL(k_hosb_init, dim3(4096), dim3(256), 0, s, 1, n, nb, L.h0.p, (const double *)nullptr,
+                       c->eig0.p, hosb.p);
+    std::vector<EigvecJob> jobs(nb);
+    for (int b = 0; b < nb; b++)
+        jobs[b] = {hosb.p + (long)b * n * n, c->eig0.p + (long)b * n, L.vosb.p + (long)b * n * n, n, n, n};
+    SC_TRY(small_sym_eig_batch(nb, jobs.data(), djobs, work, s));
+    HIP_TRY(hipStreamSynchronize(s));   // the host job array and the scoped buffers end here
+    return SCALIT_OK;
+}
+
 int block_diag(scalit_ctx *c, int bjmax, double bjtol)
 {
     hipStream_t s = c->stream;
@@ -531,9 +556,10 @@ int block_diag(scalit_ctx *c, int bjmax, double bjtol)
     HIP_TRY(hipMemcpyAsync(c->eig0.p, c->res.p, sizeof(double) * c->N, hipMemcpyDeviceToDevice, s));
     for (int l = c->sF - 1; l >= 0; l--) {
         Level &L = c->lv[l];
-        if (L.dim == 1) {
+        if (L.dim == 1 && sizeof(double) * (size_t)L.n * L.n > 160 * 1024) {
+            SC_TRY(diag_level_dense(c, l, s));
+        } else if (L.dim == 1) {
             size_t lds = sizeof(double) * (size_t)L.n * L.n;
-            if (lds > 160 * 1024) return SCALIT_EINVAL;
             allow_full_lds((const void *)k_osb_jacobi_ndk1);
             hipLaunchKernelGGL(k_osb_jacobi_ndk1, dim3(L.blk), dim3(64), lds, s, L.n, L.blk, L.h0.p, c->eig0.p,
                                L.vosb.p, bjmax, bjtol, (int *)nullptr);

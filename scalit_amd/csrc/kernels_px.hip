@@ -18,6 +18,7 @@
 // of a tile (batches of 8 and more), or one wave per vector on the VALU.
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "outer_mode.hpp"
@@ -378,6 +379,94 @@ static void level1(const Level &L, const T *vosb, const T *eig0, const T *shifts
                                           active, s);
 }
 
+// Level-1 rotations too large for the fused kernel's LDS (n1 > 256: the
+// combined (r, R) layer of an energy-cut basis holds hundreds of points):
+//   out_b(i, v) = sum_j A_b(i, j) in_b(j, v)
+// for every block b of n x n column-major A (V for the V rotation, the
+// block's transpose for V^T), optionally followed by the PX_KERNEL divide
+// (DIV).  One thread per output row i, so every load of A is a coalesced run
+// down a column; in_b of up to BR_VB vectors is staged in LDS and read by
+// broadcast.  Each A value is read once per BR_VB vectors.
+constexpr int BR_VB = 16;
+template <typename T, typename SI, typename SO, bool DIV>
+__global__ void __launch_bounds__(256) k_blockrot(int n, int nblk, int nvec, const T *__restrict__ A,
+                                                  const T *__restrict__ eig0, const T *__restrict__ shifts, T shift0,
+                                                  int nosb, T de, T beta, const SI *__restrict__ x, long ldx,
+                                                  SO *__restrict__ y, long ldy, const int *__restrict__ active)
+{
+    extern __shared__ char smem_raw[];
+    T *sx = (T *)smem_raw;   // in_b(j, v0 + vv) at j + n vv
+    const int b = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+    const long base = (long)b * n;
+    const T *Ab = A + (long)b * n * n;
+    const int ic = min(i, n - 1);   // rows past n compute a valid row and store nothing
+    for (int v0 = 0; v0 < nvec; v0 += BR_VB) {
+        const int nv = min(BR_VB, nvec - v0);
+        __syncthreads();   // the previous pass is done with sx
+        for (int e = threadIdx.x; e < n * nv; e += 256) {
+            const int j = e % n, vv = e / n, v = v0 + vv;
+            sx[e] = vec_active(active, v) ? ld_cvt<T, SI>(x + (long)v * ldx, base + j) : (T)0;
+        }
+        __syncthreads();
+        T acc[BR_VB];
+#pragma unroll
+        for (int vv = 0; vv < BR_VB; vv++) acc[vv] = (T)0;
+        int j = 0;
+        for (; j + 8 <= n; j += 8) {
+            T a[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) a[u] = Ab[ic + (long)n * (j + u)];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+#pragma unroll
+                for (int vv = 0; vv < BR_VB; vv++)
+                    if (vv < nv) acc[vv] += a[u] * sx[(j + u) + n * vv];
+        }
+        for (; j < n; j++) {
+            const T a = Ab[ic + (long)n * j];
+#pragma unroll
+            for (int vv = 0; vv < BR_VB; vv++)
+                if (vv < nv) acc[vv] += a * sx[j + n * vv];
+        }
+        if (i < n) {
+#pragma unroll
+            for (int vv = 0; vv < BR_VB; vv++) {
+                const int v = v0 + vv;
+                if (vv < nv && vec_active(active, v)) {
+                    T val = acc[vv];
+                    if constexpr (DIV) val = px_kernel<T>(nosb, val, eig0[base + i], shifts ? shifts[v] : shift0, de, beta);
+                    st_cvt<T, SO>(y + (long)v * ldy, base + i, val);
+                }
+            }
+        }
+    }
+}
+
+// Level 1 in two passes through ztmp: Z = V1^T X divided by the kernel, then
+// Y = V1 Z (vosbt: the per-block transposes of VOSB level 1).
+template <typename T, typename SI, typename SO>
+static void level1_big(const Level &L, const T *vosb, const T *vosbt, const T *eig0, const T *shifts, T shift0,
+                       int nosb, T de, T beta, int nvec, const SI *x, long ldx, SO *y, long ldy, T *ztmp, long N,
+                       const int *active, hipStream_t s)
+{
+    const int n = L.n, nb = L.blk;
+    const dim3 g((n + 255) / 256, nb);
+    const size_t lds = (size_t)n * BR_VB * sizeof(T);
+    allow_full_lds((const void *)k_blockrot<T, SI, T, true>);
+    allow_full_lds((const void *)k_blockrot<T, T, SO, false>);
+    hipLaunchKernelGGL((k_blockrot<T, SI, T, true>), g, dim3(256), lds, s, n, nb, nvec, vosbt, eig0, shifts, shift0,
+                       nosb, de, beta, x, ldx, ztmp, N, active);
+    hipLaunchKernelGGL((k_blockrot<T, T, SO, false>), g, dim3(256), lds, s, n, nb, nvec, vosb, eig0, shifts, shift0,
+                       nosb, de, beta, ztmp, N, y, ldy, active);
+}
+
+// the fused level-1 kernel holds a block's rotation in LDS; larger ones take level1_big
+template <typename T>
+static bool l1_big(int n1)
+{
+    return n1 > 256 || l1_lds<T>(n1, false) > PX_LDS_MAX;
+}
+
 template <typename T, typename S>
 static int px_impl(scalit_ctx *c, int nosb, const T *const *vosb, const T *const *vosbt, const T *eig0, T *work,
                    const T *shifts, T shift0, T de, T beta, int nvec, const S *x, long ldx, S *y, long ldy,
@@ -385,10 +474,22 @@ static int px_impl(scalit_ctx *c, int nosb, const T *const *vosb, const T *const
 {
     const long N = c->N;
     const int sF = c->sF;
-    if (c->lv[0].n > 256 || l1_lds<T>(c->lv[0].n, false) > PX_LDS_MAX) return SCALIT_EINVAL;   // level-1 block too large for LDS
+    const bool big = l1_big<T>(c->lv[0].n);
+    if (big && (long)BR_VB * c->lv[0].n * sizeof(T) > PX_LDS_MAX) return SCALIT_EINVAL;
+    T *ztmp = work + 2L * nvec * N;   // level1_big's intermediate (work holds 3 nvec N then)
+    auto lev1 = [&](const auto *xin, long ldi, auto *yout, long ldo) {
+        using SI = std::remove_const_t<std::remove_pointer_t<decltype(xin)>>;
+        using SO = std::remove_pointer_t<decltype(yout)>;
+        if (big)
+            level1_big<T, SI, SO>(c->lv[0], vosb[0], vosbt[0], eig0, shifts, shift0, nosb, de, beta, nvec, xin, ldi,
+                                  yout, ldo, ztmp, N, active, s);
+        else
+            level1<T, SI, SO>(c->lv[0], vosb[0], eig0, shifts, shift0, nosb, de, beta, nvec, xin, ldi, yout, ldo,
+                              active, s);
+    };
     if (sF == 1) {
         // a single level: input and output are the caller's vectors
-        level1<T, S, S>(c->lv[0], vosb[0], eig0, shifts, shift0, nosb, de, beta, nvec, x, ldx, y, ldy, active, s);
+        lev1(x, ldx, y, ldy);
         HIP_TRY(hipGetLastError());
         return SCALIT_OK;
     }
@@ -402,7 +503,7 @@ static int px_impl(scalit_ctx *c, int nosb, const T *const *vosb, const T *const
         std::swap(cur, dst);
     }
     // level 1: V1^T, the kernel, V1
-    level1<T, T, T>(c->lv[0], vosb[0], eig0, shifts, shift0, nosb, de, beta, nvec, cur, N, dst, N, active, s);
+    lev1((const T *)cur, N, dst, N);
     std::swap(cur, dst);
     // V for levels 2..sF; the last writes the output vectors
     for (int l = 1; l < sF - 1; l++) {
@@ -426,13 +527,14 @@ __global__ void k_transpose_blocks(int n, long total, const T *__restrict__ src,
     }
 }
 
-// The V^T rotations read per-block transposes of VOSB levels >= 2, built
+// The V^T rotations read per-block transposes of VOSB levels >= 2 (and of
+// level 1 when it takes level1_big), built
 // here on first use after the preconditioner changed (kernel buffers of a
 // new generation: recorded graphs are re-recorded).
 template <typename T>
 static int ensure_transposes(scalit_ctx *c, DBuf<T> Level::*vb, DBuf<T> Level::*vt, hipStream_t s)
 {
-    for (int l = 1; l < c->sF; l++) {
+    for (int l = l1_big<T>(c->lv[0].n) ? 0 : 1; l < c->sF; l++) {
         Level &L = c->lv[l];
         if ((L.*vt).p) continue;
         const long total = (long)L.blk * L.n * L.n;
@@ -451,7 +553,7 @@ int px_apply_f64(scalit_ctx *c, int nosb, const double *shifts_dev, double shift
     if (!c->has_precond) return SCALIT_ESTATE;
     {
         double *old = c->work.p;
-        SC_TRY(c->work.alloc((size_t)2 * nvec * c->N));
+        SC_TRY(c->work.alloc((size_t)(l1_big<double>(c->lv[0].n) ? 3 : 2) * nvec * c->N));
         if (c->work.p != old) c->gen++;
     }
     SC_TRY(ensure_transposes<double>(c, &Level::vosb, &Level::vosbt, s));
@@ -471,7 +573,7 @@ int px_apply_f32(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec,
     SC_TRY(precond_to_f32(c));
     {
         float *old = c->work_f.p;
-        SC_TRY(c->work_f.alloc((size_t)2 * nvec * c->N));
+        SC_TRY(c->work_f.alloc((size_t)(l1_big<float>(c->lv[0].n) ? 3 : 2) * nvec * c->N));
         if (c->work_f.p != old) c->gen++;
     }
     SC_TRY(ensure_transposes<float>(c, &Level::vosb_f, &Level::vosbt_f, s));
@@ -491,7 +593,7 @@ int px_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec
     SC_TRY(precond_to_f32(c));
     {
         float *old = c->work_f.p;
-        SC_TRY(c->work_f.alloc((size_t)2 * nvec * c->N));
+        SC_TRY(c->work_f.alloc((size_t)(l1_big<float>(c->lv[0].n) ? 3 : 2) * nvec * c->N));
         if (c->work_f.p != old) c->gen++;
     }
     SC_TRY(ensure_transposes<float>(c, &Level::vosb_f, &Level::vosbt_f, s));

@@ -377,6 +377,7 @@ struct PistState {
     std::vector<int> last_m;
     std::vector<std::vector<int>> last_ind;   // window index of each returned eigenvalue
     std::vector<int> last_cnt;                // Krylov size of each window's last eigensolve
+    std::vector<int> last_ret;                // each window's PIST_CONV return code
     long ldv = 0;
     // QMR parameters of the OSB_QMR callback
     int qmr_type = -1, qmr_max = 1000;
@@ -773,6 +774,7 @@ int pist_conv(scalit_ctx *c, int nw, const double *e0, const PistParams &pp, con
     std::vector<int> ind(m);
     P.last_ind.assign(nw, std::vector<int>(m, 0));
     P.last_cnt.assign(nw, 0);
+    P.last_ret.assign(nw, 0);
     P.nwin = nw;
     P.m_max = m_max;
     P.m_last = m;
@@ -866,6 +868,7 @@ int pist_conv(scalit_ctx *c, int nw, const double *e0, const PistParams &pp, con
             P.last_ind[k][i] = pr[i].second;
         }
         if (res_out) res_out[k] = res[k];
+        P.last_ret[k] = ret_out[k];
     }
     if (stats_out) {
         stats_out[0] = stats[0];
@@ -882,7 +885,7 @@ int pist_vectors(scalit_ctx *c, int k, double *psi, long ldp, hipStream_t s)
     if (!c->pist || k < 0 || k >= c->pist->nwin) return SCALIT_EINVAL;
     PistState &P = *c->pist;
     const int cnt = P.last_cnt[k], m = P.m_last, m_max = P.m_max;
-    if (cnt <= 0) return SCALIT_ESTATE;
+    if (cnt <= 0 || P.last_ret[k] <= 0) return SCALIT_ESTATE;   // unconverged: no Ritz vectors
     SC_TRY(P.psi_z.alloc((size_t)cnt * cnt));
     SC_TRY(P.eigw.alloc((size_t)m_max));
     SC_TRY(P.zw.alloc(small_sym_eig_work(m_max)));
@@ -905,7 +908,8 @@ int pist_vectors_all(scalit_ctx *c, double *psi, long ldp, int ncols, hipStream_
     if ((long)nw * m > ncols) return SCALIT_EINVAL;   // psi must hold every window's m vectors
     int cmax = 0;
     for (int k = 0; k < nw; k++) {
-        if (P.last_cnt[k] <= 0) return SCALIT_ESTATE;
+        // a window that stopped unconverged has no trustworthy Ritz vectors
+        if (P.last_cnt[k] <= 0 || P.last_ret[k] <= 0) return SCALIT_ESTATE;
         cmax = std::max(cmax, P.last_cnt[k]);
     }
     SC_TRY(P.psi_z.alloc((size_t)nw * cmax * cmax));

@@ -261,6 +261,112 @@ def config_problem(name: str, seed: int = 1234, grid=None) -> Problem:
     return build_problem(n1, n2, n3, pot, name=name, seed=seed)
 
 
+# ----------------------------------------------------------------------------
+# sequential truncation
+# ----------------------------------------------------------------------------
+@dataclasses.dataclass
+class TruncatedProblem:
+    """The triatomic in a truncated basis: PO-DVR contraction of r and R
+    (m1 and m2 points), then the energy-cut combined (r, R) layer of nkeep
+    points as level 1 and theta as level 2 with DEP on it (sF = 2,
+    sDEP = (F, T)): ScalIT's iterate operator for a ja3-type basis."""
+    name: str
+    sN: tuple                  # (nkeep, n3)
+    h0: list                   # [H0 of the combined layer, L^2]
+    dep: list                  # [None, DEP at the kept points]
+    res: np.ndarray            # V - v_r - v_R on (kept point, theta), point fastest
+    r: np.ndarray              # r and R of each kept point
+    R: np.ndarray
+    cos_theta: np.ndarray
+    potential: Potential
+    seed: int
+    raw: tuple                 # the primitive grid (n1, n2, n3)
+    contracted: tuple          # (m1, m2)
+
+    @property
+    def N(self) -> int:
+        return int(np.prod(self.sN))
+
+    def with_potential(self, pot: Potential) -> "TruncatedProblem":
+        return dataclasses.replace(self, potential=pot, res=_residual(pot, self.r, self.R, self.cos_theta))
+
+
+def morse_1d(q: np.ndarray, De: float, a: float, re: float) -> np.ndarray:
+    return De * (1.0 - np.exp(-a * (q - re))) ** 2
+
+
+def _residual(pot: Potential, r: np.ndarray, R: np.ndarray, c: np.ndarray) -> np.ndarray:
+    """V(r_a, R_a, theta_k) - v_r(r_a) - v_R(R_a), kept point a fastest."""
+    v = np.empty((len(c), len(r)))
+    for k, ck in enumerate(c):
+        v[k] = _point_potential(pot, r, R, ck)
+    v -= (morse_1d(r, *pot.morse_r) + morse_1d(R, *pot.morse_R))[None, :]
+    return np.ascontiguousarray(v).reshape(-1)
+
+
+def _point_potential(pot: Potential, r: np.ndarray, R: np.ndarray, c: float) -> np.ndarray:
+    v = morse_1d(r, *pot.morse_r) + morse_1d(R, *pot.morse_R) + pot.p2 * 0.5 * (3.0 * c * c - 1.0)
+    for amp, r0, R0, wd in pot.gauss:
+        v = v + amp * np.exp(-((r - r0) ** 2 + (R - R0) ** 2) / (2.0 * wd * wd))
+    return v
+
+
+def truncated_problem(n1: int, n2: int, n3: int, pot: Potential, m1: int, m2: int, nkeep: int, podvr, combine_2d,
+                      name: str = "truncated", seed: int = 0) -> TruncatedProblem:
+    """Sequential truncation of the n1 x n2 x n3 triatomic: the PO-DVR of r
+    and R against the 1-D Morse reference potentials v_r, v_R of `pot`
+    (DVR_Step2/3), then the nkeep (r, R) product points of lowest
+    v_r + v_R as one combined layer (saveH0CBALL).  `podvr(x, t, v, m)` and
+    `combine_2d(h1, v1, h2, v2, nkeep)` do the two steps (the library's
+    scalit_podvr / scalit_combine_2d, see `library_truncation`)."""
+    r, tr = sinc2_dvr(n1, *R_RANGE, MU)
+    R, tR = sinc2_dvr(n2, *BIGR_RANGE, MU)
+    c, l2 = legendre_dvr_l2(n3)
+    xr, hr, _ = podvr(r, tr, morse_1d(r, *pot.morse_r), m1)
+    xR, hR, _ = podvr(R, tR, morse_1d(R, *pot.morse_R), m2)
+    i1, i2, hc = combine_2d(hr, morse_1d(xr, *pot.morse_r), hR, morse_1d(xR, *pot.morse_R), nkeep)
+    rk, Rk = xr[i1], xR[i2]
+    dep = 1.0 / (2.0 * MU * rk ** 2) + 1.0 / (2.0 * MU * Rk ** 2)
+    res = _residual(pot, rk, Rk, c)
+    return TruncatedProblem(name, (nkeep, n3), [np.asfortranarray(hc), l2], [None, dep], res, rk, Rk, c, pot, seed,
+                            (n1, n2, n3), (m1, m2))
+
+
+def library_truncation():
+    """(podvr, combine_2d) through libscalit_hip.so (the eigensolves run on
+    the current GPU)."""
+    import ctypes as C
+
+    from . import _lib as L
+
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+
+    def podvr(x, t, v, m):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        t = np.asfortranarray(t, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        x0, h0, e = np.zeros(m), np.zeros((m, m), order="F"), np.zeros(m)
+        L.check(L.lib().scalit_podvr(len(x), x.ctypes.data_as(dp), t.ctypes.data_as(dp), v.ctypes.data_as(dp), m,
+                                     x0.ctypes.data_as(dp), h0.ctypes.data_as(dp), e.ctypes.data_as(dp)),
+                "scalit_podvr")
+        return x0, h0, e
+
+    def combine_2d(h1, v1, h2, v2, nkeep):
+        h1 = np.asfortranarray(h1, dtype=np.float64)
+        h2 = np.asfortranarray(h2, dtype=np.float64)
+        v1 = np.ascontiguousarray(v1, dtype=np.float64)
+        v2 = np.ascontiguousarray(v2, dtype=np.float64)
+        i1, i2 = np.zeros(nkeep, dtype=np.int32), np.zeros(nkeep, dtype=np.int32)
+        hc = np.zeros((nkeep, nkeep), order="F")
+        L.check(L.lib().scalit_combine_2d(len(v1), h1.ctypes.data_as(dp), v1.ctypes.data_as(dp), len(v2),
+                                          h2.ctypes.data_as(dp), v2.ctypes.data_as(dp), nkeep, i1.ctypes.data_as(ip),
+                                          i2.ctypes.data_as(ip), hc.ctypes.data_as(dp)), "scalit_combine_2d")
+        return i1, i2, hc
+
+    return podvr, combine_2d
+
+
 def outer_layer(prob: Problem, seed: int, eps: float = 2e-4, potential: bool = False) -> np.ndarray:
     """A dense per-point outermost layer (the sNDVR = T form ScalIT's ja3 writes,
     OUTH(NIN, n3, n3) of src/iterate/module/osb.data.h:62): for every inner grid

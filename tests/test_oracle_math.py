@@ -242,3 +242,86 @@ def test_hx_omp_equals_serial(grid):
         tmp = np.zeros(op.N)
         op.hx_omp(x, y, tmp, ntype)
         assert np.array_equal(y, op.hx(x, ntype))
+
+
+def test_px_omp_and_threaded_pist_equal_serial():
+    """orc_mypx_omp (the threaded CPU baseline's P*X) equals orc_mypx bit for
+    bit for every kernel, and PIST_CONV with the threaded operators returns
+    the serial run's code, iteration count and eigenvalues."""
+    prob = P.build_problem(10, 12, 8, P.gaussian_potential(5), seed=5)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=-0.01)
+    _, eig0 = op.diag(1000, 1e-5)
+    srt = np.sort(eig0)
+    op.set_shift(float(srt[30]), float(srt[50] - srt[30]), 0.3)
+    x = P.random_vectors(op.N, 1, 4)[0]
+    for kind in (O.TOSB, O.TOSBD1, O.TOSBD2):
+        for npc in (O.TA0, O.TA1):
+            assert np.array_equal(op.px_omp(x, npc, kind), op.px(x, npc, kind))
+    op.set_shift(-0.01)
+    e0 = float(prob.res.min())
+    r1, e1, _, c1 = op.pist_conv(x, e0, 1e-10, 4, 60, 12, 6, 2, qmr_tol=1e-8)
+    r2, e2, _, c2 = op.pist_conv(x, e0, 1e-10, 4, 60, 12, 6, 2, qmr_tol=1e-8, threaded=True)
+    assert r1 == r2 and r1 > 0 and c1.qmr_iters == c2.qmr_iters
+    assert np.array_equal(e1, e2)
+
+
+@pytest.mark.parametrize("n,m", [(24, 8), (64, 20), (40, 40)])
+def test_podvr_restatement(n, m):
+    """The oracle's DVR_Step2/3: with Phi = U_m W the PO-DVR basis functions
+    on the primitive grid (built here from a full eigh of h = T + diag(v)),
+    H0 = Phi^T h Phi and Phi^T diag(x) Phi = diag(x0), its eigenvalues are the
+    m lowest of h, and W follows DIAG's sign rule."""
+    x, t = P.sinc2_dvr(n, *P.R_RANGE, P.MU)
+    v = P.morse_1d(x, *P.MORSE_R)
+    x0, h0, e = O.podvr(x, t, v, m)
+    h = t + np.diag(v)
+    ev, u = np.linalg.eigh(h)
+    um = u[:, :m]
+    xm = um.T @ (x[:, None] * um)
+    x0n, wn = np.linalg.eigh(xm)
+    wn = O.diag_signs(wn)
+    phi = um @ wn
+    scale = np.abs(h0).max()
+    assert np.allclose(h0, phi.T @ h @ phi, rtol=0, atol=1e-12 * scale)
+    assert np.allclose(phi.T @ (x[:, None] * phi), np.diag(x0), rtol=0, atol=1e-12 * np.abs(x).max())
+    assert np.allclose(np.linalg.eigvalsh(h0), ev[:m], rtol=0, atol=1e-12 * scale)
+    assert np.allclose(e, ev[:m], rtol=0, atol=1e-13 * np.abs(ev[:m]).max())
+    # (m = n: the PO-DVR points are the primitive grid again, to rounding)
+    assert np.all(np.diff(x0) > 0) and x.min() - 1e-12 <= x0[0] and x0[-1] <= x.max() + 1e-12
+
+
+def test_combine_2d_restatement():
+    """The oracle's saveH0CBALL: keeping every point gives the Kronecker sum
+    H1 (+) H2 in the reference's order (a = i + n1 j); keeping fewer gives its
+    principal submatrix on the points of lowest v1 + v2, in loop order."""
+    g = np.random.default_rng(3)
+    n1, n2 = 7, 5
+    h1 = (lambda a: a + a.T)(g.standard_normal((n1, n1)))
+    h2 = (lambda a: a + a.T)(g.standard_normal((n2, n2)))
+    v1, v2 = g.uniform(0, 1, n1), g.uniform(0, 1, n2)
+    ks = np.kron(np.eye(n2), h1) + np.kron(h2, np.eye(n1))
+    i1, i2, hc = O.combine_2d(h1, v1, h2, v2, n1 * n2)
+    assert np.array_equal(i1 + n1 * i2, np.arange(n1 * n2)) and np.allclose(hc, ks, rtol=0, atol=1e-15)
+    nk = 17
+    i1, i2, hc = O.combine_2d(h1, v1, h2, v2, nk)
+    a = i1 + n1 * i2
+    e = (v1[None, :] + v2[:, None]).reshape(-1)
+    assert np.all(np.diff(a) > 0) and set(a) == set(np.argsort(e)[:nk])
+    assert np.allclose(hc, ks[np.ix_(a, a)], rtol=0, atol=1e-15)
+
+
+def test_truncated_operator_spectrum():
+    """The truncated operator (PO-DVR 16 x 16 of a 40 x 40 x 8 grid, 160
+    points kept) reproduces the lowest levels of the full direct-product
+    operator variationally: from above, within 2e-4 relative."""
+    import scipy.sparse.linalg as sla
+
+    pot = P.gaussian_potential(3)
+    tp = P.truncated_problem(40, 40, 8, pot, 16, 16, 160, O.podvr, O.combine_2d)
+    op = O.Operator(tp.sN, tp.h0, tp.res, tp.dep)
+    ev = np.linalg.eigvalsh(op.dense())[:8]
+    full = P.build_problem(40, 40, 8, pot)
+    opf = O.Operator(full.sN, full.h0, full.res, full.dep)
+    a = sla.LinearOperator((opf.N, opf.N), matvec=lambda v: opf.hx(v), dtype=np.float64)
+    evf = np.sort(sla.eigsh(a, k=10, which="SA", tol=1e-12)[0])[:8]
+    assert np.all(ev >= evf - 1e-12) and np.all(np.abs(ev - evf) <= 2e-4 * np.abs(evf))
