@@ -293,6 +293,7 @@ def gradients_bench(ctx, prob):
     ctx.block_diag(1000, 1e-5)
     e0 = float(prob.res.min())
     ctx.set_shift(e0)
+    ctx.set_pist_orth(2)   # a Krylov space of ~300 vectors: one MGS pass loses orthogonality
     x0 = ctx.to_device(P.random_vectors(prob.N, 1, 91))
     dvdp = ctx.to_device(prob.potential.gradient(prob.r, prob.R, prob.cos_theta))
     out = {}
@@ -311,6 +312,11 @@ def gradients_bench(ctx, prob):
         if with_grad:
             out[key]["parameters"] = int(grad.shape[1])
             out[key]["gradients_finite"] = bool(np.isfinite(grad).all())
+    psi = ctx.pist_vectors(0, 50)
+    rr = ctx.hx(psi) - torch.as_tensor(eig, device=psi.device)[:, None] * psi
+    out["max_ritz_residual"] = float((rr * rr).sum(dim=1).sqrt().max())
+    out["orthogonalisation_passes"] = 2
+    ctx.set_pist_orth(1)
     return out
 
 

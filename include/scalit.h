@@ -153,6 +153,14 @@ int scalit_pist_conv(scalit_ctx *ctx, double e0, double etol, int ntype, const d
 int scalit_pist_conv_batched(scalit_ctx *ctx, int nwin, const double *e0, double etol, int ntype, const double *x,
                              long ld, int startcnt, int stepcnt, int stepeig, int m, int m_max, int qmr_type,
                              int qmrmax, double qmrtol, int prec, double *eig, int *ret, double *res, int *stats);
+/* Orthogonalisation passes per PIST Krylov vector: 1 (the default) is the
+ * reference's MGS_ORTH (src/iterate/libiterate/orth/gsorth.f90:42-72), one
+ * modified Gram-Schmidt pass; 2 orthogonalises each new vector a second time
+ * against the basis.  No counterpart in the reference: a long PIST solve (a
+ * Krylov space of hundreds of vectors, many levels at once) loses
+ * orthogonality with one pass and its Rayleigh-Ritz step then returns
+ * spurious values; two passes keep the basis orthonormal to rounding. */
+int scalit_set_pist_orth(scalit_ctx *ctx, int passes);
 /* Ritz vectors of the last PIST solve of window `win` for its m eigenvalues:
  * psi (device, N x m, leading dimension ld), unit norm.  SCALIT_ESTATE when
  * that window's solve did not converge (its return code was <= 0). */
@@ -290,8 +298,9 @@ int scalit_pist_conv_multi(int njob, scalit_pist_job *jobs, double etol, int nty
  * potential at x.  Keeps the m lowest eigenvectors of T + diag(v) and returns
  * the m new grid points x0 (ascending: the eigenvalues of the position
  * operator in that basis), the m x m Hamiltonian h0 = W^T diag(E) W on them
- * (W with DIAG's sign convention, src/libutil/serial/base/mm.f90:66-84) and,
- * when e != NULL, the m eigenvalues E.  n <= 1024. */
+ * (each column of W with its largest-magnitude entry positive: the reference
+ * leaves these signs to DSYEV, and every choice gives an equivalent
+ * operator) and, when e != NULL, the m eigenvalues E.  n <= 1024. */
 int scalit_podvr(int n, const double *x, const double *t, const double *v, int m, double *x0, double *h0, double *e);
 /* saveH0CBALL (src/ham/ja3/module/ja3.h0:154-199): the energy-cut combined
  * layer of two coordinates.  Keeps the nkeep product points (i, j) with the
@@ -316,6 +325,14 @@ int scalit_load_data(const char *path, long n, double *data, int bin_mode);
  * direct access, one record whose length is the array's: the raw n doubles. */
 int scalit_save_data_dir(const char *path, long n, const double *data);
 int scalit_load_data_dir(const char *path, long n, double *data);
+/* P2SaveDataGrid's per-point direct-access records
+ * (src/libutil/parallel/p_io/p2io_grid.f90:62-83), the OUTH / H0GM files of
+ * ScalIT's ja3 generator: record p holds the ng x ng block of grid point p
+ * (column-major, RECL = ng*ng doubles, no record markers).  outh is the
+ * operator's layout, npts x ng x ng with the point index fastest (as
+ * scalit_set_outh takes it); npts records are read or written. */
+int scalit_load_data_grid(const char *path, long npts, int ng, double *outh);
+int scalit_save_data_grid(const char *path, long npts, int ng, const double *outh);
 /* calOSB / calDataInfo (src/iterate/module/osb.init.h:2-49): 0-based start of
  * each level's piece in the concatenated H0 (sN(l)^2 each, none for the sNDVR
  * layer), DEP (myDim(l) for each dependent level) and VOSB (myBlk(l) sN(l)^2)

@@ -463,14 +463,13 @@ class RefOperator:
 # --------------------------------------------------------------------------
 # sequential truncation (the checker for scalit_podvr / scalit_combine_2d)
 # --------------------------------------------------------------------------
-def diag_signs(z: np.ndarray) -> np.ndarray:
-    """DIAG's sign convention (src/libutil/serial/base/mm.f90:66-84): in the
-    first row with no zero entry, every eigenvector column is made >= 0."""
+def column_signs(z: np.ndarray) -> np.ndarray:
+    """Every eigenvector column with its largest-magnitude entry positive
+    (DVR_Step3 leaves the signs to DSYEV; any choice gives an equivalent
+    PO-DVR operator, and this one does not hang on rounding-level entries)."""
     z = np.array(z, dtype=np.float64, order="F")
-    for i in range(z.shape[0]):
-        if z[i, 0] != 0.0 and np.all(z[i, :] != 0.0):
-            z[:, z[i, :] < 0] *= -1.0
-            break
+    im = np.abs(z).argmax(axis=0)
+    z[:, z[im, np.arange(z.shape[1])] < 0] *= -1.0
     return z
 
 
@@ -485,7 +484,7 @@ def podvr(x, t, v, m):
     um = u[:, :m]
     xm = um.T @ (x[:, None] * um)
     x0, w = np.linalg.eigh(0.5 * (xm + xm.T))
-    w = diag_signs(w)
+    w = column_signs(w)
     h0 = w.T @ (e[:m, None] * w)
     return x0, 0.5 * (h0 + h0.T), e[:m]
 

@@ -91,6 +91,15 @@ class Context:
         return a
 
     @staticmethod
+    def load_grid_file(path: str, npts: int, ng: int) -> np.ndarray:
+        """An OUTH / H0GM file of per-point direct records (P2SaveDataGrid,
+        src/libutil/parallel/p_io/p2io_grid.f90:62-83) in the operator's
+        layout: npts x ng x ng values, point index fastest."""
+        a = np.zeros(int(npts) * ng * ng)
+        check(L.lib().scalit_load_data_grid(path.encode(), int(npts), int(ng), _np_dp(a)), f"load {path}")
+        return a
+
+    @staticmethod
     def save_file(path: str, a: np.ndarray, mode: str = "seq"):
         a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
         if mode == "dir":
@@ -101,12 +110,15 @@ class Context:
 
     @classmethod
     def from_files(cls, sN, sDEP, h0_file: str, res_file: str | None = None, dep_file: str | None = None,
-                   outh_file: str | None = None, bin_mode: bool = True, device: int = 0, e0: float = 0.0):
+                   outh_file: str | None = None, bin_mode: bool = True, device: int = 0, e0: float = 0.0,
+                   outh_per_point: bool = False):
         """An operator from the files an iterate job reads (loadInitData,
         osb.io.h:2-62, 211-235): H0 of every level concatenated, sequential
         (binary when bin_mode, else text); RES, DEP (dependent levels
-        concatenated) and, for the sNDVR = T layer, OUTH as direct-access
-        records.  With OUTH the job has no RES file and RES is zero."""
+        concatenated) and, for the sNDVR = T layer, OUTH as one direct-access
+        record (loadOutH) or, with outh_per_point, one record per inner grid
+        point as the ja3 generator writes it.  With OUTH the job has no RES
+        file and RES is zero."""
         ndvr = outh_file is not None
         lay = cls.layout(sN, sDEP, ndvr)
         h0 = cls.load_file(h0_file, lay["h0_len"], "seq" if bin_mode else "text")
@@ -122,7 +134,10 @@ class Context:
                 st = lay["dep_start"][l]
                 ctx.set_dep(l + 1, dep[st:st + ctx.dim[l]])
         if ndvr:
-            ctx.set_outh(cls.load_file(outh_file, lay["outh_len"], "dir"))
+            if outh_per_point:
+                ctx.set_outh(cls.load_grid_file(outh_file, ctx.dim[-1], ctx.sN[-1]))
+            else:
+                ctx.set_outh(cls.load_file(outh_file, lay["outh_len"], "dir"))
         ctx.set_res(res)
         ctx.set_shift(e0)
         return ctx
@@ -349,6 +364,10 @@ class Context:
                                                stepeig, m, m_max, qmr_type, qmrmax, qmrtol, prec, eig, ret, res,
                                                stats), "scalit_pist_conv_batched")
         return np.array(ret[:]), np.array(eig[:]).reshape(nw, m), np.array(res[:]), np.array(stats[:])
+
+    def set_pist_orth(self, passes: int):
+        """1: the reference's one MGS pass per Krylov vector; 2: twice."""
+        check(L.lib().scalit_set_pist_orth(self._h, passes), "scalit_set_pist_orth")
 
     def pist_vectors(self, win: int, m: int):
         psi = torch.empty((m, self.N), dtype=torch.float64, device=self.dev)

@@ -555,6 +555,13 @@ int scalit_lan_conv(scalit_ctx *c, double e0, double etol, int ntype, const doub
     return lan_conv(c, e0, etol, ntype, x, startcnt, stepcnt, stepeig, m, m_max, eig, res, c->stream);
 }
 
+int scalit_set_pist_orth(scalit_ctx *c, int passes)
+{
+    if (!c || passes < 1 || passes > 2) return SCALIT_EINVAL;
+    c->pist_orth = passes;
+    return SCALIT_OK;
+}
+
 int scalit_eig_gradients(scalit_ctx *c, int nk, const double *psi, long ld, int np, const double *dvdp, long lddv,
                          double *out)
 {

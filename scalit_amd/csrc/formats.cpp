@@ -12,6 +12,12 @@
 //   src/libutil/serial/io/iodirect.f90:10-36, 68-93): RECL is the array's
 //   IOLENGTH, so the file is the N doubles and nothing else.
 //
+// * per-point direct-access records (P2SaveDataGrid,
+//   src/libutil/parallel/p_io/p2io_grid.f90:62-83, the OUTH / H0GM files of
+//   the ja3 generator): record p (1-based) holds the NG x NG block of grid
+//   point p, column-major, RECL = NG*NG doubles.  The operator keeps OUTH
+//   with the point index fastest, so reading and writing transpose.
+//
 // scalit_layout restates calOSB/calDataInfo (src/iterate/module/osb.init.h:2-49):
 // where each level's piece starts in the concatenated H0, DEP and VOSB arrays.
 #include <cerrno>
@@ -130,6 +136,34 @@ int scalit_load_data_dir(const char *path, long n, double *data)
     File f(path, "rb");
     if (!f.f) return SCALIT_EIO;
     return read_all(f.f, data, (size_t)n * sizeof(double)) ? SCALIT_OK : SCALIT_EIO;
+}
+
+int scalit_load_data_grid(const char *path, long npts, int ng, double *outh)
+{
+    if (!path || npts < 1 || ng < 1 || !outh) return SCALIT_EINVAL;
+    File in(path, "rb");
+    if (!in.f) return SCALIT_EIO;
+    const long nn = (long)ng * ng;
+    std::vector<double> rec(nn);
+    for (long p = 0; p < npts; p++) {
+        if (!read_all(in.f, rec.data(), sizeof(double) * nn)) return SCALIT_EIO;
+        for (long e = 0; e < nn; e++) outh[p + npts * e] = rec[e];
+    }
+    return SCALIT_OK;
+}
+
+int scalit_save_data_grid(const char *path, long npts, int ng, const double *outh)
+{
+    if (!path || npts < 1 || ng < 1 || !outh) return SCALIT_EINVAL;
+    File out(path, "wb");
+    if (!out.f) return SCALIT_EIO;
+    const long nn = (long)ng * ng;
+    std::vector<double> rec(nn);
+    for (long p = 0; p < npts; p++) {
+        for (long e = 0; e < nn; e++) rec[e] = outh[p + npts * e];
+        if (!write_all(out.f, rec.data(), sizeof(double) * nn)) return SCALIT_EIO;
+    }
+    return SCALIT_OK;
 }
 
 int scalit_layout(int sF, const int *sN, const int *sDEP, int ndvr, long *h0_start, long *dep_start,

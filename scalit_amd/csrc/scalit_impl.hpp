@@ -124,6 +124,9 @@ struct scalit_ctx {
     // fused 3-level fast path is available (sF==3, sDEP=(F,F,T) or (F,F) + OUTH, sizes fit)
     bool fast3 = false;
     scalit::PistState *pist = nullptr;
+    // orthogonalisation passes per PIST Krylov vector: 1 = the reference's
+    // MGS_ORTH, 2 = once more against the basis (scalit_set_pist_orth)
+    int pist_orth = 1;
     // bumped whenever a device buffer that operator kernels read is
     // (re)allocated: recorded graphs of older generations are re-recorded
     long gen = 0;

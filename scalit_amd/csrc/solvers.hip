@@ -665,16 +665,27 @@ static int pist_linsolv(scalit_ctx *c, PistState &P, const PistParams &pp, int n
 }
 
 // MGS_ORTH for the active windows: VJ_k(:, jnew) from W_k against columns 0..jnew-1.
-// ok[k] = 0 on a zero norm (before or after the projections).
+// ok[k] = 0 on a zero norm (before or after the projections).  With
+// c->pist_orth == 2 the new column is orthogonalised against the previous ones
+// a second time and normalised again: one MGS pass loses orthogonality once
+// the Krylov vectors of a long solve become nearly dependent (the spectral
+// transform has converged towards the lowest levels), and Rayleigh-Ritz on a
+// basis that is no longer orthonormal gives spurious values; "twice is
+// enough" (Parlett) restores it.  One pass is the reference's MGS_ORTH.
 static int pist_mgs(scalit_ctx *c, PistState &P, int nw, int m_max, int jnew, const int *act_dev, std::vector<int> &ok,
                     hipStream_t s)
 {
     const long N = c->N;
     const long ldw = P.ldv * m_max;
     double *col = P.vj.p + (long)jnew * P.ldv;   // window 0's column jnew; window k at + k*ldw
+    const bool twice = c->pist_orth >= 2;
     if (launch_mgs_one(nw, N, jnew, P.vj.p, P.ldv, ldw, P.w.p, P.ldv, col, act_dev, P.ok.p, nullptr, s)) {
-        HIP_TRY(hipMemcpyAsync(ok.data(), P.ok.p, sizeof(int) * nw, hipMemcpyDeviceToHost, s));
+        // (the kernel holds its input in registers before writing: in place is fine)
+        if (twice) launch_mgs_one(nw, N, jnew, P.vj.p, P.ldv, ldw, col, ldw, col, act_dev, P.ok.p + nw, nullptr, s);
+        std::vector<int> both(2 * (size_t)nw, 1);
+        HIP_TRY(hipMemcpyAsync(both.data(), P.ok.p, sizeof(int) * (twice ? 2 : 1) * nw, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        for (int k = 0; k < nw; k++) ok[k] = both[k] && (!twice || both[nw + k]);
         return SCALIT_OK;
     }
     // VJ(:, M0) = WJ ; normw = ||WJ||
@@ -692,10 +703,22 @@ static int pist_mgs(scalit_ctx *c, PistState &P, int nw, int m_max, int jnew, co
         std::swap(pa, pb);
     }
     launch_normalize(nw, N, col, ldw, col, ldw, pa, P.ok.p, act_dev, s);
-    HIP_TRY(hipMemcpyAsync(ok.data(), P.ok.p, sizeof(int) * nw, hipMemcpyDeviceToHost, s));
+    if (twice) {
+        // the second pass on the normalised column, in place
+        launch_mgs_step(nw, N, -1, 0, P.vj.p, P.ldv, ldw, col, ldw, nullptr, P.part.p, act_dev, s);
+        pa = P.part.p;
+        pb = P.part2.p;
+        for (int i = 0; i < jnew; i++) {
+            int inext = (i + 1 < jnew) ? i + 1 : -1;
+            launch_mgs_step(nw, N, i, inext, P.vj.p, P.ldv, ldw, col, ldw, pa, pb, act_dev, s);
+            std::swap(pa, pb);
+        }
+        launch_normalize(nw, N, col, ldw, col, ldw, pa, P.ok.p + nw, act_dev, s);
+    }
+    std::vector<int> both(2 * (size_t)nw, 1);
+    HIP_TRY(hipMemcpyAsync(both.data(), P.ok.p, sizeof(int) * (twice ? 2 : 1) * nw, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    for (int k = 0; k < nw; k++)
-        if (nrm[k] == 0.0) ok[k] = 0;
+    for (int k = 0; k < nw; k++) ok[k] = both[k] && (!twice || both[nw + k]) && nrm[k] != 0.0;
     return SCALIT_OK;
 }
 
@@ -762,7 +785,7 @@ int pist_conv(scalit_ctx *c, int nw, const double *e0, const PistParams &pp, con
     SC_TRY(P.part.alloc((size_t)2 * nw * NPART));
     SC_TRY(P.part2.alloc((size_t)2 * nw * NPART));
     SC_TRY(P.tmp.alloc((size_t)std::max<long>(nw, 64)));
-    SC_TRY(P.ok.alloc(nw));
+    SC_TRY(P.ok.alloc(2 * (size_t)nw));   // the zero-norm flags of both orthogonalisation passes
     SC_TRY(P.active.alloc(nw));
     SC_TRY(P.eigw.alloc((size_t)2 * nw * m_max));
     SC_TRY(P.zw.alloc(small_sym_eig_work(m_max)));

@@ -6,9 +6,13 @@
 //    the 1-D Hamiltonian h = T + diag(v) on the primitive grid x, keep its m
 //    lowest eigenvectors U, diagonalise the position operator in that basis,
 //    X = U^T diag(x) U = W diag(x0) W^T, and return the new grid x0 with the
-//    1-D Hamiltonian H0 = W^T diag(E) W on it.  The eigenvectors W get
-//    DIAG's sign convention (src/libutil/serial/base/mm.f90:48-87): in the
-//    first row with no zero entry, every column is made non-negative.
+//    1-D Hamiltonian H0 = W^T diag(E) W on it.  DVR_Step3 leaves the signs of
+//    W to DSYEV; a sign flip of column a flips row and column a of H0, an
+//    equivalent operator (D H0 D with D = diag(+-1), the same spectrum with
+//    every diagonal potential).  Here each column of W has its
+//    largest-magnitude entry positive, a choice that, unlike DIAG's
+//    first-full-row rule (src/libutil/serial/base/mm.f90:66-84), does not
+//    hang on entries at rounding level.
 //  * scalit_combine_2d: the energy-cut combined layer of two coordinates,
 //    saveH0CBALL (src/ham/ja3/module/ja3.h0:154-199): of the n1 x n2 product
 //    points keep those with the lowest v1(i) + v2(j) and build the layer's
@@ -46,24 +50,18 @@ static int dense_eig(int n, const std::vector<double> &a, std::vector<double> &w
     return SCALIT_OK;
 }
 
-// DIAG's sign convention (mm.f90:66-84) on the n x n column-major z
-static void diag_signs(int n, std::vector<double> &z)
+// every column of the n x n column-major z with its largest-magnitude entry
+// positive (the first such entry on a tie)
+static void column_signs(int n, std::vector<double> &z)
 {
-    int row = -1;
-    for (int i = 0; i < n && row < 0; i++) {
-        if (z[i] == 0.0) continue;
-        bool all = true;
-        for (int j = 0; j < n; j++)
-            if (z[i + (long)n * j] == 0.0) {
-                all = false;
-                break;
-            }
-        if (all) row = i;
+    for (int j = 0; j < n; j++) {
+        double *c = z.data() + (long)n * j;
+        int im = 0;
+        for (int i = 1; i < n; i++)
+            if (std::fabs(c[i]) > std::fabs(c[im])) im = i;
+        if (c[im] < 0.0)
+            for (int i = 0; i < n; i++) c[i] = -c[i];
     }
-    if (row < 0) return;
-    for (int j = 0; j < n; j++)
-        if (z[row + (long)n * j] < 0.0)
-            for (int i = 0; i < n; i++) z[i + (long)n * j] = -z[i + (long)n * j];
 }
 
 }  // namespace scalit
@@ -91,7 +89,7 @@ extern "C" int scalit_podvr(int n, const double *x, const double *t, const doubl
             xm[j + (long)m * i] = acc;
         }
     SC_TRY(dense_eig(m, xm, w, z, nullptr));
-    diag_signs(m, z);
+    column_signs(m, z);
     // H0(I,J) = sum_k W(k,I) E(k) W(k,J)
     for (int i = 0; i < m; i++) {
         x0[i] = w[i];

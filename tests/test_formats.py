@@ -41,7 +41,7 @@ def ref_lib():
     from oracle import oracle as O
 
     R = O.ref()
-    for f in ("savedata_", "loaddata_", "fio_save_dir_", "fio_load_dir_"):
+    for f in ("savedata_", "loaddata_", "fio_save_dir_", "fio_load_dir_", "fio_save_grid_"):
         getattr(R, f).restype = C.c_int
     return R
 
@@ -105,6 +105,28 @@ def test_direct_access_matches_fortran_runtime(tmp_path):
     b2 = p2.encode()
     assert R.fio_load_dir_(C.byref(C.c_long(a.size)), dptr(back), b2, C.c_long(len(b2)))
     assert np.array_equal(back, a)
+
+
+@pytest.mark.ref
+def test_per_point_grid_records_match_fortran_runtime(tmp_path):
+    """The OUTH / H0GM file of P2SaveDataGrid (one direct record of NG x NG
+    per grid point) written by the Fortran runtime reads back into the
+    operator's point-fastest layout, and the library's writer gives the same
+    bytes."""
+    R = ref_lib()
+    n, ng = 37, 9
+    a = np.asfortranarray(sample(n * ng * ng, 5).reshape(n, ng, ng, order="F"))
+    p1, p2 = str(tmp_path / "outh_1.dat"), str(tmp_path / "outh_2.dat")
+    b1 = p1.encode()
+    assert R.fio_save_grid_(C.byref(C.c_long(n)), C.byref(C.c_long(ng)), a.ctypes.data_as(C.POINTER(C.c_double)), b1,
+                            C.c_long(len(b1)))
+    out = np.zeros(n * ng * ng)
+    assert L.lib().scalit_load_data_grid(b1, n, ng, dptr(out)) == L.SCALIT_OK
+    assert np.array_equal(out, a.reshape(-1, order="F"))
+    assert L.lib().scalit_save_data_grid(p2.encode(), n, ng, dptr(out)) == L.SCALIT_OK
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+    # a file shorter than npts records is an error
+    assert L.lib().scalit_load_data_grid(b1, n + 1, ng, dptr(np.zeros((n + 1) * ng * ng))) == L.SCALIT_EIO
 
 
 def test_list_directed_syntax(tmp_path):

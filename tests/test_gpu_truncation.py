@@ -76,7 +76,6 @@ def test_truncated_operator_small():
     ctx.block_diag(1000, 1e-8)
     vosb, eig0 = ctx.get_precond()
     n1, n3 = tp.sN
-    h0 = tp.h0[0]
     # level 2 (theta) was diagonalised first: EIG0 seen by level 1 is its result
     for b in (0, n3 // 2, n3 - 1):
         v = vosb[0][b * n1 * n1:(b + 1) * n1 * n1].reshape(n1, n1, order="F")
@@ -123,10 +122,16 @@ def test_config3_mixed_precision_100_levels():
     ec = float(e1[0]) + 0.004 + 0.002 * float(P.rng(1234).uniform())
     ctx.set_shift(ec)
     ctx.block_diag(1000, 1e-8)
+    ctx.set_pist_orth(2)   # a Krylov space of hundreds of vectors
     args = (ec, 1e-10, 100, 800, 200, 50, 20)
     r64, ev64, res64, st64 = ctx.pist_conv(x0, *args, qmrtol=1e-6)
     print("config3 fp64", r64, res64, st64, ev64[[0, -1]], flush=True)
     assert r64 > 0
+    psi = ctx.pist_vectors(0, 100)
+    rr = ctx.hx(psi) - torch.as_tensor(ev64, device=psi.device)[:, None] * psi
+    rn = (rr * rr).sum(dim=1).sqrt().cpu().numpy()
+    print(f"config3 fp64 Ritz residuals max {rn.max():.1e}", flush=True)
+    assert rn.max() <= 1e-8
     r32, ev32, res32, st32 = ctx.pist_conv(x0, *args, qmrtol=1e-6, prec=L.FP32)
     err = np.abs(ev32 - ev64) / np.abs(ev64)
     print("config3 fp32-inner", r32, res32, st32, flush=True)
