@@ -83,6 +83,20 @@ int scalit_set_res(scalit_ctx *ctx, const double *res);
 int scalit_set_outh(scalit_ctx *ctx, const double *outh);
 /* sOSBW%mE0 / mDE / mBeta (src/iterate/module/a_osbtype.f90:26-29) */
 int scalit_set_shift(scalit_ctx *ctx, double e0, double de, double beta);
+/* sOSB, the preconditioner kernel of the solvers' P*X (TOSB, TOSBD1 or
+ * TOSBD2; the input read at src/iterate/module/mosb.print.h:245, used by
+ * EPX at src/iterate/module/mosb.px.h:40-46): QMR, PIST and the mixed-precision
+ * inner solves then divide by that kernel, with DE and beta from
+ * scalit_set_shift.  Default TOSB. */
+int scalit_set_osb(scalit_ctx *ctx, int nosb);
+/* No counterpart in the reference (its QMR loop, qmr.f90:56-175, calls the
+ * operators once per step).  FP64 QMR solves on a fused 3-level operator
+ * (sF = 3, DEP on level 3 only, levels of at most 48 points, at most 65,536
+ * points) run every iteration in one persistent launch when on = 1, one
+ * kernel per step of the iteration when on = 0; -1 (the default) follows the
+ * environment variable SCALIT_QMR_PERSIST (unset: 1).  Same results to
+ * rounding: only the order of the terms of the dot products differs. */
+int scalit_set_qmr_persistent(scalit_ctx *ctx, int on);
 
 /* OSB set-up on the GPU: myDiag (src/iterate/module/osb.diag.h:53-151) with
  * HosbDiag/MJACOBI (src/iterate/libiterate/diag/hosbdiag.f90:5-75,

@@ -475,13 +475,15 @@ def column_signs(z: np.ndarray) -> np.ndarray:
 
 def podvr(x, t, v, m):
     """DVR_Step2 + DVR_Step3 (src/ham/libham/dvr/dvr.f90:84-160) with numpy's
-    LAPACK eigensolver: h = T + diag(v), U its m lowest eigenvectors,
+    LAPACK eigensolver: h = T + diag(v), U its m lowest eigenvectors (each
+    column's largest-magnitude entry positive: the signs of U decide the
+    signs that normalise W),
     X(I,J) = sum_k U(k,I) x(k) U(k,J) = W diag(x0) W^T, H0 = W^T diag(E) W.
     Returns x0, H0, E (the m lowest eigenvalues of h)."""
     x = f64(x)
     h = np.array(t, dtype=np.float64) + np.diag(f64(v))
     e, u = np.linalg.eigh(0.5 * (h + h.T))
-    um = u[:, :m]
+    um = column_signs(u[:, :m])
     xm = um.T @ (x[:, None] * um)
     x0, w = np.linalg.eigh(0.5 * (xm + xm.T))
     w = column_signs(w)

@@ -69,6 +69,8 @@ SIGNATURES = {
     "scalit_eig_gradients_noalloc": (_i, [_vp, _i, _vp, _l, _i, _vp, _l, _dp]),
     "scalit_podvr": (_i, [_i, _dp, _dp, _dp, _i, _dp, _dp, _dp]),
     "scalit_set_pist_orth": (_i, [_vp, _i]),
+    "scalit_set_osb": (_i, [_vp, _i]),
+    "scalit_set_qmr_persistent": (_i, [_vp, _i]),
     "scalit_load_data_grid": (_i, [C.c_char_p, _l, _i, _dp]),
     "scalit_save_data_grid": (_i, [C.c_char_p, _l, _i, _dp]),
     "scalit_combine_2d": (_i, [_i, _dp, _dp, _i, _dp, _dp, _i, _ip, _ip, _dp]),

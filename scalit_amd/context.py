@@ -369,6 +369,16 @@ class Context:
         """1: the reference's one MGS pass per Krylov vector; 2: twice."""
         check(L.lib().scalit_set_pist_orth(self._h, passes), "scalit_set_pist_orth")
 
+    def set_qmr_persistent(self, on: int):
+        """FP64 QMR on small fused 3-level operators: 1 one persistent launch
+        per solve, 0 one kernel per step, -1 as SCALIT_QMR_PERSIST says."""
+        check(L.lib().scalit_set_qmr_persistent(self._h, on), "scalit_set_qmr_persistent")
+
+    def set_osb(self, nosb: int):
+        """sOSB: the preconditioner kernel of QMR / PIST (TOSB, TOSBD1,
+        TOSBD2), with DE and beta from set_shift."""
+        check(L.lib().scalit_set_osb(self._h, nosb), "scalit_set_osb")
+
     def pist_vectors(self, win: int, m: int):
         psi = torch.empty((m, self.N), dtype=torch.float64, device=self.dev)
         check(L.lib().scalit_pist_vectors(self._h, win, _ptr(psi), self.N), "scalit_pist_vectors")

@@ -103,8 +103,16 @@ static std::map<std::tuple<int, const void *, int, size_t>, int> g_resident;
 
 static void allow_full_lds_locked(int dev, const void *fn)
 {
-    if (g_lds_raised.insert({dev, fn}).second)
-        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (!g_lds_raised.insert({dev, fn}).second) return;
+    // the CU's 160 KB less the kernel's static LDS; a refusal leaves the
+    // default limit (the launch then reports what does not fit) and is
+    // cleared here, so that it is not reported by a later, unrelated call
+    hipFuncAttributes fa{};
+    size_t stat = 0;
+    if (hipFuncGetAttributes(&fa, fn) == hipSuccess) stat = fa.sharedSizeBytes;
+    else (void)hipGetLastError();
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(160 * 1024 - stat)) != hipSuccess)
+        (void)hipGetLastError();
 }
 
 void allow_full_lds(const void *fn)
@@ -315,6 +323,20 @@ int scalit_set_shift(scalit_ctx *c, double e0, double de, double beta)
     return SCALIT_OK;
 }
 
+int scalit_set_qmr_persistent(scalit_ctx *c, int on)
+{
+    if (!c || on < -1 || on > 1) return SCALIT_EINVAL;
+    c->qmr_persist = on;
+    return SCALIT_OK;
+}
+
+int scalit_set_osb(scalit_ctx *c, int nosb)
+{
+    if (!c || (nosb != SCALIT_TOSB && nosb != SCALIT_TOSBD1 && nosb != SCALIT_TOSBD2)) return SCALIT_EINVAL;
+    c->osb = nosb;
+    return SCALIT_OK;
+}
+
 static int operator_ready(const scalit_ctx *c)
 {
     if (!c->has_res) return SCALIT_ESTATE;
@@ -434,9 +456,9 @@ int scalit_px_lp(scalit_ctx *c, int prec, int npc, int nvec, const void *x, long
     SC_TRY(ensure_f32(c));
     float e0 = (npc == SCALIT_TA0) ? 0.0f : (float)c->e0;
     if (prec == SCALIT_FP32)
-        return px_apply_f32(c, shifts, e0, nvec, (const float *)x, ldx, (float *)y, ldy, nullptr, pick(c, stream));
+        return px_apply_f32(c, SCALIT_TOSB, shifts, e0, nvec, (const float *)x, ldx, (float *)y, ldy, nullptr, pick(c, stream));
     if (prec == SCALIT_BF16)
-        return px_apply_bf16(c, shifts, e0, nvec, (const bf16_t *)x, ldx, (bf16_t *)y, ldy, nullptr, pick(c, stream));
+        return px_apply_bf16(c, SCALIT_TOSB, shifts, e0, nvec, (const bf16_t *)x, ldx, (bf16_t *)y, ldy, nullptr, pick(c, stream));
     return SCALIT_EINVAL;
 }
 

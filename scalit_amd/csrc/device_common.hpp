@@ -1,6 +1,6 @@
 // Device helpers shared by the operator kernels (H*X, P*X): the 16x16x4 MFMA
-// wrappers with their lane layouts, storage <-> compute conversions and the
-// per-vector activity mask.
+// wrappers with their lane layouts, storage <-> compute conversions, the
+// preconditioner's per-value kernel and the per-vector activity mask.
 #pragma once
 
 #include "scalit_impl.hpp"
@@ -52,6 +52,24 @@ template <typename T, typename S>
 __device__ __forceinline__ void st_cvt(S *p, long i, T v) { p[i] = (S)v; }
 template <>
 __device__ __forceinline__ void st_cvt<float, bf16_t>(bf16_t *p, long i, float v) { p[i] = f32_to_bf16(v); }
+
+// PX_KERNEL (osb.px.h:156-211) for one value
+template <typename T>
+__device__ __forceinline__ T px_kernel(int nosb, T y, T eig, T e0, T de, T beta)
+{
+    T ade = eig - e0;
+    if (nosb == SCALIT_TOSBD1) {
+        if (fabs(ade) <= de) return ade < (T)0 ? -y * ((T)1 / de) : y * ((T)1 / de);
+        return y / ade;
+    }
+    if (nosb == SCALIT_TOSBD2) {
+        T b1 = (T)1 + beta, al = -beta * de, t;
+        if (fabs(ade) <= de) t = ade < (T)0 ? (-al + b1 * ade) : (al + b1 * ade);
+        else t = ade;
+        return y / t;
+    }
+    return y / ade;
+}
 
 __device__ __forceinline__ bool vec_active(const int *active, int v) { return active == nullptr || active[v] != 0; }
 

@@ -25,24 +25,6 @@
 
 namespace scalit {
 
-// PX_KERNEL (osb.px.h:156-211) for one value
-template <typename T>
-__device__ __forceinline__ T px_kernel(int nosb, T y, T eig, T e0, T de, T beta)
-{
-    T ade = eig - e0;
-    if (nosb == SCALIT_TOSBD1) {
-        if (fabs(ade) <= de) return ade < (T)0 ? -y * ((T)1 / de) : y * ((T)1 / de);
-        return y / ade;
-    }
-    if (nosb == SCALIT_TOSBD2) {
-        T b1 = (T)1 + beta, al = -beta * de, t;
-        if (fabs(ade) <= de) t = ade < (T)0 ? (-al + b1 * ade) : (al + b1 * ade);
-        else t = ade;
-        return y / t;
-    }
-    return y / ade;
-}
-
 // Y(k,i,b) = sum_j V_b(j,i) X(k,j,b)  (transpose, VTX)  or  V_b(i,j) (VX);
 // one thread per output value, any n.
 template <typename T, typename S, typename SO>
@@ -566,8 +548,8 @@ int px_apply_f64(scalit_ctx *c, int nosb, const double *shifts_dev, double shift
                                    nvec, x, ldx, y, ldy, active, s);
 }
 
-int px_apply_f32(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const float *x, long ldx, float *y,
-                 long ldy, const int *active, hipStream_t s)
+int px_apply_f32(scalit_ctx *c, int nosb, const float *shifts_dev, float shift0, int nvec, const float *x, long ldx,
+                 float *y, long ldy, const int *active, hipStream_t s)
 {
     if (!c->has_precond) return SCALIT_ESTATE;
     SC_TRY(precond_to_f32(c));
@@ -582,12 +564,12 @@ int px_apply_f32(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec,
         vosb[l] = c->lv[l].vosb_f.p;
         vosbt[l] = c->lv[l].vosbt_f.p;
     }
-    return px_impl<float, float>(c, SCALIT_TOSB, vosb, vosbt, c->eig0_f.p, c->work_f.p, shifts_dev, shift0, (float)c->de,
+    return px_impl<float, float>(c, nosb, vosb, vosbt, c->eig0_f.p, c->work_f.p, shifts_dev, shift0, (float)c->de,
                                  (float)c->beta, nvec, x, ldx, y, ldy, active, s);
 }
 
-int px_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const bf16_t *x, long ldx,
-                  bf16_t *y, long ldy, const int *active, hipStream_t s)
+int px_apply_bf16(scalit_ctx *c, int nosb, const float *shifts_dev, float shift0, int nvec, const bf16_t *x,
+                  long ldx, bf16_t *y, long ldy, const int *active, hipStream_t s)
 {
     if (!c->has_precond) return SCALIT_ESTATE;
     SC_TRY(precond_to_f32(c));
@@ -602,7 +584,7 @@ int px_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec
         vosb[l] = c->lv[l].vosb_f.p;
         vosbt[l] = c->lv[l].vosbt_f.p;
     }
-    return px_impl<float, bf16_t>(c, SCALIT_TOSB, vosb, vosbt, c->eig0_f.p, c->work_f.p, shifts_dev, shift0, (float)c->de,
+    return px_impl<float, bf16_t>(c, nosb, vosb, vosbt, c->eig0_f.p, c->work_f.p, shifts_dev, shift0, (float)c->de,
                                   (float)c->beta, nvec, x, ldx, y, ldy, active, s);
 }
 

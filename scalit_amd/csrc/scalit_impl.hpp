@@ -127,6 +127,13 @@ struct scalit_ctx {
     // orthogonalisation passes per PIST Krylov vector: 1 = the reference's
     // MGS_ORTH, 2 = once more against the basis (scalit_set_pist_orth)
     int pist_orth = 1;
+    // the preconditioner kernel of the solvers' P*X (the reference's sOSB:
+    // TOSB, TOSBD1 or TOSBD2, with DE and beta from scalit_set_shift)
+    int osb = 0;
+    // FP64 QMR on small fused 3-level operators: 1 = the persistent solver,
+    // 0 = the launch-per-step iteration, -1 = as SCALIT_QMR_PERSIST says
+    // (default 1)
+    int qmr_persist = -1;
     // bumped whenever a device buffer that operator kernels read is
     // (re)allocated: recorded graphs of older generations are re-recorded
     long gen = 0;
@@ -159,10 +166,10 @@ int hx_variants(const scalit_ctx *c, int prec, int nvec, int *va, int *vb);
 // P*X (kernels_px.hip)
 int px_apply_f64(scalit_ctx *c, int nosb, const double *shifts_dev, double shift0, int nvec, const double *x,
                  long ldx, double *y, long ldy, const int *active, hipStream_t s);
-int px_apply_f32(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const float *x, long ldx, float *y,
-                 long ldy, const int *active, hipStream_t s);
-int px_apply_bf16(scalit_ctx *c, const float *shifts_dev, float shift0, int nvec, const bf16_t *x, long ldx,
-                  bf16_t *y, long ldy, const int *active, hipStream_t s);
+int px_apply_f32(scalit_ctx *c, int nosb, const float *shifts_dev, float shift0, int nvec, const float *x, long ldx,
+                 float *y, long ldy, const int *active, hipStream_t s);
+int px_apply_bf16(scalit_ctx *c, int nosb, const float *shifts_dev, float shift0, int nvec, const bf16_t *x,
+                  long ldx, bf16_t *y, long ldy, const int *active, hipStream_t s);
 int precond_to_f32(scalit_ctx *c);
 
 // OSB set-up (kernels_diag.hip)

@@ -87,4 +87,18 @@ void launch_dv_accum(long n, int nvec, const double *x, const double *g, long ld
 
 bool fast3_ok(const scalit_ctx *c);
 
+// The persistent QMR solver (kernels_qmr.hip): every iteration of the solve
+// in one launch for the fused 3-level operator on small grids (FP64).
+struct QmrPersistBufs {
+    DBuf<double> part;   // partial sums of the dot products
+    DBuf<unsigned> bar;  // barrier arrival counters
+    DBuf<int> err;       // a barrier wait gave up
+    int herr = 0;
+};
+constexpr int QP_NOT_LAUNCHED = 1;
+bool qmr_persist_ok(const scalit_ctx *c);
+int qmr_persist_solve(scalit_ctx *c, int nvec, const QmrVecs<double> &w, QmrScal *q, int *active,
+                      const double *shifts, const double *part_vz, int conv, int qmrmax, double tol, QmrPersistBufs &pb,
+                      hipStream_t s);
+
 }  // namespace scalit

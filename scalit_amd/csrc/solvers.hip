@@ -92,7 +92,7 @@ struct Ops<double> {
     static int px(scalit_ctx *c, const double *sh, int nv, const double *x, long ldx, double *y, long ldy,
                   const int *act, hipStream_t s)
     {
-        return px_apply_f64(c, SCALIT_TOSB, sh, c->e0, nv, x, ldx, y, ldy, act, s);
+        return px_apply_f64(c, c->osb, sh, c->e0, nv, x, ldx, y, ldy, act, s);
     }
 };
 template <>
@@ -105,7 +105,7 @@ struct Ops<float> {
     static int px(scalit_ctx *c, const float *sh, int nv, const float *x, long ldx, float *y, long ldy,
                   const int *act, hipStream_t s)
     {
-        return px_apply_f32(c, sh, (float)c->e0, nv, x, ldx, y, ldy, act, s);
+        return px_apply_f32(c, c->osb, sh, (float)c->e0, nv, x, ldx, y, ldy, act, s);
     }
 };
 template <>
@@ -118,7 +118,7 @@ struct Ops<bf16_t> {
     static int px(scalit_ctx *c, const float *sh, int nv, const bf16_t *x, long ldx, bf16_t *y, long ldy,
                   const int *act, hipStream_t s)
     {
-        return px_apply_bf16(c, sh, (float)c->e0, nv, x, ldx, y, ldy, act, s);
+        return px_apply_bf16(c, c->osb, sh, (float)c->e0, nv, x, ldx, y, ldy, act, s);
     }
 };
 template <typename S>
@@ -145,6 +145,7 @@ struct QmrWS {
     int g_conv = -99, g_qmrmax = -1;
     double g_tol = -1;
     long g_gen = -1;   // operator buffer generation the graph was recorded with
+    QmrPersistBufs persist;
     std::vector<QmrScal> hscal;
     std::vector<int> hact, hones;
     std::vector<typename ShiftT<S>::type> hshift;
@@ -203,6 +204,9 @@ struct QmrWS {
         scal.release();
         active.release();
         shifts.release();
+        persist.part.release();
+        persist.bar.release();
+        persist.err.release();
         nvec = 0;
     }
 };
@@ -283,6 +287,9 @@ static int env_int(const char *name, int def)
 }
 int g_qmr_use_graph = env_int("SCALIT_QMR_GRAPH", 1);
 int g_qmr_check_every = std::max(1, env_int("SCALIT_QMR_CHECK", 4));
+// FP64 solves on operators the persistent solver takes run it unless
+// SCALIT_QMR_PERSIST=0
+int g_qmr_persist = env_int("SCALIT_QMR_PERSIST", 1);
 
 // Solve (H - shift_v) x_v = b_v for the nvec columns already in w.B(); the
 // solution lands in w.X().  shifts: host, nvec values (the QMR shift E0 per
@@ -308,6 +315,23 @@ static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double t
     launch_qmr_init_z<S>(nv, n, v, w.part(0), w.part(1), w.part(9), s);
     launch_qmr_init_scal(nv, w.part(0), w.part(1), w.scal.p, w.active.p, qmrmax, s);
     HIP_TRY(hipGetLastError());
+    if constexpr (std::is_same<S, double>::value) {
+        const int want = c->qmr_persist < 0 ? g_qmr_persist : c->qmr_persist;
+        if (cb == nullptr && want != 0 && qmrmax >= 1 && qmr_persist_ok(c)) {
+            const int rc = qmr_persist_solve(c, nv, v, w.scal.p, w.active.p, w.shifts.p, w.part(9), conv, qmrmax, tol,
+                                             w.persist, s);
+            if (rc != QP_NOT_LAUNCHED) {
+                SC_TRY(rc);
+                HIP_TRY(hipMemcpyAsync(w.hscal.data(), w.scal.p, sizeof(QmrScal) * nv, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+                for (int i = 0; i < nv; i++) {
+                    if (ret) ret[i] = w.hscal[i].ret;
+                    if (res) res[i] = w.hscal[i].res;
+                }
+                return SCALIT_OK;
+            }
+        }
+    }
     const bool use_graph = g_qmr_use_graph != 0 && cb == nullptr;
     if (cb == nullptr) {
         const int prec = std::is_same<S, double>::value ? SCALIT_FP64

@@ -9,8 +9,9 @@
 //    1-D Hamiltonian H0 = W^T diag(E) W on it.  DVR_Step3 leaves the signs of
 //    W to DSYEV; a sign flip of column a flips row and column a of H0, an
 //    equivalent operator (D H0 D with D = diag(+-1), the same spectrum with
-//    every diagonal potential).  Here each column of W has its
-//    largest-magnitude entry positive, a choice that, unlike DIAG's
+//    every diagonal potential).  Here each column of U and of W has its
+//    largest-magnitude entry positive (U's signs pass into W's rows and so
+//    into which entry of a column of W is normalised), a choice that, unlike DIAG's
 //    first-full-row rule (src/libutil/serial/base/mm.f90:66-84), does not
 //    hang on entries at rounding level.
 //  * scalit_combine_2d: the energy-cut combined layer of two coordinates,
@@ -79,6 +80,8 @@ extern "C" int scalit_podvr(int n, const double *x, const double *t, const doubl
     for (long k = 0; k < (long)n * n; k++) h[k] = t[k];
     for (int i = 0; i < n; i++) h[i + (long)n * i] += v[i];
     SC_TRY(dense_eig(n, h, eh, u, nullptr));
+    // the signs of U's columns carry into the sign choice of W's below
+    column_signs(n, u);
     // DVR_Step3: X(I,J) = sum_k U(k,I) x(k) U(k,J) over the m lowest
     std::vector<double> xm((size_t)m * m), w, z;
     for (int i = 0; i < m; i++)

@@ -35,10 +35,14 @@ def op_ctx(grid, e0, seed=1234, kind="jitter"):
     return prob, op, ctx
 
 
+@pytest.mark.parametrize("persist", [1, 0])
 @pytest.mark.parametrize("qmr_type", [-1, 0, -2])
-def test_qmr_matches_oracle(qmr_type):
-    print(f"qmr type={qmr_type}", flush=True)
+def test_qmr_matches_oracle(qmr_type, persist):
+    """QMR at 24 x 32 x 24 through the persistent solver (one launch per
+    solve) and through one kernel per step of the iteration."""
+    print(f"qmr type={qmr_type} persistent={persist}", flush=True)
     prob, op, ctx = op_ctx((24, 32, 24), e0=-0.003)
+    ctx.set_qmr_persistent(persist)
     vosb, eig0 = op.diag()
     ctx.set_precond(vosb, eig0)
     b = P.random_vectors(prob.N, 1, 31)
@@ -52,8 +56,11 @@ def test_qmr_matches_oracle(qmr_type):
     ctx.close()
 
 
-def test_qmr_batched_shifts_and_breakdown():
+@pytest.mark.parametrize("persist", [1, 0])
+def test_qmr_batched_shifts_and_breakdown(persist):
+    print(f"qmr batched persistent={persist}", flush=True)
     prob, op, ctx = op_ctx((24, 32, 24), e0=-0.003)
+    ctx.set_qmr_persistent(persist)
     vosb, eig0 = op.diag()
     ctx.set_precond(vosb, eig0)
     b = P.random_vectors(prob.N, 4, 32)
@@ -70,6 +77,49 @@ def test_qmr_batched_shifts_and_breakdown():
     # iteration limit: -QMRMAX
     ret, _, _ = ctx.qmr(ctx.to_device(b[:1]), ntype=-1, qmrmax=2, qmrtol=1e-14)
     assert ret[0] == -2
+    ctx.close()
+
+
+@pytest.mark.parametrize("nosb", [O.TOSBD1, O.TOSBD2])
+@pytest.mark.parametrize("persist", [1, 0])
+def test_qmr_osbd_kernels(nosb, persist):
+    """sOSB = TOSBD1 / TOSBD2: QMR's preconditioner divides by the OSBD
+    kernel (window DE, beta) on both solver paths, against the oracle's QMR
+    with the same kernel."""
+    print(f"qmr osbd kind={nosb} persistent={persist}", flush=True)
+    prob, op, ctx = op_ctx((24, 32, 24), e0=0.002)
+    vosb, eig0 = op.diag()
+    ctx.set_precond(vosb, eig0)
+    ctx.set_qmr_persistent(persist)
+    ctx.set_osb(nosb)
+    ctx.set_shift(0.002, 0.0005, 0.3)
+    op.set_shift(0.002, 0.0005, 0.3)
+    op.s.osb_kind = nosb
+    b = P.random_vectors(prob.N, 2, 33)
+    ret, x, _ = ctx.qmr(ctx.to_device(b), ntype=-1, qmrmax=400, qmrtol=1e-9)
+    x = x.cpu().numpy()
+    for v in range(2):
+        r_o, x_o, _, _ = op.qmr(b[v], qmr_type=-1, qmr_max=400, qmr_tol=1e-9)
+        assert r_o > 0 and abs(int(ret[v]) - r_o) <= 1, (ret, r_o)
+        assert np.abs(x[v] - x_o).max() <= 1e-7 * np.abs(x_o).max()
+    ctx.close()
+
+
+@pytest.mark.parametrize("persist", [1, 0])
+def test_pist_persistent_and_per_step_agree(persist):
+    """The config-1 PIST (lowest 20 levels, 24 x 32 x 24) against the golden
+    levels with either QMR path."""
+    g = gold()
+    prob = P.config_problem("pist_fp64")
+    ctx = Context.from_problem(prob, e0=g["e0"])
+    ctx.set_qmr_persistent(persist)
+    ctx.block_diag(1000, 1e-5)
+    x0 = ctx.to_device(P.random_vectors(prob.N, 1, g["start_seed"]))
+    r, eig, res, stats = ctx.pist_conv(x0, g["e0"], g["etol"], g["m"], g["m_max"], g["startcnt"], g["stepcnt"],
+                                       g["stepeig"], qmrtol=g["qmr_tol"])
+    print("pist persistent" if persist else "pist per-step", r, stats, flush=True)
+    ref = np.array(g["eigsh"])
+    assert r > 0 and (np.abs(eig - ref) / np.abs(ref)).max() <= 1e-10
     ctx.close()
 
 

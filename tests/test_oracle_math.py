@@ -270,13 +270,14 @@ def test_podvr_restatement(n, m):
     """The oracle's DVR_Step2/3: with Phi = U_m W the PO-DVR basis functions
     on the primitive grid (built here from a full eigh of h = T + diag(v)),
     H0 = Phi^T h Phi and Phi^T diag(x) Phi = diag(x0), its eigenvalues are the
-    m lowest of h, and every column of W has its largest entry positive."""
+    m lowest of h, and every column of U and of W has its largest entry
+    positive."""
     x, t = P.sinc2_dvr(n, *P.R_RANGE, P.MU)
     v = P.morse_1d(x, *P.MORSE_R)
     x0, h0, e = O.podvr(x, t, v, m)
     h = t + np.diag(v)
     ev, u = np.linalg.eigh(h)
-    um = u[:, :m]
+    um = O.column_signs(u[:, :m])
     xm = um.T @ (x[:, None] * um)
     x0n, wn = np.linalg.eigh(xm)
     wn = O.column_signs(wn)
