@@ -91,6 +91,7 @@ bool fast3_ok(const scalit_ctx *c);
 // in one launch for the fused 3-level operator on small grids (FP64).
 struct QmrPersistBufs {
     DBuf<double> part;   // partial sums of the dot products
+    DBuf<double> opart;  // partial sums of the Gram-Schmidt step
     DBuf<unsigned> bar;  // barrier arrival counters
     DBuf<int> err;       // a barrier wait gave up
     int herr = 0;
@@ -100,5 +101,23 @@ bool qmr_persist_ok(const scalit_ctx *c);
 int qmr_persist_solve(scalit_ctx *c, int nvec, const QmrVecs<double> &w, QmrScal *q, int *active,
                       const double *shifts, const double *part_vz, int conv, int qmrmax, double tol, QmrPersistBufs &pb,
                       hipStream_t s);
+// The Krylov steps of pistH0 for nw windows in one persistent launch: for
+// j = j_from..j_to, the QMR solve with basis column j - 1 as its right-hand
+// side, then MGS_ORTH of the solution into column j (orth passes).  Per
+// window: qret[v * ldg + j] = the QMR return of step j (steps not run keep
+// what the caller put there), wret[v] = 1 when every step ran, else the
+// pistH0 value that ended the window.  gram: per window ldg x ldg, the basis
+// columns' dot products, extended by one row per step.
+struct PistPersistCall {
+    double *vj;
+    long ldv, ldw;
+    double *gram;
+    int ldg, j_from, j_to, orth, zero_x, conv, qmrmax;
+    double tol;
+    int *qret, *wret;
+};
+int pist_persist_max_cols(scalit_ctx *c, int nw, int ldg);
+int pist_persist_steps(scalit_ctx *c, int nw, const PistPersistCall &pc, QmrVecs<double> w, QmrScal *q,
+                       int *active, const double *shifts, QmrPersistBufs &pb, hipStream_t s);
 
 }  // namespace scalit

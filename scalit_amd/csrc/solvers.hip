@@ -290,6 +290,10 @@ int g_qmr_check_every = std::max(1, env_int("SCALIT_QMR_CHECK", 4));
 // FP64 solves on operators the persistent solver takes run it unless
 // SCALIT_QMR_PERSIST=0
 int g_qmr_persist = env_int("SCALIT_QMR_PERSIST", 1);
+static bool qmr_persist_wanted(const scalit_ctx *c)
+{
+    return (c->qmr_persist < 0 ? g_qmr_persist : c->qmr_persist) != 0;
+}
 
 // Solve (H - shift_v) x_v = b_v for the nvec columns already in w.B(); the
 // solution lands in w.X().  shifts: host, nvec values (the QMR shift E0 per
@@ -316,8 +320,7 @@ static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double t
     launch_qmr_init_scal(nv, w.part(0), w.part(1), w.scal.p, w.active.p, qmrmax, s);
     HIP_TRY(hipGetLastError());
     if constexpr (std::is_same<S, double>::value) {
-        const int want = c->qmr_persist < 0 ? g_qmr_persist : c->qmr_persist;
-        if (cb == nullptr && want != 0 && qmrmax >= 1 && qmr_persist_ok(c)) {
+        if (cb == nullptr && qmr_persist_wanted(c) && qmrmax >= 1 && qmr_persist_ok(c)) {
             const int rc = qmr_persist_solve(c, nv, v, w.scal.p, w.active.p, w.shifts.p, w.part(9), conv, qmrmax, tol,
                                              w.persist, s);
             if (rc != QP_NOT_LAUNCHED) {
@@ -403,6 +406,12 @@ struct PistState {
     std::vector<int> last_cnt;                // Krylov size of each window's last eigensolve
     std::vector<int> last_ret;                // each window's PIST_CONV return code
     long ldv = 0;
+    // the persistent Krylov steps: the basis columns' Gram matrix per window,
+    // valid while every step of the run went through them
+    DBuf<double> gram;
+    DBuf<int> qret, wret;
+    bool gram_valid = false;
+    std::vector<int> hqret, hwret;
     // QMR parameters of the OSB_QMR callback
     int qmr_type = -1, qmr_max = 1000;
     double qmr_tol = 1e-3;
@@ -437,6 +446,9 @@ struct PistState {
             b->release();
         lejobs.release();
         lokv.release();
+        gram.release();
+        qret.release();
+        wret.release();
     }
 };
 
@@ -746,6 +758,64 @@ static int pist_mgs(scalit_ctx *c, PistState &P, int nw, int m_max, int jnew, co
     return SCALIT_OK;
 }
 
+// The Krylov steps j = j_from..j_to of pist_h0 in one persistent launch
+// (kernels_qmr.hip), with the same per-window outcome and counts as the loop
+// of pist_linsolv / pist_mgs calls.  QP_NOT_LAUNCHED when the windows do not
+// fit the device at once.
+static int pist_steps_persistent(scalit_ctx *c, PistState &P, const PistParams &pp, int nw, const double *e0,
+                                 int j_from, int j_to, std::vector<int> &act, std::vector<int> &ret,
+                                 std::vector<int> &stats, hipStream_t s)
+{
+    const long N = c->N;
+    const int ldg = pp.m_max;
+    QmrWS<double> &q = P.q64;
+    SC_TRY(q.ensure(nw, N));
+    q.hshift.assign(e0, e0 + nw);
+    HIP_TRY(hipMemcpyAsync(q.shifts.p, q.hshift.data(), 8 * (size_t)nw, hipMemcpyHostToDevice, s));
+    SC_TRY(P.gram.alloc((size_t)nw * ldg * ldg));
+    SC_TRY(P.qret.alloc((size_t)nw * ldg));
+    SC_TRY(P.wret.alloc(nw));
+    HIP_TRY(hipMemsetAsync(P.qret.p, 0x80, sizeof(int) * (size_t)nw * ldg, s));   // "not run"
+    PistPersistCall pc;
+    pc.vj = P.vj.p;
+    pc.ldv = P.ldv;
+    pc.ldw = P.ldv * pp.m_max;
+    pc.gram = P.gram.p;
+    pc.ldg = ldg;
+    pc.j_from = j_from;
+    pc.j_to = j_to;
+    pc.orth = c->pist_orth >= 2 ? 2 : 1;
+    pc.zero_x = pp.qmr_type <= 0 ? 1 : 0;
+    pc.conv = std::abs(pp.qmr_type) == 1 ? 1 : (std::abs(pp.qmr_type) == 2 ? 2 : 0);
+    pc.qmrmax = pp.qmrmax;
+    pc.tol = pp.qmrtol;
+    pc.qret = P.qret.p;
+    pc.wret = P.wret.p;
+    const int rc = pist_persist_steps(c, nw, pc, q.view(), q.scal.p, P.active.p, q.shifts.p, q.persist, s);
+    if (rc != SCALIT_OK) return rc;
+    P.hqret.resize((size_t)nw * ldg);
+    P.hwret.resize(nw);
+    HIP_TRY(hipMemcpyAsync(P.hqret.data(), P.qret.p, sizeof(int) * (size_t)nw * ldg, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(P.hwret.data(), P.wret.p, sizeof(int) * nw, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const int not_run = (int)0x80808080u;
+    for (int k = 0; k < nw; k++) {
+        if (!act[k]) continue;
+        for (int j = j_from; j <= j_to; j++) {
+            const int r = P.hqret[(size_t)k * ldg + j];
+            if (r == not_run) break;
+            stats[1] += 1;
+            if (r > 0) stats[0] += r;
+        }
+        if (P.hwret[k] != 1) {
+            ret[k] = P.hwret[k];
+            act[k] = 0;
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(P.active.p, act.data(), sizeof(int) * nw, hipMemcpyHostToDevice, s));
+    return SCALIT_OK;
+}
+
 // pistH0 for nw windows; ret[k] gets the reference's return value.
 static int pist_h0(scalit_ctx *c, PistState &P, const PistParams &pp, int nw, const double *e0, int m_low, int m_high,
                    std::vector<int> &act, std::vector<int> &ret, std::vector<int> &stats, hipStream_t s)
@@ -757,7 +827,13 @@ static int pist_h0(scalit_ctx *c, PistState &P, const PistParams &pp, int nw, co
         if (act[k]) ret[k] = m_high;
     int m_start = (m_low == 1) ? 1 : m_low - 1;
     std::vector<int> num(nw, 0), ok(nw, 1);
-    for (int j = m_start; j <= m_high - 1; j++) {
+    int rc = QP_NOT_LAUNCHED;
+    if (m_start <= m_high - 1 && P.gram_valid && pp.prec == SCALIT_FP64 && qmr_persist_wanted(c) &&
+        m_high <= pist_persist_max_cols(c, nw, m_max))
+        rc = pist_steps_persistent(c, P, pp, nw, e0, m_start, m_high - 1, act, ret, stats, s);
+    if (rc != QP_NOT_LAUNCHED) SC_TRY(rc);
+    else if (m_start <= m_high - 1) P.gram_valid = false;   // later steps cannot take the persistent path in this run
+    for (int j = m_start; rc == QP_NOT_LAUNCHED && j <= m_high - 1; j++) {
         SC_TRY(pist_linsolv(c, P, pp, nw, e0, j - 1, act, num, stats, s));
         for (int k = 0; k < nw; k++)
             if (act[k] && num[k] <= 0) {
@@ -842,6 +918,7 @@ int pist_conv(scalit_ctx *c, int nw, const double *e0, const PistParams &pp, con
             }
     }
     HIP_TRY(hipMemcpyAsync(P.active.p, act.data(), sizeof(int) * nw, hipMemcpyHostToDevice, s));
+    P.gram_valid = true;   // no Gram rows are needed before the first step
     int cnt_low = 1;
     int cnt_high = std::max({pp.startcnt, m, pp.stepeig + 1});
     while (cnt_high <= m_max) {
