@@ -766,7 +766,7 @@ __device__ void tridiag_bisect(int n, const double *d, const double *e, double *
     // T = threads per eigenvalue: a power of two <= 64 (a group never spans
     // two wavefronts, so the group reductions are lane shuffles)
     int T = 1;
-    while (T < 64 && 2 * T * n <= EIGV_NT) T *= 2;
+    while (T < 64 && 2 * T * n <= EIGV_NT) T *= 2;   // every eigenvalue in one pass of the workgroup
     const int sub = tid & (T - 1);
     const double frac = (double)(sub + 1) / (double)(T + 1);
     for (int base = 0; base < n; base += EIGV_NT / T) {
@@ -818,44 +818,70 @@ __global__ void __launch_bounds__(EIGV_NT) k_sym_eigvals(const EigvalJob *__rest
         A[i + ld * j] = (i <= j) ? a_in[i + (long)lda * j] : a_in[j + (long)lda * i];   // from the upper triangle
     }
     __syncthreads();
+    // One Householder step per column, three barriers each: wave 0 forms the
+    // reflector (v, tau) alone; every thread then takes part in p = tau A22 v;
+    // each wave forms p.v by itself (the same sum in every wave) and the
+    // rank-2 update A22 -= v w^T + w v^T uses w = p - (tau p.v / 2) v on the
+    // fly.  tau goes through red[k & 1], so wave 0 may start the next column
+    // while the other waves still read this one's.
+    const int lane = tid & 63;
     for (int k = 0; k + 2 < n; k++) {
         const int m = n - k - 1;   // trailing size; x = A(k+1.., k)
-        double ss = 0;
-        for (int i = tid; i < m; i += EIGV_NT) {
-            const double xi = A[(k + 1 + i) + ld * k];
-            ss += (i > 0) ? xi * xi : 0.0;
+        if (tid < 64) {
+            double ss = 0;
+            for (int i = 1 + lane; i < m; i += 64) {
+                const double xi = A[(k + 1 + i) + ld * k];
+                ss += xi * xi;
+            }
+            for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);   // below the subdiagonal
+            const double x0 = A[(k + 1) + ld * k];
+            if (lane == 0) d[k] = A[k + ld * k];
+            if (ss == 0.0) {   // column already reduced
+                if (lane == 0) {
+                    e[k] = x0;
+                    red[k & 1] = 0.0;
+                }
+            } else {
+                const double nrm = sqrt(x0 * x0 + ss);
+                const double alpha = x0 <= 0 ? nrm : -nrm;   // new subdiagonal value, no cancellation in v0
+                if (lane == 0) {
+                    e[k] = alpha;
+                    red[k & 1] = 2.0 / ((x0 - alpha) * (x0 - alpha) + ss);
+                }
+                for (int i = lane; i < m; i += 64) v[i] = (i == 0) ? x0 - alpha : A[(k + 1 + i) + ld * k];
+            }
         }
-        const double sigma = block_sum_1024(ss, red);   // sum of squares below the subdiagonal
-        const double x0 = A[(k + 1) + ld * k];
-        if (tid == 0) d[k] = A[k + ld * k];
-        if (sigma == 0.0) {   // column already reduced
-            if (tid == 0) e[k] = x0;
-            __syncthreads();
-            continue;
-        }
-        const double nrm = sqrt(x0 * x0 + sigma);
-        const double alpha = x0 <= 0 ? nrm : -nrm;   // new subdiagonal value, no cancellation in v0
-        if (tid == 0) e[k] = alpha;
-        for (int i = tid; i < m; i += EIGV_NT) v[i] = (i == 0) ? x0 - alpha : A[(k + 1 + i) + ld * k];
         __syncthreads();
-        const double vtv = (x0 - alpha) * (x0 - alpha) + sigma;
-        const double tau = 2.0 / vtv;
-        // p = tau A22 v
-        for (int i = tid; i < m; i += EIGV_NT) {
+        const double tau = red[k & 1];
+        if (tau == 0.0) continue;
+        // p = tau A22 v: eight threads per row (neighbouring lanes) while the
+        // rows fit, their partial sums joined by lane shuffles
+        if (m <= EIGV_NT / 8) {
+            const int i = tid >> 3, sub = tid & 7;
             double acc = 0;
-            for (int j = 0; j < m; j++) acc += A[(k + 1 + i) + ld * (k + 1 + j)] * v[j];
-            pw[i] = tau * acc;
+            if (i < m)
+                for (int j = sub; j < m; j += 8) acc += A[(k + 1 + i) + ld * (k + 1 + j)] * v[j];
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 2);
+            acc += __shfl_xor(acc, 4);
+            if (i < m && sub == 0) pw[i] = tau * acc;
+        } else {
+            for (int i = tid; i < m; i += EIGV_NT) {
+                double acc = 0;
+                for (int j = 0; j < m; j++) acc += A[(k + 1 + i) + ld * (k + 1 + j)] * v[j];
+                pw[i] = tau * acc;
+            }
         }
         __syncthreads();
         double pv = 0;
-        for (int i = tid; i < m; i += EIGV_NT) pv += pw[i] * v[i];
-        const double K = 0.5 * tau * block_sum_1024(pv, red);
-        for (int i = tid; i < m; i += EIGV_NT) pw[i] -= K * v[i];   // w = p - K v
-        __syncthreads();
-        // A22 -= v w^T + w v^T
+        for (int i = lane; i < m; i += 64) pv += pw[i] * v[i];
+        for (int o = 32; o > 0; o >>= 1) pv += __shfl_xor(pv, o);
+        const double K = 0.5 * tau * pv;
+        // A22 -= v w^T + w v^T, w = p - K v
         for (int x = tid; x < m * m; x += EIGV_NT) {
             const int i = x % m, j = x / m;
-            A[(k + 1 + i) + ld * (k + 1 + j)] -= v[i] * pw[j] + pw[i] * v[j];
+            const double wi = pw[i] - K * v[i], wj = pw[j] - K * v[j];
+            A[(k + 1 + i) + ld * (k + 1 + j)] -= v[i] * wj + wi * v[j];
         }
         __syncthreads();
     }

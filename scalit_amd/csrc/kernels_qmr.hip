@@ -29,6 +29,8 @@
 //   5 pencil Z = V3 Z;  Z.Z, V.Z
 //   6 slab   UX, UR, X, R;  HX0 = (modes 1, 2, diagonal) X
 //   7 pencil HX0 += DEP o (H3 X);  HX0.HX0, (HX0-B).(HX0-B)
+// and phases 1 and 2 of the next iteration share the barrier intervals of 6
+// and 7, so that an iteration costs five grid barriers.
 // Every workgroup adds the G partial sums of a dot product in the same fixed
 // order, so all of them take the same branch of the recurrence; the
 // arithmetic of each vector value is the launch-per-step path's (the dot
@@ -336,9 +338,58 @@ __device__ void qp_px_slabs(QpW &W)
     }
 }
 
+// Q = Z/ZVRAT/ZNRM - c Q on the own slabs, then S = (modes 1, 2, diagonal) Q
+// (phase 1 of an iteration)
+__device__ void qp_phase_q(QpW &W, bool first, double zvrat, double znrm, double cq)
+{
+    const int g = W.g, G = W.G, n3 = W.n3, P = W.P, tid = threadIdx.x;
+    for (int sl = g; sl < n3; sl += G) {
+        const long o = (long)sl * P;
+        for (int e = tid; e < P; e += QP_NT) {
+            const double z = W.Z[o + e];
+            const double qn = first ? z / zvrat / znrm : z / zvrat / znrm - cq * W.Q[o + e];
+            W.Q[o + e] = qn;
+            W.L.sA[e] = qn;
+        }
+        __syncthreads();
+        qp_slab_hx12(W, o, W.Sv);
+        __syncthreads();
+    }
+}
+
+// S += DEP o (H3 Q), S = ZVRAT_NEXT S on the own pencils; returns the partial
+// Q.S (phase 2 of an iteration)
+__device__ double qp_phase_s(QpW &W, double zvrat_next)
+{
+    const int g = W.g, G = W.G, n3 = W.n3, P = W.P, pc = W.pc, nch = W.nch, tid = threadIdx.x;
+    double qs = 0.0;
+    for (int c = g; c < nch; c += G) {
+        const int p0 = c * pc, np = min(pc, P - p0);
+        qp_load_pencil(W, W.Q, p0, np);
+        __syncthreads();
+        const float rp = 1.0f / (float)np;
+        for (int e = tid; e < np * n3; e += QP_NT) {
+            const int i = fdiv_small(e, rp), pp = e - i * np;
+            const double acc = qp_pencil_dot(W, W.L.H3, false, pp, i);
+            const long idx = p0 + pp + (long)P * i;
+            const double sv = zvrat_next * (W.Sv[idx] + W.a->dep[p0 + pp] * acc);
+            W.Sv[idx] = sv;
+            qs += W.L.pen[pp + pc * i] * sv;
+        }
+        __syncthreads();
+    }
+    return qs;
+}
+
 // The QMR iterations from the state s (qmr.f90:56-175), vz_sum = the
 // (V/VNRM).Z of the step before.  status: QMR_CONVERGED or QMR_EXIT at the
 // end.  False when a barrier wait gave up.
+//
+// Phases 1 and 2 of iteration i+1 need nothing that phases 6 and 7 of
+// iteration i produce (only the convergence test does), so they run in the
+// same barrier intervals as 6 (slabs) and 7 (pencils): five barriers per
+// iteration.  When iteration i ends the solve, what they wrote (Q, S and a
+// partial sum) is scratch that nothing reads.
 __device__ bool qp_qmr_loop(QpW &W, QmrScal &s, double vz_sum, int &status)
 {
     const QpArgs &a = *W.a;
@@ -348,59 +399,26 @@ __device__ bool qp_qmr_loop(QpW &W, QmrScal &s, double vz_sum, int &status)
     double *X = W.X, *R = W.R, *UX = W.UX, *UR = W.UR, *Q = W.Q, *Sv = W.Sv, *V = W.V, *Z = W.Z, *HX0 = W.HX0;
     double *part = W.part;
     status = QMR_RUNNING;
+    // ---- iteration 1, phases 1 and 2: the exit and breakdown checks, Q, H Q
+    if (s.vnrm == 0.0 || s.znrm == 0.0) {
+        status = QMR_EXIT;
+        return true;
+    }
+    double vzdot = vz_sum / s.znrm;
+    if (vzdot == 0.0) {
+        status = QMR_EXIT;
+        return true;
+    }
+    double zvrat_next = s.zvrat * s.znrm / s.vnrm;
+    s.vzdot = vzdot;
+    s.zvrat_next = zvrat_next;
+    qp_phase_q(W, s.iter == 1, s.zvrat, s.znrm, s.iter == 1 ? 0.0 : s.vnrm * vzdot / s.qsdot);
+    if (!qp_sync(W)) return false;
+    qp_put(part, QS_QS, G, qp_phase_s(W, zvrat_next), L.red);
+    if (!qp_sync(W)) return false;
+
     for (;;) {
         const bool first = s.iter == 1;
-        // ---- phase 1 (slabs): exit and breakdown checks, Q, modes 1 + 2 of H Q
-        if (s.vnrm == 0.0 || s.znrm == 0.0) {
-            status = QMR_EXIT;
-            return true;
-        }
-        const double vzdot = vz_sum / s.znrm;
-        if (vzdot == 0.0) {
-            status = QMR_EXIT;
-            return true;
-        }
-        const double cq = first ? 0.0 : s.vnrm * vzdot / s.qsdot;
-        const double zvrat = s.zvrat, znrm = s.znrm;
-        const double zvrat_next = zvrat * znrm / s.vnrm;
-        s.vzdot = vzdot;
-        s.zvrat_next = zvrat_next;
-        for (int sl = g; sl < n3; sl += G) {
-            const long o = (long)sl * P;
-            for (int e = tid; e < P; e += QP_NT) {
-                const double z = Z[o + e];
-                const double qn = first ? z / zvrat / znrm : z / zvrat / znrm - cq * Q[o + e];
-                Q[o + e] = qn;
-                L.sA[e] = qn;
-            }
-            __syncthreads();
-            qp_slab_hx12(W, o, Sv);
-            __syncthreads();
-        }
-        if (!qp_sync(W)) return false;
-
-        // ---- phase 2 (pencils): mode 3 of H Q, S = ZVRAT S, Q.S
-        {
-            double qs = 0.0;
-            for (int c = g; c < nch; c += G) {
-                const int p0 = c * pc, np = min(pc, P - p0);
-                qp_load_pencil(W, Q, p0, np);
-                __syncthreads();
-                const float rp = 1.0f / (float)np;
-                for (int e = tid; e < np * n3; e += QP_NT) {
-                    const int i = fdiv_small(e, rp), pp = e - i * np;
-                    const double acc = qp_pencil_dot(W, L.H3, false, pp, i);
-                    const long idx = p0 + pp + (long)P * i;
-                    const double sv = zvrat_next * (Sv[idx] + a.dep[p0 + pp] * acc);
-                    Sv[idx] = sv;
-                    qs += L.pen[pp + pc * i] * sv;
-                }
-                __syncthreads();
-            }
-            qp_put(part, QS_QS, G, qs, L.red);
-        }
-        if (!qp_sync(W)) return false;
-
         // ---- phase 3 (pencils): BETA, V = S - BETA V/VNRM, V.V, Z = V3^T V
         const double qsdot = qp_grid_sum(part, QS_QS, G);
         const double beta = qsdot / vzdot;
@@ -464,7 +482,7 @@ __device__ bool qp_qmr_loop(QpW &W, QmrScal &s, double vz_sum, int &status)
         }
         if (!qp_sync(W)) return false;
 
-        // ---- phase 6 (slabs): GAMA, ETA, the UX / UR / X / R updates, modes 1 + 2 of H X
+        // ---- GAMA, ETA; the next iteration's checks and scalars
         const double znrm_next = sqrt(qp_grid_sum(part, QS_ZZ, G));
         const double vz_next = qp_grid_sum(part, QS_VZ, G);
         const double theta = vnrmpo / (s.gamamo * fabs(beta));
@@ -474,6 +492,13 @@ __device__ bool qp_qmr_loop(QpW &W, QmrScal &s, double vz_sum, int &status)
             status = QMR_EXIT;
             return true;
         }
+        const double vzdot_n = (vnrmpo == 0.0 || znrm_next == 0.0) ? 0.0 : vz_next / znrm_next;
+        const bool next_ok = vzdot_n != 0.0;   // VNRM, ZNRM and VZDOT of iteration i+1 nonzero
+        const double zvrat_next_n = next_ok ? zvrat_next * znrm_next / vnrmpo : 0.0;
+        const double cq_n = next_ok ? vnrmpo * vzdot_n / qsdot : 0.0;
+
+        // ---- phase 6 (slabs): the UX / UR / X / R updates, modes 1 + 2 of H X;
+        // then phase 1 of iteration i+1
         {
             const double ae = eta * zvrat_next;
             const double tg = (s.thetamo * gama) * (s.thetamo * gama);
@@ -505,35 +530,41 @@ __device__ bool qp_qmr_loop(QpW &W, QmrScal &s, double vz_sum, int &status)
                 qp_put(part, QS_XX, G, xx, L.red);
                 qp_put(part, QS_UR, G, urur, L.red);
             }
+            // (Q and S of iteration i are read above, before this overwrites them)
+            if (next_ok) qp_phase_q(W, false, zvrat_next, znrm_next, cq_n);
         }
         if (!qp_sync(W)) return false;
 
-        // ---- phase 7 (pencils, conv type 1): mode 3 of H X, HX0.HX0, (HX0-B).(HX0-B)
-        if (conv == 1) {
-            double hh = 0.0, r1 = 0.0;
-            for (int c = g; c < nch; c += G) {
-                const int p0 = c * pc, np = min(pc, P - p0);
-                qp_load_pencil(W, X, p0, np);
-                __syncthreads();
-                const float rp = 1.0f / (float)np;
-                for (int e = tid; e < np * n3; e += QP_NT) {
-                    const int i = fdiv_small(e, rp), pp = e - i * np;
-                    const double acc = qp_pencil_dot(W, L.H3, false, pp, i);
-                    const long idx = p0 + pp + (long)P * i;
-                    const double hv = HX0[idx] + a.dep[p0 + pp] * acc;
-                    HX0[idx] = hv;
-                    const double r = hv - W.B[idx];
-                    hh += hv * hv;
-                    r1 += r * r;
+        // ---- phase 7 (pencils, conv type 1): mode 3 of H X, HX0.HX0,
+        // (HX0-B).(HX0-B); then phase 2 of iteration i+1
+        {
+            if (conv == 1) {
+                double hh = 0.0, r1 = 0.0;
+                for (int c = g; c < nch; c += G) {
+                    const int p0 = c * pc, np = min(pc, P - p0);
+                    qp_load_pencil(W, X, p0, np);
+                    __syncthreads();
+                    const float rp = 1.0f / (float)np;
+                    for (int e = tid; e < np * n3; e += QP_NT) {
+                        const int i = fdiv_small(e, rp), pp = e - i * np;
+                        const double acc = qp_pencil_dot(W, L.H3, false, pp, i);
+                        const long idx = p0 + pp + (long)P * i;
+                        const double hv = HX0[idx] + a.dep[p0 + pp] * acc;
+                        HX0[idx] = hv;
+                        const double r = hv - W.B[idx];
+                        hh += hv * hv;
+                        r1 += r * r;
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
+                qp_put(part, QS_HH, G, hh, L.red);
+                qp_put(part, QS_R1, G, r1, L.red);
             }
-            qp_put(part, QS_HH, G, hh, L.red);
-            qp_put(part, QS_R1, G, r1, L.red);
-            if (!qp_sync(W)) return false;
+            if (next_ok) qp_put(part, QS_QS, G, qp_phase_s(W, zvrat_next_n), L.red);
         }
+        if (!qp_sync(W)) return false;
 
-        // ---- the end of the iteration (every workgroup alike)
+        // ---- the end of iteration i (every workgroup alike)
         double res;
         if (conv == 1) {
             const double xnrm = sqrt(qp_grid_sum(part, QS_HH, G));
@@ -571,6 +602,15 @@ __device__ bool qp_qmr_loop(QpW &W, QmrScal &s, double vz_sum, int &status)
             status = QMR_EXIT;
             return true;
         }
+        // the checks at the top of iteration i+1
+        if (!next_ok) {
+            status = QMR_EXIT;
+            return true;
+        }
+        vzdot = vzdot_n;
+        zvrat_next = zvrat_next_n;
+        s.vzdot = vzdot;
+        s.zvrat_next = zvrat_next;
     }
 }
 
@@ -691,16 +731,28 @@ __device__ __forceinline__ long qp_elem(const QpW &W, int p0, float rp, int np, 
 __device__ void qp_gs_dots(const QpW &W, const double *vjv, long ldv, int j, bool gj, const double *wl,
                            const double *vl, int p0, int np, double *op, int ldg)
 {
+    constexpr int K = 16;   // loads in flight per lane
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, ne = np * W.n3;
     const float rp = 1.0f / (float)(np > 0 ? np : 1);
     for (int i = wave; i < j; i += QP_NW) {
         const double *vi = vjv + (long)i * ldv;
         const bool wantg = gj && i < j - 1;
         double ac = 0.0, ag = 0.0;
-        for (int t = lane; t < ne; t += 64) {
-            const double x = vi[qp_elem(W, p0, rp, np, t)];
-            ac += x * wl[t];
-            if (wantg) ag += x * vl[t];
+        for (int t0 = 0; t0 < ne; t0 += 64 * K) {
+            double x[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int t = t0 + lane + 64 * k;
+                x[k] = t < ne ? vi[qp_elem(W, p0, rp, np, t)] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int t = t0 + lane + 64 * k;
+                if (t < ne) {
+                    ac += x[k] * wl[t];
+                    if (wantg) ag += x[k] * vl[t];
+                }
+            }
         }
         ac = qp_wave_sum(ac);
         ag = qp_wave_sum(ag);
@@ -799,7 +851,7 @@ __device__ bool qp_gs(QpW &W, const PpArgs &pa, int j, int &ok)
     for (int t = tid; t < ne; t += QP_NT) {
         const long e = qp_elem(W, p0, rp, np, t);
         double acc = wl[t];
-#pragma unroll 8
+#pragma unroll 16
         for (int i = 0; i < j; i++) acc -= hs[i] * vjv[(long)i * pa.ldv + e];
         wl[t] = acc;
         nn += acc * acc;
@@ -831,7 +883,7 @@ __device__ bool qp_gs(QpW &W, const PpArgs &pa, int j, int &ok)
     for (int t = tid; t < ne; t += QP_NT) {
         const long e = qp_elem(W, p0, rp, np, t);
         double acc = wl[t] / nrm1;
-#pragma unroll 8
+#pragma unroll 16
         for (int i = 0; i < j; i++) acc -= hs[i] * vjv[(long)i * pa.ldv + e];
         wl[t] = acc;
         nn += acc * acc;
