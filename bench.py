@@ -57,6 +57,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pist", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--config4", choices=["a", "ab", "abc"], default="a",
+                    help="grids of the uniform-window layout: 48x64x40 (a), 80x96x64 (b), 128x128x96 (c); "
+                         "the default runs the first (about 10 s), abc takes minutes")
     ap.add_argument("--solver-run", action="store_true",
                     help="time the solver path on the 64x96x80 operator (OSB set-up, P*X at batch 1 and 16, "
                          "QMR) and print one line per measurement; prints no JSON")
@@ -185,6 +188,7 @@ def main():
     if not args.no_pist:
         out["pist"] = pist_bench(args.no_cpu)
         out["gradients_64x96x80"] = gradients_bench(ctx, prob)
+        out["uniform_windows"] = config4_bench(["batched_" + c for c in args.config4])
     ctx.close()
     print(json.dumps(out))
 
@@ -320,6 +324,41 @@ def gradients_bench(ctx, prob):
     return out
 
 
+def config4_bench(names):
+    """Batched PIST windows on the configured grids: 32 centres spread
+    uniformly over [0, 0.05] hartree, 4 levels each, FP64 (ETOL 1e-10, QMR
+    1e-6, at most 3000 QMR iterations per solve), OSB set-up outside the
+    timed region.  Windows whose QMR solves do not converge return the
+    reference's failure codes and are counted, not hidden."""
+    import torch
+
+    from scalit_amd import problems as P
+    from scalit_amd.context import Context
+
+    out = {}
+    cent = [0.05 * (k + 0.5) / 32 for k in range(32)]
+    for name in names:
+        prob = P.config_problem(name)
+        ctx = Context.from_problem(prob, e0=0.025)
+        ctx.block_diag(1000, 1e-5)
+        x0 = ctx.to_device(P.random_vectors(prob.N, 32, 5))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ret, eig, rr, st = ctx.pist_conv_batched(x0, cent, 1e-10, 4, 120, 12, 6, 2, qmrtol=1e-6, qmrmax=3000)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ok = ret > 0
+        lv = np.sort(eig[ok].reshape(-1))
+        dup = int(np.sum(np.diff(lv) <= 1e-9 * np.maximum(1.0, np.abs(lv[1:])))) if lv.size else 0
+        out[name] = {"grid": list(prob.sN), "points": int(prob.N), "windows": 32, "levels_per_window": 4,
+                     "windows_converged": int(ok.sum()), "highest_converged_centre": float(max(
+                         [cent[k] for k in range(32) if ok[k]], default=0.0)),
+                     "levels_found": int(lv.size), "duplicated_levels": dup, "seconds": round(dt, 3),
+                     "eigenpairs_per_s": round(lv.size / dt, 3), "qmr_iters": int(st[0]), "krylov_max": int(st[2])}
+        ctx.close()
+    return out
+
+
 def window_centres(levels):
     """One window per level, centred a quarter of the smaller neighbouring gap
     inside the level's interval, so that its nearest level is that level."""
@@ -360,7 +399,25 @@ def pist_bench(no_cpu):
             eig_fp64 = eig.copy()
         err = float((np.abs(eig - ref) / np.abs(ref)).max())
         res[name] = {"ret": int(r), "seconds": round(dt, 3), "eigenpairs_per_s": round(20 / dt, 3),
-                     "max_rel_err": err, "qmr_iters": int(stats[0]), "krylov": int(stats[2])}
+                     "max_rel_err": err, "qmr_iters": int(stats[0]), "krylov": int(stats[2]),
+                     "us_per_qmr_iteration": round(1e6 * dt / max(int(stats[0]), 1), 1)}
+    res["fp64"]["path"] = "persistent (the Krylov steps of each block in one launch)"
+    # the same solve with one kernel per QMR step (graph replay) and the
+    # reference's MGS_ORTH one projection at a time
+    ctx.set_qmr_persistent(0)
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r, eig, rr, stats = ctx.pist_conv(x0, g["e0"], 1e-11, 20, 200, 40, 10, 5, qmrtol=1e-6)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    ctx.set_qmr_persistent(-1)
+    res["fp64_per_step"] = {"ret": int(r), "seconds": round(dt, 3), "eigenpairs_per_s": round(20 / dt, 3),
+                            "max_rel_err": float((np.abs(eig - ref) / np.abs(ref)).max()),
+                            "qmr_iters": int(stats[0]), "krylov": int(stats[2]),
+                            "us_per_qmr_iteration": round(1e6 * dt / max(int(stats[0]), 1), 1),
+                            "path": "one kernel per QMR step, HIP graph replay"}
+    res["persistent_speedup"] = round(res["fp64_per_step"]["seconds"] / res["fp64"]["seconds"], 3)
     # Continuation along a potential-parameter scan: the levels of the plain
     # FP64 solve above (this potential) place one window per level for the
     # next scan point, the potential with its 6 Morse parameters moved by a
