@@ -68,7 +68,7 @@ def main():
                          "baseline) and print them as one JSON line, for profiling that path")
     ap.add_argument("--pmc-run", action="store_true",
                     help="workload for rocprofv3 --pmc: three 512 MiB device copies (the counter calibration), "
-                         "then --steps H*X calls; prints no JSON")
+                         "then --steps H*X calls, the OSB set-up and --steps P*X calls; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the H*X / PIST path here is built and timed on a single MI355X "
@@ -99,8 +99,13 @@ def main():
         for _ in range(args.steps):
             ctx.hx(x, out=y)
         torch.cuda.synchronize()
+        # then the OSB set-up and P*X (TOSB) of the same batch
+        ctx.block_diag(1000, 1e-5)
+        for _ in range(args.steps):
+            ctx.px(x, out=y)
+        torch.cuda.synchronize()
         ctx.close()
-        print(f"pmc run done: 3 copies of 512 MiB, {args.steps} H*X of batch {B}", flush=True)
+        print(f"pmc run done: 3 copies of 512 MiB, {args.steps} H*X and {args.steps} P*X of batch {B}", flush=True)
         return
     if args.solver_run:
         solver_run(args.steps)

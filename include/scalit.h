@@ -92,7 +92,8 @@ int scalit_set_osb(scalit_ctx *ctx, int nosb);
 /* No counterpart in the reference (its QMR loop, qmr.f90:56-175, calls the
  * operators once per step).  FP64 QMR solves on a fused 3-level operator
  * (sF = 3, DEP on level 3 only, levels of at most 48 points, at most 65,536
- * points) run every iteration in one persistent launch when on = 1, one
+ * points, n1 n2 <= 1,536) run every iteration in one persistent launch when
+ * on = 1 (and PIST runs its Krylov steps block by block the same way), one
  * kernel per step of the iteration when on = 0; -1 (the default) follows the
  * environment variable SCALIT_QMR_PERSIST (unset: 1).  Same results to
  * rounding: only the order of the terms of the dot products differs. */
@@ -312,9 +313,10 @@ int scalit_pist_conv_multi(int njob, scalit_pist_job *jobs, double etol, int nty
  * potential at x.  Keeps the m lowest eigenvectors of T + diag(v) and returns
  * the m new grid points x0 (ascending: the eigenvalues of the position
  * operator in that basis), the m x m Hamiltonian h0 = W^T diag(E) W on them
- * (each column of W with its largest-magnitude entry positive: the reference
- * leaves these signs to DSYEV, and every choice gives an equivalent
- * operator) and, when e != NULL, the m eigenvalues E.  n <= 1024. */
+ * (each column of the eigenvectors U of T + diag(v), and then of W, with its
+ * largest-magnitude entry positive: the reference leaves these signs to
+ * DSYEV, and every choice gives an equivalent operator) and, when e != NULL,
+ * the m eigenvalues E.  n <= 1024. */
 int scalit_podvr(int n, const double *x, const double *t, const double *v, int m, double *x0, double *h0, double *e);
 /* saveH0CBALL (src/ham/ja3/module/ja3.h0:154-199): the energy-cut combined
  * layer of two coordinates.  Keeps the nkeep product points (i, j) with the

@@ -939,15 +939,18 @@ __global__ void __launch_bounds__(QP_NT) k_pist_persist(PpArgs pa)
 
 // ---------------------------------------------------------------- host side
 
-// Whether the persistent solvers can run this operator: the fused 3-level
+// Whether the persistent solvers take this operator: the fused 3-level
 // layout (sF = 3, DEP on level 3 only, no sNDVR layer) with its
-// preconditioner, levels of at most QP_NMAX points, at most 65,536 points.
+// preconditioner, levels of at most QP_NMAX points, at most 65,536 points and
+// theta planes of at most 1,536 points.  Past that a plane's phase is longer than
+// the launches it saves (measured: 81 against 113 us per QMR iteration at
+// 32x40x24, 151 against 121 at 40x48x32).
 bool qmr_persist_ok(const scalit_ctx *c)
 {
     if (!c->fast3 || c->ndvr || c->sF != 3 || !c->has_precond || !c->has_res) return false;
     for (int l = 0; l < 3; l++)
         if (c->lv[l].n > QP_NMAX || c->lv[l].dep != (l == 2) || !c->lv[l].vosb.p) return false;
-    return c->N <= 65536;
+    return c->N <= 65536 && (long)c->lv[0].n * c->lv[1].n <= 1536;
 }
 
 static int env_or(const char *name, int def)
