@@ -244,6 +244,29 @@ def solver_run(reps):
         it = int(np.abs(ret).max())
         print(f"batch {nv}: QMR to 1e-8 in {it} iterations, {dt * 1e3:.2f} ms ({dt * 1e3 / max(it, 1):.3f} ms per "
               f"iteration), residuals <= {float(np.max(res)):.1e}", flush=True)
+    # the windowed kernel TOSBW: the 64 / 256 OSB states nearest E0, batch 16
+    from scalit_amd import _lib as L
+
+    x = ctx.to_device(P.random_vectors(prob.N, 16, 3))
+    y = torch.empty_like(x)
+    b = ctx.to_device(P.random_vectors(prob.N, 16, 5))
+    for cnt in (64, 256):
+        t0 = time.perf_counter()
+        ctx.set_osbw(e0, 1e-4, cnt)
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        tp = timed(lambda: ctx.px(x, nosb=L.TOSBW, out=y), reps)
+        ctx.set_osb(L.TOSBW)
+        for rep in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ret, xs, res = ctx.qmr(b, ntype=-1, qmrmax=1000, qmrtol=1e-8)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        ctx.set_osb(L.TOSB)
+        it = int(np.abs(ret).max())
+        print(f"TOSBW window {cnt}: set-up {ts * 1e3:.1f} ms, batch 16 P*X {tp:.4f} ms, QMR to 1e-8 in {it} "
+              f"iterations, {dt * 1e3:.2f} ms, residuals <= {float(np.max(res)):.1e}", flush=True)
     ctx.close()
 
 

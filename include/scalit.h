@@ -46,6 +46,7 @@ extern "C" {
 #define SCALIT_TOSB 0
 #define SCALIT_TOSBD1 1
 #define SCALIT_TOSBD2 2
+#define SCALIT_TOSBW 3   /* the windowed preconditioner; needs scalit_set_osbw */
 /* arithmetic of an operator application */
 #define SCALIT_FP64 0
 #define SCALIT_FP32 1
@@ -83,12 +84,27 @@ int scalit_set_res(scalit_ctx *ctx, const double *res);
 int scalit_set_outh(scalit_ctx *ctx, const double *outh);
 /* sOSBW%mE0 / mDE / mBeta (src/iterate/module/a_osbtype.f90:26-29) */
 int scalit_set_shift(scalit_ctx *ctx, double e0, double de, double beta);
-/* sOSB, the preconditioner kernel of the solvers' P*X (TOSB, TOSBD1 or
- * TOSBD2; the input read at src/iterate/module/mosb.print.h:245, used by
- * EPX at src/iterate/module/mosb.px.h:40-46): QMR, PIST and the mixed-precision
- * inner solves then divide by that kernel, with DE and beta from
- * scalit_set_shift.  Default TOSB. */
+/* sOSB, the preconditioner kernel of the solvers' P*X (TOSB, TOSBD1,
+ * TOSBD2 or TOSBW; the input read at src/iterate/module/mosb.print.h:245,
+ * used by EPX at src/iterate/module/mosb.px.h:40-46): QMR, PIST and the
+ * mixed-precision inner solves then divide by that kernel, with DE and beta
+ * from scalit_set_shift.  Default TOSB.  TOSBW is FP64 only (the FP32 and
+ * BF16 inner solves return SCALIT_EINVAL) and takes the per-step QMR. */
 int scalit_set_osb(scalit_ctx *ctx, int nosb);
+/* initOSBW (src/iterate/module/osb.osbw.h:2-50) for the TOSBW kernel
+ * (src/iterate/module/osb.px.h:191-206): adjustWinSize (osb.osbw.h:117-145)
+ * widens or narrows the window from de until exactly cnt OSB states have
+ * |EIG0 - e0| <= de (at most 10,000 steps, else SCALIT_ESTATE), getOSBWIndex
+ * (osb.osbw.h:79-95) takes them in grid order, getVOSBH0
+ * (src/iterate/module/osb.hij.h:133-160) forms H_W(i,j) = v_i . H v_j with
+ * EIG0 on the diagonal, and createWH (src/iterate/module/osb.pij.h:6-40)
+ * subtracts e0 from the diagonal when npc is SCALIT_TA1.  P*X with TOSBW then
+ * divides the rotated vector by EIG0 - shift and replaces the window's
+ * components by (H_W - e0)^-1 times them.  cnt 1..1024; needs the
+ * preconditioner (scalit_block_diag or scalit_set_precond, which drop the
+ * window again) and RES; not for sNDVR operators.  *hwlen gets the window's
+ * size and *de_out the width found (either may be NULL). */
+int scalit_set_osbw(scalit_ctx *ctx, int npc, double e0, double de, int cnt, int *hwlen, double *de_out);
 /* No counterpart in the reference (its QMR loop, qmr.f90:56-175, calls the
  * operators once per step).  FP64 QMR solves on a fused 3-level operator
  * (sF = 3, DEP on level 3 only, levels of at most 48 points, at most 65,536
@@ -154,7 +170,9 @@ int scalit_qmr_mixed(scalit_ctx *ctx, int prec, int qmrmax, double qmrtol, int i
 /* PIST_CONV (src/iterate/libiterate/pist/pist.f90:7-98) with pistH0
  * (pistHij.f90:328-383), LinSolv = OSB QMR (src/iterate/module/osbw.qmr.h:13),
  * H0X = H (TA0).  x is the device start vector; eig (host, m entries)
- * receives the eigenvalues ascending.  The return value is the reference's.
+ * receives the eigenvalues ascending.  The return value is the reference's
+ * (0 for its parameter errors), or a negative SCALIT_E* code when the solve
+ * cannot run (e.g. an inner precision that the preconditioner kernel refuses).
  * prec selects the inner solve (SCALIT_FP64 = plain QMR, FP32/BF16 = mixed
  * with fp64 refinement); stats (host, 4 ints, may be NULL): total QMR
  * iterations, linear solves, final Krylov size, H*X applications. */

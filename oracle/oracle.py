@@ -23,7 +23,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libscalit_ref.so")
 
 FMAX = 8
 TA0, TA1 = 0, 1
-TOSB, TOSBD1, TOSBD2 = 0, 1, 2
+TOSB, TOSBD1, TOSBD2, TOSBW = 0, 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -433,6 +433,23 @@ class RefOperator:
             ref().refdrv_px_kind(C.c_int(kind), C.c_int(npc), dptr(x), dptr(y))
         return y
 
+    def osbw(self, cnt, e0, de):
+        """Module osb's window set-up (adjustWinSize, getOSBWIndex,
+        getVOSBH0) after diag(): (0-based grid indices, width, H_W)."""
+        d = C.c_double(de)
+        n = C.c_int(0)
+        ind = np.zeros(cnt, dtype=np.int32)
+        hw = np.zeros((cnt, cnt), order="F")
+        ref().refop_osbw(C.c_int(cnt), C.c_double(e0), C.byref(d), C.byref(n), iptr(ind), dptr(hw))
+        assert n.value == cnt
+        return ind - 1, d.value, hw
+
+    def osb_vector(self, k):
+        """Module osb's getVi for the 0-based grid index k."""
+        v = np.zeros(self.op.N)
+        ref().refop_vi(C.c_int(k + 1), dptr(v))
+        return v
+
     def qmr(self, b, qmr_type=-1, qmr_max=1000, qmr_tol=1e-3):
         R = ref()
         b = f64(b)
@@ -505,3 +522,78 @@ def combine_2d(h1, v1, h2, v2, nkeep):
     h1, h2 = np.asarray(h1), np.asarray(h2)
     hc = (j[:, None] == j[None, :]) * h1[i[:, None], i[None, :]] + (i[:, None] == i[None, :]) * h2[j[:, None], j[None, :]]
     return i.astype(np.int32), j.astype(np.int32), np.asfortranarray(hc)
+
+
+# --------------------------------------------------------------------------
+# the windowed preconditioner TOSBW (the checker for scalit_set_osbw and the
+# TOSBW kernel of scalit_px)
+# --------------------------------------------------------------------------
+def adjust_win_size(eig0, e0: float, cnt: int, de: float) -> float:
+    """adjustWinSize (src/iterate/module/osb.osbw.h:117-145): double the
+    width while fewer than cnt OSB states have |EIG0 - e0| <= width, halve
+    towards the last smaller width while more do, until exactly cnt do."""
+    d = np.abs(f64(eig0) - e0)
+    de1 = 0.0
+    k = int((d <= de).sum())
+    it = 0
+    while k != cnt:
+        it += 1
+        if it > 10000:
+            raise RuntimeError("adjustWinSize does not settle (ties in EIG0)")
+        if k < cnt:
+            de1, de = de, 2.0 * de
+        else:
+            de = 0.5 * (de1 + de)
+        k = int((d <= de).sum())
+    return de
+
+
+def osbw_index(eig0, e0: float, de: float, cnt: int) -> np.ndarray:
+    """getOSBWIndex (src/iterate/module/osb.osbw.h:79-95), 0-based."""
+    return np.nonzero(np.abs(f64(eig0) - e0) <= de)[0][:cnt]
+
+
+def osb_vector(op: "Operator", k: int) -> np.ndarray:
+    """getLevelIndex + getLevelVi (src/iterate/module/osb.vi.h:11-73): the
+    OSB state k (0-based, level 1 fastest) is the tensor product of column
+    c_l of level l's block b_l, c_l the state's level-l digit and b_l the
+    number formed by its digits above level l."""
+    v = np.ones(1)
+    stride = 1
+    for l in range(op.sF):
+        n = op.sN[l]
+        c = (k // stride) % n
+        stride *= n
+        b = k // stride
+        blk = op.VOSB[l][b * n * n:(b + 1) * n * n].reshape(n, n, order="F")
+        v = np.kron(blk[:, c], v)
+    return v
+
+
+def osbw_setup(op: "Operator", e0: float, de: float, cnt: int, npc: int = TA1):
+    """initOSBW (src/iterate/module/osb.osbw.h:2-50) with getVOSBH0
+    (src/iterate/module/osb.hij.h:133-160) and createWH
+    (src/iterate/module/osb.pij.h:6-40): returns (indices, width, H_W, HWR,
+    window vectors), HWR = H_W less e0 on the diagonal under TA1."""
+    de = adjust_win_size(op.EIG0, e0, cnt, de)
+    ind = osbw_index(op.EIG0, e0, de, cnt)
+    vw = np.stack([osb_vector(op, int(k)) for k in ind], axis=1)
+    hv = np.stack([op.hx(vw[:, j], TA0) for j in range(cnt)], axis=1)
+    hw = vw.T @ hv
+    hw = 0.5 * (hw + hw.T)
+    hw[np.diag_indices(cnt)] = op.EIG0[ind]
+    hwr = hw - (e0 if npc == TA1 else 0.0) * np.eye(cnt)
+    return ind, de, hw, hwr, vw
+
+
+def px_tosbw(op: "Operator", x, win, npc: int = TA1) -> np.ndarray:
+    """MYPX with nOSB = TOSBW (src/iterate/module/osb.px.h:191-206): the
+    rotated vector divided by EIG0 - E0 except on the window, whose
+    components are replaced by HWR^-1 times them.  With z = V^T x this is
+    P_TOSB x + V_W (HWR^-1 z_W - z_W / (EIG0_W - E0)), z_W = V_W^T x."""
+    ind, _, _, hwr, vw = win
+    x = f64(x)
+    e0 = op.s.E0 if npc == TA1 else 0.0
+    zw = vw.T @ x
+    corr = np.linalg.solve(hwr, zw) - zw / (op.EIG0[ind] - e0)
+    return op.px(x, npc=npc, kind=TOSB) + vw @ corr

@@ -77,3 +77,43 @@ subroutine refop_px(nosbk, npc, x, y) bind(C, name='refop_px')
   real(c_double), intent(out) :: y(*)
   call MYPX(nosbk, npc, myLen, x, y)
 end subroutine
+
+! The window set-up of TOSBW as the module does it (initOSBW,
+! src/iterate/module/osb.osbw.h:2-50, without the LAPACK factorisation):
+! adjustWinSize, getOSBWSize, getOSBWIndex and getVOSBH0
+! (src/iterate/module/osb.hij.h:133-160).  ind gets 1-based grid indices,
+! hw the window's H (EIG0 on the diagonal); both sized by the caller for cnt.
+subroutine refop_osbw(cnt, e0, de, wlen, wix, hmat) bind(C, name='refop_osbw')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: cnt
+  real(c_double), value :: e0
+  real(c_double), intent(inout) :: de
+  integer(c_int), intent(out) :: wlen
+  integer(c_int), intent(out) :: wix(cnt)
+  real(c_double), intent(out) :: hmat(cnt, cnt)
+  integer(kind=8) :: n8
+  call adjustWinSize(cnt, e0, de)
+  n8 = getOSBWSize(e0, de)
+  wlen = int(n8)
+  if (n8 /= cnt) return
+  call getOSBWIndex(e0, de, n8, wix)
+  call getVOSBH0(n8, wix, hmat)
+end subroutine
+
+! One OSB state's vector (getVi, src/iterate/module/osb.vi.h:74-80) for the
+! 1-based grid index k.
+subroutine refop_vi(k, vi) bind(C, name='refop_vi')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: k
+  real(c_double), intent(out) :: vi(*)
+  integer(kind=8) :: one
+  integer :: kix(1), bk(sF, 1), col(sF, 1)
+  one = 1
+  kix(1) = k
+  call getLevelIndex(one, kix, bk, col)
+  call getVi(bk(1:sF, 1), col(1:sF, 1), vi)
+end subroutine

@@ -18,7 +18,7 @@ SCALIT_ENOMEM = -1003
 SCALIT_ESTATE = -1004
 SCALIT_EIO = -1005
 TA0, TA1 = 0, 1
-TOSB, TOSBD1, TOSBD2 = 0, 1, 2
+TOSB, TOSBD1, TOSBD2, TOSBW = 0, 1, 2, 3
 FP64, FP32, BF16 = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
@@ -70,6 +70,7 @@ SIGNATURES = {
     "scalit_podvr": (_i, [_i, _dp, _dp, _dp, _i, _dp, _dp, _dp]),
     "scalit_set_pist_orth": (_i, [_vp, _i]),
     "scalit_set_osb": (_i, [_vp, _i]),
+    "scalit_set_osbw": (_i, [_vp, _i, _d, _d, _i, _vp, _vp]),
     "scalit_set_qmr_persistent": (_i, [_vp, _i]),
     "scalit_load_data_grid": (_i, [C.c_char_p, _l, _i, _dp]),
     "scalit_save_data_grid": (_i, [C.c_char_p, _l, _i, _dp]),

@@ -376,8 +376,16 @@ class Context:
 
     def set_osb(self, nosb: int):
         """sOSB: the preconditioner kernel of QMR / PIST (TOSB, TOSBD1,
-        TOSBD2), with DE and beta from set_shift."""
+        TOSBD2, or TOSBW after set_osbw), with DE and beta from set_shift."""
         check(L.lib().scalit_set_osb(self._h, nosb), "scalit_set_osb")
+
+    def set_osbw(self, e0: float, de: float, cnt: int, npc: int = L.TA1):
+        """initOSBW (src/iterate/module/osb.osbw.h:2-50): the window of the
+        cnt OSB states nearest e0 (adjustWinSize from width de) for the TOSBW
+        kernel; returns (window size, width found)."""
+        n, d = C.c_int(), C.c_double()
+        check(L.lib().scalit_set_osbw(self._h, npc, e0, de, cnt, C.byref(n), C.byref(d)), "scalit_set_osbw")
+        return n.value, d.value
 
     def pist_vectors(self, win: int, m: int):
         psi = torch.empty((m, self.N), dtype=torch.float64, device=self.dev)

@@ -149,6 +149,7 @@ static void drop_f32(scalit_ctx *c)
     c->res_f.release();
     c->eig0_f.release();
     c->outh_f.release();
+    c->osbw_n = 0;   // a TOSBW window belongs to the operator it was built on
     c->gen++;
 }
 
@@ -332,7 +333,7 @@ int scalit_set_qmr_persistent(scalit_ctx *c, int on)
 
 int scalit_set_osb(scalit_ctx *c, int nosb)
 {
-    if (!c || (nosb != SCALIT_TOSB && nosb != SCALIT_TOSBD1 && nosb != SCALIT_TOSBD2)) return SCALIT_EINVAL;
+    if (!c || nosb < SCALIT_TOSB || nosb > SCALIT_TOSBW) return SCALIT_EINVAL;
     c->osb = nosb;
     return SCALIT_OK;
 }
@@ -376,6 +377,7 @@ int scalit_set_precond(scalit_ctx *c, const double *const *vosb, const double *e
     HIP_TRY(hipMemcpy(c->eig0.p, eig0, sizeof(double) * c->N, hipMemcpyHostToDevice));
     c->eig0_f.release();
     c->has_precond = true;
+    c->osbw_n = 0;
     c->gen++;
     return SCALIT_OK;
 }
@@ -442,7 +444,7 @@ int scalit_px(scalit_ctx *c, int nosb, int npc, int nvec, const double *x, long 
     CTX_GUARD(c);
     if (!c->has_precond) return SCALIT_ESTATE;
     if (nvec < 1 || !x || !y || ldx < c->N || ldy < c->N || x == y) return SCALIT_EINVAL;
-    if (nosb != SCALIT_TOSB && nosb != SCALIT_TOSBD1 && nosb != SCALIT_TOSBD2) return SCALIT_EINVAL;
+    if (nosb < SCALIT_TOSB || nosb > SCALIT_TOSBW) return SCALIT_EINVAL;
     double e0 = (npc == SCALIT_TA0) ? 0.0 : c->e0;
     return px_apply_f64(c, nosb, shifts, e0, nvec, x, ldx, y, ldy, nullptr, pick(c, stream));
 }
@@ -497,7 +499,7 @@ int scalit_pist_conv(scalit_ctx *c, double e0, double etol, int ntype, const dou
     double r = 0;
     int rc = scalit_pist_conv_batched(c, 1, &e0, etol, ntype, x, c ? c->N : 0, startcnt, stepcnt, stepeig, m, m_max,
                                       qmr_type, qmrmax, qmrtol, prec, eig, &ret, &r, stats);
-    if (rc != SCALIT_OK) return rc < 0 && rc <= SCALIT_EINVAL ? 0 : rc;
+    if (rc != SCALIT_OK) return rc;   // the reference's parameter errors come back as ret = 0
     if (res) *res = r;
     return ret;
 }

@@ -570,6 +570,7 @@ int block_diag(scalit_ctx *c, int bjmax, double bjtol)
         }
     }
     c->has_precond = true;
+    c->osbw_n = 0;
     // fp32 copies are rebuilt on demand
     for (int l = 0; l < c->sF; l++) {
         c->lv[l].vosb_f.release();

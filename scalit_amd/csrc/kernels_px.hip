@@ -458,16 +458,29 @@ static int px_impl(scalit_ctx *c, int nosb, const T *const *vosb, const T *const
     const int sF = c->sF;
     const bool big = l1_big<T>(c->lv[0].n);
     if (big && (long)BR_VB * c->lv[0].n * sizeof(T) > PX_LDS_MAX) return SCALIT_EINVAL;
+    const bool win = nosb == SCALIT_TOSBW;
+    if (win && (!std::is_same<T, double>::value || !std::is_same<S, double>::value || c->osbw_n <= 0 || c->ndvr))
+        return c->osbw_n <= 0 ? SCALIT_ESTATE : SCALIT_EINVAL;
     T *ztmp = work + 2L * nvec * N;   // level1_big's intermediate (work holds 3 nvec N then)
+    // TOSBW: the TOSB divide by EIG0 with +inf at the window, the window's
+    // components solved beside it
+    const T *eig1 = eig0;
+    if constexpr (std::is_same<T, double>::value)
+        if (win) eig1 = c->osbw_eig0.p;
+    const int kern1 = win ? SCALIT_TOSB : nosb;
     auto lev1 = [&](const auto *xin, long ldi, auto *yout, long ldo) {
         using SI = std::remove_const_t<std::remove_pointer_t<decltype(xin)>>;
         using SO = std::remove_pointer_t<decltype(yout)>;
+        if constexpr (std::is_same<SI, double>::value && std::is_same<SO, double>::value)
+            if (win) osbw_gather(c, nvec, xin, ldi, active, s);
         if (big)
-            level1_big<T, SI, SO>(c->lv[0], vosb[0], vosbt[0], eig0, shifts, shift0, nosb, de, beta, nvec, xin, ldi,
+            level1_big<T, SI, SO>(c->lv[0], vosb[0], vosbt[0], eig1, shifts, shift0, kern1, de, beta, nvec, xin, ldi,
                                   yout, ldo, ztmp, N, active, s);
         else
-            level1<T, SI, SO>(c->lv[0], vosb[0], eig0, shifts, shift0, nosb, de, beta, nvec, xin, ldi, yout, ldo,
+            level1<T, SI, SO>(c->lv[0], vosb[0], eig1, shifts, shift0, kern1, de, beta, nvec, xin, ldi, yout, ldo,
                               active, s);
+        if constexpr (std::is_same<SI, double>::value && std::is_same<SO, double>::value)
+            if (win) osbw_scatter(c, nvec, yout, ldo, active, s);
     };
     if (sF == 1) {
         // a single level: input and output are the caller's vectors
@@ -537,6 +550,11 @@ int px_apply_f64(scalit_ctx *c, int nosb, const double *shifts_dev, double shift
         double *old = c->work.p;
         SC_TRY(c->work.alloc((size_t)(l1_big<double>(c->lv[0].n) ? 3 : 2) * nvec * c->N));
         if (c->work.p != old) c->gen++;
+    }
+    if (nosb == SCALIT_TOSBW && c->osbw_n > 0) {
+        double *old = c->osbw_d.p;
+        SC_TRY(c->osbw_d.alloc((size_t)nvec * c->osbw_n));
+        if (c->osbw_d.p != old) c->gen++;
     }
     SC_TRY(ensure_transposes<double>(c, &Level::vosb, &Level::vosbt, s));
     const double *vosb[SC_FMAX], *vosbt[SC_FMAX];

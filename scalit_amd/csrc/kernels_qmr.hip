@@ -948,6 +948,7 @@ __global__ void __launch_bounds__(QP_NT) k_pist_persist(PpArgs pa)
 bool qmr_persist_ok(const scalit_ctx *c)
 {
     if (!c->fast3 || c->ndvr || c->sF != 3 || !c->has_precond || !c->has_res) return false;
+    if (c->osb == SCALIT_TOSBW) return false;   // the window solve couples level-1 blocks
     for (int l = 0; l < 3; l++)
         if (c->lv[l].n > QP_NMAX || c->lv[l].dep != (l == 2) || !c->lv[l].vosb.p) return false;
     return c->N <= 65536 && (long)c->lv[0].n * c->lv[1].n <= 1536;

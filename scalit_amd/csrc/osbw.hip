@@ -1,0 +1,255 @@
+// The windowed OSB preconditioner TOSBW (src/iterate/module/osb.px.h:191-206,
+// set up by initOSBW, src/iterate/module/osb.osbw.h:2-50): the OSB states
+// whose EIG0 lies within DE of E0 (DE adjusted until exactly mCnt do,
+// adjustWinSize, osb.osbw.h:117-145) are treated exactly: P*X divides the
+// other rotated components by EIG0 - E0 as TOSB does, and replaces the window
+// components by the solution of (H_W - E0) y = x, with H_W the Hamiltonian in
+// the window's OSB states (getVOSBH0, src/iterate/module/osb.hij.h:133-160:
+// H_W(i,j) = v_i . H v_j off the diagonal, EIG0 on it).
+//
+// Here the window is set up on the GPU: every window state's OSB vector is a
+// tensor product of one column per level (getLevelVi,
+// src/iterate/module/osb.vi.h), written by one kernel for all states; H
+// applies to all of them in one batched H*X; one kernel forms the upper
+// triangle of H_W; and (H_W - E0)^-1 is formed once from its eigenvalues and
+// eigenvectors (a symmetric solve equal to the reference's DSYTRF/DSYTRS up to
+// rounding).  P*X keeps the fused level-1 pass of TOSB: the window's
+// components of V1^T x are gathered and solved before it (one matrix-vector
+// product of the window's size per vector), its divide reads EIG0 with +inf
+// at the window (so those components become 0 rather than being divided),
+// and V1 times the solved components is added to its output.
+#include <cmath>
+#include <vector>
+
+#include "scalit_impl.hpp"
+
+namespace scalit {
+
+struct OsbwLevels {
+    int sF;
+    int n[SC_FMAX];
+    const double *v[SC_FMAX];
+};
+
+// X(:, k) = the OSB vector of grid index ind[k]: prod_l V_l[b_l](i_l, c_l)
+__global__ void k_osbw_basis(long N, int cnt, OsbwLevels L, const int *__restrict__ ind, double *__restrict__ x,
+                             long ldx)
+{
+    const int k = blockIdx.y;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N || k >= cnt) return;
+    long ci = ind[k], ei = e;
+    double val = 1.0;
+    // level l's block index is the product index of the higher levels' columns
+    long cols[SC_FMAX];
+    for (int l = 0; l < L.sF; l++) {
+        cols[l] = ci % L.n[l];
+        ci /= L.n[l];
+    }
+    for (int l = 0; l < L.sF; l++) {
+        const int n = L.n[l];
+        const long il = ei % n;
+        ei /= n;
+        long b = 0;
+        for (int m = L.sF - 1; m > l; m--) b = b * L.n[m] + cols[m];
+        val *= L.v[l][b * n * n + il + (long)n * cols[l]];
+    }
+    x[(long)k * ldx + e] = val;
+}
+
+// H_W(i, j) = X(:, i) . HX(:, j) for i < j, both triangles; one workgroup per pair
+__global__ void __launch_bounds__(256) k_osbw_gram(long N, int cnt, const double *__restrict__ x, long ldx,
+                                                   const double *__restrict__ hx, long ldh, double *__restrict__ hw)
+{
+    __shared__ double red[4];
+    const int i = blockIdx.x, j = blockIdx.y;
+    if (i >= j) return;
+    const double *xi = x + (long)i * ldx, *hj = hx + (long)j * ldh;
+    double acc = 0.0;
+    for (long e = threadIdx.x; e < N; e += 256) acc += xi[e] * hj[e];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double v = red[0] + red[1] + red[2] + red[3];
+        hw[i + (long)cnt * j] = v;
+        hw[j + (long)cnt * i] = v;
+    }
+}
+
+// M = Z diag(1 / w) Z^T (the inverse of a symmetric matrix from its eigenpairs)
+__global__ void k_osbw_inverse(int n, const double *__restrict__ z, const double *__restrict__ w,
+                               double *__restrict__ m)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * n) return;
+    const int i = (int)(e % n), j = (int)(e / n);
+    double acc = 0.0;
+    for (int k = 0; k < n; k++) acc += z[i + (long)n * k] * z[j + (long)n * k] / w[k];
+    m[e] = acc;
+}
+
+// EIG0 with +inf at the window's states
+__global__ void k_osbw_mark(int n, const int *__restrict__ ind, double *__restrict__ e)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) e[ind[k]] = INFINITY;
+}
+
+// d = M z, z(k) = V1[b_k](:, c_k) . x(b_k block): the window's components of
+// V1^T x solved; one workgroup per vector, one wave per window state
+__global__ void __launch_bounds__(256) k_osbw_gather(int n1, int n, const int *__restrict__ ind,
+                                                     const double *__restrict__ v1, const double *__restrict__ m,
+                                                     const double *__restrict__ x, long ldx, double *__restrict__ d,
+                                                     const int *__restrict__ active)
+{
+    extern __shared__ double zw[];
+    const int v = blockIdx.x;
+    if (active && !active[v]) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int k = wv; k < n; k += 4) {
+        const long g = ind[k], b = g / n1;
+        const double *col = v1 + b * n1 * n1 + (long)n1 * (g - b * n1);
+        const double *xb = x + (long)v * ldx + b * n1;
+        double acc = 0.0;
+        for (int i = lane; i < n1; i += 64) acc += col[i] * xb[i];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) zw[k] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) {
+        double acc = 0.0;
+        for (int k = 0; k < n; k++) acc += m[i + (long)n * k] * zw[k];
+        d[(long)v * n + i] = acc;
+    }
+}
+
+// y(b block) += sum over the run's states k of V1[b](:, c_k) d(k): one
+// workgroup per run of window states in one level-1 block and vector (runs
+// touch disjoint blocks)
+__global__ void __launch_bounds__(256) k_osbw_scatter(int n1, int n, const int *__restrict__ ind,
+                                                      const int *__restrict__ runs, const double *__restrict__ v1,
+                                                      const double *__restrict__ d, double *__restrict__ y, long ldy,
+                                                      const int *__restrict__ active)
+{
+    const int r = blockIdx.x, v = blockIdx.y;
+    if (active && !active[v]) return;
+    const int k0 = runs[r], k1 = runs[r + 1];
+    const long b = ind[k0] / n1;
+    const double *vb = v1 + b * n1 * n1;
+    double *yb = y + (long)v * ldy + b * n1;
+    for (int i = threadIdx.x; i < n1; i += 256) {
+        double acc = 0.0;
+        for (int k = k0; k < k1; k++) acc += vb[i + (long)n1 * (ind[k] - b * n1)] * d[(long)v * n + k];
+        yb[i] += acc;
+    }
+}
+
+void osbw_gather(scalit_ctx *c, int nvec, const double *x, long ldx, const int *active, hipStream_t s)
+{
+    const int n = c->osbw_n;
+    hipLaunchKernelGGL(k_osbw_gather, dim3(nvec), dim3(256), sizeof(double) * (size_t)n, s, c->lv[0].n, n,
+                       c->osbw_ind.p, c->lv[0].vosb.p, c->osbw_m.p, x, ldx, c->osbw_d.p, active);
+}
+
+void osbw_scatter(scalit_ctx *c, int nvec, double *y, long ldy, const int *active, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_osbw_scatter, dim3(c->osbw_nruns, nvec), dim3(256), 0, s, c->lv[0].n, c->osbw_n,
+                       c->osbw_ind.p, c->osbw_runs.p, c->lv[0].vosb.p, c->osbw_d.p, y, ldy, active);
+}
+
+}  // namespace scalit
+
+using namespace scalit;
+
+extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int cnt, int *hwlen, double *de_out)
+{
+    if (!c || cnt < 1 || cnt > 1024 || !(de > 0.0) || (npc != SCALIT_TA0 && npc != SCALIT_TA1)) return SCALIT_EINVAL;
+    if (!c->has_precond || !c->has_res) return SCALIT_ESTATE;
+    if (c->ndvr) return SCALIT_EINVAL;   // the level-1 rotation of an sNDVR operator is the outer layer
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const long N = c->N;
+    c->osbw_n = 0;
+    std::vector<double> eig(N);
+    HIP_TRY(hipMemcpyAsync(eig.data(), c->eig0.p, 8 * (size_t)N, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // adjustWinSize: the reference's search for a width holding exactly cnt states
+    auto count = [&](double w) {
+        long k = 0;
+        for (long i = 0; i < N; i++)
+            if (std::fabs(eig[i] - e0) <= w) k++;
+        return k;
+    };
+    double de1 = 0.0;
+    long cnt0 = count(de);
+    int it = 0;
+    while (cnt0 != cnt) {
+        if (++it > 10000) return SCALIT_ESTATE;   // the search does not settle (ties in EIG0)
+        if (cnt0 < cnt) {
+            de1 = de;
+            de = 2.0 * de;
+        } else {
+            de = 0.5 * (de1 + de);
+        }
+        cnt0 = count(de);
+    }
+    // getOSBWIndex: the window's states in grid order
+    std::vector<int> ind;
+    for (long i = 0; i < N && (int)ind.size() < cnt; i++)
+        if (std::fabs(eig[i] - e0) <= de) ind.push_back((int)i);
+    const int n = (int)ind.size();
+    SC_TRY(c->osbw_ind.alloc(n));
+    HIP_TRY(hipMemcpyAsync(c->osbw_ind.p, ind.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    // the window's OSB vectors and H applied to them
+    const long ld = (N + 63) & ~63L;
+    DBuf<double> xv, hv, hw, w, z, work;
+    SC_TRY(xv.alloc((size_t)n * ld));
+    SC_TRY(hv.alloc((size_t)n * ld));
+    OsbwLevels L;
+    L.sF = c->sF;
+    for (int l = 0; l < c->sF; l++) {
+        L.n[l] = c->lv[l].n;
+        L.v[l] = c->lv[l].vosb.p;
+    }
+    hipLaunchKernelGGL(k_osbw_basis, dim3((unsigned)((N + 255) / 256), n), dim3(256), 0, s, N, n, L,
+                       c->osbw_ind.p, xv.p, ld);
+    HIP_TRY(hipGetLastError());
+    SC_TRY(hx_apply_f64(c, nullptr, 0.0, n, xv.p, ld, hv.p, ld, false, nullptr, s));   // HijX: MYHX(TA0)
+    SC_TRY(hw.alloc((size_t)n * n));
+    hipLaunchKernelGGL(k_osbw_gram, dim3(n, n), dim3(256), 0, s, N, n, xv.p, ld, hv.p, ld, hw.p);
+    HIP_TRY(hipGetLastError());
+    // the diagonal is EIG0 (getVOSBH0), less the shift (createWH: TA1)
+    const double shift = npc == SCALIT_TA0 ? 0.0 : e0;
+    std::vector<double> hd(n);
+    for (int k = 0; k < n; k++) hd[k] = eig[ind[k]] - shift;
+    HIP_TRY(hipMemcpy2DAsync(hw.p, 8 * (size_t)(n + 1), hd.data(), 8, 8, n, hipMemcpyHostToDevice, s));
+    SC_TRY(w.alloc(n));
+    SC_TRY(z.alloc((size_t)n * n));
+    SC_TRY(work.alloc(small_sym_eig_work(n)));
+    SC_TRY(small_sym_eig(n, hw.p, n, w.p, z.p, n, work.p, s));
+    SC_TRY(c->osbw_m.alloc((size_t)n * n));
+    hipLaunchKernelGGL(k_osbw_inverse, dim3((unsigned)(((long)n * n + 255) / 256)), dim3(256), 0, s, n, z.p, w.p,
+                       c->osbw_m.p);
+    HIP_TRY(hipGetLastError());
+    // runs of window states in one level-1 block (the indices ascend)
+    const int n1 = c->lv[0].n;
+    std::vector<int> runs{0};
+    for (int k = 1; k < n; k++)
+        if (ind[k] / n1 != ind[k - 1] / n1) runs.push_back(k);
+    runs.push_back(n);
+    SC_TRY(c->osbw_runs.alloc(runs.size()));
+    HIP_TRY(hipMemcpyAsync(c->osbw_runs.p, runs.data(), sizeof(int) * runs.size(), hipMemcpyHostToDevice, s));
+    SC_TRY(c->osbw_eig0.alloc(N));
+    HIP_TRY(hipMemcpyAsync(c->osbw_eig0.p, c->eig0.p, 8 * (size_t)N, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_osbw_mark, dim3((n + 255) / 256), dim3(256), 0, s, n, c->osbw_ind.p, c->osbw_eig0.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    c->osbw_n = n;
+    c->osbw_nruns = (int)runs.size() - 1;
+    c->osbw_shift = shift;
+    c->gen++;
+    if (hwlen) *hwlen = n;
+    if (de_out) *de_out = de;
+    return SCALIT_OK;
+}

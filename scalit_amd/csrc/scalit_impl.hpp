@@ -130,6 +130,16 @@ struct scalit_ctx {
     // the preconditioner kernel of the solvers' P*X (the reference's sOSB:
     // TOSB, TOSBD1 or TOSBD2, with DE and beta from scalit_set_shift)
     int osb = 0;
+    // the TOSBW window (scalit_set_osbw): grid indices of its states
+    // (ascending), the starts of its runs of states in one level-1 block
+    // (osbw_nruns + 1), the inverse of its shifted Hamiltonian (osbw_n x
+    // osbw_n), EIG0 with +inf at the window (the divide then leaves the
+    // window's components 0) and the window's part of P*X per vector
+    // (osbw_d, nvec x osbw_n); osbw_n = 0 when no window is set up
+    scalit::DBuf<int> osbw_ind, osbw_runs;
+    scalit::DBuf<double> osbw_m, osbw_eig0, osbw_d;
+    int osbw_n = 0, osbw_nruns = 0;
+    double osbw_shift = 0.0;
     // FP64 QMR on small fused 3-level operators: 1 = the persistent solver,
     // 0 = the launch-per-step iteration, -1 = as SCALIT_QMR_PERSIST says
     // (default 1)
@@ -171,6 +181,12 @@ int px_apply_f32(scalit_ctx *c, int nosb, const float *shifts_dev, float shift0,
 int px_apply_bf16(scalit_ctx *c, int nosb, const float *shifts_dev, float shift0, int nvec, const bf16_t *x,
                   long ldx, bf16_t *y, long ldy, const int *active, hipStream_t s);
 int precond_to_f32(scalit_ctx *c);
+// TOSBW around the level-1 rotation (osbw.hip): osbw_gather forms
+// d = (H_W - E0)^-1 (V1^T x)(window) from level 1's input x; the level-1 pass
+// then divides by osbw_eig0 (the window's components become 0) and
+// osbw_scatter adds V1(:, window) d to its output
+void osbw_gather(scalit_ctx *c, int nvec, const double *x, long ldx, const int *active, hipStream_t s);
+void osbw_scatter(scalit_ctx *c, int nvec, double *y, long ldy, const int *active, hipStream_t s);
 
 // OSB set-up (kernels_diag.hip)
 int block_diag(scalit_ctx *c, int bjmax, double bjtol);

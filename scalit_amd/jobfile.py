@@ -191,9 +191,9 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
     if job.sJOB != JOB_BOUND or job.sCX:
         raise ValueError("only real bound-state jobs (sJOB = 1, sCX = F) are on this path")
     path = lambda k: os.path.join(base_dir, job.files[k]) if k in job.files else None
-    nosb = {TOSB: L.TOSB, TOSBD1: L.TOSBD1, TOSBD2: L.TOSBD2}.get(job.sOSB)
+    nosb = {TOSB: L.TOSB, TOSBD1: L.TOSBD1, TOSBD2: L.TOSBD2, TOSBW: L.TOSBW}.get(job.sOSB)
     if nosb is None:
-        raise ValueError("the OSBW windowed preconditioner (sOSB = 4) is not on this path")
+        raise ValueError(f"sOSB = {job.sOSB} is not a preconditioner kernel (1..4)")
     ctx = Context.from_files(job.sN, job.sDEP, h0_file=path("H0"), res_file=path("RES"), dep_file=path("DEP"),
                              outh_file=path("OUTH"), device=device)
     ctx.set_shift(job.osbw_e0, job.osbw_de, job.osbw_beta)
@@ -205,6 +205,14 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
         if job.sVOSB > 0:
             ctx.save_precond(path("VOSB"), path("EIG"), mode)
     m0 = job.conv_num
+    if nosb == L.TOSBW:
+        # OSB_PISTCONV (osbw.pistconv.h:26-31): sPC = TA1, then initOSBW with
+        # the job's window; a window of no states fails the job (returns 0)
+        if job.osbw_cnt < 1:
+            ctx.close()
+            return 0, np.zeros(m0), 0.0
+        ctx.set_osbw(job.osbw_e0, job.osbw_de, job.osbw_cnt, L.TA1)
+    ctx.set_osb(nosb)
     e0, etol, start, step, stepeig, nmax = pist_params(job, m0)
     x0 = np.random.default_rng(seed).random((1, ctx.N))
     r, eig, res, _ = ctx.pist_conv(ctx.to_device(x0), e0, etol, m0, nmax, start, step, stepeig,

@@ -120,3 +120,10 @@ eig0.dat
     job.sVOSB = -42
     r2, eig2, _ = J.run_pist_job(job, base_dir=str(tmp_path))
     assert r2 == r and np.array_equal(eig2, eig)
+    # sOSB selects the solvers' preconditioner kernel: OSBD, and the window of
+    # the 24 OSB states nearest the job's E0 (initOSBW before PIST)
+    for kind, cnt in ((J.TOSBD1, 0), (J.TOSBW, 24)):
+        job.sOSB, job.osbw_de, job.osbw_cnt = kind, 1.0e-3, cnt
+        r3, eig3, _ = J.run_pist_job(job, base_dir=str(tmp_path))
+        print(f"job with sOSB = {kind}: ret {r3}", flush=True)
+        assert r3 > 0 and (np.abs(np.sort(eig3) - ref) / np.abs(ref)).max() <= 1e-9

@@ -105,6 +105,55 @@ def test_qmr_osbd_kernels(nosb, persist):
     ctx.close()
 
 
+@pytest.mark.parametrize("npc", [L.TA1, L.TA0])
+def test_tosbw_window_px_and_solvers(npc):
+    """The windowed preconditioner TOSBW on 12 x 10 x 8: scalit_set_osbw
+    against the oracle's restatement of initOSBW from the same EIG0 (width
+    and window size exact), TOSBW P*X against the oracle's px_tosbw to 1e-10
+    relative (the window inverse is formed from an eigendecomposition here and
+    by a solve in the oracle), then QMR with sOSB = TOSBW (residual of the
+    solution <= 1e-8 relative) and PIST for the 8 levels nearest E0 against a
+    dense eigensolve (1e-10 relative).  The FP32 inner solves refuse TOSBW."""
+    print(f"tosbw npc={npc}", flush=True)
+    prob, op, ctx = op_ctx((12, 10, 8), e0=0.0)
+    vosb, eig0 = op.diag()
+    ctx.set_precond(vosb, eig0)
+    srt = np.sort(eig0)
+    e0 = float(0.5 * (srt[40] + srt[41]))
+    ctx.set_shift(e0)
+    op.set_shift(e0)
+    x = P.random_vectors(prob.N, 3, 41)
+    for cnt, de in ((16, 1e-6), (60, 2e-3)):
+        n, d = ctx.set_osbw(e0, de, cnt, npc)
+        win = O.osbw_setup(op, e0, de, cnt, npc)
+        assert n == cnt and d == win[1], (n, d, win[1])
+        print(f"window {cnt}: width {d:.3e}, cond(HWR) {np.linalg.cond(win[3]):.1e}", flush=True)
+        y = ctx.px(ctx.to_device(x), nosb=L.TOSBW, npc=npc).cpu().numpy()
+        for v in range(3):
+            yo = O.px_tosbw(op, x[v], win, npc)
+            assert np.abs(y[v] - yo).max() <= 1e-10 * np.abs(yo).max()
+    if npc != L.TA1:
+        ctx.close()
+        return
+    b = P.random_vectors(prob.N, 1, 43)
+    ret0, _, _ = ctx.qmr(ctx.to_device(b), ntype=-1, qmrmax=400, qmrtol=1e-9)
+    ctx.set_osb(L.TOSBW)
+    ret, xs, _ = ctx.qmr(ctx.to_device(b), ntype=-1, qmrmax=400, qmrtol=1e-9)
+    print(f"QMR iterations: TOSB {int(ret0[0])}, TOSBW {int(ret[0])}", flush=True)
+    assert ret[0] > 0
+    h = op.hx(xs.cpu().numpy()[0], O.TA1)
+    assert np.linalg.norm(h - b[0]) <= 1e-8 * np.linalg.norm(b[0])
+    ev = np.linalg.eigvalsh(op.dense())
+    near = np.sort(ev[np.argsort(np.abs(ev - e0))[:8]])
+    x0 = ctx.to_device(P.random_vectors(prob.N, 1, 7))
+    r, eig, _, stats = ctx.pist_conv(x0, e0, 1e-12, 8, 200, 30, 10, 4, qmrtol=1e-9)
+    print("PIST with TOSBW", r, stats, flush=True)
+    assert r > 0 and (np.abs(np.sort(eig) - near) / np.abs(near)).max() <= 1e-10
+    with pytest.raises(Exception):
+        ctx.pist_conv(x0, e0, 1e-10, 8, 200, 30, 10, 4, qmrtol=1e-6, prec=L.FP32)
+    ctx.close()
+
+
 @pytest.mark.parametrize("persist", [1, 0])
 def test_pist_persistent_and_per_step_agree(persist):
     """The config-1 PIST (lowest 20 levels, 24 x 32 x 24) against the golden

@@ -326,3 +326,64 @@ def test_truncated_operator_spectrum():
     a = sla.LinearOperator((opf.N, opf.N), matvec=lambda v: opf.hx(v), dtype=np.float64)
     evf = np.sort(sla.eigsh(a, k=10, which="SA", tol=1e-12)[0])[:8]
     assert np.all(ev >= evf - 1e-12) and np.all(np.abs(ev - evf) <= 2e-4 * np.abs(evf))
+
+
+def _osbw_problem(grid=(5, 6, 4), seed=3):
+    prob = tiny(grid, seed)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    _, eig0 = op.diag()
+    srt = np.sort(eig0)
+    e0 = float(0.5 * (srt[20] + srt[21]))
+    op.set_shift(e0)
+    return prob, op, e0
+
+
+def test_osb_vectors_are_the_preconditioner_eigenvectors():
+    """Every OSB state's vector (getLevelVi restated) is an eigenvector of the
+    reference-pinned TOSB preconditioner with eigenvalue 1 / (EIG0_k - E0),
+    and the vectors are orthonormal: a wrong digit, block or transposed
+    block fails this."""
+    _, op, e0 = _osbw_problem()
+    vs = np.stack([O.osb_vector(op, k) for k in range(op.N)], axis=1)
+    assert np.abs(vs.T @ vs - np.eye(op.N)).max() <= 1e-12
+    for k in (0, 7, 31, op.N - 1):
+        p = op.px(vs[:, k], npc=O.TA1, kind=O.TOSB)
+        assert np.abs(p - vs[:, k] / (op.EIG0[k] - e0)).max() <= 1e-10 * np.abs(p).max()
+
+
+def test_osbw_window_whole_grid_reproduces_spectrum():
+    """With every state in the window and the diagonal taken as v_i . H v_i
+    (instead of EIG0), H_W is V^T H V: its eigenvalues are H's, against a
+    dense eigensolve of the Kronecker sum."""
+    prob, op, e0 = _osbw_problem((4, 5, 3))
+    ind, de, hw, hwr, vw = O.osbw_setup(op, e0, 1e-3, op.N)
+    assert sorted(ind) == list(range(op.N))
+    full = hw.copy()
+    hv = np.stack([op.hx(vw[:, j], O.TA0) for j in range(op.N)], axis=1)
+    full[np.diag_indices(op.N)] = np.einsum("ij,ij->j", vw, hv)
+    ev = np.linalg.eigvalsh(full)
+    ref = np.linalg.eigvalsh(kron_dense(prob))
+    assert np.abs(ev - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("npc", [O.TA0, O.TA1])
+def test_osbw_window_and_kernel_identities(npc):
+    """adjustWinSize settles on exactly cnt states, taken in grid order; the
+    TOSBW P*X inverts HWR on the window's span (P V_W HWR a = V_W a) and is
+    TOSB on its orthogonal complement."""
+    _, op, e0 = _osbw_problem()
+    for cnt, de in ((1, 1e-6), (12, 1e-6), (12, 10.0), (40, 1e-4)):
+        w = O.osbw_setup(op, e0, de, cnt, npc)
+        ind, de2 = w[0], w[1]
+        assert len(ind) == cnt and np.all(np.diff(ind) > 0)
+        assert int((np.abs(op.EIG0 - e0) <= de2).sum()) == cnt
+    ind, _, hw, hwr, vw = w
+    assert np.allclose(hwr, hw - (e0 if npc == O.TA1 else 0.0) * np.eye(len(ind)), rtol=0, atol=0)
+    a = P.rng(4).standard_normal(len(ind))
+    y = O.px_tosbw(op, vw @ (hwr @ a), w, npc)
+    assert np.abs(y - vw @ a).max() <= 1e-10 * np.abs(vw @ a).max()
+    x = P.random_vectors(op.N, 1, 8)[0]
+    x = x - vw @ (vw.T @ x)
+    y = O.px_tosbw(op, x, w, npc)
+    t = op.px(x, npc=npc, kind=O.TOSB)
+    assert np.abs(y - t).max() <= 1e-12 * np.abs(t).max()

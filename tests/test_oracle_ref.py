@@ -270,3 +270,24 @@ def test_problem_legendre_builders_match_reference():
         yr = np.zeros((n, n), order="F")
         R.yjpolys_(O._ci(n), O._ci(n - 1), O.dptr(xr), O.dptr(yr))
         assert np.abs(y - yr).max() <= 1e-12 * np.abs(yr).max()
+
+
+@pytest.mark.parametrize("cnt,de", [(1, 1e-6), (16, 1e-6), (16, 5.0), (60, 2e-3)])
+def test_osbw_setup_matches_reference(cnt, de):
+    """The window set-up of the reference's own module osb (adjustWinSize,
+    getOSBWIndex, getVOSBH0 with HijX = MYHX(TA0)) against the oracle's
+    restatement: width and indices bit-exact, H_W to 1e-13 of its largest
+    entry (the oracle forms v_i . H v_j as one matrix product), every window
+    state's vector (getVi) bit-exact."""
+    prob = P.config_problem("pist_fp64", grid=(12, 10, 8))
+    op, rop = make_ops(prob, e0=0.0)
+    _, eig0 = op.diag()
+    rop.diag()
+    srt = np.sort(eig0)
+    e0 = float(0.5 * (srt[40] + srt[41]))
+    ind, de2, hw, _, vw = O.osbw_setup(op, e0, de, cnt)
+    rind, rde, rhw = rop.osbw(cnt, e0, de)
+    assert de2 == rde and np.array_equal(ind, rind)
+    assert np.abs(hw - rhw).max() <= 1e-13 * np.abs(rhw).max()
+    for j in range(0, cnt, max(1, cnt // 5)):
+        assert np.array_equal(vw[:, j], rop.osb_vector(int(ind[j])))
