@@ -42,6 +42,19 @@ extern "C" {
 /* H*X shift selector: src/iterate/module/osb.hx.h:82-89 */
 #define SCALIT_TA0 0   /* Y = H X */
 #define SCALIT_TA1 1   /* Y = (H - E0) X */
+/* complex operators (absorbing potentials): src/iterate/module/osb.data.h:18-22 */
+#define SCALIT_TAAP0 2   /* (H + i AP) X */
+#define SCALIT_TMAP0 3   /* (H - i AP) X */
+#define SCALIT_TAAP 100  /* (H + i AP - E0) X */
+#define SCALIT_TAAPP 101 /* (H + i APP - E0) X */
+#define SCALIT_TAAPR 102 /* (H + i APR - E0) X */
+#define SCALIT_TMAP 200  /* (H - i AP - E0) X */
+#define SCALIT_TMAPP 201 /* (H - i APP - E0) X */
+#define SCALIT_TMAPR 202 /* (H - i APR - E0) X */
+/* absorbing-potential arrays: osb.data.h:61 */
+#define SCALIT_AP 0
+#define SCALIT_APP 1
+#define SCALIT_APR 2
 /* preconditioner kernels: src/iterate/module/osb.px.h:156-211 */
 #define SCALIT_TOSB 0
 #define SCALIT_TOSBD1 1

@@ -444,6 +444,34 @@ class RefOperator:
         assert n.value == cnt
         return ind - 1, d.value, hw
 
+    def set_ap(self, which, a):
+        ref().refop_set_ap(C.c_int(which), dptr(f64(a)))
+
+    def hx_dx(self, x, ntype):
+        """Module osb's MYHX_DX on a complex vector."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        ref().refop_hx_dx(C.c_int(ntype), C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data))
+        return y
+
+    def px_dx(self, x, kind, npc):
+        """Module osb's MYPX_DX (kind TOSB / TOSBD1 / TOSBD2 as the oracle numbers them)."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        ref().refop_px_dx(C.c_int(kind + 1), C.c_int(npc), C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data))
+        return y
+
+    def qmr_dx(self, b, kind, hc, pc, ntype=-1, qmr_max=1000, qmr_tol=1e-3):
+        """QMR_CX through OSB_QMRDX's operators: (code, x, res)."""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros_like(b)
+        res = C.c_double(0.0)
+        R = ref()
+        R.refop_qmr_dx.restype = C.c_int
+        r = R.refop_qmr_dx(C.c_int(kind + 1), C.c_int(hc), C.c_int(pc), C.c_int(ntype), C.c_int(qmr_max),
+                           C.c_double(qmr_tol), C.c_void_p(b.ctypes.data), C.c_void_p(x.ctypes.data), C.byref(res))
+        return r, x, res.value
+
     def osb_vector(self, k):
         """Module osb's getVi for the 0-based grid index k."""
         v = np.zeros(self.op.N)
@@ -597,3 +625,224 @@ def px_tosbw(op: "Operator", x, win, npc: int = TA1) -> np.ndarray:
     zw = vw.T @ x
     corr = np.linalg.solve(hwr, zw) - zw / (op.EIG0[ind] - e0)
     return op.px(x, npc=npc, kind=TOSB) + vw @ corr
+
+
+# --------------------------------------------------------------------------
+# complex (absorbing-potential) operators and solvers: the checker for
+# scalit_hx_cx / scalit_px_cx / scalit_qmr_cx / scalit_pist_conv_sx
+# --------------------------------------------------------------------------
+TAAP0, TMAP0 = 2, 3                    # osb.data.h:18
+TAAP, TAAPP, TAAPR = 100, 101, 102     # osb.data.h:20
+TMAP, TMAPP, TMAPR = 200, 201, 202     # osb.data.h:22
+AP, APP, APR = 0, 1, 2
+
+
+def _imag_diag(ntype: int, aps):
+    """The imaginary diagonal that nType / nPC adds: +-AP, +-APP, +-APR
+    (osb.hxdx.h:196-218, osb.mypxdx.h)."""
+    ap, app, apr = aps
+    return {TAAP0: ap, TMAP0: -ap, TAAP: ap, TAAPP: app, TAAPR: apr,
+            TMAP: -ap, TMAPP: -app, TMAPR: -apr}.get(ntype)
+
+
+def _shift_of(op: "Operator", ntype: int) -> float:
+    return 0.0 if ntype in (TA0, TAAP0, TMAP0) else op.s.E0
+
+
+def hx_dx(op: "Operator", x, ntype: int, aps) -> np.ndarray:
+    """MYHX_DX (src/iterate/module/osb.hxdx.h:177-240): Y = (RES - E0 +- i A) X
+    plus the real level products applied to the real and imaginary parts."""
+    x = np.asarray(x, dtype=np.complex128)
+    y = op.hx(x.real, TA0) + 1j * op.hx(x.imag, TA0) - _shift_of(op, ntype) * x
+    a = _imag_diag(ntype, aps)
+    if a is not None:
+        y = y + 1j * a * x
+    return y
+
+
+def osb_basis(op: "Operator") -> np.ndarray:
+    """Every OSB state's vector as the columns of V (tiny grids)."""
+    if getattr(op, "_vdense", None) is None:
+        op._vdense = np.stack([osb_vector(op, k) for k in range(op.N)], axis=1)
+    return op._vdense
+
+
+def px_dx_denominator(op: "Operator", nosb: int, npc: int, aps) -> np.ndarray:
+    """The complex divisor of MYPX_DX's kernel (osb.mypxdx.h): EIG0 - E0,
+    clamped by TOSBD1 / TOSBD2 (DE, beta), plus +-i A by nPC."""
+    e0 = _shift_of(op, npc)
+    de = abs(op.s.DE)
+    ade = op.EIG0 - e0
+    if nosb == TOSBD1:
+        ade = np.where(np.abs(ade) <= de, np.where(ade < 0, -de, de), ade)
+    elif nosb == TOSBD2:
+        beta = op.s.beta + 1.0
+        alpha = -op.s.beta * de
+        ade = np.where(np.abs(ade) <= de, alpha + beta * ade, ade)
+    a = _imag_diag(npc, aps)
+    return ade + (0j if a is None else 1j * a)
+
+
+def px_dx(op: "Operator", x, nosb: int, npc: int, aps) -> np.ndarray:
+    """MYPX_DX (src/iterate/module/osb.mypxdx.h): V (kernel)^-1 V^T X with the
+    real OSB rotations applied to complex X."""
+    v = osb_basis(op)
+    z = v.T @ np.asarray(x, dtype=np.complex128)
+    return v @ (z / px_dx_denominator(op, nosb, npc, aps))
+
+
+def dot_cx(x, y) -> complex:
+    """DOT_CX (src/libutil/serial/base/vector_cx.f90:227-234): x^T y."""
+    return complex(np.sum(x * y))
+
+
+def qmr_cx(hx, px, b, ntype: int = -1, qmrmax: int = 1000, qmrtol: float = 1e-3, x0=None):
+    """QMR_CX (src/iterate/libiterate/qmr/qmrcx.f90:4-179) for complex
+    symmetric systems: the real QMR's recurrence with the bilinear x^T y for
+    VZDOT / QSDOT and Hermitian norms.  Returns (code, x, res)."""
+    b = np.asarray(b, dtype=np.complex128)
+    n = b.size
+    nrm = lambda v: np.sqrt(abs(np.vdot(v, v)))
+    x = np.zeros(n, complex) if (ntype <= 0 or x0 is None) else np.array(x0, dtype=complex)
+    zvrat = 1.0
+    r = b.copy()
+    v = b.copy()
+    bnrm = nrm(b)
+    vnrm = bnrm
+    z = px(b)
+    znrm = nrm(z)
+    gamamo, eta = 1.0, -1.0 + 0j
+    qsdot = 0j
+    q = s = ux = ur = None
+    thetamo = 0.0
+    res = 0.0
+    code = 0
+    for i in range(1, qmrmax + 1):
+        if vnrm == 0.0 or znrm == 0.0:
+            break
+        zvrat = zvrat * znrm / vnrm
+        v = v / vnrm
+        z = z / znrm
+        vzdot = dot_cx(v, z)
+        if abs(vzdot) == 0.0:
+            break
+        q = z.copy() if i == 1 else z - (vnrm * vzdot) / qsdot * q
+        s = zvrat * hx(q)
+        qsdot = dot_cx(q, s)
+        beta = qsdot / vzdot
+        if abs(qsdot) == 0.0 or abs(beta) == 0.0:
+            break
+        v = s - beta * v
+        vnrmpo = nrm(v)
+        z = px(v) / zvrat
+        znrm = nrm(z)
+        theta = vnrmpo / (gamamo * abs(beta))
+        gama = 1.0 / np.sqrt(1.0 + theta * theta)
+        eta = -eta * vnrm * gama * gama / (beta * gamamo * gamamo)
+        if gama == 0.0:
+            break
+        if i == 1:
+            ux = (eta * zvrat) * q
+            ur = eta * s
+        else:
+            ux = (eta * zvrat) * q + ((thetamo * gama) ** 2) * ux
+            ur = eta * s + ((thetamo * gama) ** 2) * ur
+        x = x + ux
+        r = r - ur
+        t = abs(ntype)
+        if t == 1:
+            hx0 = hx(x)
+            xnrm = nrm(hx0)
+            res = nrm(hx0 - b) / xnrm
+        elif t == 2:
+            res = max(nrm(r) / bnrm, np.sqrt(abs(np.vdot(ux, ux) / np.vdot(x, x))),
+                      np.sqrt(abs(np.vdot(ur, ur) / np.vdot(r, r))))
+        else:
+            res = nrm(r) / bnrm
+        if abs(res) < qmrtol:
+            code = i
+            break
+        code = -i
+        vnrm, gamamo, thetamo = vnrmpo, gama, theta
+    return code, x, float(res)
+
+
+def mgs_orth_sx(w, vj: list) -> bool:
+    """MGS_ORTH_SX (src/iterate/libiterate/orth/gsorth_sx.f90:47-74): one
+    modified Gram-Schmidt step under x^T y, normalised by the complex
+    sqrt(w^T w); appends to vj."""
+    w = np.array(w, dtype=complex)
+    for v in vj:
+        w = w - dot_cx(v, w) * v
+    nw = np.sqrt(complex(dot_cx(w, w)))
+    if nw == 0:
+        return False
+    vj.append(w / nw)
+    return True
+
+
+def _op_cx(vals: np.ndarray, ntype: int) -> np.ndarray:
+    """getOpCX (src/libutil/serial/base/util_cx.f90:375-420) for the two
+    measures PIST_CONV_SX uses: 1 = real part, 6 = |z|^2."""
+    return vals.real.copy() if ntype == 1 else vals.real ** 2 + vals.imag ** 2
+
+
+def pist_conv_sx(hx0, linsolv, x, e0, etol, ntype, m, m_max, startcnt, stepcnt, stepeig, eigvals=np.linalg.eigvals):
+    """PIST_CONV_SX (src/iterate/libiterate/pist/pist_sx.f90:9-110) with
+    pistH0_SX (pistHij_sx.f90:58-121): Krylov vectors from the shifted solves,
+    MGS_ORTH_SX, HMAT(i,j) = v_i^T H0 v_j, complex eigenvalues (ZGEEV: here
+    eigvals), windows by real part nearest e0 (getWindow_CX, nDiffType 1),
+    convergence by getDiffMax_CX of |z|^2 (nConvType 6), ascending real parts
+    (Reorder_CX).  ntype > 0 is refused (the reference passes getSepMax_CX
+    five arguments for its four)."""
+    if ntype > 0:
+        raise ValueError("PIST_CONV_SX with NTYPE > 0 is not defined")
+    if stepeig >= m_max or startcnt > m_max or m > m_max:
+        return 0, np.zeros(m, complex), 0.0
+    x = np.asarray(x, dtype=complex)
+    cnt_low, cnt_high = 1, max(startcnt, m, stepeig + 1)
+    vj, hmat = [], np.zeros((m_max, m_max), complex)
+    code, res, eig = 0, 0.0, np.zeros(m, complex)
+    while cnt_high <= m_max:
+        # pistH0_SX
+        if cnt_low == 1:
+            t = np.sqrt(complex(dot_cx(x, x)))
+            if t == 0:
+                return -cnt_high, eig, res
+            vj = [x / t]
+            m_start = 1
+        else:
+            m_start = cnt_low - 1
+        for j in range(m_start, cnt_high):
+            num, w, _ = linsolv(vj[j - 1])
+            if num <= 0:
+                return -cnt_high, eig, res
+            if not mgs_orth_sx(w, vj):
+                return -cnt_high, eig, res
+        for j in range(cnt_low, cnt_high + 1):
+            wj = hx0(vj[j - 1])
+            for i in range(1, cnt_high + 1):
+                hmat[i - 1, j - 1] = dot_cx(vj[i - 1], wj)
+        for j in range(1, cnt_low):
+            wj = hx0(vj[j - 1])
+            for i in range(cnt_low, cnt_high + 1):
+                hmat[i - 1, j - 1] = dot_cx(vj[i - 1], wj)
+        num = cnt_high - stepeig
+        tmp = eigvals(hmat[:num, :num])
+        ind, _ = get_window(e0, _op_cx(tmp, 1), m)     # 1-based, as getWindow0's
+        old = tmp[ind - 1]
+        tmp = eigvals(hmat[:cnt_high, :cnt_high])
+        ind, _ = get_window(e0, _op_cx(tmp, 1), m)
+        eig = tmp[ind - 1]
+        res = get_diff_max(_op_cx(eig, 6), _op_cx(old, 6))
+        if res <= etol:
+            code = cnt_high
+            break
+        code = -cnt_high
+        cnt_low = cnt_high + 1
+        if cnt_low >= m_max:
+            code = -m_max
+            break
+        cnt_high = min(cnt_high + stepcnt, m_max)
+    order = np.argsort(_op_cx(eig, 1), kind="stable")
+    return code, eig[order], res

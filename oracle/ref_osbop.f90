@@ -117,3 +117,66 @@ subroutine refop_vi(k, vi) bind(C, name='refop_vi')
   call getLevelIndex(one, kix, bk, col)
   call getVi(bk(1:sF, 1), col(1:sF, 1), vi)
 end subroutine
+
+! The absorbing potentials of module osb (AP, APP, APR; osb.data.h:61,
+! loaded by osb.io.h:20-30): which 0 = AP, 1 = APP, 2 = APR.
+subroutine refop_set_ap(which, a) bind(C, name='refop_set_ap')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: which
+  real(c_double), intent(in) :: a(*)
+  if (.not. allocated(AP)) allocate(AP(myLen))
+  if (.not. allocated(APP)) allocate(APP(myLen))
+  if (.not. allocated(APR)) allocate(APR(myLen))
+  select case (which)
+  case (0)
+    AP(1:myLen) = a(1:myLen)
+  case (1)
+    APP(1:myLen) = a(1:myLen)
+  case default
+    APR(1:myLen) = a(1:myLen)
+  end select
+end subroutine
+
+! Module osb's MYHX_DX (src/iterate/module/osb.hxdx.h:177-240) and MYPX_DX
+! (osb.mypxdx.h) on complex vectors; nType / nPC as osb.data.h:18-22, nOSB
+! as refop_px.
+subroutine refop_hx_dx(ntype, x, y) bind(C, name='refop_hx_dx')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: ntype
+  complex(c_double_complex), intent(in) :: x(*)
+  complex(c_double_complex), intent(out) :: y(*)
+  call MYHX_DX(ntype, myLen, x, y)
+end subroutine
+
+subroutine refop_px_dx(nosbk, npc, x, y) bind(C, name='refop_px_dx')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: nosbk, npc
+  complex(c_double_complex), intent(in) :: x(*)
+  complex(c_double_complex), intent(out) :: y(*)
+  call MYPX_DX(nosbk, npc, myLen, x, y)
+end subroutine
+
+! OSB_QMRDX (src/iterate/module/osbw.qmrdx.h:16-27): QMR_CX with HX_DX =
+! MYHX_DX(sHC) and PX_DX = MYPX_DX(sOSB, sPC); nOSB as refop_px.
+function refop_qmr_dx(nosbk, hc, pc, ntype, qmax, tol, b, x, qres) bind(C, name='refop_qmr_dx') result(r)
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: nosbk, hc, pc, ntype, qmax
+  real(c_double), value :: tol
+  complex(c_double_complex), intent(in) :: b(*)
+  complex(c_double_complex), intent(inout) :: x(*)
+  real(c_double), intent(out) :: qres
+  integer(c_int) :: r
+  integer :: QMR_CX
+  sOSB = nosbk
+  sHC = hc
+  sPC = pc
+  r = QMR_CX(ntype, qmax, tol, myLen, b, HX_DX, PX_DX, x, qres)
+end function

@@ -18,6 +18,10 @@ SCALIT_ENOMEM = -1003
 SCALIT_ESTATE = -1004
 SCALIT_EIO = -1005
 TA0, TA1 = 0, 1
+TAAP0, TMAP0 = 2, 3
+TAAP, TAAPP, TAAPR = 100, 101, 102
+TMAP, TMAPP, TMAPR = 200, 201, 202
+AP, APP, APR = 0, 1, 2
 TOSB, TOSBD1, TOSBD2, TOSBW = 0, 1, 2, 3
 FP64, FP32, BF16 = 0, 1, 2
 
@@ -71,6 +75,12 @@ SIGNATURES = {
     "scalit_set_pist_orth": (_i, [_vp, _i]),
     "scalit_set_osb": (_i, [_vp, _i]),
     "scalit_set_osbw": (_i, [_vp, _i, _d, _d, _i, _vp, _vp]),
+    "scalit_set_ap": (_i, [_vp, _i, _vp]),
+    "scalit_hx_cx": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    "scalit_px_cx": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
+    "scalit_qmr_cx": (_i, [_vp, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp]),
+    "scalit_pist_conv_sx": (_i, [_vp, _d, _d, _i, _vp, _i, _i, _i, _i, _i, _i, _i, _d, _i, _vp, _vp, _vp, _vp]),
+    "scalit_small_eig_cx": (_i, [_i, _vp, _i, _vp]),
     "scalit_set_qmr_persistent": (_i, [_vp, _i]),
     "scalit_load_data_grid": (_i, [C.c_char_p, _l, _i, _dp]),
     "scalit_save_data_grid": (_i, [C.c_char_p, _l, _i, _dp]),

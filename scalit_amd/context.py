@@ -387,6 +387,64 @@ class Context:
         check(L.lib().scalit_set_osbw(self._h, npc, e0, de, cnt, C.byref(n), C.byref(d)), "scalit_set_osbw")
         return n.value, d.value
 
+    # ------------------------------------------------------------ complex jobs
+    def to_device_cx(self, z) -> torch.Tensor:
+        """Complex vectors (b, N) in the split layout: (b, 2, N) float64, the
+        real plane then the imaginary plane of each vector."""
+        z = np.atleast_2d(np.asarray(z, dtype=np.complex128))
+        return self.to_device(np.stack([z.real, z.imag], axis=1).reshape(-1, self.N)).reshape(-1, 2, self.N)
+
+    @staticmethod
+    def from_device_cx(t: torch.Tensor) -> np.ndarray:
+        a = t.detach().cpu().numpy().reshape(t.shape[0], 2, -1)
+        return a[:, 0] + 1j * a[:, 1]
+
+    def set_ap(self, which: int, a: np.ndarray):
+        """AP / APP / APR, the absorbing potentials (osb.data.h:61)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(L.lib().scalit_set_ap(self._h, which, _np_dp(a)), "scalit_set_ap")
+
+    def hx_cx(self, x: torch.Tensor, ntype: int = L.TMAP):
+        """MYHX_DX (src/iterate/module/osb.hxdx.h:177-240) on split vectors."""
+        x = x.contiguous()
+        y = torch.empty_like(x)
+        check(L.lib().scalit_hx_cx(self._h, ntype, x.shape[0], _ptr(x), _ptr(y), _stream()), "scalit_hx_cx")
+        return y
+
+    def px_cx(self, x: torch.Tensor, nosb: int = L.TOSB, npc: int = L.TMAP):
+        """MYPX_DX (src/iterate/module/osb.mypxdx.h) on split vectors."""
+        x = x.contiguous()
+        y = torch.empty_like(x)
+        check(L.lib().scalit_px_cx(self._h, nosb, npc, x.shape[0], _ptr(x), _ptr(y), _stream()), "scalit_px_cx")
+        return y
+
+    def qmr_cx(self, b: torch.Tensor, qmr_type: int = -1, hc: int = L.TMAP, pc: int = L.TMAP, qmrmax: int = 1000,
+               qmrtol: float = 1e-3, x0=None):
+        """OSB_QMRDX (src/iterate/module/osbw.qmrdx.h:16-27): QMR_CX for one
+        split right-hand side (1, 2, N).  Returns (QMR_CX's value, x, res)."""
+        b = b.contiguous()
+        x = torch.zeros_like(b) if x0 is None else x0.clone().contiguous()
+        ret, res = C.c_int(), C.c_double()
+        check(L.lib().scalit_qmr_cx(self._h, qmr_type, hc, pc, qmrmax, qmrtol, _ptr(b), _ptr(x), C.byref(ret),
+                                    C.byref(res)), "scalit_qmr_cx")
+        return ret.value, x, res.value
+
+    def pist_conv_sx(self, x0: torch.Tensor, e0: float, etol: float, m: int, m_max: int, startcnt: int, stepcnt: int,
+                     stepeig: int = 1, ntype: int = 0, qmr_type: int = -1, qmrmax: int = 1000, qmrtol: float = 1e-3,
+                     pc: int = L.TMAP):
+        """OSB_PISTCONVDX1 (src/iterate/module/osbw.pistconv.h:98-140): complex
+        resonance energies nearest e0 (by real part) of H - i AP.  Returns
+        (PIST_CONV_SX's value, eigenvalues ascending by real part, res, stats)."""
+        x0 = x0.contiguous()
+        eig = (C.c_double * (2 * m))()
+        ret, res = C.c_int(), C.c_double()
+        stats = (C.c_int * 4)()
+        check(L.lib().scalit_pist_conv_sx(self._h, e0, etol, ntype, _ptr(x0), startcnt, stepcnt, stepeig, m, m_max,
+                                          qmr_type, qmrmax, qmrtol, pc, eig, C.byref(ret), C.byref(res), stats),
+              "scalit_pist_conv_sx")
+        e = np.array(eig[:])
+        return ret.value, e[0::2] + 1j * e[1::2], res.value, np.array(stats[:])
+
     def pist_vectors(self, win: int, m: int):
         psi = torch.empty((m, self.N), dtype=torch.float64, device=self.dev)
         check(L.lib().scalit_pist_vectors(self._h, win, _ptr(psi), self.N), "scalit_pist_vectors")

@@ -224,6 +224,7 @@ int scalit_destroy(scalit_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     release_solvers(c);
+    release_complex(c);
     for (int l = 0; l < c->sF; l++) {
         Level &L = c->lv[l];
         L.h0.release();

@@ -1,0 +1,717 @@
+// Complex (absorbing-potential) jobs: the resonance path of the reference's
+// OSB module.
+//
+//  * MYHX_DX (src/iterate/module/osb.hxdx.h:177-240): Y = (RES - E0 +- i A) X
+//    plus the real level products, on complex X.  The real operator applies to
+//    the real and imaginary planes as one batch of 2 nvec real vectors (the
+//    fused H*X kernels unchanged); one kernel adds the +-i A X term.
+//  * MYPX_DX (src/iterate/module/osb.mypxdx.h): kernels_px.hip, px_apply_cx.
+//  * QMR_CX (src/iterate/libiterate/qmr/qmrcx.f90:4-179) as OSB_QMRDX
+//    (src/iterate/module/osbw.qmrdx.h:16-27) calls it: HX_DX = MYHX_DX(sHC),
+//    PX_DX = MYPX_DX(sOSB, sPC).  Vectors stay on the GPU; the recurrence's
+//    scalars are formed on the host from deterministic two-stage reductions.
+//  * PIST_CONV_SX (src/iterate/libiterate/pist/pist_sx.f90:9-110) with
+//    pistH0_SX (pistHij_sx.f90:58-121) and MGS_ORTH_SX (orth/gsorth_sx.f90:
+//    47-74), as OSB_PISTCONVDX1 (src/iterate/module/osbw.pistconv.h:98-140)
+//    drives it: H0X = MYHX_DX(TMAP0), LinSolv = OSB_QMRDX with sHC = TMAP and
+//    sPC = TMAP (sAP) or TA1.  The Gram-Schmidt projections chain on the
+//    device (no host round trip per projection); the Krylov matrix's complex
+//    eigenvalues (ZGEEV in the reference) come from a Hessenberg QR on the
+//    host, over at most M_MAX values as the reference's root rank does.
+//
+// Complex vectors use the split layout: vector v's real part at x + 2 v N and
+// its imaginary part at x + (2 v + 1) N.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "scalit_impl.hpp"
+
+namespace scalit {
+
+using cd = std::complex<double>;
+
+constexpr int CX_G = 512;   // blocks of a reduction's first stage
+
+struct CxState {
+    DBuf<double> vec;    // QMR vectors: R, UX, UR, Q, S, V, Z, HX0 (2 N each)
+    DBuf<double> part;   // first-stage partial sums, CX_G x 2 per slot
+    DBuf<double> scal;   // reduced values, 2 per slot
+    std::vector<double> hscal;
+    DBuf<double> vj;     // PIST_CONV_SX's Krylov basis
+    DBuf<double> wj;     // its solve and H0X vectors
+    int slots = 0;
+};
+
+static CxState &cxstate(scalit_ctx *c)
+{
+    if (!c->cx) c->cx = new CxState();
+    return *c->cx;
+}
+
+void release_complex(scalit_ctx *c)
+{
+    delete c->cx;
+    c->cx = nullptr;
+}
+
+const double *cx_imag_diag(const scalit_ctx *c, int ntype, double *sign, int *rc)
+{
+    int which = -1;
+    double sg = 0.0;
+    switch (ntype) {
+    case SCALIT_TAAP0:
+    case SCALIT_TAAP: which = SCALIT_AP; sg = 1.0; break;
+    case SCALIT_TAAPP: which = SCALIT_APP; sg = 1.0; break;
+    case SCALIT_TAAPR: which = SCALIT_APR; sg = 1.0; break;
+    case SCALIT_TMAP0:
+    case SCALIT_TMAP: which = SCALIT_AP; sg = -1.0; break;
+    case SCALIT_TMAPP: which = SCALIT_APP; sg = -1.0; break;
+    case SCALIT_TMAPR: which = SCALIT_APR; sg = -1.0; break;
+    case SCALIT_TA0:
+    case SCALIT_TA1: break;
+    default: *rc = SCALIT_EINVAL; return nullptr;
+    }
+    *sign = sg;
+    if (which < 0) return nullptr;
+    if (!c->has_ap[which]) {
+        *rc = SCALIT_ESTATE;
+        return nullptr;
+    }
+    return c->ap[which].p;
+}
+
+// y += i sign a x on split complex vectors
+__global__ void k_cx_imag(long N, int nvec, const double *__restrict__ a, double sign, const double *__restrict__ x,
+                          double *__restrict__ y)
+{
+    const int v = blockIdx.y;
+    const double *xr = x + 2L * v * N, *xi = xr + N;
+    double *yr = y + 2L * v * N, *yi = yr + N;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        const double t = sign * a[e];
+        const double r = xr[e], i = xi[e];
+        yr[e] -= t * i;
+        yi[e] += t * r;
+    }
+}
+
+// first stage of sum_e a_e b_e (conj = 0, the bilinear DOT_CX) or
+// sum_e conj(a_e) b_e (conj = 1): one (re, im) partial per block
+__global__ void __launch_bounds__(256) k_cx_dot(long N, const double *__restrict__ a, const double *__restrict__ b,
+                                                int conj, double *__restrict__ part)
+{
+    __shared__ double sr[256], si[256];
+    const double *ar = a, *ai = a + N, *br = b, *bi = b + N;
+    double r = 0.0, i = 0.0;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < N; e += (long)gridDim.x * 256) {
+        const double xr = ar[e], xi = conj ? -ai[e] : ai[e], yr = br[e], yi = bi[e];
+        r += xr * yr - xi * yi;
+        i += xr * yi + xi * yr;
+    }
+    sr[threadIdx.x] = r;
+    si[threadIdx.x] = i;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sr[threadIdx.x] += sr[threadIdx.x + o];
+            si[threadIdx.x] += si[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sr[0];
+        part[2 * blockIdx.x + 1] = si[0];
+    }
+}
+
+// second stage in a fixed order; op 1 takes the principal complex square root
+__global__ void __launch_bounds__(256) k_cx_fin(int G, const double *__restrict__ part, int op,
+                                                double *__restrict__ out)
+{
+    __shared__ double sr[256], si[256];
+    double r = 0.0, i = 0.0;
+    for (int b = threadIdx.x; b < G; b += 256) {
+        r += part[2 * b];
+        i += part[2 * b + 1];
+    }
+    sr[threadIdx.x] = r;
+    si[threadIdx.x] = i;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sr[threadIdx.x] += sr[threadIdx.x + o];
+            si[threadIdx.x] += si[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double a = sr[0], b = si[0];
+        if (op == 1) {
+            const double m = hypot(a, b);
+            const double re = sqrt(0.5 * (m + a)), im = copysign(sqrt(0.5 * (m - a)), b);
+            a = re;
+            b = im;
+        }
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+// out = a x + b y (x or y may be null for a zero term; out may alias either)
+__global__ void k_cx_axpby(long N, double a_r, double a_i, const double *x, double b_r, double b_i, const double *y,
+                           double *out)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        double r = 0.0, i = 0.0;
+        if (x) {
+            const double xr = x[e], xi = x[e + N];
+            r += a_r * xr - a_i * xi;
+            i += a_r * xi + a_i * xr;
+        }
+        if (y) {
+            const double yr = y[e], yi = y[e + N];
+            r += b_r * yr - b_i * yi;
+            i += b_r * yi + b_i * yr;
+        }
+        out[e] = r;
+        out[e + N] = i;
+    }
+}
+
+// y -= s x, s a reduced device value
+__global__ void k_cx_axpy_dev(long N, const double *__restrict__ s, const double *__restrict__ x, double *__restrict__ y)
+{
+    const double sr = s[0], si = s[1];
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        const double xr = x[e], xi = x[e + N];
+        y[e] -= sr * xr - si * xi;
+        y[e + N] -= sr * xi + si * xr;
+    }
+}
+
+// y /= s, s a reduced device value (left unchanged when s = 0)
+__global__ void k_cx_div_dev(long N, const double *__restrict__ s, double *__restrict__ y)
+{
+    const double sr = s[0], si = s[1];
+    const double den = sr * sr + si * si;
+    if (den == 0.0) return;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        const double yr = y[e], yi = y[e + N];
+        y[e] = (yr * sr + yi * si) / den;
+        y[e + N] = (yi * sr - yr * si) / den;
+    }
+}
+
+static unsigned cx_grid(long N) { return (unsigned)std::min<long>((N + 255) / 256, 2048); }
+static int cx_blocks(long N) { return (int)std::min<long>((N + 255) / 256, CX_G); }
+
+static int cx_slots(CxState &X, int n)
+{
+    if (X.slots >= n) return SCALIT_OK;
+    SC_TRY(X.part.alloc((size_t)n * CX_G * 2));
+    SC_TRY(X.scal.alloc((size_t)n * 2));
+    X.hscal.assign((size_t)n * 2, 0.0);
+    X.slots = n;
+    return SCALIT_OK;
+}
+
+// slot <- sum a b (or conj(a) b); op 1: its square root
+static void cx_dot(CxState &X, long N, const double *a, const double *b, int conj, int slot, int op, hipStream_t s)
+{
+    const int G = cx_blocks(N);
+    double *part = X.part.p + (size_t)slot * CX_G * 2;
+    hipLaunchKernelGGL(k_cx_dot, dim3(G), dim3(256), 0, s, N, a, b, conj, part);
+    hipLaunchKernelGGL(k_cx_fin, dim3(1), dim3(256), 0, s, G, (const double *)part, op, X.scal.p + 2 * slot);
+}
+
+// the first n slots to the host
+static int cx_read(CxState &X, int n, hipStream_t s)
+{
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(X.hscal.data(), X.scal.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SCALIT_OK;
+}
+
+static cd slot_val(const CxState &X, int k) { return cd(X.hscal[2 * k], X.hscal[2 * k + 1]); }
+
+static void cx_axpby(long N, cd a, const double *x, cd b, const double *y, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cx_axpby, dim3(cx_grid(N)), dim3(256), 0, s, N, a.real(), a.imag(), x, b.real(), b.imag(),
+                       y, out);
+}
+
+// MYHX_DX on nvec split complex vectors
+static int hx_apply_cx(scalit_ctx *c, int ntype, int nvec, const double *x, double *y, hipStream_t s)
+{
+    int rc = SCALIT_OK;
+    double sign = 0.0;
+    const double *a = cx_imag_diag(c, ntype, &sign, &rc);
+    SC_TRY(rc);
+    const long N = c->N;
+    const double sh = (ntype == SCALIT_TA0 || ntype == SCALIT_TAAP0 || ntype == SCALIT_TMAP0) ? 0.0 : c->e0;
+    SC_TRY(hx_apply_f64(c, nullptr, sh, 2 * nvec, x, N, y, N, false, nullptr, s));
+    if (a)
+        hipLaunchKernelGGL(k_cx_imag, dim3(cx_grid(N), nvec), dim3(256), 0, s, N, nvec, a, sign, x, y);
+    HIP_TRY(hipGetLastError());
+    return SCALIT_OK;
+}
+
+// QMR_CX (qmrcx.f90:4-179).  x (device, split) holds the start vector when
+// ntype > 0; *ret gets QMR_CX's return value.
+static int qmr_cx_run(scalit_ctx *c, int nosb, int hc, int pc, int ntype, int qmrmax, double tol, const double *b,
+                      double *x, int *ret, double *res_out, hipStream_t s)
+{
+    CxState &X = cxstate(c);
+    const long N = c->N, L = 2 * N;
+    SC_TRY(X.vec.alloc((size_t)8 * L));
+    SC_TRY(cx_slots(X, 4));
+    double *R = X.vec.p, *UX = R + L, *UR = UX + L, *Q = UR + L, *S = Q + L, *V = S + L, *Z = V + L, *HX0 = Z + L;
+    auto nrm = [&](const double *v, int slot) { cx_dot(X, N, v, v, 1, slot, 0, s); };
+    auto hx = [&](const double *in, double *out) { return hx_apply_cx(c, hc, 1, in, out, s); };
+    auto px = [&](const double *in, double *out) { return px_apply_cx(c, nosb, pc, 1, in, out, s); };
+    int code = -1;
+    double res = 0.0;
+    if (ntype <= 0) HIP_TRY(hipMemsetAsync(x, 0, sizeof(double) * L, s));
+    HIP_TRY(hipMemcpyAsync(R, b, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(V, b, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
+    double zvrat = 1.0;
+    nrm(b, 0);
+    SC_TRY(px(b, Z));
+    nrm(Z, 1);
+    SC_TRY(cx_read(X, 2, s));
+    const double bnrm = std::sqrt(std::abs(slot_val(X, 0).real()));
+    double vnrm = bnrm;
+    double znrm = std::sqrt(std::abs(slot_val(X, 1).real()));
+    double gamamo = 1.0, thetamo = 0.0;
+    cd eta(-1.0, 0.0), qsdot(0.0, 0.0);
+    code = 0;
+    for (int i = 1; i <= qmrmax; i++) {
+        if (vnrm == 0.0 || znrm == 0.0) break;
+        zvrat = zvrat * znrm / vnrm;
+        cx_axpby(N, cd(1.0 / vnrm, 0.0), V, 0.0, nullptr, V, s);
+        cx_axpby(N, cd(1.0 / znrm, 0.0), Z, 0.0, nullptr, Z, s);
+        cx_dot(X, N, V, Z, 0, 0, 0, s);
+        SC_TRY(cx_read(X, 1, s));
+        const cd vzdot = slot_val(X, 0);
+        if (std::abs(vzdot) == 0.0) break;
+        if (i == 1)
+            HIP_TRY(hipMemcpyAsync(Q, Z, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
+        else
+            cx_axpby(N, 1.0, Z, -(vnrm * vzdot) / qsdot, Q, Q, s);
+        SC_TRY(hx(Q, S));
+        cx_axpby(N, zvrat, S, 0.0, nullptr, S, s);
+        cx_dot(X, N, Q, S, 0, 0, 0, s);
+        SC_TRY(cx_read(X, 1, s));
+        qsdot = slot_val(X, 0);
+        const cd beta = qsdot / vzdot;
+        if (std::abs(qsdot) == 0.0 || std::abs(beta) == 0.0) break;
+        cx_axpby(N, 1.0, S, -beta, V, V, s);
+        nrm(V, 0);
+        SC_TRY(px(V, Z));
+        cx_axpby(N, 1.0 / zvrat, Z, 0.0, nullptr, Z, s);
+        nrm(Z, 1);
+        SC_TRY(cx_read(X, 2, s));
+        const double vnrmpo = std::sqrt(std::abs(slot_val(X, 0).real()));
+        znrm = std::sqrt(std::abs(slot_val(X, 1).real()));
+        const double theta = vnrmpo / (gamamo * std::abs(beta));
+        const double gama = 1.0 / std::sqrt(1.0 + theta * theta);
+        eta = -eta * vnrm * gama * gama / (beta * gamamo * gamamo);
+        if (gama == 0.0) break;
+        if (i == 1) {
+            cx_axpby(N, eta * zvrat, Q, 0.0, nullptr, UX, s);
+            cx_axpby(N, eta, S, 0.0, nullptr, UR, s);
+        } else {
+            const double tg = (thetamo * gama) * (thetamo * gama);
+            cx_axpby(N, eta * zvrat, Q, tg, UX, UX, s);
+            cx_axpby(N, eta, S, tg, UR, UR, s);
+        }
+        cx_axpby(N, 1.0, x, 1.0, UX, x, s);
+        cx_axpby(N, 1.0, R, -1.0, UR, R, s);
+        const int t = std::abs(ntype);
+        if (t == 1) {
+            SC_TRY(hx(x, HX0));
+            nrm(HX0, 0);
+            cx_axpby(N, 1.0, HX0, -1.0, b, HX0, s);
+            nrm(HX0, 1);
+            SC_TRY(cx_read(X, 2, s));
+            res = std::sqrt(std::abs(slot_val(X, 1).real())) / std::sqrt(std::abs(slot_val(X, 0).real()));
+        } else if (t == 2) {
+            nrm(R, 0);
+            nrm(UX, 1);
+            nrm(x, 2);
+            nrm(UR, 3);
+            SC_TRY(cx_read(X, 4, s));
+            const double r1 = std::sqrt(std::abs(slot_val(X, 0).real())) / bnrm;
+            const double r2 = std::sqrt(std::abs(slot_val(X, 1).real() / slot_val(X, 2).real()));
+            const double r3 = std::sqrt(std::abs(slot_val(X, 3).real() / slot_val(X, 0).real()));
+            res = std::max(r1, std::max(r2, r3));
+        } else {
+            nrm(R, 0);
+            SC_TRY(cx_read(X, 1, s));
+            res = std::sqrt(std::abs(slot_val(X, 0).real())) / bnrm;
+        }
+        if (std::abs(res) < tol) {
+            code = i;
+            break;
+        }
+        code = -i;
+        vnrm = vnrmpo;
+        gamamo = gama;
+        thetamo = theta;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    *ret = code;
+    if (res_out) *res_out = res;
+    return SCALIT_OK;
+}
+
+// ---------------------------------------------------------------- host eigenvalues
+// Eigenvalues of a general complex matrix (ZGEEV with JOBVL = JOBVR = 'N'):
+// Householder reduction to upper Hessenberg form, then implicit single-shift
+// QR sweeps (Wilkinson shifts, exceptional shifts at 10 and 20 sweeps without
+// deflation) on the active block, deflating on negligible subdiagonals.
+static int cx_eigvals(int n, std::vector<cd> &A, std::vector<cd> &w)
+{
+    auto a = [&](int i, int j) -> cd & { return A[i + (size_t)n * j]; };
+    w.assign(n, cd(0.0, 0.0));
+    for (int k = 0; k + 2 < n; k++) {
+        double xn = 0.0;
+        for (int i = k + 1; i < n; i++) xn += std::norm(a(i, k));
+        xn = std::sqrt(xn);
+        if (xn == 0.0) continue;
+        const cd x0 = a(k + 1, k);
+        const cd ph = std::abs(x0) == 0.0 ? cd(1.0, 0.0) : x0 / std::abs(x0);
+        const cd alpha = -ph * xn;
+        std::vector<cd> v(n, cd(0.0, 0.0));
+        for (int i = k + 1; i < n; i++) v[i] = a(i, k);
+        v[k + 1] -= alpha;
+        double vn = 0.0;
+        for (int i = k + 1; i < n; i++) vn += std::norm(v[i]);
+        vn = std::sqrt(vn);
+        if (vn == 0.0) continue;
+        for (int i = k + 1; i < n; i++) v[i] /= vn;
+        // A <- (I - 2 v v^H) A (I - 2 v v^H)
+        for (int j = 0; j < n; j++) {
+            cd t(0.0, 0.0);
+            for (int i = k + 1; i < n; i++) t += std::conj(v[i]) * a(i, j);
+            for (int i = k + 1; i < n; i++) a(i, j) -= 2.0 * v[i] * t;
+        }
+        for (int i = 0; i < n; i++) {
+            cd t(0.0, 0.0);
+            for (int j = k + 1; j < n; j++) t += a(i, j) * v[j];
+            for (int j = k + 1; j < n; j++) a(i, j) -= 2.0 * t * std::conj(v[j]);
+        }
+        for (int i = k + 2; i < n; i++) a(i, k) = cd(0.0, 0.0);
+    }
+    const double eps = 2.220446049250313e-16;
+    int ihi = n - 1, iter = 0, total = 0;
+    while (ihi >= 0) {
+        int l = ihi;
+        for (; l > 0; l--) {
+            const double h = std::abs(a(l - 1, l - 1)) + std::abs(a(l, l));
+            if (std::abs(a(l, l - 1)) <= eps * (h == 0.0 ? 1.0 : h)) {
+                a(l, l - 1) = cd(0.0, 0.0);
+                break;
+            }
+        }
+        if (l == ihi) {
+            w[ihi] = a(ihi, ihi);
+            ihi--;
+            iter = 0;
+            continue;
+        }
+        if (++total > 60 * n + 60) return SCALIT_ESTATE;
+        iter++;
+        cd mu;
+        if (iter == 10 || iter == 20) {
+            mu = a(ihi, ihi) + 0.75 * std::abs(a(ihi, ihi - 1).real());
+        } else {
+            const cd p = a(ihi - 1, ihi - 1), q = a(ihi - 1, ihi), r = a(ihi, ihi - 1), d = a(ihi, ihi);
+            const cd half = 0.5 * (p - d);
+            const cd disc = std::sqrt(half * half + q * r);
+            const cd m1 = d - q * r / (half + disc), m2 = d - q * r / (half - disc);
+            const bool ok1 = std::abs(half + disc) > 0.0, ok2 = std::abs(half - disc) > 0.0;
+            if (ok1 && ok2) mu = std::abs(m1 - d) <= std::abs(m2 - d) ? m1 : m2;
+            else mu = ok1 ? m1 : (ok2 ? m2 : d);
+        }
+        // implicit QR sweep over rows/columns l..ihi
+        for (int k = l; k < ihi; k++) {
+            cd x, y;
+            if (k == l) {
+                x = a(l, l) - mu;
+                y = a(l + 1, l);
+            } else {
+                x = a(k, k - 1);
+                y = a(k + 1, k - 1);
+            }
+            const double ax = std::abs(x), ay = std::abs(y);
+            if (ay == 0.0) continue;
+            const double rr = std::hypot(ax, ay);
+            double cs;
+            cd sn;
+            if (ax == 0.0) {
+                cs = 0.0;
+                sn = std::conj(y) / ay;
+            } else {
+                cs = ax / rr;
+                sn = (x / ax) * std::conj(y) / rr;
+            }
+            // rows k, k+1 from the left
+            for (int j = std::max(l, k - 1); j <= ihi; j++) {
+                const cd u = a(k, j), v = a(k + 1, j);
+                a(k, j) = cs * u + sn * v;
+                a(k + 1, j) = -std::conj(sn) * u + cs * v;
+            }
+            if (k > l) a(k + 1, k - 1) = cd(0.0, 0.0);
+            // columns k, k+1 from the right
+            for (int i = l; i <= std::min(k + 2, ihi); i++) {
+                const cd u = a(i, k), v = a(i, k + 1);
+                a(i, k) = cs * u + std::conj(sn) * v;
+                a(i, k + 1) = -sn * u + cs * v;
+            }
+        }
+    }
+    return SCALIT_OK;
+}
+
+// util_cx.f90 getOpCX: 1 = real part, 6 = |z|^2
+static double op_cx(cd z, int t) { return t == 1 ? z.real() : z.real() * z.real() + z.imag() * z.imag(); }
+
+// getWindow0 (util.f90:669-717) on op_cx(., 1): the m values whose real part
+// is nearest e0, first index winning ties
+static void window_cx(double e0, const std::vector<cd> &vec, int m, std::vector<cd> &out)
+{
+    const int n = (int)vec.size();
+    out.assign(m, cd(0.0, 0.0));
+    if (n < m) {
+        for (int i = 0; i < n; i++) out[i] = vec[i];
+        for (int i = std::max(n - 1, 0); i < m; i++) out[i] = vec[n - 1];
+        return;
+    }
+    std::vector<char> used(n, 0);
+    for (int i = 0; i < m; i++) {
+        int idx = -1;
+        double best = 0.0;
+        for (int j = 0; j < n; j++) {
+            if (used[j]) continue;
+            const double d = std::fabs(op_cx(vec[j], 1) - e0);
+            if (idx < 0 || best > d) {
+                idx = j;
+                best = d;
+            }
+        }
+        out[i] = vec[idx];
+        used[idx] = 1;
+    }
+}
+
+struct PistSx {
+    double e0, etol;
+    int ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, pc;
+    double qmrtol;
+};
+
+// PIST_CONV_SX.  x0: device split start vector; eig: host, 2 m doubles
+// (re, im pairs, ascending real part); *ret the reference's return value.
+static int pist_sx_run(scalit_ctx *c, const PistSx &p, const double *x0, double *eig, int *ret, double *res_out,
+                       int *stats, hipStream_t s)
+{
+    CxState &X = cxstate(c);
+    const long N = c->N, L = 2 * N;
+    const int m = p.m, mm = p.m_max;
+    *ret = 0;
+    if (stats)
+        for (int k = 0; k < 4; k++) stats[k] = 0;
+    if (p.stepeig >= mm || p.startcnt > mm || m > mm) return SCALIT_OK;   // PIST_CONV_SX = 0
+    SC_TRY(X.vj.alloc((size_t)mm * L));
+    SC_TRY(X.wj.alloc((size_t)2 * L));
+    SC_TRY(cx_slots(X, std::max(mm, 4)));
+    double *W = X.wj.p, *HW = W + L;
+    std::vector<cd> hmat((size_t)mm * mm, cd(0.0, 0.0));
+    auto H = [&](int i, int j) -> cd & { return hmat[(i - 1) + (size_t)mm * (j - 1)]; };
+    auto VJ = [&](int j) { return X.vj.p + (size_t)(j - 1) * L; };
+    int cnt_low = 1, cnt_high = std::max(std::max(p.startcnt, m), p.stepeig + 1);
+    std::vector<cd> tmp, old(m), ev(m);
+    double res = 0.0;
+    int code = 0;
+    auto eigs = [&](int num, std::vector<cd> &w) {
+        std::vector<cd> A((size_t)num * num);
+        for (int j = 1; j <= num; j++)
+            for (int i = 1; i <= num; i++) A[(i - 1) + (size_t)num * (j - 1)] = H(i, j);
+        return cx_eigvals(num, A, w);
+    };
+    while (cnt_high <= mm) {
+        // pistH0_SX (pistHij_sx.f90:58-121)
+        int m_start;
+        if (cnt_low == 1) {
+            m_start = 1;
+            cx_dot(X, N, x0, x0, 0, 0, 1, s);
+            SC_TRY(cx_read(X, 1, s));
+            const cd t = slot_val(X, 0);
+            if (t == cd(0.0, 0.0)) {
+                code = -cnt_high;
+                break;
+            }
+            cx_axpby(N, 1.0 / t, x0, 0.0, nullptr, VJ(1), s);
+        } else {
+            m_start = cnt_low - 1;
+        }
+        bool fail = false;
+        for (int j = m_start; j <= cnt_high - 1 && !fail; j++) {
+            int num = 0;
+            double qres = 0.0;
+            SC_TRY(qmr_cx_run(c, c->osb, SCALIT_TMAP, p.pc, p.qmr_type, p.qmrmax, p.qmrtol, VJ(j), W, &num, &qres, s));
+            if (stats) {
+                stats[0] += std::abs(num);
+                stats[1] += 1;
+            }
+            if (num <= 0) {
+                fail = true;
+                break;
+            }
+            // MGS_ORTH_SX: the projections chained on the device
+            double *vn = VJ(j + 1);
+            HIP_TRY(hipMemcpyAsync(vn, W, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
+            for (int i = 1; i <= j; i++) {
+                cx_dot(X, N, VJ(i), vn, 0, 0, 0, s);
+                hipLaunchKernelGGL(k_cx_axpy_dev, dim3(cx_grid(N)), dim3(256), 0, s, N, (const double *)X.scal.p,
+                                   (const double *)VJ(i), vn);
+            }
+            cx_dot(X, N, vn, vn, 0, 1, 1, s);
+            SC_TRY(cx_read(X, 2, s));
+            if (slot_val(X, 1) == cd(0.0, 0.0)) {
+                fail = true;
+                break;
+            }
+            hipLaunchKernelGGL(k_cx_div_dev, dim3(cx_grid(N)), dim3(256), 0, s, N, (const double *)(X.scal.p + 2), vn);
+        }
+        if (fail) {
+            code = -cnt_high;
+            break;
+        }
+        // HMAT(i, j) = v_i^T H0 v_j
+        for (int j = 1; j <= cnt_high; j++) {
+            const int i0 = j >= cnt_low ? 1 : cnt_low;
+            SC_TRY(hx_apply_cx(c, SCALIT_TMAP0, 1, VJ(j), HW, s));
+            if (stats) stats[3] += 1;
+            for (int i = i0; i <= cnt_high; i++) cx_dot(X, N, VJ(i), HW, 0, i - i0, 0, s);
+            SC_TRY(cx_read(X, cnt_high - i0 + 1, s));
+            for (int i = i0; i <= cnt_high; i++) H(i, j) = slot_val(X, i - i0);
+        }
+        int num = cnt_high - p.stepeig;
+        if (eigs(num, tmp) != SCALIT_OK) {
+            code = -num;
+            break;
+        }
+        window_cx(p.e0, tmp, m, old);
+        if (eigs(cnt_high, tmp) != SCALIT_OK) {
+            code = -cnt_high;
+            break;
+        }
+        window_cx(p.e0, tmp, m, ev);
+        // getDiffMax_CX with nConvType 6 (util.f90:84-100 on |z|^2)
+        res = std::fabs(op_cx(ev[0], 6) - op_cx(old[0], 6));
+        for (int k = 1; k < m; k++) res = std::max(res, std::fabs(op_cx(ev[k], 6) - op_cx(old[k], 6)));
+        if (stats) stats[2] = cnt_high;
+        if (res <= p.etol) {
+            code = cnt_high;
+            break;
+        }
+        code = -cnt_high;
+        cnt_low = cnt_high + 1;
+        if (cnt_low >= mm) {
+            code = -mm;
+            break;
+        }
+        cnt_high = std::min(cnt_high + p.stepcnt, mm);
+    }
+    // Reorder_CX('A', 1): ascending real part, first index first among ties
+    std::vector<int> ord(m);
+    for (int k = 0; k < m; k++) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ev[a].real() < ev[b].real(); });
+    for (int k = 0; k < m; k++) {
+        eig[2 * k] = ev[ord[k]].real();
+        eig[2 * k + 1] = ev[ord[k]].imag();
+    }
+    *ret = code;
+    if (res_out) *res_out = res;
+    return SCALIT_OK;
+}
+
+}  // namespace scalit
+
+using namespace scalit;
+
+extern "C" {
+
+int scalit_set_ap(scalit_ctx *c, int which, const double *ap)
+{
+    if (!c || !ap || which < SCALIT_AP || which > SCALIT_APR) return SCALIT_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+    SC_TRY(c->ap[which].alloc(c->N));
+    HIP_TRY(hipMemcpy(c->ap[which].p, ap, sizeof(double) * c->N, hipMemcpyHostToDevice));
+    c->has_ap[which] = true;
+    c->gen++;
+    return SCALIT_OK;
+}
+
+int scalit_hx_cx(scalit_ctx *c, int ntype, int nvec, const double *x, double *y, scalit_stream stream)
+{
+    if (!c || nvec < 1 || !x || !y || x == y) return SCALIT_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->has_res) return SCALIT_ESTATE;
+    return hx_apply_cx(c, ntype, nvec, x, y, (hipStream_t)stream);
+}
+
+int scalit_px_cx(scalit_ctx *c, int nosb, int npc, int nvec, const double *x, double *y, scalit_stream stream)
+{
+    if (!c || nvec < 1 || !x || !y || x == y) return SCALIT_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+    return px_apply_cx(c, nosb, npc, nvec, x, y, (hipStream_t)stream);
+}
+
+int scalit_qmr_cx(scalit_ctx *c, int qmr_type, int hc, int pc, int qmrmax, double qmrtol, const double *b, double *x,
+                  int *ret, double *res)
+{
+    if (!c || !b || !x || !ret || b == x || qmrmax < 0) return SCALIT_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->has_res || !c->has_precond) return SCALIT_ESTATE;
+    if (c->osb == SCALIT_TOSBW) return SCALIT_EINVAL;
+    HIP_TRY(hipDeviceSynchronize());
+    return qmr_cx_run(c, c->osb, hc, pc, qmr_type, qmrmax, qmrtol, b, x, ret, res, c->stream);
+}
+
+int scalit_pist_conv_sx(scalit_ctx *c, double e0, double etol, int ntype, const double *x0, int startcnt, int stepcnt,
+                        int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int pc, double *eig,
+                        int *ret, double *res, int *stats)
+{
+    if (!c || !x0 || !eig || !ret || m < 1 || m_max < 1 || ntype > 0) return SCALIT_EINVAL;
+    if (pc != SCALIT_TMAP && pc != SCALIT_TA1) return SCALIT_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->has_res || !c->has_precond || !c->has_ap[SCALIT_AP]) return SCALIT_ESTATE;
+    if (c->osb == SCALIT_TOSBW) return SCALIT_EINVAL;
+    HIP_TRY(hipDeviceSynchronize());
+    PistSx p{e0, etol, ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, pc, qmrtol};
+    return pist_sx_run(c, p, x0, eig, ret, res, stats, c->stream);
+}
+
+int scalit_small_eig_cx(int n, const double *a, int lda, double *w)
+{
+    if (n < 1 || !a || !w || lda < n) return SCALIT_EINVAL;
+    std::vector<cd> A((size_t)n * n), ev;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++)
+            A[i + (size_t)n * j] = cd(a[2 * (i + (size_t)lda * j)], a[2 * (i + (size_t)lda * j) + 1]);
+    SC_TRY(cx_eigvals(n, A, ev));
+    for (int i = 0; i < n; i++) {
+        w[2 * i] = ev[i].real();
+        w[2 * i + 1] = ev[i].imag();
+    }
+    return SCALIT_OK;
+}
+
+}  // extern "C"

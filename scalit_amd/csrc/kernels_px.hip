@@ -527,9 +527,10 @@ __global__ void k_transpose_blocks(int n, long total, const T *__restrict__ src,
 // here on first use after the preconditioner changed (kernel buffers of a
 // new generation: recorded graphs are re-recorded).
 template <typename T>
-static int ensure_transposes(scalit_ctx *c, DBuf<T> Level::*vb, DBuf<T> Level::*vt, hipStream_t s)
+static int ensure_transposes(scalit_ctx *c, DBuf<T> Level::*vb, DBuf<T> Level::*vt, hipStream_t s,
+                             bool with_l1 = false)
 {
-    for (int l = l1_big<T>(c->lv[0].n) ? 0 : 1; l < c->sF; l++) {
+    for (int l = (with_l1 || l1_big<T>(c->lv[0].n)) ? 0 : 1; l < c->sF; l++) {
         Level &L = c->lv[l];
         if ((L.*vt).p) continue;
         const long total = (long)L.blk * L.n * L.n;
@@ -604,6 +605,99 @@ int px_apply_bf16(scalit_ctx *c, int nosb, const float *shifts_dev, float shift0
     }
     return px_impl<float, bf16_t>(c, nosb, vosb, vosbt, c->eig0_f.p, c->work_f.p, shifts_dev, shift0, (float)c->de,
                                   (float)c->beta, nvec, x, ldx, y, ldy, active, s);
+}
+
+// The complex kernel of MYPX_DX (src/iterate/module/osb.mypxdx.h): z / d with
+// d = EIG0 - E0 (TOSBD1: clamped to +-DE inside the window; TOSBD2:
+// alpha + beta (EIG0 - E0) there, alpha = -mBeta DE, beta = mBeta + 1) plus
+// i sign A, on split complex vectors in place (ld N per plane).
+__global__ void k_cx_div(long N, int nvec, const double *__restrict__ eig0, double e0, int nosb, double de,
+                         double mbeta, const double *__restrict__ a, double sign, double *__restrict__ z)
+{
+    const int v = blockIdx.y;
+    double *zr = z + 2L * v * N, *zi = zr + N;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        double ade = eig0[e] - e0;
+        if (nosb == SCALIT_TOSBD1) {
+            if (fabs(ade) <= de) ade = ade < 0.0 ? -de : de;
+        } else if (nosb == SCALIT_TOSBD2) {
+            if (fabs(ade) <= de) ade = -mbeta * de + (mbeta + 1.0) * ade;
+        }
+        const double di = a ? sign * a[e] : 0.0;
+        const double xr = zr[e], xi = zi[e];
+        if (di == 0.0) {
+            zr[e] = xr / ade;
+            zi[e] = xi / ade;
+        } else {
+            // (xr + i xi) / (ade + i di), scaled against overflow as Smith's division
+            if (fabs(ade) >= fabs(di)) {
+                const double r = di / ade, den = ade + di * r;
+                zr[e] = (xr + xi * r) / den;
+                zi[e] = (xi - xr * r) / den;
+            } else {
+                const double r = ade / di, den = di + ade * r;
+                zr[e] = (xr * r + xi) / den;
+                zi[e] = (xi * r - xr) / den;
+            }
+        }
+    }
+}
+
+int px_apply_cx(scalit_ctx *c, int nosb, int npc, int nvec, const double *x, double *y, hipStream_t s)
+{
+    if (!c->has_precond) return SCALIT_ESTATE;
+    if (nosb != SCALIT_TOSB && nosb != SCALIT_TOSBD1 && nosb != SCALIT_TOSBD2) return SCALIT_EINVAL;
+    int rc = SCALIT_OK;
+    double sign = 0.0;
+    const double *a = cx_imag_diag(c, npc, &sign, &rc);
+    SC_TRY(rc);
+    const long N = c->N;
+    const int sF = c->sF, nr = 2 * nvec;
+    const Level &L1 = c->lv[0];
+    if ((long)BR_VB * L1.n * sizeof(double) > PX_LDS_MAX) return SCALIT_EINVAL;
+    {
+        double *old = c->work.p;
+        SC_TRY(c->work.alloc((size_t)3 * nr * N));
+        if (c->work.p != old) c->gen++;
+    }
+    SC_TRY(ensure_transposes<double>(c, &Level::vosb, &Level::vosbt, s, true));
+    const double e0 = (npc == SCALIT_TA0 || npc == SCALIT_TAAP0 || npc == SCALIT_TMAP0) ? 0.0 : c->e0;
+    double *w0 = c->work.p, *w1 = w0 + (long)nr * N, *zt = w1 + (long)nr * N;
+    // V^T for levels sF..2 (1-based) on the real and imaginary planes
+    const double *cur = x;
+    double *dst = w0;
+    for (int l = sF - 1; l >= 1; l--) {
+        rotate<double, double, double>(c->lv[l], c->lv[l].vosb.p, c->lv[l].vosbt.p, true, nr, cur, N, dst, N,
+                                       nullptr, s);
+        cur = dst;
+        dst = (dst == w0) ? w1 : w0;
+    }
+    // level 1: V1^T, the complex divide, V1
+    const int n = L1.n, nb = L1.blk;
+    const dim3 g((n + 255) / 256, nb);
+    const size_t lds = (size_t)n * BR_VB * sizeof(double);
+    allow_full_lds((const void *)k_blockrot<double, double, double, false>);
+    hipLaunchKernelGGL((k_blockrot<double, double, double, false>), g, dim3(256), lds, s, n, nb, nr, L1.vosbt.p,
+                       c->eig0.p, (const double *)nullptr, 0.0, SCALIT_TOSB, 0.0, 0.0, cur, N, zt, N,
+                       (const int *)nullptr);
+    hipLaunchKernelGGL(k_cx_div, dim3((unsigned)std::min<long>((N + 255) / 256, 2048), nvec), dim3(256), 0, s, N,
+                       nvec, c->eig0.p, e0, nosb, std::fabs(c->de), c->beta, a, sign, zt);
+    double *out1 = sF == 1 ? y : dst;
+    hipLaunchKernelGGL((k_blockrot<double, double, double, false>), g, dim3(256), lds, s, n, nb, nr, L1.vosb.p,
+                       c->eig0.p, (const double *)nullptr, 0.0, SCALIT_TOSB, 0.0, 0.0, (const double *)zt, N, out1,
+                       N, (const int *)nullptr);
+    cur = out1;
+    dst = (out1 == w0) ? w1 : w0;
+    // V for levels 2..sF; the last writes y
+    for (int l = 1; l < sF; l++) {
+        double *o = (l == sF - 1) ? y : dst;
+        rotate<double, double, double>(c->lv[l], c->lv[l].vosb.p, c->lv[l].vosbt.p, false, nr, cur, N, o, N, nullptr,
+                                       s);
+        cur = o;
+        dst = (o == w0) ? w1 : w0;
+    }
+    HIP_TRY(hipGetLastError());
+    return SCALIT_OK;
 }
 
 }  // namespace scalit

@@ -93,6 +93,7 @@ struct Level {
 };
 
 struct PistState;   // defined in solvers.hip
+struct CxState;     // defined in complex.hip
 
 // autotuned H*X pass variants for one (precision, batch size)
 struct TuneEntry {
@@ -140,6 +141,11 @@ struct scalit_ctx {
     scalit::DBuf<double> osbw_m, osbw_eig0, osbw_d;
     int osbw_n = 0, osbw_nruns = 0;
     double osbw_shift = 0.0;
+    // the absorbing potentials AP, APP, APR of resonance and CRP jobs (N
+    // values each) and the complex solvers' work buffers
+    scalit::DBuf<double> ap[3];
+    bool has_ap[3] = {false, false, false};
+    scalit::CxState *cx = nullptr;
     // FP64 QMR on small fused 3-level operators: 1 = the persistent solver,
     // 0 = the launch-per-step iteration, -1 = as SCALIT_QMR_PERSIST says
     // (default 1)
@@ -186,6 +192,13 @@ int precond_to_f32(scalit_ctx *c);
 // then divides by osbw_eig0 (the window's components become 0) and
 // osbw_scatter adds V1(:, window) d to its output
 void osbw_gather(scalit_ctx *c, int nvec, const double *x, long ldx, const int *active, hipStream_t s);
+// Complex vectors in split layout: vector v's real part at x + 2 v N, its
+// imaginary part at x + (2 v + 1) N.  px_apply_cx is MYPX_DX (kernels_px.hip);
+// cx_imag_diag returns the +-AP / APP / APR diagonal an nType or nPC adds
+// (nullptr when none, SCALIT_ESTATE in *rc when that array is not set).
+int px_apply_cx(scalit_ctx *c, int nosb, int npc, int nvec, const double *x, double *y, hipStream_t s);
+const double *cx_imag_diag(const scalit_ctx *c, int ntype, double *sign, int *rc);
+void release_complex(scalit_ctx *c);
 void osbw_scatter(scalit_ctx *c, int nvec, double *y, long ldy, const int *active, hipStream_t s);
 
 // OSB set-up (kernels_diag.hip)

@@ -388,6 +388,20 @@ def outer_layer(prob: Problem, seed: int, eps: float = 2e-4, potential: bool = F
     return a.reshape(-1, order="F")   # a[p, i, j] at p + NIN (i + n3 j)
 
 
+def absorbing_potential(prob: Problem, strength: float = 0.02, start: float = 0.7, coord: str = "R") -> np.ndarray:
+    """A quadratic absorbing potential on the grid, strength ((x - x0) /
+    (x_max - x0))^2 beyond x0 = x_min + start (x_max - x_min) along R (or r):
+    the AP / APP / APR arrays of a resonance or CRP job (which the reference
+    loads from files, src/iterate/module/osb.io.h:20-30)."""
+    x = prob.R if coord == "R" else prob.r
+    lo, hi = float(x.min()), float(x.max())
+    x0 = lo + start * (hi - lo)
+    a1 = np.where(x > x0, strength * ((x - x0) / (hi - x0)) ** 2, 0.0)
+    n1, n2, n3 = prob.sN
+    full = a1[None, :, None] if coord == "R" else a1[None, None, :]
+    return np.ascontiguousarray(np.broadcast_to(full, (n3, n2, n1))).reshape(-1)
+
+
 def random_vectors(n: int, b: int, seed: int) -> np.ndarray:
     """B standard-normal vectors of length n, shape (b, n)."""
     return rng(seed).standard_normal((b, n))

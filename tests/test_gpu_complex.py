@@ -1,0 +1,150 @@
+"""Complex (absorbing-potential) jobs on the GPU through the C ABI: MYHX_DX,
+MYPX_DX, QMR_CX and PIST_CONV_SX against the oracle's restatements (each
+pinned against the reference's compiled MYHX_DX / MYPX_DX / QMR_CX and by
+dense algebra), and resonance energies against a dense eigensolve of
+H - i AP.
+
+Tolerances: operators 1e-12 relative; QMR_CX the oracle's return code
+(+-1; for the full criterion, whose |UR|/|R| term is rounding noise near
+the tolerance, within 5 iterations or 5% of them) and solutions to 1e-7; resonance energies 1e-10 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from scalit_amd import _lib as L
+from scalit_amd import problems as P
+from scalit_amd.context import Context
+
+pytestmark = pytest.mark.gpu
+
+ALL_TYPES = [L.TA0, L.TA1, L.TAAP0, L.TMAP0, L.TAAP, L.TAAPP, L.TAAPR, L.TMAP, L.TMAPP, L.TMAPR]
+
+
+def cx_setup(grid, strength=0.02, seed=3):
+    prob = P.build_problem(*grid, P.gaussian_potential(seed), seed=seed)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    vosb, eig0 = op.diag()
+    srt = np.sort(eig0)
+    e0 = float(0.5 * (srt[len(srt) // 8] + srt[len(srt) // 8 + 1]))
+    de = float(srt[len(srt) // 8 + 10] - e0)
+    op.set_shift(e0, de, 0.3)
+    ctx = Context.from_problem(prob, e0=e0)
+    ctx.set_precond(vosb, eig0)
+    ctx.set_shift(e0, de, 0.3)
+    aps = (P.absorbing_potential(prob, strength), P.absorbing_potential(prob, 0.01, 0.5),
+           P.absorbing_potential(prob, 0.01, 0.5, coord="r"))
+    for k, a in enumerate(aps):
+        ctx.set_ap(k, a)
+    return prob, op, ctx, aps, e0
+
+
+def rand_cx(n, b, seed):
+    return P.random_vectors(n, b, seed) + 1j * P.random_vectors(n, b, seed + 1)
+
+
+@pytest.mark.parametrize("grid", [(12, 10, 8), (24, 32, 24)])
+def test_hx_cx_matches_oracle(grid):
+    print(f"hx_cx {grid}", flush=True)
+    prob, op, ctx, aps, _ = cx_setup(grid)
+    x = rand_cx(prob.N, 2, 21)
+    xd = ctx.to_device_cx(x)
+    for nt in ALL_TYPES:
+        y = ctx.from_device_cx(ctx.hx_cx(xd, nt))
+        for v in range(2):
+            ref = O.hx_dx(op, x[v], nt, aps)
+            assert np.abs(y[v] - ref).max() <= 1e-12 * np.abs(ref).max(), nt
+    ctx.close()
+
+
+def test_px_cx_matches_oracle():
+    """Every kernel with every nPC on 12 x 10 x 8 (the oracle's dense OSB
+    basis), and on 24 x 32 x 24 the real limit (TA1, no absorbing term)
+    against the GPU's real P*X of both planes."""
+    print("px_cx", flush=True)
+    prob, op, ctx, aps, _ = cx_setup((12, 10, 8))
+    x = rand_cx(prob.N, 2, 23)
+    xd = ctx.to_device_cx(x)
+    for kind in (L.TOSB, L.TOSBD1, L.TOSBD2):
+        for npc in ALL_TYPES:
+            y = ctx.from_device_cx(ctx.px_cx(xd, kind, npc))
+            for v in range(2):
+                ref = O.px_dx(op, x[v], kind, npc, aps)
+                assert np.abs(y[v] - ref).max() <= 1e-12 * np.abs(ref).max(), (kind, npc)
+    ctx.close()
+    prob, op, ctx, aps, _ = cx_setup((24, 32, 24))
+    x = rand_cx(prob.N, 1, 25)
+    y = ctx.from_device_cx(ctx.px_cx(ctx.to_device_cx(x), L.TOSB, L.TA1))[0]
+    re = ctx.px(ctx.to_device(x.real)).cpu().numpy()[0]
+    im = ctx.px(ctx.to_device(x.imag)).cpu().numpy()[0]
+    assert np.abs(y - (re + 1j * im)).max() <= 1e-12 * np.abs(y).max()
+    ctx.close()
+
+
+@pytest.mark.parametrize("qmr_type", [-1, 0, -2])
+@pytest.mark.parametrize("pc", [L.TMAP, L.TA1])
+def test_qmr_cx_matches_oracle(qmr_type, pc):
+    print(f"qmr_cx type={qmr_type} pc={pc}", flush=True)
+    prob, op, ctx, aps, e0 = cx_setup((12, 10, 8))
+    b = rand_cx(prob.N, 1, 27)
+    r_o, x_o, _ = O.qmr_cx(lambda v: O.hx_dx(op, v, O.TMAP, aps), lambda v: O.px_dx(op, v, O.TOSB, pc, aps), b[0],
+                           qmr_type, 500, 1e-9)
+    r, x, res = ctx.qmr_cx(ctx.to_device_cx(b), qmr_type, L.TMAP, pc, 500, 1e-9)
+    slack = max(5, r_o // 20) if qmr_type == -2 else 1
+    assert r_o > 0 and r > 0 and abs(r - r_o) <= slack, (r, r_o)
+    x = ctx.from_device_cx(x)[0]
+    assert np.abs(x - x_o).max() <= 1e-7 * np.abs(x_o).max()
+    dense = op.dense() - e0 * np.eye(prob.N) - 1j * np.diag(aps[0])
+    assert np.linalg.norm(dense @ x - b[0]) <= 1e-8 * np.linalg.norm(b[0])
+    ctx.close()
+
+
+@pytest.mark.parametrize("strength", [0.002, 0.02])
+def test_pist_conv_sx_resonances(strength):
+    """The 4 resonance energies whose real parts are nearest E0 on
+    8 x 9 x 6, against a dense eigensolve of H - i AP and the oracle's
+    PIST_CONV_SX (same return code)."""
+    print(f"pist_sx strength={strength}", flush=True)
+    prob, op, ctx, aps, _ = cx_setup((8, 9, 6), strength)
+    ev = np.linalg.eigvals(op.dense() - 1j * np.diag(aps[0]))
+    srt = np.sort(ev.real)
+    e0 = float(0.5 * (srt[40] + srt[41]))
+    ctx.set_shift(e0)
+    op.set_shift(e0)
+    near = ev[np.argsort(np.abs(ev.real - e0), kind="stable")[:4]]
+    near = near[np.argsort(near.real)]
+    x = np.atleast_2d(P.random_vectors(prob.N, 1, 3)[0] + 0j)
+    r, eig, res, stats = ctx.pist_conv_sx(ctx.to_device_cx(x), e0, 1e-12, 4, 160, 16, 4, 2, qmr_type=-1,
+                                          qmrmax=500, qmrtol=1e-10)
+    print("pist_sx", r, eig, res, stats, flush=True)
+    assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
+    lin = lambda v: O.qmr_cx(lambda u: O.hx_dx(op, u, O.TMAP, aps), lambda u: O.px_dx(op, u, O.TOSB, O.TMAP, aps),
+                             v, -1, 500, 1e-10)
+    r_o, eig_o, _ = O.pist_conv_sx(lambda v: O.hx_dx(op, v, O.TMAP0, aps), lin, x[0], e0, 1e-12, 0, 4, 160, 16, 4, 2)
+    assert r == r_o, (r, r_o)
+    ctx.close()
+
+
+def test_pist_conv_sx_config1_grid():
+    """Resonances of the 24 x 32 x 24 operator with an absorbing potential in
+    R: two start vectors give the same 4 energies to 1e-9, each with a
+    negative imaginary part."""
+    print("pist_sx 24x32x24", flush=True)
+    prob, op, ctx, aps, _ = cx_setup((24, 32, 24), 0.01)
+    srt = np.sort(op.EIG0)
+    e0 = float(0.5 * (srt[30] + srt[31]))
+    ctx.set_shift(e0)
+    out = []
+    for seed in (5, 6):
+        x = np.atleast_2d(P.random_vectors(prob.N, 1, seed)[0] + 0j)
+        r, eig, res, stats = ctx.pist_conv_sx(ctx.to_device_cx(x), e0, 1e-12, 4, 160, 20, 5, 2, qmr_type=-1,
+                                              qmrmax=3000, qmrtol=1e-10)
+        print("pist_sx 24x32x24", seed, r, eig, res, stats, flush=True)
+        assert r > 0
+        out.append(eig)
+    assert np.abs(out[0] - out[1]).max() <= 1e-9 * np.abs(out[0]).max()
+    assert (out[0].imag < 0).all()
+    with pytest.raises(L.ScalitError):
+        ctx.pist_conv_sx(ctx.to_device_cx(np.ones((1, prob.N), complex)), e0, 1e-12, 4, 120, 20, 5, 2, ntype=1)
+    ctx.close()

@@ -387,3 +387,87 @@ def test_osbw_window_and_kernel_identities(npc):
     y = O.px_tosbw(op, x, w, npc)
     t = op.px(x, npc=npc, kind=O.TOSB)
     assert np.abs(y - t).max() <= 1e-12 * np.abs(t).max()
+
+
+def _cx_problem(grid=(6, 7, 5), strength=0.02):
+    prob = tiny(grid, 3)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    op.diag()
+    ap = P.absorbing_potential(prob, strength)
+    aps = (ap, P.absorbing_potential(prob, 0.01, 0.5), P.absorbing_potential(prob, 0.01, 0.5, coord="r"))
+    a_dense = op.dense() - 1j * np.diag(ap)
+    ev = np.linalg.eigvals(a_dense)
+    srt = np.sort(ev.real)
+    e0 = float(0.5 * (srt[20] + srt[21]))
+    op.set_shift(e0)
+    return prob, op, aps, ev, e0
+
+
+@pytest.mark.parametrize("ntype", [O.TA0, O.TA1, O.TAAP0, O.TMAP0, O.TAAP, O.TAAPP, O.TAAPR, O.TMAP, O.TMAPP,
+                                   O.TMAPR])
+def test_hx_dx_is_the_dense_complex_operator(ntype):
+    """MYHX_DX restated equals (H - E0 + i s A) x with H the brute-force
+    Kronecker sum, for every nType (its shift and absorbing term)."""
+    prob, op, aps, _, e0 = _cx_problem()
+    x = P.rng(9).standard_normal(op.N) + 1j * P.rng(10).standard_normal(op.N)
+    shift = 0.0 if ntype in (O.TA0, O.TAAP0, O.TMAP0) else e0
+    a = {O.TAAP0: aps[0], O.TMAP0: -aps[0], O.TAAP: aps[0], O.TAAPP: aps[1], O.TAAPR: aps[2], O.TMAP: -aps[0],
+         O.TMAPP: -aps[1], O.TMAPR: -aps[2]}.get(ntype, 0.0 * aps[0])
+    dense = kron_dense(prob) - shift * np.eye(op.N) + 1j * np.diag(a)
+    y = O.hx_dx(op, x, ntype, aps)
+    assert np.abs(y - dense @ x).max() <= 1e-12 * np.abs(y).max()
+
+
+def test_px_dx_real_kernel_is_the_real_preconditioner():
+    """With no absorbing term MYPX_DX is the real MYPX on the real and
+    imaginary parts (TOSB, TOSBD1, TOSBD2), and with one it inverts
+    V (EIG0 - E0 - i A) V^T exactly."""
+    prob, op, aps, _, e0 = _cx_problem()
+    op.set_shift(e0, 2e-3, 0.3)
+    x = P.rng(11).standard_normal(op.N) + 1j * P.rng(12).standard_normal(op.N)
+    for kind in (O.TOSB, O.TOSBD1, O.TOSBD2):
+        y = O.px_dx(op, x, kind, O.TA1, aps)
+        ref = op.px(x.real, npc=O.TA1, kind=kind) + 1j * op.px(x.imag, npc=O.TA1, kind=kind)
+        if kind == O.TOSBD2:
+            continue   # the complex TOSBD2 divisor is alpha + beta ADE on both sides (osb.mypxdx.h), the real one +-alpha
+        assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max(), kind
+    v = O.osb_basis(op)
+    d = op.EIG0 - e0 - 1j * aps[0]
+    y = O.px_dx(op, x, O.TOSB, O.TMAP, aps)
+    assert np.abs(v @ (d * (v.T @ y)) - x).max() <= 1e-11 * np.abs(x).max()
+
+
+def test_qmr_cx_solves_the_absorbing_system():
+    """QMR_CX restated solves (H - E0 - i AP) x = b with the OSB
+    preconditioner (either nPC) to the requested residual."""
+    prob, op, aps, _, e0 = _cx_problem()
+    dense = kron_dense(prob) - e0 * np.eye(op.N) - 1j * np.diag(aps[0])
+    b = P.rng(13).standard_normal(op.N) + 1j * P.rng(14).standard_normal(op.N)
+    for pc in (O.TMAP, O.TA1):
+        r, x, res = O.qmr_cx(lambda v: O.hx_dx(op, v, O.TMAP, aps), lambda v: O.px_dx(op, v, O.TOSB, pc, aps), b,
+                             -1, 500, 1e-10)
+        assert r > 0 and res < 1e-10
+        assert np.linalg.norm(dense @ x - b) <= 1e-9 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("strength", [0.002, 0.02])
+def test_pist_conv_sx_resonances_against_dense(strength):
+    """PIST_CONV_SX restated (complex eigenvalues of the Krylov matrix by
+    numpy's ZGEEV) finds the 4 complex eigenvalues of H - i AP whose real
+    parts are nearest E0, against a dense eigensolve, to 1e-10 relative."""
+    prob, op, aps, _, _ = _cx_problem(strength=strength)
+    ev = np.linalg.eigvals(kron_dense(prob) - 1j * np.diag(aps[0]))
+    srt = np.sort(ev.real)
+    e0 = float(0.5 * (srt[20] + srt[21]))
+    op.set_shift(e0)
+    near = ev[np.argsort(np.abs(ev.real - e0), kind="stable")[:4]]
+    near = near[np.argsort(near.real)]
+    x = P.rng(3).standard_normal(op.N) + 0j
+    lin = lambda v: O.qmr_cx(lambda u: O.hx_dx(op, u, O.TMAP, aps), lambda u: O.px_dx(op, u, O.TOSB, O.TMAP, aps),
+                             v, -1, 500, 1e-10)
+    r, eig, res = O.pist_conv_sx(lambda v: O.hx_dx(op, v, O.TMAP0, aps), lin, x, e0, 1e-12, 0, 4, 80, 16, 4, 2)
+    assert r > 0
+    assert np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
+    with pytest.raises(ValueError):
+        O.pist_conv_sx(None, None, x, e0, 1e-12, 1, 4, 80, 16, 4, 2)
+    assert O.pist_conv_sx(None, None, x, e0, 1e-12, 0, 4, 4, 16, 4, 2)[0] == 0

@@ -291,3 +291,69 @@ def test_osbw_setup_matches_reference(cnt, de):
     assert np.abs(hw - rhw).max() <= 1e-13 * np.abs(rhw).max()
     for j in range(0, cnt, max(1, cnt // 5)):
         assert np.array_equal(vw[:, j], rop.osb_vector(int(ind[j])))
+
+
+def _cx_ops(grid=(8, 7, 6), e0=None):
+    prob = P.config_problem("pist_fp64", grid=grid)
+    op, rop = make_ops(prob, e0=0.0)
+    _, eig0 = op.diag()
+    rop.diag()
+    srt = np.sort(eig0)
+    e = float(0.5 * (srt[30] + srt[31])) if e0 is None else e0
+    op.set_shift(e, float(srt[40] - srt[30]), 0.3)
+    rop.set_shift(e, float(srt[40] - srt[30]), 0.3)
+    aps = (P.absorbing_potential(prob, 0.02), P.absorbing_potential(prob, 0.015, 0.6),
+           P.absorbing_potential(prob, 0.01, 0.5, coord="r"))
+    for k, a in enumerate(aps):
+        rop.set_ap(k, a)
+    return prob, op, rop, aps
+
+
+@pytest.mark.parametrize("ntype", [O.TA0, O.TA1, O.TAAP0, O.TMAP0, O.TAAP, O.TAAPP, O.TAAPR, O.TMAP, O.TMAPP,
+                                   O.TMAPR])
+def test_hx_dx_matches_reference(ntype):
+    """MYHX_DX of the compiled module osb (osb.hxdx.h:177-240) against the
+    oracle's restatement, every nType, to 1e-14 of the result (the oracle
+    adds the diagonal after the level products)."""
+    prob, op, rop, aps = _cx_ops()
+    rng = P.rng(5)
+    x = rng.standard_normal(op.N) + 1j * rng.standard_normal(op.N)
+    y, yr = O.hx_dx(op, x, ntype, aps), rop.hx_dx(x, ntype)
+    assert np.abs(y - yr).max() <= 1e-14 * np.abs(yr).max()
+
+
+@pytest.mark.parametrize("kind", [O.TOSB, O.TOSBD1, O.TOSBD2])
+@pytest.mark.parametrize("npc", [O.TA0, O.TA1, O.TAAP0, O.TMAP0, O.TAAP, O.TAAPP, O.TAAPR, O.TMAP, O.TMAPP,
+                                 O.TMAPR])
+def test_px_dx_matches_reference(kind, npc):
+    """MYPX_DX of the compiled module osb (osb.mypxdx.h) against the oracle's
+    restatement (dense OSB basis, complex divisor), every kernel and nPC,
+    to 1e-12 of the result; E0 inside the EIG0 spectrum and DE wide enough
+    that the OSBD branches run."""
+    prob, op, rop, aps = _cx_ops()
+    rng = P.rng(6)
+    x = rng.standard_normal(op.N) + 1j * rng.standard_normal(op.N)
+    y, yr = O.px_dx(op, x, kind, npc, aps), rop.px_dx(x, kind, npc)
+    assert np.abs(y - yr).max() <= 1e-12 * np.abs(yr).max()
+
+
+@pytest.mark.parametrize("ntype", [-1, 0, -2])
+@pytest.mark.parametrize("pc", [O.TMAP, O.TA1])
+def test_qmr_cx_matches_reference(ntype, pc):
+    """QMR_CX (qmrcx.f90) as OSB_QMRDX runs it (HX_DX = MYHX_DX(TMAP), PX_DX =
+    MYPX_DX(TOSB, pc)), compiled, against the oracle's restatement driven by
+    the same compiled operators (so that only the recurrence differs) and by
+    the oracle's own: the same return code for the criteria |R|/|B| and
+    |HX - B|/|HX|; for the full criterion (-2), whose |UR|/|R| term is
+    rounding noise once R is near the tolerance, within 5 iterations;
+    solutions to 1e-8."""
+    prob, op, rop, aps = _cx_ops()
+    rng = P.rng(7)
+    b = rng.standard_normal(op.N) + 1j * rng.standard_normal(op.N)
+    r_ref, x_ref, _ = rop.qmr_dx(b, O.TOSB, O.TMAP, pc, ntype, 400, 1e-9)
+    slack = 0 if ntype != -2 else 5
+    for hx, px in ((lambda v: rop.hx_dx(v, O.TMAP), lambda v: rop.px_dx(v, O.TOSB, pc)),
+                   (lambda v: O.hx_dx(op, v, O.TMAP, aps), lambda v: O.px_dx(op, v, O.TOSB, pc, aps))):
+        r, x, _ = O.qmr_cx(hx, px, b, ntype, 400, 1e-9)
+        assert r_ref > 0 and abs(r - r_ref) <= slack, (r, r_ref)
+        assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
