@@ -222,6 +222,57 @@ int scalit_pist_vectors_all(scalit_ctx *ctx, double *psi, long ld, int ncols);
 int scalit_lan_conv(scalit_ctx *ctx, double e0, double etol, int ntype, const double *x, int startcnt, int stepcnt,
                     int stepeig, int m, int m_max, double *eig, double *res);
 
+/* ------------------------------------------------------------------------ */
+/* absorbing-potential (resonance) jobs: complex vectors in the split layout */
+/* (vector v's real part at x + 2 v N, its imaginary part at x + (2 v + 1) N; */
+/* the reference's double complex arrays are interleaved)                   */
+/* ------------------------------------------------------------------------ */
+
+/* AP / APP / APR (src/iterate/module/osb.data.h:61, loaded by osb.io.h:20-30:
+ * a resonance job sets AP = APP; a CRP job AP = APP + APR).  Host, N values. */
+int scalit_set_ap(scalit_ctx *ctx, int which, const double *ap);
+/* MYHX_DX (src/iterate/module/osb.hxdx.h:177-240): Y = (RES - E0 +- i A) X +
+ * the level products, ntype one of SCALIT_TA0 ... SCALIT_TMAPR (E0 = 0 for
+ * TA0, TAAP0, TMAP0).  x, y device, nvec split vectors. */
+int scalit_hx_cx(scalit_ctx *ctx, int ntype, int nvec, const double *x, double *y, scalit_stream stream);
+/* MYPX_DX (src/iterate/module/osb.mypxdx.h): V (kernel)^-1 V^T X with the
+ * complex divisor EIG0 - E0 (clamped by TOSBD1 / TOSBD2) +- i A of npc.
+ * nosb TOSB, TOSBD1 or TOSBD2 (TOSBW: SCALIT_EINVAL). */
+int scalit_px_cx(scalit_ctx *ctx, int nosb, int npc, int nvec, const double *x, double *y, scalit_stream stream);
+/* OSB_QMRDX (src/iterate/module/osbw.qmrdx.h:16-27): QMR_CX
+ * (src/iterate/libiterate/qmr/qmrcx.f90:4-179) with HX_DX = MYHX_DX(hc) and
+ * PX_DX = MYPX_DX(sOSB, pc).  b, x device (one split vector; x is the start
+ * vector when qmr_type > 0); *ret gets QMR_CX's value, *res its residual. */
+int scalit_qmr_cx(scalit_ctx *ctx, int qmr_type, int hc, int pc, int qmrmax, double qmrtol, const double *b, double *x,
+                  int *ret, double *res);
+/* OSB_PISTCONVDX1 (src/iterate/module/osbw.pistconv.h:98-140): PIST_CONV_SX
+ * (src/iterate/libiterate/pist/pist_sx.f90:9-110) with H0X = MYHX_DX(TMAP0),
+ * LinSolv = OSB_QMRDX with hc = TMAP and pc = SCALIT_TMAP (sAP) or SCALIT_TA1,
+ * the operator shift E0 of scalit_set_shift and the window centre e0.  x0
+ * device (one split vector); eig host, 2 m doubles (re, im of each energy,
+ * ascending real part); *ret gets PIST_CONV_SX's value.  ntype > 0 is
+ * SCALIT_EINVAL (the reference passes getSepMax_CX five arguments for four). */
+int scalit_pist_conv_sx(scalit_ctx *ctx, double e0, double etol, int ntype, const double *x0, int startcnt,
+                        int stepcnt, int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int pc,
+                        double *eig, int *ret, double *res, int *stats);
+/* osbCRP (src/iterate/module/osbw.crp.h:7-49): the cumulative reaction
+ * probability N(E) = 4 sum of the nm Lanczos eigenvalues (LAN_DX,
+ * src/iterate/libiterate/lanczos/lan_dx.f90:299-325, full re-orthogonalisation)
+ * of APR^1/2 (H - i AP - E)^-1 APP (H + i AP - E)^-1 APR^1/2 (CRP_HXDX,
+ * osbw.crp.h:79-131), each inverse an OSB_QMRDX solve whose preconditioner
+ * includes the absorbing term when sap != 0 (sAP).  Needs APP and APR; sets
+ * AP = APP + APR as osbCRP does.  x0: device, ncrp split start vectors (one
+ * per energy, as the reference draws one per energy); crp host, ncrp values
+ * (0 where the Lanczos or its eigensolve fails, as calOSBCRP returns);
+ * stats (host, 2 ints, may be NULL): QMR iterations, QMR solves.  The ctx's
+ * shift is restored afterwards. */
+int scalit_crp(scalit_ctx *ctx, int ncrp, const double *energies, int nm, const double *x0, int qmr_type,
+               int qmrmax, double qmrtol, int sap, double *crp, int *stats);
+/* ZGEEV('N', 'N') as PIST_CONV_SX calls it (pist_sx.f90:62,71), on the host:
+ * the eigenvalues of a general complex n x n matrix (a interleaved re/im,
+ * column-major, leading dimension lda) into w (2 n doubles). */
+int scalit_small_eig_cx(int n, const double *a, int lda, double *w);
+
 /* Hellmann-Feynman gradients: dE_k/dp_j = sum_grid psi_k^2 dV/dp_j.
  * psi device N x nk (ld), dvdp device N x np (lddv); out host nk x np
  * (row-major).  Uses the adjoint product's dV = x o g with x = g = psi_k. */

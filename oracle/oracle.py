@@ -846,3 +846,104 @@ def pist_conv_sx(hx0, linsolv, x, e0, etol, ntype, m, m_max, startcnt, stepcnt, 
         cnt_high = min(cnt_high + stepcnt, m_max)
     order = np.argsort(_op_cx(eig, 1), kind="stable")
     return code, eig[order], res
+
+
+def lan_hij_dx(apply, x, m: int):
+    """lanHij_dx (src/iterate/libiterate/lanczos/lanHij_dx.f90:59-97) with
+    LAN_MGS_ORTH_CX (orth/gsorth_cx.f90:132-162): m Lanczos steps on a
+    Hermitian operator with one full re-orthogonalisation pass per step.
+    Returns (code, alpha, beta)."""
+    x = np.asarray(x, dtype=complex)
+    alpha, beta = np.zeros(m), np.zeros(m)
+    t = np.sqrt(np.vdot(x, x).real)
+    if t == 0.0:
+        return 0, alpha, beta
+    vj = [x / t]
+    for j in range(1, m + 1):
+        w = apply(vj[j - 1])
+        if j != 1:
+            w = w - beta[j - 2] * vj[j - 2]
+        alpha[j - 1] = np.vdot(w, vj[j - 1]).real
+        vn = w.copy()
+        for v in vj:
+            vn = vn - np.vdot(v, vn) * v
+        nw = np.sqrt(np.vdot(vn, vn).real)
+        beta[j - 1] = nw
+        if nw == 0.0:
+            return -j, alpha, beta
+        vj.append(vn / nw)
+    return m, alpha, beta
+
+
+def lan_dx(apply, x, m: int):
+    """LAN_DX (src/iterate/libiterate/lanczos/lan_dx.f90:299-325): lanHij_dx,
+    then the tridiagonal's eigenvalues (DSTEV).  Returns (code, eigenvalues
+    or None)."""
+    code, alpha, beta = lan_hij_dx(apply, x, m)
+    if code <= 0:
+        return code, None
+    return m, tridiag_eigvals(alpha, beta[:m - 1])
+
+
+def crp_dx(op: "Operator", app, apr, energies, nm: int, x0s, qmr_type=-1, qmrmax=1000, qmrtol=1e-3, sap=True,
+           kind: int = TOSB):
+    """osbCRP / calOSBCRP / CRP_HXDX (src/iterate/module/osbw.crp.h:7-131):
+    N(E) = 4 sum of the nm Lanczos eigenvalues of APR^1/2 (H - i AP - E)^-1
+    APP (H + i AP - E)^-1 APR^1/2, AP = APP + APR, each inverse by QMR_CX
+    with the OSB preconditioner (including the absorbing term when sap)."""
+    app, apr = f64(app), f64(apr)
+    ap, sq = app + apr, np.sqrt(apr)
+    aps = (ap, app, apr)
+    keep = op.s.E0
+    out = []
+    for e, x in zip(energies, x0s):
+        op.s.E0 = e
+
+        def apply(v):
+            y = sq * v
+            t = qmr_cx(lambda u: hx_dx(op, u, TMAP, aps), lambda u: px_dx(op, u, kind, TMAP if sap else TA1, aps), y,
+                       qmr_type, qmrmax, qmrtol)[1]
+            y = app * t
+            t = qmr_cx(lambda u: hx_dx(op, u, TAAP, aps), lambda u: px_dx(op, u, kind, TAAP if sap else TA1, aps), y,
+                       qmr_type, qmrmax, qmrtol)[1]
+            return sq * t
+
+        code, ev = lan_dx(apply, x, nm)
+        out.append(4.0 * float(np.sum(ev)) if code > 0 else 0.0)
+    op.s.E0 = keep
+    return np.array(out)
+
+
+_CXOP = C.CFUNCTYPE(None, _ip, C.c_void_p, C.c_void_p)
+
+
+def ref_lanhij_dx(apply, x, m: int):
+    """The reference's compiled lanHij_dx (lanHij_dx.f90:59-97) on a Python
+    Hermitian operator: (code, alpha, beta)."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    n = x.size
+
+    def cb(np_, xp, yp):
+        k = np_[0]
+        xv = np.ctypeslib.as_array(C.cast(xp, _dp), shape=(2 * k,)).view(np.complex128)
+        yv = np.ctypeslib.as_array(C.cast(yp, _dp), shape=(2 * k,)).view(np.complex128)
+        yv[:] = apply(xv.copy())
+
+    f = _CXOP(cb)
+    alpha, beta = np.zeros(m), np.zeros(m)
+    R = ref()
+    R.lanhij_dx_.restype = C.c_int
+    code = R.lanhij_dx_(C.byref(C.c_int(n)), C.c_void_p(x.ctypes.data), C.byref(C.c_int(m)), f, dptr(alpha),
+                        dptr(beta))
+    return code, alpha, beta
+
+
+def ref_mgs_orth_sx(w, vj: np.ndarray, m: int):
+    """The reference's compiled MGS_ORTH_SX (gsorth_sx.f90:47-74): vj is
+    N x (m + 1) complex, column-major; column m + 1 is written."""
+    w = np.ascontiguousarray(w, dtype=np.complex128)
+    R = ref()
+    R.mgs_orth_sx_.restype = C.c_int
+    ok = R.mgs_orth_sx_(C.byref(C.c_int(w.size)), C.c_void_p(w.ctypes.data), C.byref(C.c_int(m)),
+                        C.c_void_p(vj.ctypes.data))
+    return ok != 0

@@ -445,6 +445,19 @@ class Context:
         e = np.array(eig[:])
         return ret.value, e[0::2] + 1j * e[1::2], res.value, np.array(stats[:])
 
+    def crp(self, energies, nm: int, x0: torch.Tensor, qmr_type: int = -1, qmrmax: int = 1000, qmrtol: float = 1e-3,
+            sap: bool = True):
+        """osbCRP (src/iterate/module/osbw.crp.h:7-49): N(E) at each energy from
+        nm Lanczos steps on the CRP operator; x0 (ncrp, 2, N) start vectors.
+        Returns (N(E) per energy, [QMR iterations, QMR solves])."""
+        e = np.ascontiguousarray(energies, dtype=np.float64)
+        x0 = x0.contiguous()
+        out = np.zeros(len(e))
+        stats = (C.c_int * 2)()
+        check(L.lib().scalit_crp(self._h, len(e), _np_dp(e), nm, _ptr(x0), qmr_type, qmrmax, qmrtol, 1 if sap else 0,
+                                 _np_dp(out), stats), "scalit_crp")
+        return out, np.array(stats[:])
+
     def pist_vectors(self, win: int, m: int):
         psi = torch.empty((m, self.N), dtype=torch.float64, device=self.dev)
         check(L.lib().scalit_pist_vectors(self._h, win, _ptr(psi), self.N), "scalit_pist_vectors")

@@ -642,6 +642,111 @@ static int pist_sx_run(scalit_ctx *c, const PistSx &p, const double *x0, double 
     return SCALIT_OK;
 }
 
+// ---------------------------------------------------------------- CRP
+// AP = APP + APR and SQ_APR = sqrt(APR), as osbCRP sets them
+__global__ void k_crp_prep(long N, const double *__restrict__ app, const double *__restrict__ apr,
+                           double *__restrict__ ap, double *__restrict__ sq)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        ap[e] = app[e] + apr[e];
+        sq[e] = sqrt(apr[e]);
+    }
+}
+
+// y = a o x on one split complex vector
+__global__ void k_cx_diag(long N, const double *__restrict__ a, const double *__restrict__ x, double *__restrict__ y)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        y[e] = a[e] * x[e];
+        y[e + N] = a[e] * x[e + N];
+    }
+}
+
+struct CrpArgs {
+    int nm, qmr_type, qmrmax, sap;
+    double qmrtol;
+};
+
+// CRP_HXDX (src/iterate/module/osbw.crp.h:79-131): out = APR^1/2 (H - i AP - E)^-1
+// APP (H + i AP - E)^-1 APR^1/2 in, the two solves by OSB_QMRDX
+static int crp_apply(scalit_ctx *c, const CrpArgs &a, const double *sq, const double *in, double *out, double *t1,
+                     double *t2, int *its, hipStream_t s)
+{
+    const long N = c->N;
+    const unsigned g = cx_grid(N);
+    int n1 = 0, n2 = 0;
+    double r = 0.0;
+    hipLaunchKernelGGL(k_cx_diag, dim3(g), dim3(256), 0, s, N, sq, in, t1);
+    SC_TRY(qmr_cx_run(c, c->osb, SCALIT_TMAP, a.sap ? SCALIT_TMAP : SCALIT_TA1, a.qmr_type, a.qmrmax, a.qmrtol, t1, t2,
+                      &n1, &r, s));
+    hipLaunchKernelGGL(k_cx_diag, dim3(g), dim3(256), 0, s, N, (const double *)c->ap[SCALIT_APP].p,
+                       (const double *)t2, t1);
+    SC_TRY(qmr_cx_run(c, c->osb, SCALIT_TAAP, a.sap ? SCALIT_TAAP : SCALIT_TA1, a.qmr_type, a.qmrmax, a.qmrtol, t1, t2,
+                      &n2, &r, s));
+    hipLaunchKernelGGL(k_cx_diag, dim3(g), dim3(256), 0, s, N, sq, (const double *)t2, out);
+    HIP_TRY(hipGetLastError());
+    *its += std::abs(n1) + std::abs(n2);
+    return SCALIT_OK;
+}
+
+// calOSBCRP (osbw.crp.h:54-71) with LAN_DX (src/iterate/libiterate/lanczos/
+// lan_dx.f90:299-325) and lanHij_dx (lanHij_dx.f90:59-97): nm steps of
+// Lanczos with full re-orthogonalisation (LAN_MGS_ORTH_CX, orth/gsorth_cx.f90:
+// 132-162) on the Hermitian CRP operator, the tridiagonal's eigenvalues
+// (DSTEV), 4 times their sum; 0 when the Lanczos or the eigensolve fails.
+static int crp_one(scalit_ctx *c, const CrpArgs &a, const double *sq, const double *x0, double *crp, int *its,
+                   hipStream_t s)
+{
+    CxState &X = cxstate(c);
+    const long N = c->N, L = 2 * N;
+    const int M = a.nm;
+    double *V = X.vj.p, *W = X.wj.p, *T1 = W + L, *T2 = T1 + L;
+    auto VJ = [&](int j) { return V + (size_t)(j - 1) * L; };
+    std::vector<double> alpha(M), beta(M);
+    *crp = 0.0;
+    cx_dot(X, N, x0, x0, 1, 0, 0, s);
+    SC_TRY(cx_read(X, 1, s));
+    const double t = std::sqrt(slot_val(X, 0).real());
+    if (t == 0.0) return SCALIT_OK;
+    cx_axpby(N, 1.0 / t, x0, 0.0, nullptr, VJ(1), s);
+    for (int j = 1; j <= M; j++) {
+        SC_TRY(crp_apply(c, a, sq, VJ(j), W, T1, T2, its, s));
+        if (j != 1) cx_axpby(N, 1.0, W, -beta[j - 2], VJ(j - 1), W, s);
+        cx_dot(X, N, W, VJ(j), 1, 0, 0, s);
+        SC_TRY(cx_read(X, 1, s));
+        alpha[j - 1] = slot_val(X, 0).real();
+        double *vn = VJ(j + 1);
+        HIP_TRY(hipMemcpyAsync(vn, W, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
+        for (int i = 1; i <= j; i++) {
+            cx_dot(X, N, VJ(i), vn, 1, 0, 0, s);
+            hipLaunchKernelGGL(k_cx_axpy_dev, dim3(cx_grid(N)), dim3(256), 0, s, N, (const double *)X.scal.p,
+                               (const double *)VJ(i), vn);
+        }
+        cx_dot(X, N, vn, vn, 1, 1, 0, s);
+        SC_TRY(cx_read(X, 2, s));
+        const double nw = std::sqrt(slot_val(X, 1).real());
+        beta[j - 1] = nw;
+        if (nw == 0.0) return SCALIT_OK;   // lanHij_dx = -j: LAN_DX fails, the CRP is 0
+        cx_axpby(N, 1.0 / nw, vn, 0.0, nullptr, vn, s);
+    }
+    std::vector<cd> T((size_t)M * M, cd(0.0, 0.0)), ev;
+    for (int k = 0; k < M; k++) {
+        T[k + (size_t)M * k] = alpha[k];
+        if (k + 1 < M) {
+            T[k + 1 + (size_t)M * k] = beta[k];
+            T[k + (size_t)M * (k + 1)] = beta[k];
+        }
+    }
+    if (cx_eigvals(M, T, ev) != SCALIT_OK) return SCALIT_OK;
+    double sum = 0.0;
+    std::vector<double> w(M);
+    for (int k = 0; k < M; k++) w[k] = ev[k].real();
+    std::sort(w.begin(), w.end());
+    for (int k = 0; k < M; k++) sum += w[k];
+    *crp = 4.0 * sum;
+    return SCALIT_OK;
+}
+
 }  // namespace scalit
 
 using namespace scalit;
@@ -697,6 +802,47 @@ int scalit_pist_conv_sx(scalit_ctx *c, double e0, double etol, int ntype, const 
     HIP_TRY(hipDeviceSynchronize());
     PistSx p{e0, etol, ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, pc, qmrtol};
     return pist_sx_run(c, p, x0, eig, ret, res, stats, c->stream);
+}
+
+int scalit_crp(scalit_ctx *c, int ncrp, const double *energies, int nm, const double *x0, int qmr_type, int qmrmax,
+               double qmrtol, int sap, double *crp, int *stats)
+{
+    if (!c || ncrp < 1 || !energies || nm < 1 || !x0 || !crp) return SCALIT_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->has_res || !c->has_precond || !c->has_ap[SCALIT_APP] || !c->has_ap[SCALIT_APR]) return SCALIT_ESTATE;
+    if (c->osb == SCALIT_TOSBW) return SCALIT_EINVAL;
+    HIP_TRY(hipDeviceSynchronize());
+    hipStream_t s = c->stream;
+    CxState &X = cxstate(c);
+    const long N = c->N, L = 2 * N;
+    SC_TRY(c->ap[SCALIT_AP].alloc(N));
+    SC_TRY(X.vj.alloc((size_t)(nm + 1) * L));
+    SC_TRY(X.wj.alloc((size_t)3 * L));
+    SC_TRY(cx_slots(X, 4));
+    DBuf<double> sq;
+    SC_TRY(sq.alloc(N));
+    hipLaunchKernelGGL(k_crp_prep, dim3(cx_grid(N)), dim3(256), 0, s, N, (const double *)c->ap[SCALIT_APP].p,
+                       (const double *)c->ap[SCALIT_APR].p, c->ap[SCALIT_AP].p, sq.p);
+    HIP_TRY(hipGetLastError());
+    c->has_ap[SCALIT_AP] = true;
+    const double e_keep = c->e0;
+    CrpArgs a{nm, qmr_type, qmrmax, sap ? 1 : 0, qmrtol};
+    int its = 0;
+    for (int k = 0; k < ncrp; k++) {
+        c->e0 = energies[k];   // sOSBW%mE0 = engy(i)
+        const int rc = crp_one(c, a, sq.p, x0 + (size_t)k * L, crp + k, &its, s);
+        if (rc != SCALIT_OK) {
+            c->e0 = e_keep;
+            return rc;
+        }
+    }
+    c->e0 = e_keep;
+    HIP_TRY(hipStreamSynchronize(s));
+    if (stats) {
+        stats[0] = its;
+        stats[1] = ncrp * nm * 2;
+    }
+    return SCALIT_OK;
 }
 
 int scalit_small_eig_cx(int n, const double *a, int lda, double *w)

@@ -148,3 +148,28 @@ def test_pist_conv_sx_config1_grid():
     with pytest.raises(L.ScalitError):
         ctx.pist_conv_sx(ctx.to_device_cx(np.ones((1, prob.N), complex)), e0, 1e-12, 4, 120, 20, 5, 2, ntype=1)
     ctx.close()
+
+
+def test_crp_matches_oracle():
+    """osbCRP on 6 x 7 x 5 (product absorbing potential in r, reactant in R)
+    at three energies, 12 Lanczos steps each (24 complex QMR solves per
+    energy): N(E) against the oracle's restatement with the same start
+    vectors, to 1e-8 relative; the caller's shift is left as it was."""
+    print("crp", flush=True)
+    prob = P.build_problem(6, 7, 5, P.gaussian_potential(3), seed=3)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    vosb, eig0 = op.diag()
+    ctx = Context.from_problem(prob, e0=0.0)
+    ctx.set_precond(vosb, eig0)
+    app = P.absorbing_potential(prob, 0.01, 0.6, coord="r")
+    apr = P.absorbing_potential(prob, 0.01, 0.6, coord="R")
+    ctx.set_ap(L.APP, app)
+    ctx.set_ap(L.APR, apr)
+    h = np.sort(np.linalg.eigvalsh(op.dense()))
+    energies = [float(h[10] + 1e-4), float(h[25] + 1e-4), float(h[40] + 1e-4)]
+    x0 = np.stack([P.random_vectors(prob.N, 1, 40 + k)[0] + 0j for k in range(3)])
+    crp, stats = ctx.crp(energies, 12, ctx.to_device_cx(x0), qmr_type=-1, qmrmax=500, qmrtol=1e-12)
+    ref = O.crp_dx(op, app, apr, energies, 12, list(x0), -1, 500, 1e-12, True)
+    print("crp", crp, ref, stats, flush=True)
+    assert np.abs(crp - ref).max() <= 1e-8 * np.abs(ref).max() and (ref > 0).all()
+    ctx.close()

@@ -471,3 +471,30 @@ def test_pist_conv_sx_resonances_against_dense(strength):
     with pytest.raises(ValueError):
         O.pist_conv_sx(None, None, x, e0, 1e-12, 1, 4, 80, 16, 4, 2)
     assert O.pist_conv_sx(None, None, x, e0, 1e-12, 0, 4, 4, 16, 4, 2)[0] == 0
+
+
+def test_crp_operator_and_lanczos_against_dense():
+    """osbCRP restated: its operator is APR^1/2 (H + i AP - E)^-1 APP
+    (H - i AP - E)^-1 APR^1/2 (AP = APP + APR; CRP_HXDX's order of solves),
+    so N(E) from nm Lanczos steps equals the same Lanczos on that dense
+    Hermitian matrix to 1e-9, and lies below 4 trace (the exact N(E))."""
+    prob = tiny((6, 7, 5), 3)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    op.diag()
+    app = P.absorbing_potential(prob, 0.01, 0.6, coord="r")
+    apr = P.absorbing_potential(prob, 0.01, 0.6, coord="R")
+    h = kron_dense(prob)
+    e = float(np.sort(np.linalg.eigvalsh(h))[25] + 1e-4)
+    ap = app + apr
+    ap_p = h + 1j * np.diag(ap) - e * np.eye(op.N)
+    ap_m = h - 1j * np.diag(ap) - e * np.eye(op.N)
+    s = np.diag(np.sqrt(apr))
+    pm = s @ np.linalg.solve(ap_p, np.diag(app) @ np.linalg.solve(ap_m, s))
+    assert np.abs(pm - pm.conj().T).max() <= 1e-12 * np.abs(pm).max()
+    x = P.rng(4).standard_normal(op.N) + 0j
+    for nm in (6, 20):
+        crp = O.crp_dx(op, app, apr, [e], nm, [x], -1, 500, 1e-12, True)[0]
+        code, ev = O.lan_dx(lambda v: pm @ v, x, nm)
+        assert code == nm
+        assert abs(crp - 4 * ev.sum()) <= 1e-9 * abs(crp)
+        assert 0 < crp <= 4 * np.trace(pm).real * (1 + 1e-12)

@@ -357,3 +357,29 @@ def test_qmr_cx_matches_reference(ntype, pc):
         r, x, _ = O.qmr_cx(hx, px, b, ntype, 400, 1e-9)
         assert r_ref > 0 and abs(r - r_ref) <= slack, (r, r_ref)
         assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+
+
+def test_lan_hij_dx_and_mgs_orth_sx_match_reference():
+    """lanHij_dx (lanHij_dx.f90:59-97) and MGS_ORTH_SX (gsorth_sx.f90:47-74),
+    compiled, against the oracle's restatements: the Lanczos coefficients of
+    a Hermitian operator of rank 12 for 8 steps to 1e-12, and one complex-
+    symmetric Gram-Schmidt step to 1e-13."""
+    rng = np.random.default_rng(3)
+    n = 50
+    b = rng.standard_normal((n, 12)) + 1j * rng.standard_normal((n, 12))
+    h = b @ b.conj().T / 40.0
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    code_r, a_r, b_r = O.ref_lanhij_dx(lambda v: h @ v, x, 8)
+    code, a, bb = O.lan_hij_dx(lambda v: h @ v, x, 8)
+    assert code == code_r == 8
+    assert np.abs(a - a_r).max() <= 1e-12 * np.abs(a_r).max() and np.abs(bb - b_r).max() <= 1e-12 * np.abs(b_r).max()
+    vj = np.zeros((n, 4), dtype=np.complex128, order="F")
+    vs = []
+    for k in range(3):
+        v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        assert O.mgs_orth_sx(v, vs)
+        vj[:, k] = vs[-1]
+    w = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert O.ref_mgs_orth_sx(w, vj, 3)
+    assert O.mgs_orth_sx(w, vs)
+    assert np.abs(vj[:, 3] - vs[3]).max() <= 1e-13
