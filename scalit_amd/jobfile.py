@@ -177,23 +177,38 @@ def pist_params(job: IterateJob, m0: int):
 
 
 def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: int = 1):
-    """The bound-state job of an iterate input on this library: operator from
+    """The job of an iterate input on this library (progPistEig,
+    src/iterate/module/osbw.progEig.h:6-80, real H0, sCX = F): operator from
     the job's files, preconditioner from the set-up (or from the VOSB/EIG
-    files when sVOSB < 0; saved there when sVOSB > 0), then PIST for
-    M0 = conv_num levels around conv_e0.  Returns (ret, eigenvalues, res).
-    The start vector is uniform on [0, 1) like the reference's RANDOM_NUMBER,
-    from a seeded generator."""
+    files when sVOSB < 0; saved there when sVOSB > 0), then
+
+    * sJOB = 1, bound states: PIST for M0 = conv_num levels around conv_e0;
+      returns (ret, eigenvalues, res);
+    * sJOB = 2, resonances (OSB_PISTCONVDX1): AP = APP from its file, the
+      complex-symmetric PIST for M0 energies of H - i AP whose real parts are
+      nearest conv_e0, with the absorbing term in the preconditioner when
+      sAP; returns (ret, complex energies, res);
+    * sJOB = 4-6, CRP (osbCRP): APP and APR from their files, N(E) at
+      conv_num energies conv_e0 + k conv_tol from conv_gap Lanczos steps
+      each; returns (number of energies, N(E) per energy, 0.0).
+
+    Start vectors are uniform on [0, 1) (real and imaginary parts for the
+    complex jobs) like the reference's RANDOM_NUMBER / randMat_CX, from a
+    seeded generator."""
     import torch
 
     from . import _lib as L
     from .context import Context
 
-    if job.sJOB != JOB_BOUND or job.sCX:
-        raise ValueError("only real bound-state jobs (sJOB = 1, sCX = F) are on this path")
+    if job.sCX or job.sJOB not in (JOB_BOUND, JOB_RES1, JOB_CRP, JOB_CRP1, JOB_CRP2):
+        raise ValueError("only real-H0 jobs (sCX = F): bound states (sJOB = 1), resonances with the symmetric "
+                         "PIST (sJOB = 2) and CRP (sJOB = 4-6)")
     path = lambda k: os.path.join(base_dir, job.files[k]) if k in job.files else None
     nosb = {TOSB: L.TOSB, TOSBD1: L.TOSBD1, TOSBD2: L.TOSBD2, TOSBW: L.TOSBW}.get(job.sOSB)
     if nosb is None:
         raise ValueError(f"sOSB = {job.sOSB} is not a preconditioner kernel (1..4)")
+    if nosb == L.TOSBW and job.sJOB != JOB_BOUND:
+        raise ValueError("the windowed preconditioner (sOSB = 4) is on the bound-state path only")
     ctx = Context.from_files(job.sN, job.sDEP, h0_file=path("H0"), res_file=path("RES"), dep_file=path("DEP"),
                              outh_file=path("OUTH"), device=device)
     ctx.set_shift(job.osbw_e0, job.osbw_de, job.osbw_beta)
@@ -205,6 +220,32 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
         if job.sVOSB > 0:
             ctx.save_precond(path("VOSB"), path("EIG"), mode)
     m0 = job.conv_num
+    rng = np.random.default_rng(seed)
+    if job.sJOB == JOB_RES1:
+        # osb.io.h:20-23: APP from its file, APR = 0, AP = APP
+        app = Context.load_file(path("APP"), ctx.N, "dir")
+        ctx.set_ap(L.AP, app)
+        ctx.set_ap(L.APP, app)
+        ctx.set_ap(L.APR, np.zeros(ctx.N))
+        ctx.set_osb(nosb)
+        e0, etol, start, step, stepeig, nmax = pist_params(job, m0)
+        x0 = rng.random(ctx.N) + 1j * rng.random(ctx.N)
+        r, eig, res, _ = ctx.pist_conv_sx(ctx.to_device_cx(x0), e0, etol, m0, nmax, start, step, stepeig,
+                                          qmrmax=job.qmr_max, qmrtol=job.qmr_tol, pc=L.TMAP if job.sAP else L.TA1)
+        torch.cuda.synchronize()
+        ctx.close()
+        return r, eig, res
+    if job.sJOB in (JOB_CRP, JOB_CRP1, JOB_CRP2):
+        ctx.set_ap(L.APP, Context.load_file(path("APP"), ctx.N, "dir"))
+        ctx.set_ap(L.APR, Context.load_file(path("APR"), ctx.N, "dir"))
+        ctx.set_osb(nosb)
+        energies = [job.conv_e0 + k * job.conv_tol for k in range(m0)]   # osbw.progEig.h:66-68
+        x0 = rng.random((m0, ctx.N)) + 1j * rng.random((m0, ctx.N))
+        crp, _ = ctx.crp(energies, job.conv_gap, ctx.to_device_cx(x0), qmrmax=job.qmr_max, qmrtol=job.qmr_tol,
+                         sap=bool(job.sAP))
+        torch.cuda.synchronize()
+        ctx.close()
+        return m0, crp, 0.0
     if nosb == L.TOSBW:
         # OSB_PISTCONV (osbw.pistconv.h:26-31): sPC = TA1, then initOSBW with
         # the job's window; a window of no states fails the job (returns 0)
@@ -214,7 +255,7 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
         ctx.set_osbw(job.osbw_e0, job.osbw_de, job.osbw_cnt, L.TA1)
     ctx.set_osb(nosb)
     e0, etol, start, step, stepeig, nmax = pist_params(job, m0)
-    x0 = np.random.default_rng(seed).random((1, ctx.N))
+    x0 = rng.random((1, ctx.N))
     r, eig, res, _ = ctx.pist_conv(ctx.to_device(x0), e0, etol, m0, nmax, start, step, stepeig,
                                    qmrmax=job.qmr_max, qmrtol=job.qmr_tol, prec=L.FP64)
     torch.cuda.synchronize()

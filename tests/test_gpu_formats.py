@@ -10,6 +10,7 @@ H*X and P*X, bit for bit, as one built from the arrays.
 import numpy as np
 import pytest
 
+from oracle import oracle as O
 from scalit_amd import problems as P
 from scalit_amd.context import Context
 
@@ -127,3 +128,61 @@ eig0.dat
         r3, eig3, _ = J.run_pist_job(job, base_dir=str(tmp_path))
         print(f"job with sOSB = {kind}: ret {r3}", flush=True)
         assert r3 > 0 and (np.abs(np.sort(eig3) - ref) / np.abs(ref)).max() <= 1e-9
+
+
+def _cx_job_text(prob, files, sjob, conv, extra):
+    import os
+
+    return f"""3 {prob.sN[0]} {prob.sN[1]} {prob.sN[2]}
+F F T
+{sjob} 1
+F F T T
+1000 1.0D-5 500 1.0D-10
+{conv}
+0.0 1.0 0.0 0
+0 0 0 0 0
+{os.path.basename(files['h0_file'])}
+{os.path.basename(files['res_file'])}
+{os.path.basename(files['dep_file'])}
+""" + "".join(e + "\n" for e in extra)
+
+
+def test_resonance_and_crp_jobs_from_input_files(tmp_path):
+    """A resonance job (sJOB = 2, APP file) and a CRP job (sJOB = 4, APP and
+    APR files) of 8 x 9 x 6 from iterate input files: the 4 resonance energies
+    whose real parts are nearest the job's E0 against a dense eigensolve of
+    H - i APP (1e-10), and N(E) at 3 energies from 10 Lanczos steps against
+    the oracle's osbCRP with the same start vectors (1e-8)."""
+    from scalit_amd import jobfile as J
+
+    prob = P.build_problem(8, 9, 6, P.gaussian_potential(3), seed=3)
+    files = write_operator(tmp_path, prob, True)
+    app = P.absorbing_potential(prob, 0.01, 0.6, coord="R")
+    apr = P.absorbing_potential(prob, 0.01, 0.6, coord="r")
+    Context.save_file(str(tmp_path / "app.dat"), app, "dir")
+    Context.save_file(str(tmp_path / "apr.dat"), apr, "dir")
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    h = op.dense()
+    ev = np.linalg.eigvals(h - 1j * np.diag(app))
+    srt = np.sort(ev.real)
+    e0 = float(0.5 * (srt[40] + srt[41]))
+    near = ev[np.argsort(np.abs(ev.real - e0), kind="stable")[:4]]
+    near = near[np.argsort(near.real)]
+    text = _cx_job_text(prob, files, 2, f"{e0!r} 1.0D-12 16 4 160 4 2", ["app.dat"])
+    (tmp_path / "res.in").write_text(text)
+    job = J.read_iterate_input(str(tmp_path / "res.in"))
+    r, eig, res = J.run_pist_job(job, base_dir=str(tmp_path))
+    print("resonance job", r, eig, res, flush=True)
+    assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
+    hs = np.sort(np.linalg.eigvalsh(h))
+    e_start, e_step = float(hs[20] + 1e-4), 2e-4
+    text = _cx_job_text(prob, files, 4, f"{e_start!r} {e_step!r} 0 0 0 3 10", ["app.dat", "apr.dat"])
+    (tmp_path / "crp.in").write_text(text)
+    job = J.read_iterate_input(str(tmp_path / "crp.in"))
+    n, crp, _ = J.run_pist_job(job, base_dir=str(tmp_path), seed=5)
+    rng = np.random.default_rng(5)
+    x0 = rng.random((3, prob.N)) + 1j * rng.random((3, prob.N))
+    op.diag()
+    ref = O.crp_dx(op, app, apr, [e_start + k * e_step for k in range(3)], 10, list(x0), -1, 500, 1e-10, True)
+    print("crp job", crp, ref, flush=True)
+    assert n == 3 and np.abs(crp - ref).max() <= 1e-8 * np.abs(ref).max()
