@@ -194,6 +194,7 @@ def main():
         out["pist"] = pist_bench(args.no_cpu)
         out["gradients_64x96x80"] = gradients_bench(ctx, prob)
         out["uniform_windows"] = config4_bench(["batched_" + c for c in args.config4])
+        out["absorbing_potential_24x32x24"] = complex_bench()
     ctx.close()
     print(json.dumps(out))
 
@@ -349,6 +350,51 @@ def gradients_bench(ctx, prob):
     out["max_ritz_residual"] = float((rr * rr).sum(dim=1).sqrt().max())
     out["orthogonalisation_passes"] = 2
     ctx.set_pist_orth(1)
+    return out
+
+
+def complex_bench():
+    """The resonance and CRP paths on the 24x32x24 operator with an absorbing
+    potential in R (strength 0.01 hartree beyond 70% of the R range): the 4
+    resonance energies whose real parts are nearest a level (PIST_CONV_SX,
+    ETOL 1e-12, QMR 1e-10) and N(E) at 3 energies from 10 Lanczos steps
+    (product absorbing potential in r), OSB set-up outside the timed region."""
+    import torch
+
+    from scalit_amd import _lib as L
+    from scalit_amd import problems as P
+    from scalit_amd.context import Context
+
+    prob = P.config_problem("pist_fp64")
+    ctx = Context.from_problem(prob, e0=0.0)
+    ctx.block_diag(1000, 1e-5)
+    _, eig0 = ctx.get_precond()
+    srt = np.sort(eig0)
+    e0 = float(0.5 * (srt[30] + srt[31]))
+    ctx.set_shift(e0)
+    ap = P.absorbing_potential(prob, 0.01)
+    ctx.set_ap(L.AP, ap)
+    x0 = ctx.to_device_cx(P.random_vectors(prob.N, 1, 5)[0] + 0j)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r, eig, res, st = ctx.pist_conv_sx(x0, e0, 1e-12, 4, 160, 20, 5, 2, qmr_type=-1, qmrmax=3000, qmrtol=1e-10)
+    dt = time.perf_counter() - t0
+    out = {"resonance": {"ret": int(r), "seconds": round(dt, 3), "energies": [[float(z.real), float(z.imag)]
+                                                                            for z in eig],
+                         "qmr_iters": int(st[0]), "solves": int(st[1]), "krylov": int(st[2]),
+                         "us_per_qmr_iteration": round(dt * 1e6 / max(1, int(st[0])), 1)}}
+    ctx.set_ap(L.APP, P.absorbing_potential(prob, 0.01, 0.7, coord="r"))
+    ctx.set_ap(L.APR, ap)
+    energies = [e0, e0 + 2e-4, e0 + 4e-4]
+    xs = ctx.to_device_cx(np.stack([P.random_vectors(prob.N, 1, 7 + k)[0] + 0j for k in range(3)]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    crp, cst = ctx.crp(energies, 10, xs, qmr_type=-1, qmrmax=3000, qmrtol=1e-10)
+    dt = time.perf_counter() - t0
+    out["crp"] = {"energies": energies, "crp": [float(v) for v in crp], "seconds": round(dt, 3),
+                  "qmr_iters": int(cst[0]), "solves": int(cst[1]),
+                  "us_per_qmr_iteration": round(dt * 1e6 / max(1, int(cst[0])), 1)}
+    ctx.close()
     return out
 
 
