@@ -255,6 +255,13 @@ int scalit_qmr_cx(scalit_ctx *ctx, int qmr_type, int hc, int pc, int qmrmax, dou
 int scalit_pist_conv_sx(scalit_ctx *ctx, double e0, double etol, int ntype, const double *x0, int startcnt,
                         int stepcnt, int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int pc,
                         double *eig, int *ret, double *res, int *stats);
+/* OSB_PISTCONVDX2 (src/iterate/module/osbw.pistconv.h:204-244): PIST_CONV_CX
+ * (src/iterate/libiterate/pist/pist_cx.f90, pistHij_cx.f90) -- as
+ * scalit_pist_conv_sx with the Hermitian products x^H y and real norms of
+ * MGS_ORTH_CX (orth/gsorth_cx.f90:51-79) for the Krylov basis and its matrix. */
+int scalit_pist_conv_cx(scalit_ctx *ctx, double e0, double etol, int ntype, const double *x0, int startcnt,
+                        int stepcnt, int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int pc,
+                        double *eig, int *ret, double *res, int *stats);
 /* osbCRP (src/iterate/module/osbw.crp.h:7-49): the cumulative reaction
  * probability N(E) = 4 sum of the nm Lanczos eigenvalues (LAN_DX,
  * src/iterate/libiterate/lanczos/lan_dx.f90:299-325, full re-orthogonalisation)

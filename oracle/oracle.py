@@ -781,20 +781,38 @@ def mgs_orth_sx(w, vj: list) -> bool:
     return True
 
 
+def mgs_orth_cx(w, vj: list) -> bool:
+    """MGS_ORTH_CX (src/iterate/libiterate/orth/gsorth_cx.f90:51-79): one
+    modified Gram-Schmidt step under x^H y, normalised by the real norm;
+    appends to vj."""
+    w = np.array(w, dtype=complex)
+    for v in vj:
+        w = w - np.vdot(v, w) * v
+    nw = np.sqrt(np.vdot(w, w).real)
+    if nw == 0:
+        return False
+    vj.append(w / nw)
+    return True
+
+
 def _op_cx(vals: np.ndarray, ntype: int) -> np.ndarray:
     """getOpCX (src/libutil/serial/base/util_cx.f90:375-420) for the two
     measures PIST_CONV_SX uses: 1 = real part, 6 = |z|^2."""
     return vals.real.copy() if ntype == 1 else vals.real ** 2 + vals.imag ** 2
 
 
-def pist_conv_sx(hx0, linsolv, x, e0, etol, ntype, m, m_max, startcnt, stepcnt, stepeig, eigvals=np.linalg.eigvals):
+def pist_conv_sx(hx0, linsolv, x, e0, etol, ntype, m, m_max, startcnt, stepcnt, stepeig, eigvals=np.linalg.eigvals,
+                 hermitian=False):
     """PIST_CONV_SX (src/iterate/libiterate/pist/pist_sx.f90:9-110) with
     pistH0_SX (pistHij_sx.f90:58-121): Krylov vectors from the shifted solves,
     MGS_ORTH_SX, HMAT(i,j) = v_i^T H0 v_j, complex eigenvalues (ZGEEV: here
     eigvals), windows by real part nearest e0 (getWindow_CX, nDiffType 1),
     convergence by getDiffMax_CX of |z|^2 (nConvType 6), ascending real parts
     (Reorder_CX).  ntype > 0 is refused (the reference passes getSepMax_CX
-    five arguments for its four)."""
+    five arguments for its four).  hermitian: PIST_CONV_CX (pist_cx.f90 with
+    pistHij_cx.f90), the same loop with x^H y, real norms and MGS_ORTH_CX."""
+    dot = (lambda a, b: complex(np.vdot(a, b))) if hermitian else dot_cx
+    orth = mgs_orth_cx if hermitian else mgs_orth_sx
     if ntype > 0:
         raise ValueError("PIST_CONV_SX with NTYPE > 0 is not defined")
     if stepeig >= m_max or startcnt > m_max or m > m_max:
@@ -806,7 +824,7 @@ def pist_conv_sx(hx0, linsolv, x, e0, etol, ntype, m, m_max, startcnt, stepcnt, 
     while cnt_high <= m_max:
         # pistH0_SX
         if cnt_low == 1:
-            t = np.sqrt(complex(dot_cx(x, x)))
+            t = np.sqrt(dot(x, x).real) if hermitian else np.sqrt(complex(dot_cx(x, x)))
             if t == 0:
                 return -cnt_high, eig, res
             vj = [x / t]
@@ -817,16 +835,16 @@ def pist_conv_sx(hx0, linsolv, x, e0, etol, ntype, m, m_max, startcnt, stepcnt, 
             num, w, _ = linsolv(vj[j - 1])
             if num <= 0:
                 return -cnt_high, eig, res
-            if not mgs_orth_sx(w, vj):
+            if not orth(w, vj):
                 return -cnt_high, eig, res
         for j in range(cnt_low, cnt_high + 1):
             wj = hx0(vj[j - 1])
             for i in range(1, cnt_high + 1):
-                hmat[i - 1, j - 1] = dot_cx(vj[i - 1], wj)
+                hmat[i - 1, j - 1] = dot(vj[i - 1], wj)
         for j in range(1, cnt_low):
             wj = hx0(vj[j - 1])
             for i in range(cnt_low, cnt_high + 1):
-                hmat[i - 1, j - 1] = dot_cx(vj[i - 1], wj)
+                hmat[i - 1, j - 1] = dot(vj[i - 1], wj)
         num = cnt_high - stepeig
         tmp = eigvals(hmat[:num, :num])
         ind, _ = get_window(e0, _op_cx(tmp, 1), m)     # 1-based, as getWindow0's
@@ -936,6 +954,16 @@ def ref_lanhij_dx(apply, x, m: int):
     code = R.lanhij_dx_(C.byref(C.c_int(n)), C.c_void_p(x.ctypes.data), C.byref(C.c_int(m)), f, dptr(alpha),
                         dptr(beta))
     return code, alpha, beta
+
+
+def ref_mgs_orth_cx(w, vj: np.ndarray, m: int):
+    """The reference's compiled MGS_ORTH_CX (gsorth_cx.f90:51-79)."""
+    w = np.ascontiguousarray(w, dtype=np.complex128)
+    R = ref()
+    R.mgs_orth_cx_.restype = C.c_int
+    ok = R.mgs_orth_cx_(C.byref(C.c_int(w.size)), C.c_void_p(w.ctypes.data), C.byref(C.c_int(m)),
+                        C.c_void_p(vj.ctypes.data))
+    return ok != 0
 
 
 def ref_mgs_orth_sx(w, vj: np.ndarray, m: int):

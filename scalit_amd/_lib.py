@@ -81,6 +81,7 @@ SIGNATURES = {
     "scalit_qmr_cx": (_i, [_vp, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp]),
     "scalit_pist_conv_sx": (_i, [_vp, _d, _d, _i, _vp, _i, _i, _i, _i, _i, _i, _i, _d, _i, _vp, _vp, _vp, _vp]),
     "scalit_small_eig_cx": (_i, [_i, _vp, _i, _vp]),
+    "scalit_pist_conv_cx": (_i, [_vp, _d, _d, _i, _vp, _i, _i, _i, _i, _i, _i, _i, _d, _i, _vp, _vp, _vp, _vp]),
     "scalit_crp": (_i, [_vp, _i, _vp, _i, _vp, _i, _i, _d, _i, _vp, _vp]),
     "scalit_set_qmr_persistent": (_i, [_vp, _i]),
     "scalit_load_data_grid": (_i, [C.c_char_p, _l, _i, _dp]),

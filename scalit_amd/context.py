@@ -431,17 +431,18 @@ class Context:
 
     def pist_conv_sx(self, x0: torch.Tensor, e0: float, etol: float, m: int, m_max: int, startcnt: int, stepcnt: int,
                      stepeig: int = 1, ntype: int = 0, qmr_type: int = -1, qmrmax: int = 1000, qmrtol: float = 1e-3,
-                     pc: int = L.TMAP):
+                     pc: int = L.TMAP, hermitian: bool = False):
         """OSB_PISTCONVDX1 (src/iterate/module/osbw.pistconv.h:98-140): complex
-        resonance energies nearest e0 (by real part) of H - i AP.  Returns
-        (PIST_CONV_SX's value, eigenvalues ascending by real part, res, stats)."""
+        resonance energies nearest e0 (by real part) of H - i AP; with
+        hermitian, OSB_PISTCONVDX2's PIST_CONV_CX (x^H y products).  Returns
+        (the PIST value, eigenvalues ascending by real part, res, stats)."""
         x0 = x0.contiguous()
         eig = (C.c_double * (2 * m))()
         ret, res = C.c_int(), C.c_double()
         stats = (C.c_int * 4)()
-        check(L.lib().scalit_pist_conv_sx(self._h, e0, etol, ntype, _ptr(x0), startcnt, stepcnt, stepeig, m, m_max,
-                                          qmr_type, qmrmax, qmrtol, pc, eig, C.byref(ret), C.byref(res), stats),
-              "scalit_pist_conv_sx")
+        fn = L.lib().scalit_pist_conv_cx if hermitian else L.lib().scalit_pist_conv_sx
+        check(fn(self._h, e0, etol, ntype, _ptr(x0), startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, qmrtol,
+                 pc, eig, C.byref(ret), C.byref(res), stats), "scalit_pist_conv_sx/cx")
         e = np.array(eig[:])
         return ret.value, e[0::2] + 1j * e[1::2], res.value, np.array(stats[:])
 

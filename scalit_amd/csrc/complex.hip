@@ -10,7 +10,9 @@
 //    (src/iterate/module/osbw.qmrdx.h:16-27) calls it: HX_DX = MYHX_DX(sHC),
 //    PX_DX = MYPX_DX(sOSB, sPC).  Vectors stay on the GPU; the recurrence's
 //    scalars are formed on the host from deterministic two-stage reductions.
-//  * PIST_CONV_SX (src/iterate/libiterate/pist/pist_sx.f90:9-110) with
+//  * PIST_CONV_SX (src/iterate/libiterate/pist/pist_sx.f90:9-110) and its
+//    Hermitian twin PIST_CONV_CX (pist_cx.f90, pistHij_cx.f90, MGS_ORTH_CX
+//    orth/gsorth_cx.f90:51-79; OSB_PISTCONVDX2, osbw.pistconv.h:204-244) with
 //    pistH0_SX (pistHij_sx.f90:58-121) and MGS_ORTH_SX (orth/gsorth_sx.f90:
 //    47-74), as OSB_PISTCONVDX1 (src/iterate/module/osbw.pistconv.h:98-140)
 //    drives it: H0X = MYHX_DX(TMAP0), LinSolv = OSB_QMRDX with sHC = TMAP and
@@ -126,7 +128,8 @@ __global__ void __launch_bounds__(256) k_cx_dot(long N, const double *__restrict
     }
 }
 
-// second stage in a fixed order; op 1 takes the principal complex square root
+// second stage in a fixed order; op 1 takes the principal complex square root,
+// op 2 the square root of the real part (DSQRT(DBLE(.)) of a Hermitian norm)
 __global__ void __launch_bounds__(256) k_cx_fin(int G, const double *__restrict__ part, int op,
                                                 double *__restrict__ out)
 {
@@ -153,6 +156,9 @@ __global__ void __launch_bounds__(256) k_cx_fin(int G, const double *__restrict_
             const double re = sqrt(0.5 * (m + a)), im = copysign(sqrt(0.5 * (m - a)), b);
             a = re;
             b = im;
+        } else if (op == 2) {
+            a = sqrt(a);
+            b = 0.0;
         }
         out[0] = a;
         out[1] = b;
@@ -513,6 +519,9 @@ struct PistSx {
     double e0, etol;
     int ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, pc;
     double qmrtol;
+    // 0: PIST_CONV_SX (x^T y, complex square-root norms, MGS_ORTH_SX);
+    // 1: PIST_CONV_CX (x^H y, real norms, MGS_ORTH_CX, pistHij_cx.f90)
+    int herm;
 };
 
 // PIST_CONV_SX.  x0: device split start vector; eig: host, 2 m doubles
@@ -547,9 +556,10 @@ static int pist_sx_run(scalit_ctx *c, const PistSx &p, const double *x0, double 
     while (cnt_high <= mm) {
         // pistH0_SX (pistHij_sx.f90:58-121)
         int m_start;
+        const int cj = p.herm, nop = p.herm ? 2 : 1;
         if (cnt_low == 1) {
             m_start = 1;
-            cx_dot(X, N, x0, x0, 0, 0, 1, s);
+            cx_dot(X, N, x0, x0, cj, 0, nop, s);
             SC_TRY(cx_read(X, 1, s));
             const cd t = slot_val(X, 0);
             if (t == cd(0.0, 0.0)) {
@@ -577,11 +587,11 @@ static int pist_sx_run(scalit_ctx *c, const PistSx &p, const double *x0, double 
             double *vn = VJ(j + 1);
             HIP_TRY(hipMemcpyAsync(vn, W, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
             for (int i = 1; i <= j; i++) {
-                cx_dot(X, N, VJ(i), vn, 0, 0, 0, s);
+                cx_dot(X, N, VJ(i), vn, cj, 0, 0, s);
                 hipLaunchKernelGGL(k_cx_axpy_dev, dim3(cx_grid(N)), dim3(256), 0, s, N, (const double *)X.scal.p,
                                    (const double *)VJ(i), vn);
             }
-            cx_dot(X, N, vn, vn, 0, 1, 1, s);
+            cx_dot(X, N, vn, vn, cj, 1, nop, s);
             SC_TRY(cx_read(X, 2, s));
             if (slot_val(X, 1) == cd(0.0, 0.0)) {
                 fail = true;
@@ -598,7 +608,7 @@ static int pist_sx_run(scalit_ctx *c, const PistSx &p, const double *x0, double 
             const int i0 = j >= cnt_low ? 1 : cnt_low;
             SC_TRY(hx_apply_cx(c, SCALIT_TMAP0, 1, VJ(j), HW, s));
             if (stats) stats[3] += 1;
-            for (int i = i0; i <= cnt_high; i++) cx_dot(X, N, VJ(i), HW, 0, i - i0, 0, s);
+            for (int i = i0; i <= cnt_high; i++) cx_dot(X, N, VJ(i), HW, cj, i - i0, 0, s);
             SC_TRY(cx_read(X, cnt_high - i0 + 1, s));
             for (int i = i0; i <= cnt_high; i++) H(i, j) = slot_val(X, i - i0);
         }
@@ -800,7 +810,21 @@ int scalit_pist_conv_sx(scalit_ctx *c, double e0, double etol, int ntype, const 
     if (!c->has_res || !c->has_precond || !c->has_ap[SCALIT_AP]) return SCALIT_ESTATE;
     if (c->osb == SCALIT_TOSBW) return SCALIT_EINVAL;
     HIP_TRY(hipDeviceSynchronize());
-    PistSx p{e0, etol, ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, pc, qmrtol};
+    PistSx p{e0, etol, ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, pc, qmrtol, 0};
+    return pist_sx_run(c, p, x0, eig, ret, res, stats, c->stream);
+}
+
+int scalit_pist_conv_cx(scalit_ctx *c, double e0, double etol, int ntype, const double *x0, int startcnt, int stepcnt,
+                        int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int pc, double *eig,
+                        int *ret, double *res, int *stats)
+{
+    if (!c || !x0 || !eig || !ret || m < 1 || m_max < 1 || ntype > 0) return SCALIT_EINVAL;
+    if (pc != SCALIT_TMAP && pc != SCALIT_TA1) return SCALIT_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->has_res || !c->has_precond || !c->has_ap[SCALIT_AP]) return SCALIT_ESTATE;
+    if (c->osb == SCALIT_TOSBW) return SCALIT_EINVAL;
+    HIP_TRY(hipDeviceSynchronize());
+    PistSx p{e0, etol, ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, pc, qmrtol, 1};
     return pist_sx_run(c, p, x0, eig, ret, res, stats, c->stream);
 }
 

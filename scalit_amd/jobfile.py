@@ -184,10 +184,11 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
 
     * sJOB = 1, bound states: PIST for M0 = conv_num levels around conv_e0;
       returns (ret, eigenvalues, res);
-    * sJOB = 2, resonances (OSB_PISTCONVDX1): AP = APP from its file, the
-      complex-symmetric PIST for M0 energies of H - i AP whose real parts are
-      nearest conv_e0, with the absorbing term in the preconditioner when
-      sAP; returns (ret, complex energies, res);
+    * sJOB = 2 / 3, resonances (OSB_PISTCONVDX1 / OSB_PISTCONVDX2): AP = APP
+      from its file, the complex-symmetric (2) or Hermitian-product (3) PIST
+      for M0 energies of H - i AP whose real parts are nearest conv_e0, with
+      the absorbing term in the preconditioner when sAP; returns (ret,
+      complex energies, res);
     * sJOB = 4-6, CRP (osbCRP): APP and APR from their files, N(E) at
       conv_num energies conv_e0 + k conv_tol from conv_gap Lanczos steps
       each; returns (number of energies, N(E) per energy, 0.0).
@@ -200,9 +201,9 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
     from . import _lib as L
     from .context import Context
 
-    if job.sCX or job.sJOB not in (JOB_BOUND, JOB_RES1, JOB_CRP, JOB_CRP1, JOB_CRP2):
-        raise ValueError("only real-H0 jobs (sCX = F): bound states (sJOB = 1), resonances with the symmetric "
-                         "PIST (sJOB = 2) and CRP (sJOB = 4-6)")
+    if job.sCX or job.sJOB not in (JOB_BOUND, JOB_RES1, JOB_RES2, JOB_CRP, JOB_CRP1, JOB_CRP2):
+        raise ValueError("only real-H0 jobs (sCX = F): bound states (sJOB = 1), resonances (sJOB = 2, 3) and "
+                         "CRP (sJOB = 4-6)")
     path = lambda k: os.path.join(base_dir, job.files[k]) if k in job.files else None
     nosb = {TOSB: L.TOSB, TOSBD1: L.TOSBD1, TOSBD2: L.TOSBD2, TOSBW: L.TOSBW}.get(job.sOSB)
     if nosb is None:
@@ -221,7 +222,7 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
             ctx.save_precond(path("VOSB"), path("EIG"), mode)
     m0 = job.conv_num
     rng = np.random.default_rng(seed)
-    if job.sJOB == JOB_RES1:
+    if job.sJOB in (JOB_RES1, JOB_RES2):
         # osb.io.h:20-23: APP from its file, APR = 0, AP = APP
         app = Context.load_file(path("APP"), ctx.N, "dir")
         ctx.set_ap(L.AP, app)
@@ -231,7 +232,8 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
         e0, etol, start, step, stepeig, nmax = pist_params(job, m0)
         x0 = rng.random(ctx.N) + 1j * rng.random(ctx.N)
         r, eig, res, _ = ctx.pist_conv_sx(ctx.to_device_cx(x0), e0, etol, m0, nmax, start, step, stepeig,
-                                          qmrmax=job.qmr_max, qmrtol=job.qmr_tol, pc=L.TMAP if job.sAP else L.TA1)
+                                          qmrmax=job.qmr_max, qmrtol=job.qmr_tol, pc=L.TMAP if job.sAP else L.TA1,
+                                          hermitian=job.sJOB == JOB_RES2)
         torch.cuda.synchronize()
         ctx.close()
         return r, eig, res

@@ -100,12 +100,13 @@ def test_qmr_cx_matches_oracle(qmr_type, pc):
     ctx.close()
 
 
+@pytest.mark.parametrize("hermitian", [False, True])
 @pytest.mark.parametrize("strength", [0.002, 0.02])
-def test_pist_conv_sx_resonances(strength):
+def test_pist_conv_sx_resonances(strength, hermitian):
     """The 4 resonance energies whose real parts are nearest E0 on
     8 x 9 x 6, against a dense eigensolve of H - i AP and the oracle's
-    PIST_CONV_SX (same return code)."""
-    print(f"pist_sx strength={strength}", flush=True)
+    PIST_CONV_SX (or, hermitian, PIST_CONV_CX; same return code)."""
+    print(f"pist_sx strength={strength} hermitian={hermitian}", flush=True)
     prob, op, ctx, aps, _ = cx_setup((8, 9, 6), strength)
     ev = np.linalg.eigvals(op.dense() - 1j * np.diag(aps[0]))
     srt = np.sort(ev.real)
@@ -116,12 +117,13 @@ def test_pist_conv_sx_resonances(strength):
     near = near[np.argsort(near.real)]
     x = np.atleast_2d(P.random_vectors(prob.N, 1, 3)[0] + 0j)
     r, eig, res, stats = ctx.pist_conv_sx(ctx.to_device_cx(x), e0, 1e-12, 4, 160, 16, 4, 2, qmr_type=-1,
-                                          qmrmax=500, qmrtol=1e-10)
+                                          qmrmax=500, qmrtol=1e-10, hermitian=hermitian)
     print("pist_sx", r, eig, res, stats, flush=True)
     assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
     lin = lambda v: O.qmr_cx(lambda u: O.hx_dx(op, u, O.TMAP, aps), lambda u: O.px_dx(op, u, O.TOSB, O.TMAP, aps),
                              v, -1, 500, 1e-10)
-    r_o, eig_o, _ = O.pist_conv_sx(lambda v: O.hx_dx(op, v, O.TMAP0, aps), lin, x[0], e0, 1e-12, 0, 4, 160, 16, 4, 2)
+    r_o, eig_o, _ = O.pist_conv_sx(lambda v: O.hx_dx(op, v, O.TMAP0, aps), lin, x[0], e0, 1e-12, 0, 4, 160, 16, 4, 2,
+                                   hermitian=hermitian)
     assert r == r_o, (r, r_o)
     ctx.close()
 

@@ -174,6 +174,10 @@ def test_resonance_and_crp_jobs_from_input_files(tmp_path):
     r, eig, res = J.run_pist_job(job, base_dir=str(tmp_path))
     print("resonance job", r, eig, res, flush=True)
     assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
+    job.sJOB = J.JOB_RES2   # the Hermitian-product PIST of OSB_PISTCONVDX2
+    r, eig, res = J.run_pist_job(job, base_dir=str(tmp_path))
+    print("resonance job (sJOB = 3)", r, eig, res, flush=True)
+    assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
     hs = np.sort(np.linalg.eigvalsh(h))
     e_start, e_step = float(hs[20] + 1e-4), 2e-4
     text = _cx_job_text(prob, files, 4, f"{e_start!r} {e_step!r} 0 0 0 3 10", ["app.dat", "apr.dat"])

@@ -450,8 +450,9 @@ def test_qmr_cx_solves_the_absorbing_system():
         assert np.linalg.norm(dense @ x - b) <= 1e-9 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("hermitian", [False, True])
 @pytest.mark.parametrize("strength", [0.002, 0.02])
-def test_pist_conv_sx_resonances_against_dense(strength):
+def test_pist_conv_sx_resonances_against_dense(strength, hermitian):
     """PIST_CONV_SX restated (complex eigenvalues of the Krylov matrix by
     numpy's ZGEEV) finds the 4 complex eigenvalues of H - i AP whose real
     parts are nearest E0, against a dense eigensolve, to 1e-10 relative."""
@@ -465,7 +466,8 @@ def test_pist_conv_sx_resonances_against_dense(strength):
     x = P.rng(3).standard_normal(op.N) + 0j
     lin = lambda v: O.qmr_cx(lambda u: O.hx_dx(op, u, O.TMAP, aps), lambda u: O.px_dx(op, u, O.TOSB, O.TMAP, aps),
                              v, -1, 500, 1e-10)
-    r, eig, res = O.pist_conv_sx(lambda v: O.hx_dx(op, v, O.TMAP0, aps), lin, x, e0, 1e-12, 0, 4, 80, 16, 4, 2)
+    r, eig, res = O.pist_conv_sx(lambda v: O.hx_dx(op, v, O.TMAP0, aps), lin, x, e0, 1e-12, 0, 4, 80, 16, 4, 2,
+                                 hermitian=hermitian)
     assert r > 0
     assert np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
     with pytest.raises(ValueError):

@@ -383,3 +383,11 @@ def test_lan_hij_dx_and_mgs_orth_sx_match_reference():
     assert O.ref_mgs_orth_sx(w, vj, 3)
     assert O.mgs_orth_sx(w, vs)
     assert np.abs(vj[:, 3] - vs[3]).max() <= 1e-13
+    # MGS_ORTH_CX (gsorth_cx.f90:51-79), the Hermitian twin
+    vh = []
+    vj = np.zeros((n, 4), dtype=np.complex128, order="F")
+    for k in range(3):
+        assert O.mgs_orth_cx(rng.standard_normal(n) + 1j * rng.standard_normal(n), vh)
+        vj[:, k] = vh[-1]
+    assert O.ref_mgs_orth_cx(w, vj, 3) and O.mgs_orth_cx(w, vh)
+    assert np.abs(vj[:, 3] - vh[3]).max() <= 1e-13
