@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--pist-run", action="store_true",
                     help="run only the PIST measurements on the 24x32x24 workload (no H*X bench, no host "
                          "baseline) and print them as one JSON line, for profiling that path")
+    ap.add_argument("--complex-run", action="store_true",
+                    help="run only the resonance and CRP measurements on the 24x32x24 operator with an absorbing "
+                         "potential and print them as one JSON line, for profiling that path")
     ap.add_argument("--pmc-run", action="store_true",
                     help="workload for rocprofv3 --pmc: three 512 MiB device copies (the counter calibration), "
                          "then --steps H*X calls, the OSB set-up and --steps P*X calls; prints no JSON")
@@ -112,6 +115,9 @@ def main():
         return
     if args.pist_run:
         print(json.dumps({"pist": pist_bench(True)}), flush=True)
+        return
+    if args.complex_run:
+        print(json.dumps({"absorbing_potential_24x32x24": complex_bench()}), flush=True)
         return
     prob = P.config_problem("hx_bench")
     ctx = Context.from_problem(prob)

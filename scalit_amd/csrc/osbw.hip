@@ -97,17 +97,20 @@ __global__ void k_osbw_mark(int n, const int *__restrict__ ind, double *__restri
 }
 
 // d = M z, z(k) = V1[b_k](:, c_k) . x(b_k block): the window's components of
-// V1^T x solved; one workgroup per vector, one wave per window state
-__global__ void __launch_bounds__(256) k_osbw_gather(int n1, int n, const int *__restrict__ ind,
-                                                     const double *__restrict__ v1, const double *__restrict__ m,
-                                                     const double *__restrict__ x, long ldx, double *__restrict__ d,
-                                                     const int *__restrict__ active)
+// V1^T x solved; one workgroup of 16 waves per vector, one wave per window
+// state for the dots, then each row of M z split over GATHER_KS partial sums
+// (reduced through LDS in a fixed order)
+constexpr int GATHER_NT = 1024, GATHER_KS = 4;
+__global__ void __launch_bounds__(GATHER_NT) k_osbw_gather(int n1, int n, const int *__restrict__ ind,
+                                                           const double *__restrict__ v1, const double *__restrict__ m,
+                                                           const double *__restrict__ x, long ldx,
+                                                           double *__restrict__ d, const int *__restrict__ active)
 {
-    extern __shared__ double zw[];
+    extern __shared__ double zw[];   // n values of z, then GATHER_KS x n partial sums
     const int v = blockIdx.x;
     if (active && !active[v]) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int k = wv; k < n; k += 4) {
+    for (int k = wv; k < n; k += GATHER_NT / 64) {
         const long g = ind[k], b = g / n1;
         const double *col = v1 + b * n1 * n1 + (long)n1 * (g - b * n1);
         const double *xb = x + (long)v * ldx + b * n1;
@@ -117,9 +120,18 @@ __global__ void __launch_bounds__(256) k_osbw_gather(int n1, int n, const int *_
         if (lane == 0) zw[k] = acc;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 256) {
+    double *part = zw + n;
+    const int rows = GATHER_NT / GATHER_KS, q = threadIdx.x / rows, kq = (n + GATHER_KS - 1) / GATHER_KS;
+    const int k0 = q * kq, k1 = min(n, k0 + kq);
+    for (int i = threadIdx.x % rows; i < n; i += rows) {
         double acc = 0.0;
-        for (int k = 0; k < n; k++) acc += m[i + (long)n * k] * zw[k];
+        for (int k = k0; k < k1; k++) acc += m[i + (long)n * k] * zw[k];
+        part[q * n + i] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += GATHER_NT) {
+        double acc = part[i];
+        for (int j = 1; j < GATHER_KS; j++) acc += part[j * n + i];
         d[(long)v * n + i] = acc;
     }
 }
@@ -148,8 +160,8 @@ __global__ void __launch_bounds__(256) k_osbw_scatter(int n1, int n, const int *
 void osbw_gather(scalit_ctx *c, int nvec, const double *x, long ldx, const int *active, hipStream_t s)
 {
     const int n = c->osbw_n;
-    hipLaunchKernelGGL(k_osbw_gather, dim3(nvec), dim3(256), sizeof(double) * (size_t)n, s, c->lv[0].n, n,
-                       c->osbw_ind.p, c->lv[0].vosb.p, c->osbw_m.p, x, ldx, c->osbw_d.p, active);
+    hipLaunchKernelGGL(k_osbw_gather, dim3(nvec), dim3(GATHER_NT), sizeof(double) * (size_t)n * (1 + GATHER_KS), s,
+                       c->lv[0].n, n, c->osbw_ind.p, c->lv[0].vosb.p, c->osbw_m.p, x, ldx, c->osbw_d.p, active);
 }
 
 void osbw_scatter(scalit_ctx *c, int nvec, double *y, long ldy, const int *active, hipStream_t s)

@@ -154,6 +154,35 @@ def test_tosbw_window_px_and_solvers(npc):
     ctx.close()
 
 
+def test_tosbw_bench_grid():
+    """TOSBW on the 64 x 96 x 80 operator (level-1 blocks of 64 on the fused
+    MFMA kernel), a window of the 64 OSB states nearest E0: P*X of 4 vectors
+    against the oracle's px_tosbw on the GPU's own VOSB / EIG0 (1e-10), and a
+    batch-16 QMR with TOSBW to 1e-8."""
+    print("tosbw 64x96x80", flush=True)
+    prob = P.config_problem("hx_bench")
+    e0 = float(np.sort(prob.res)[50])
+    ctx = Context.from_problem(prob, e0=e0)
+    ctx.block_diag(1000, 1e-5)
+    vosb, eig0 = ctx.get_precond()
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=e0)
+    op.set_precond(vosb, eig0)
+    n, d = ctx.set_osbw(e0, 1e-4, 64)
+    win = O.osbw_setup(op, e0, 1e-4, 64)
+    assert n == 64 and d == win[1]
+    x = P.random_vectors(prob.N, 4, 51)
+    y = ctx.px(ctx.to_device(x), nosb=L.TOSBW).cpu().numpy()
+    for v in range(4):
+        yo = O.px_tosbw(op, x[v], win)
+        assert np.abs(y[v] - yo).max() <= 1e-10 * np.abs(yo).max(), v
+    ctx.set_osb(L.TOSBW)
+    b = ctx.to_device(P.random_vectors(prob.N, 16, 5))
+    ret, _, res = ctx.qmr(b, ntype=-1, qmrmax=200, qmrtol=1e-8)
+    print("TOSBW QMR", ret, max(res), flush=True)
+    assert (np.array(ret) > 0).all()
+    ctx.close()
+
+
 @pytest.mark.parametrize("persist", [1, 0])
 def test_pist_persistent_and_per_step_agree(persist):
     """The config-1 PIST (lowest 20 levels, 24 x 32 x 24) against the golden
