@@ -37,10 +37,27 @@ using cd = std::complex<double>;
 constexpr int CX_G = 512;   // blocks of a reduction's first stage
 
 struct CxState {
-    DBuf<double> vec;    // QMR vectors: R, UX, UR, Q, S, V, Z, HX0 (2 N each)
+    DBuf<double> vec;    // QMR vectors: R, UX, UR, Q, S, V, Z, HX0, X, B (2 N each)
     DBuf<double> part;   // first-stage partial sums, CX_G x 2 per slot
     DBuf<double> scal;   // reduced values, 2 per slot
     std::vector<double> hscal;
+    DBuf<double> qs;     // QMR_CX's scalars on the device
+    // one QMR_CX iteration as a HIP graph, for the operator generation, the
+    // shift E0 / DE / beta (kernel arguments) and the solver options it was
+    // recorded with
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long g_gen = -1;
+    int g_key[4] = {0, 0, 0, 0};
+    double g_dkey[4] = {0.0, 0.0, 0.0, 0.0};
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~CxState() { drop_graph(); }
     DBuf<double> vj;     // PIST_CONV_SX's Krylov basis
     DBuf<double> wj;     // its solve and H0X vectors
     int slots = 0;
@@ -216,6 +233,7 @@ static int cx_blocks(long N) { return (int)std::min<long>((N + 255) / 256, CX_G)
 static int cx_slots(CxState &X, int n)
 {
     if (X.slots >= n) return SCALIT_OK;
+    X.drop_graph();   // the recorded QMR iteration addresses the partial sums
     SC_TRY(X.part.alloc((size_t)n * CX_G * 2));
     SC_TRY(X.scal.alloc((size_t)n * 2));
     X.hscal.assign((size_t)n * 2, 0.0);
@@ -265,113 +283,429 @@ static int hx_apply_cx(scalit_ctx *c, int ntype, int nvec, const double *x, doub
     return SCALIT_OK;
 }
 
+// ---------------------------------------------------------------- QMR_CX
+// The recurrence's scalars live on the device (QS below): each step's vector
+// kernel writes one partial sum per block for the reductions it feeds, and a
+// one-workgroup step kernel sums them in a fixed order and advances the
+// scalars.  A stop flag set by a breakdown, by convergence or by a zero norm
+// makes every later kernel of the solve return at once, so the host launches
+// iterations in groups and reads the flag once per group.
+enum QsIdx {
+    QS_ZVRAT, QS_VNRM, QS_ZNRM, QS_VZ, QS_VZI, QS_QS, QS_QSI, QS_BETA, QS_BETAI, QS_ETA, QS_ETAI, QS_GAMAMO,
+    QS_THETAMO, QS_THETA, QS_GAMA, QS_VNRMPO, QS_BNRM, QS_RES, QS_STOP, QS_CODE, QS_ITER, QS_CQ, QS_CQI, QS_TG,
+    QS_INVV, QS_INVZ, QS_COUNT
+};
+constexpr int QS_LEN = 32;
+
+__device__ __forceinline__ bool qs_stopped(const double *qs) { return qs[QS_STOP] != 0.0; }
+
+// block partial sums of up to four quantities into part[slot][block][2]
+template <int K>
+__device__ __forceinline__ void cx_block_sums(double (&v)[K][2], double *part)
+{
+    __shared__ double sh[K][2][256];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        sh[k][0][threadIdx.x] = v[k][0];
+        sh[k][1][threadIdx.x] = v[k][1];
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                sh[k][0][threadIdx.x] += sh[k][0][threadIdx.x + o];
+                sh[k][1][threadIdx.x] += sh[k][1][threadIdx.x + o];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            part[(size_t)k * CX_G * 2 + 2 * blockIdx.x] = sh[k][0][0];
+            part[(size_t)k * CX_G * 2 + 2 * blockIdx.x + 1] = sh[k][1][0];
+        }
+}
+
+// V *= 1/VNRM, Z *= 1/ZNRM; partial V^T Z
+__global__ void __launch_bounds__(256) k_cxq_vz(long N, const double *__restrict__ qs, double *__restrict__ V,
+                                                double *__restrict__ Z, double *__restrict__ part)
+{
+    if (qs_stopped(qs)) return;
+    const double iv = qs[QS_INVV], iz = qs[QS_INVZ];
+    double acc[1][2] = {{0.0, 0.0}};
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < N; e += (long)gridDim.x * 256) {
+        const double vr = V[e] * iv, vi = V[e + N] * iv, zr = Z[e] * iz, zi = Z[e + N] * iz;
+        V[e] = vr;
+        V[e + N] = vi;
+        Z[e] = zr;
+        Z[e + N] = zi;
+        acc[0][0] += vr * zr - vi * zi;
+        acc[0][1] += vr * zi + vi * zr;
+    }
+    cx_block_sums<1>(acc, part);
+}
+
+// Q = Z (first iteration) or Z + CQ Q
+__global__ void k_cxq_q(long N, const double *__restrict__ qs, int first, const double *__restrict__ Z,
+                        double *__restrict__ Q)
+{
+    if (qs_stopped(qs)) return;
+    const double cr = qs[QS_CQ], ci = qs[QS_CQI];
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        double r = Z[e], i = Z[e + N];
+        if (!first) {
+            const double qr = Q[e], qi = Q[e + N];
+            r += cr * qr - ci * qi;
+            i += cr * qi + ci * qr;
+        }
+        Q[e] = r;
+        Q[e + N] = i;
+    }
+}
+
+// S *= ZVRAT; partial Q^T S
+__global__ void __launch_bounds__(256) k_cxq_qs(long N, const double *__restrict__ qs, const double *__restrict__ Q,
+                                                double *__restrict__ S, double *__restrict__ part)
+{
+    if (qs_stopped(qs)) return;
+    const double zv = qs[QS_ZVRAT];
+    double acc[1][2] = {{0.0, 0.0}};
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < N; e += (long)gridDim.x * 256) {
+        const double sr = S[e] * zv, si = S[e + N] * zv, qr = Q[e], qi = Q[e + N];
+        S[e] = sr;
+        S[e + N] = si;
+        acc[0][0] += qr * sr - qi * si;
+        acc[0][1] += qr * si + qi * sr;
+    }
+    cx_block_sums<1>(acc, part);
+}
+
+// V = S - BETA V; partial |V|^2
+__global__ void __launch_bounds__(256) k_cxq_v(long N, const double *__restrict__ qs, const double *__restrict__ S,
+                                               double *__restrict__ V, double *__restrict__ part)
+{
+    if (qs_stopped(qs)) return;
+    const double br = qs[QS_BETA], bi = qs[QS_BETAI];
+    double acc[1][2] = {{0.0, 0.0}};
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < N; e += (long)gridDim.x * 256) {
+        const double vr0 = V[e], vi0 = V[e + N];
+        const double vr = S[e] - (br * vr0 - bi * vi0), vi = S[e + N] - (br * vi0 + bi * vr0);
+        V[e] = vr;
+        V[e + N] = vi;
+        acc[0][0] += vr * vr + vi * vi;
+    }
+    cx_block_sums<1>(acc, part);
+}
+
+// Z /= ZVRAT; partial |Z|^2
+__global__ void __launch_bounds__(256) k_cxq_z(long N, const double *__restrict__ qs, double *__restrict__ Z,
+                                               double *__restrict__ part)
+{
+    if (qs_stopped(qs)) return;
+    const double iz = 1.0 / qs[QS_ZVRAT];
+    double acc[1][2] = {{0.0, 0.0}};
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < N; e += (long)gridDim.x * 256) {
+        const double zr = Z[e] * iz, zi = Z[e + N] * iz;
+        Z[e] = zr;
+        Z[e + N] = zi;
+        acc[0][0] += zr * zr + zi * zi;
+    }
+    cx_block_sums<1>(acc, part);
+}
+
+// UX = CQ Q + TG UX, UR = ETA S + TG UR (TG = 0 and no read on the first
+// iteration), X += UX, R -= UR; partials |R|^2, |UX|^2, |X|^2, |UR|^2
+__global__ void __launch_bounds__(256) k_cxq_x(long N, const double *__restrict__ qs, int first,
+                                               const double *__restrict__ Q, const double *__restrict__ S,
+                                               double *__restrict__ UX, double *__restrict__ UR,
+                                               double *__restrict__ X, double *__restrict__ R,
+                                               double *__restrict__ part)
+{
+    if (qs_stopped(qs)) return;
+    const double cr = qs[QS_CQ], ci = qs[QS_CQI], er = qs[QS_ETA], ei = qs[QS_ETAI], tg = qs[QS_TG];
+    double acc[4][2] = {};
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < N; e += (long)gridDim.x * 256) {
+        const double qr = Q[e], qi = Q[e + N], sr = S[e], si = S[e + N];
+        double uxr = cr * qr - ci * qi, uxi = cr * qi + ci * qr;
+        double urr = er * sr - ei * si, uri = er * si + ei * sr;
+        if (!first) {
+            uxr += tg * UX[e];
+            uxi += tg * UX[e + N];
+            urr += tg * UR[e];
+            uri += tg * UR[e + N];
+        }
+        UX[e] = uxr;
+        UX[e + N] = uxi;
+        UR[e] = urr;
+        UR[e + N] = uri;
+        const double xr = X[e] + uxr, xi = X[e + N] + uxi, rr = R[e] - urr, ri = R[e + N] - uri;
+        X[e] = xr;
+        X[e + N] = xi;
+        R[e] = rr;
+        R[e + N] = ri;
+        acc[0][0] += rr * rr + ri * ri;
+        acc[1][0] += uxr * uxr + uxi * uxi;
+        acc[2][0] += xr * xr + xi * xi;
+        acc[3][0] += urr * urr + uri * uri;
+    }
+    cx_block_sums<4>(acc, part);
+}
+
+// partials |HX0|^2 and |HX0 - B|^2 (criterion |HX - B| / |HX|)
+__global__ void __launch_bounds__(256) k_cxq_hx0(long N, const double *__restrict__ qs, const double *__restrict__ H,
+                                                 const double *__restrict__ B, double *__restrict__ part)
+{
+    if (qs_stopped(qs)) return;
+    double acc[2][2] = {};
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < N; e += (long)gridDim.x * 256) {
+        const double hr = H[e], hi = H[e + N], dr = hr - B[e], di = hi - B[e + N];
+        acc[0][0] += hr * hr + hi * hi;
+        acc[1][0] += dr * dr + di * di;
+    }
+    cx_block_sums<2>(acc, part);
+}
+
+enum CxqStep { CXQ_BEGIN, CXQ_AFTER_VZ, CXQ_AFTER_QS, CXQ_THETA, CXQ_END };
+
+// the recurrence's scalar steps (qmrcx.f90:56-175), one workgroup; part holds
+// the partial sums of slots 0..(nsum-1), G blocks each
+__global__ void __launch_bounds__(256) k_cxq_step(int step, int G, int nsum, const double *__restrict__ part,
+                                                  double *__restrict__ qs, int ntype, double tol, int first)
+{
+    __shared__ double sr[6][256], si[6][256];
+    if (qs_stopped(qs)) return;
+    for (int k = 0; k < nsum; k++) {
+        double r = 0.0, i = 0.0;
+        for (int b = threadIdx.x; b < G; b += 256) {
+            r += part[(size_t)k * CX_G * 2 + 2 * b];
+            i += part[(size_t)k * CX_G * 2 + 2 * b + 1];
+        }
+        sr[k][threadIdx.x] = r;
+        si[k][threadIdx.x] = i;
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int k = 0; k < nsum; k++) {
+                sr[k][threadIdx.x] += sr[k][threadIdx.x + o];
+                si[k][threadIdx.x] += si[k][threadIdx.x + o];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    using C2 = double2;
+    auto cmul = [](C2 a, C2 b) { return C2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; };
+    auto cdiv = [](C2 a, C2 b) {
+        const double d = b.x * b.x + b.y * b.y;
+        return C2{(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+    };
+    auto cabs = [](C2 a) { return hypot(a.x, a.y); };
+    switch (step) {
+    case CXQ_BEGIN: {
+        const double vn = qs[QS_VNRM], zn = qs[QS_ZNRM];
+        if (vn == 0.0 || zn == 0.0) {
+            qs[QS_STOP] = 1.0;
+            return;
+        }
+        qs[QS_ZVRAT] = qs[QS_ZVRAT] * zn / vn;
+        qs[QS_INVV] = 1.0 / vn;
+        qs[QS_INVZ] = 1.0 / zn;
+        qs[QS_ITER] += 1.0;
+        break;
+    }
+    case CXQ_AFTER_VZ: {
+        const C2 vz{sr[0][0], si[0][0]};
+        qs[QS_VZ] = vz.x;
+        qs[QS_VZI] = vz.y;
+        if (cabs(vz) == 0.0) {
+            qs[QS_STOP] = 1.0;
+            return;
+        }
+        if (!first) {
+            const C2 c = cdiv(C2{qs[QS_VNRM] * vz.x, qs[QS_VNRM] * vz.y}, C2{qs[QS_QS], qs[QS_QSI]});
+            qs[QS_CQ] = -c.x;
+            qs[QS_CQI] = -c.y;
+        }
+        break;
+    }
+    case CXQ_AFTER_QS: {
+        const C2 q{sr[0][0], si[0][0]};
+        qs[QS_QS] = q.x;
+        qs[QS_QSI] = q.y;
+        const C2 beta = cdiv(q, C2{qs[QS_VZ], qs[QS_VZI]});
+        qs[QS_BETA] = beta.x;
+        qs[QS_BETAI] = beta.y;
+        if (cabs(q) == 0.0 || cabs(beta) == 0.0) qs[QS_STOP] = 1.0;
+        break;
+    }
+    case CXQ_THETA: {
+        const double vnrmpo = sqrt(fabs(sr[0][0])), znrm = sqrt(fabs(sr[1][0]));
+        const C2 beta{qs[QS_BETA], qs[QS_BETAI]};
+        const double gamamo = qs[QS_GAMAMO], vnrm = qs[QS_VNRM];
+        const double theta = vnrmpo / (gamamo * cabs(beta));
+        const double gama = 1.0 / sqrt(1.0 + theta * theta);
+        // ETA = -ETA VNRM GAMA^2 / (BETA GAMAMO^2)
+        C2 eta = cdiv(C2{-qs[QS_ETA] * vnrm * gama * gama, -qs[QS_ETAI] * vnrm * gama * gama},
+                      C2{beta.x * gamamo * gamamo, beta.y * gamamo * gamamo});
+        qs[QS_VNRMPO] = vnrmpo;
+        qs[QS_ZNRM] = znrm;
+        qs[QS_THETA] = theta;
+        qs[QS_GAMA] = gama;
+        qs[QS_ETA] = eta.x;
+        qs[QS_ETAI] = eta.y;
+        if (gama == 0.0) {
+            qs[QS_STOP] = 1.0;
+            return;
+        }
+        const C2 cq = cmul(eta, C2{qs[QS_ZVRAT], 0.0});
+        qs[QS_CQ] = cq.x;
+        qs[QS_CQI] = cq.y;
+        const double tg = qs[QS_THETAMO] * gama;
+        qs[QS_TG] = first ? 0.0 : tg * tg;
+        break;
+    }
+    case CXQ_END: {
+        const int t = ntype < 0 ? -ntype : ntype;
+        double res;
+        if (t == 1) {
+            res = sqrt(fabs(sr[1][0])) / sqrt(fabs(sr[0][0]));
+        } else if (t == 2) {
+            const double r1 = sqrt(fabs(sr[0][0])) / qs[QS_BNRM];
+            const double r2 = sqrt(fabs(sr[1][0] / sr[2][0]));
+            const double r3 = sqrt(fabs(sr[3][0] / sr[0][0]));
+            res = fmax(r1, fmax(r2, r3));
+        } else {
+            res = sqrt(fabs(sr[0][0])) / qs[QS_BNRM];
+        }
+        qs[QS_RES] = res;
+        const double it = qs[QS_ITER];
+        if (fabs(res) < tol) {
+            qs[QS_CODE] = it;
+            qs[QS_STOP] = 1.0;
+            return;
+        }
+        qs[QS_CODE] = -it;
+        qs[QS_VNRM] = qs[QS_VNRMPO];
+        qs[QS_GAMAMO] = qs[QS_GAMA];
+        qs[QS_THETAMO] = qs[QS_THETA];
+        break;
+    }
+    }
+}
+
 // QMR_CX (qmrcx.f90:4-179).  x (device, split) holds the start vector when
 // ntype > 0; *ret gets QMR_CX's return value.
-static int qmr_cx_run(scalit_ctx *c, int nosb, int hc, int pc, int ntype, int qmrmax, double tol, const double *b,
-                      double *x, int *ret, double *res_out, hipStream_t s)
+static int qmr_cx_run(scalit_ctx *c, int nosb, int hc, int pc, int ntype, int qmrmax, double tol, const double *b_in,
+                      double *x_out, int *ret, double *res_out, hipStream_t s)
 {
     CxState &X = cxstate(c);
     const long N = c->N, L = 2 * N;
-    SC_TRY(X.vec.alloc((size_t)8 * L));
-    SC_TRY(cx_slots(X, 4));
+    {
+        double *old = X.vec.p;
+        double *old_qs = X.qs.p;
+        SC_TRY(X.vec.alloc((size_t)10 * L));
+        SC_TRY(cx_slots(X, 4));
+        SC_TRY(X.qs.alloc(QS_LEN));
+        if (X.vec.p != old || X.qs.p != old_qs) X.drop_graph();
+    }
     double *R = X.vec.p, *UX = R + L, *UR = UX + L, *Q = UR + L, *S = Q + L, *V = S + L, *Z = V + L, *HX0 = Z + L;
+    // the solve works on its own copies of x and b, so that the recorded
+    // iteration serves every right-hand side
+    double *x = HX0 + L, *b = x + L;
+    double *qs = X.qs.p, *part = X.part.p;
+    const int G = cx_blocks(N), t = std::abs(ntype);
+    const unsigned g = cx_grid(N);
     auto nrm = [&](const double *v, int slot) { cx_dot(X, N, v, v, 1, slot, 0, s); };
-    auto hx = [&](const double *in, double *out) { return hx_apply_cx(c, hc, 1, in, out, s); };
-    auto px = [&](const double *in, double *out) { return px_apply_cx(c, nosb, pc, 1, in, out, s); };
-    int code = -1;
-    double res = 0.0;
+    HIP_TRY(hipMemcpyAsync(b, b_in, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
     if (ntype <= 0) HIP_TRY(hipMemsetAsync(x, 0, sizeof(double) * L, s));
+    else HIP_TRY(hipMemcpyAsync(x, x_out, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(R, b, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(V, b, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
-    double zvrat = 1.0;
     nrm(b, 0);
-    SC_TRY(px(b, Z));
+    SC_TRY(px_apply_cx(c, nosb, pc, 1, b, Z, s));
     nrm(Z, 1);
     SC_TRY(cx_read(X, 2, s));
-    const double bnrm = std::sqrt(std::abs(slot_val(X, 0).real()));
-    double vnrm = bnrm;
-    double znrm = std::sqrt(std::abs(slot_val(X, 1).real()));
-    double gamamo = 1.0, thetamo = 0.0;
-    cd eta(-1.0, 0.0), qsdot(0.0, 0.0);
-    code = 0;
-    for (int i = 1; i <= qmrmax; i++) {
-        if (vnrm == 0.0 || znrm == 0.0) break;
-        zvrat = zvrat * znrm / vnrm;
-        cx_axpby(N, cd(1.0 / vnrm, 0.0), V, 0.0, nullptr, V, s);
-        cx_axpby(N, cd(1.0 / znrm, 0.0), Z, 0.0, nullptr, Z, s);
-        cx_dot(X, N, V, Z, 0, 0, 0, s);
-        SC_TRY(cx_read(X, 1, s));
-        const cd vzdot = slot_val(X, 0);
-        if (std::abs(vzdot) == 0.0) break;
-        if (i == 1)
-            HIP_TRY(hipMemcpyAsync(Q, Z, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
-        else
-            cx_axpby(N, 1.0, Z, -(vnrm * vzdot) / qsdot, Q, Q, s);
-        SC_TRY(hx(Q, S));
-        cx_axpby(N, zvrat, S, 0.0, nullptr, S, s);
-        cx_dot(X, N, Q, S, 0, 0, 0, s);
-        SC_TRY(cx_read(X, 1, s));
-        qsdot = slot_val(X, 0);
-        const cd beta = qsdot / vzdot;
-        if (std::abs(qsdot) == 0.0 || std::abs(beta) == 0.0) break;
-        cx_axpby(N, 1.0, S, -beta, V, V, s);
-        nrm(V, 0);
-        SC_TRY(px(V, Z));
-        cx_axpby(N, 1.0 / zvrat, Z, 0.0, nullptr, Z, s);
-        nrm(Z, 1);
-        SC_TRY(cx_read(X, 2, s));
-        const double vnrmpo = std::sqrt(std::abs(slot_val(X, 0).real()));
-        znrm = std::sqrt(std::abs(slot_val(X, 1).real()));
-        const double theta = vnrmpo / (gamamo * std::abs(beta));
-        const double gama = 1.0 / std::sqrt(1.0 + theta * theta);
-        eta = -eta * vnrm * gama * gama / (beta * gamamo * gamamo);
-        if (gama == 0.0) break;
-        if (i == 1) {
-            cx_axpby(N, eta * zvrat, Q, 0.0, nullptr, UX, s);
-            cx_axpby(N, eta, S, 0.0, nullptr, UR, s);
-        } else {
-            const double tg = (thetamo * gama) * (thetamo * gama);
-            cx_axpby(N, eta * zvrat, Q, tg, UX, UX, s);
-            cx_axpby(N, eta, S, tg, UR, UR, s);
-        }
-        cx_axpby(N, 1.0, x, 1.0, UX, x, s);
-        cx_axpby(N, 1.0, R, -1.0, UR, R, s);
-        const int t = std::abs(ntype);
+    // the initial state: ZVRAT = 1, VNRM = BNRM, GAMAMO = 1, ETA = -1, code 0
+    std::vector<double> q0(QS_LEN, 0.0);
+    q0[QS_ZVRAT] = 1.0;
+    q0[QS_BNRM] = std::sqrt(std::abs(slot_val(X, 0).real()));
+    q0[QS_VNRM] = q0[QS_BNRM];
+    q0[QS_ZNRM] = std::sqrt(std::abs(slot_val(X, 1).real()));
+    q0[QS_GAMAMO] = 1.0;
+    q0[QS_ETA] = -1.0;
+    HIP_TRY(hipMemcpyAsync(qs, q0.data(), sizeof(double) * QS_LEN, hipMemcpyHostToDevice, s));
+    auto iteration = [&](int first) -> int {
+        hipLaunchKernelGGL(k_cxq_step, dim3(1), dim3(256), 0, s, (int)CXQ_BEGIN, G, 0, (const double *)part, qs,
+                           ntype, tol, first);
+        hipLaunchKernelGGL(k_cxq_vz, dim3(G), dim3(256), 0, s, N, (const double *)qs, V, Z, part);
+        hipLaunchKernelGGL(k_cxq_step, dim3(1), dim3(256), 0, s, (int)CXQ_AFTER_VZ, G, 1, (const double *)part, qs,
+                           ntype, tol, first);
+        hipLaunchKernelGGL(k_cxq_q, dim3(g), dim3(256), 0, s, N, (const double *)qs, first, (const double *)Z, Q);
+        SC_TRY(hx_apply_cx(c, hc, 1, Q, S, s));
+        hipLaunchKernelGGL(k_cxq_qs, dim3(G), dim3(256), 0, s, N, (const double *)qs, (const double *)Q, S, part);
+        hipLaunchKernelGGL(k_cxq_step, dim3(1), dim3(256), 0, s, (int)CXQ_AFTER_QS, G, 1, (const double *)part, qs,
+                           ntype, tol, first);
+        hipLaunchKernelGGL(k_cxq_v, dim3(G), dim3(256), 0, s, N, (const double *)qs, (const double *)S, V, part);
+        SC_TRY(px_apply_cx(c, nosb, pc, 1, V, Z, s));
+        hipLaunchKernelGGL(k_cxq_z, dim3(G), dim3(256), 0, s, N, (const double *)qs, Z, part + (size_t)CX_G * 2);
+        hipLaunchKernelGGL(k_cxq_step, dim3(1), dim3(256), 0, s, (int)CXQ_THETA, G, 2, (const double *)part, qs,
+                           ntype, tol, first);
+        hipLaunchKernelGGL(k_cxq_x, dim3(G), dim3(256), 0, s, N, (const double *)qs, first, (const double *)Q,
+                           (const double *)S, UX, UR, x, R, part);
         if (t == 1) {
-            SC_TRY(hx(x, HX0));
-            nrm(HX0, 0);
-            cx_axpby(N, 1.0, HX0, -1.0, b, HX0, s);
-            nrm(HX0, 1);
-            SC_TRY(cx_read(X, 2, s));
-            res = std::sqrt(std::abs(slot_val(X, 1).real())) / std::sqrt(std::abs(slot_val(X, 0).real()));
-        } else if (t == 2) {
-            nrm(R, 0);
-            nrm(UX, 1);
-            nrm(x, 2);
-            nrm(UR, 3);
-            SC_TRY(cx_read(X, 4, s));
-            const double r1 = std::sqrt(std::abs(slot_val(X, 0).real())) / bnrm;
-            const double r2 = std::sqrt(std::abs(slot_val(X, 1).real() / slot_val(X, 2).real()));
-            const double r3 = std::sqrt(std::abs(slot_val(X, 3).real() / slot_val(X, 0).real()));
-            res = std::max(r1, std::max(r2, r3));
-        } else {
-            nrm(R, 0);
-            SC_TRY(cx_read(X, 1, s));
-            res = std::sqrt(std::abs(slot_val(X, 0).real())) / bnrm;
+            SC_TRY(hx_apply_cx(c, hc, 1, x, HX0, s));
+            hipLaunchKernelGGL(k_cxq_hx0, dim3(G), dim3(256), 0, s, N, (const double *)qs, (const double *)HX0, b,
+                               part);
         }
-        if (std::abs(res) < tol) {
-            code = i;
-            break;
+        hipLaunchKernelGGL(k_cxq_step, dim3(1), dim3(256), 0, s, (int)CXQ_END, G, t == 2 ? 4 : (t == 1 ? 2 : 1),
+                           (const double *)part, qs, ntype, tol, first);
+        HIP_TRY(hipGetLastError());
+        return SCALIT_OK;
+    };
+    double hq[QS_LEN];
+    hq[QS_CODE] = 0.0;
+    hq[QS_RES] = 0.0;
+    int done = 0;
+    bool stop = false;
+    while (done < qmrmax && !stop) {
+        const int k = std::min(done == 0 ? 1 : 8, qmrmax - done);
+        for (int i = 0; i < k; i++) {
+            if (done + i == 0) {
+                SC_TRY(iteration(1));   // the first (and any allocation or tuning it needs) runs as launched
+                continue;
+            }
+            const int key[4] = {nosb, hc, pc, ntype};
+            const double dkey[4] = {tol, c->e0, c->de, c->beta};
+            if (!X.exec || X.g_gen != c->gen || !std::equal(key, key + 4, X.g_key) ||
+                !std::equal(dkey, dkey + 4, X.g_dkey)) {
+                X.drop_graph();
+                HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                const int rc = iteration(0);
+                hipGraph_t gr = nullptr;
+                const hipError_t e = hipStreamEndCapture(s, &gr);
+                if (rc != SCALIT_OK) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    return rc;
+                }
+                HIP_TRY(e);
+                X.graph = gr;
+                HIP_TRY(hipGraphInstantiate(&X.exec, gr, nullptr, nullptr, 0));
+                X.g_gen = c->gen;
+                std::copy(key, key + 4, X.g_key);
+                std::copy(dkey, dkey + 4, X.g_dkey);
+            }
+            HIP_TRY(hipGraphLaunch(X.exec, s));
         }
-        code = -i;
-        vnrm = vnrmpo;
-        gamamo = gama;
-        thetamo = theta;
+        done += k;
+        HIP_TRY(hipMemcpyAsync(hq, qs, sizeof(double) * QS_LEN, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        stop = hq[QS_STOP] != 0.0;
     }
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(x_out, x, sizeof(double) * L, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
-    *ret = code;
-    if (res_out) *res_out = res;
+    *ret = (int)hq[QS_CODE];
+    if (res_out) *res_out = hq[QS_RES];
     return SCALIT_OK;
 }
 
