@@ -57,13 +57,20 @@ __global__ void k_osbw_basis(long N, int cnt, OsbwLevels L, const int *__restric
     x[(long)k * ldx + e] = val;
 }
 
-// H_W(i, j) = X(:, i) . HX(:, j) for i < j, both triangles; one workgroup per pair
+// H_W(i, j) = X(:, i) . HX(:, j) for i < j, both triangles, and
+// H_W(i, i) = EIG0(ind(i)) - shift; one workgroup per pair
 __global__ void __launch_bounds__(256) k_osbw_gram(long N, int cnt, const double *__restrict__ x, long ldx,
-                                                   const double *__restrict__ hx, long ldh, double *__restrict__ hw)
+                                                   const double *__restrict__ hx, long ldh,
+                                                   const double *__restrict__ eig0, const int *__restrict__ ind,
+                                                   double shift, double *__restrict__ hw)
 {
     __shared__ double red[4];
     const int i = blockIdx.x, j = blockIdx.y;
-    if (i >= j) return;
+    if (i > j) return;
+    if (i == j) {
+        if (threadIdx.x == 0) hw[i + (long)cnt * i] = eig0[ind[i]] - shift;
+        return;
+    }
     const double *xi = x + (long)i * ldx, *hj = hx + (long)j * ldh;
     double acc = 0.0;
     for (long e = threadIdx.x; e < N; e += 256) acc += xi[e] * hj[e];
@@ -212,7 +219,7 @@ extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int
         if (std::fabs(eig[i] - e0) <= de) ind.push_back((int)i);
     const int n = (int)ind.size();
     SC_TRY(c->osbw_ind.alloc(n));
-    HIP_TRY(hipMemcpyAsync(c->osbw_ind.p, ind.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpy(c->osbw_ind.p, ind.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     // the window's OSB vectors and H applied to them
     const long ld = (N + 63) & ~63L;
     DBuf<double> xv, hv, hw, w, z, work;
@@ -229,13 +236,11 @@ extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int
     HIP_TRY(hipGetLastError());
     SC_TRY(hx_apply_f64(c, nullptr, 0.0, n, xv.p, ld, hv.p, ld, false, nullptr, s));   // HijX: MYHX(TA0)
     SC_TRY(hw.alloc((size_t)n * n));
-    hipLaunchKernelGGL(k_osbw_gram, dim3(n, n), dim3(256), 0, s, N, n, xv.p, ld, hv.p, ld, hw.p);
-    HIP_TRY(hipGetLastError());
     // the diagonal is EIG0 (getVOSBH0), less the shift (createWH: TA1)
     const double shift = npc == SCALIT_TA0 ? 0.0 : e0;
-    std::vector<double> hd(n);
-    for (int k = 0; k < n; k++) hd[k] = eig[ind[k]] - shift;
-    HIP_TRY(hipMemcpy2DAsync(hw.p, 8 * (size_t)(n + 1), hd.data(), 8, 8, n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_osbw_gram, dim3(n, n), dim3(256), 0, s, N, n, xv.p, ld, hv.p, ld, c->eig0.p,
+                       c->osbw_ind.p, shift, hw.p);
+    HIP_TRY(hipGetLastError());
     SC_TRY(w.alloc(n));
     SC_TRY(z.alloc((size_t)n * n));
     SC_TRY(work.alloc(small_sym_eig_work(n)));
@@ -251,7 +256,7 @@ extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int
         if (ind[k] / n1 != ind[k - 1] / n1) runs.push_back(k);
     runs.push_back(n);
     SC_TRY(c->osbw_runs.alloc(runs.size()));
-    HIP_TRY(hipMemcpyAsync(c->osbw_runs.p, runs.data(), sizeof(int) * runs.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpy(c->osbw_runs.p, runs.data(), sizeof(int) * runs.size(), hipMemcpyHostToDevice));
     SC_TRY(c->osbw_eig0.alloc(N));
     HIP_TRY(hipMemcpyAsync(c->osbw_eig0.p, c->eig0.p, 8 * (size_t)N, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_osbw_mark, dim3((n + 255) / 256), dim3(256), 0, s, n, c->osbw_ind.p, c->osbw_eig0.p);
