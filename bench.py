@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--pist-run", action="store_true",
                     help="run only the PIST measurements on the 24x32x24 workload (no H*X bench, no host "
                          "baseline) and print them as one JSON line, for profiling that path")
+    ap.add_argument("--multi-run", action="store_true",
+                    help="run only the multi-problem QMR measurement on the three config-4 grids and print it as "
+                         "one JSON line")
     ap.add_argument("--complex-run", action="store_true",
                     help="run only the resonance and CRP measurements on the 24x32x24 operator with an absorbing "
                          "potential and print them as one JSON line, for profiling that path")
@@ -115,6 +118,9 @@ def main():
         return
     if args.pist_run:
         print(json.dumps({"pist": pist_bench(True)}), flush=True)
+        return
+    if args.multi_run:
+        print(json.dumps({"qmr_multi": qmr_multi_bench()}), flush=True)
         return
     if args.complex_run:
         print(json.dumps({"absorbing_potential_24x32x24": complex_bench()}), flush=True)
@@ -201,6 +207,7 @@ def main():
         out["gradients_64x96x80"] = gradients_bench(ctx, prob)
         out["uniform_windows"] = config4_bench(["batched_" + c for c in args.config4])
         out["absorbing_potential_24x32x24"] = complex_bench()
+        out["qmr_multi"] = qmr_multi_bench()
     ctx.close()
     print(json.dumps(out))
 
@@ -357,6 +364,59 @@ def gradients_bench(ctx, prob):
     out["orthogonalisation_passes"] = 2
     ctx.set_pist_orth(1)
     return out
+
+
+def qmr_multi_bench():
+    """FP64 QMR on several operators of uneven grids with 4 right-hand sides
+    each, shifts between the 4th and 5th OSB levels of each, to 1e-8: all
+    problems in lock step with one graph launch per iteration
+    (scalit_qmr_multi) against the same solves one problem after another (the
+    one-kernel-per-step path); on the three config-4 grids (48x64x40,
+    80x96x64, 128x128x96) and on three small grids (24x32x24, 20x28x16,
+    16x20x12), where single solves are launch-bound.  OSB set-ups outside the
+    timed regions; the second of two runs is timed."""
+    import torch
+
+    from scalit_amd import problems as P
+    from scalit_amd.context import Context
+
+    def one(probs):
+        runs = []
+        for k, prob in enumerate(probs):
+            ctx = Context.from_problem(prob)
+            ctx.block_diag(1000, 1e-5)
+            ctx.set_qmr_persistent(0)
+            _, eig0 = ctx.get_precond()
+            srt = np.sort(eig0)
+            e0 = float(0.5 * (srt[3] + srt[4]))
+            shifts = [e0 + 2e-5 * j for j in range(4)]
+            runs.append((prob, ctx, ctx.to_device(P.random_vectors(prob.N, 4, 60 + k)), shifts))
+        problems = [(ctx, b, sh) for _, ctx, b, sh in runs]
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            multi = Context.qmr_multi(problems, ntype=-1, qmrmax=300, qmrtol=1e-8)
+            t_multi = time.perf_counter() - t0
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            alone = [ctx.qmr(b, ntype=-1, qmrmax=300, qmrtol=1e-8, shifts=sh) for _, ctx, b, sh in runs]
+            t_alone = time.perf_counter() - t0
+        same = all(bool((m[1] == a[1]).all()) and np.array_equal(m[0], np.array(a[0])) for m, a in zip(multi, alone))
+        out = {"grids": [list(p.sN) for p, _, _, _ in runs], "rhs_per_problem": 4,
+               "qmr_iterations": [int(np.abs(m[0]).max()) for m in multi],
+               "converged": [bool((m[0] > 0).all()) for m in multi],
+               "seconds_one_graph_per_iteration": round(t_multi, 4),
+               "seconds_one_problem_after_another": round(t_alone, 4),
+               "speedup": round(t_alone / t_multi, 3), "bit_identical": same}
+        for _, ctx, _, _ in runs:
+            ctx.close()
+        return out
+
+    big = one([P.config_problem(n) for n in ("batched_a", "batched_b", "batched_c")])
+    small = one([P.build_problem(*g, P.jittered_potential(21 + i), seed=21 + i)
+                 for i, g in enumerate([(24, 32, 24), (20, 28, 16), (16, 20, 12)])])
+    return {"config4_grids": big, "small_grids": small}
 
 
 def complex_bench():

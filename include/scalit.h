@@ -391,6 +391,26 @@ typedef struct scalit_pist_job {
 int scalit_pist_conv_multi(int njob, scalit_pist_job *jobs, double etol, int ntype, int startcnt, int stepcnt,
                            int stepeig, int m, int m_max, int qmr_type, int qmrmax, double qmrtol, int prec);
 
+/* No counterpart in the reference (QMR, qmr.f90:5-175, solves one operator
+ * per call).  FP64 QMR solves of several operators (one job per context, all
+ * contexts on one device, grids of any sizes) in lock step: one iteration of
+ * every job is recorded as one HIP graph, a branch per job, and each graph
+ * launch advances all of them.  Results are those of scalit_qmr with the
+ * persistent solver switched off (scalit_set_qmr_persistent(ctx, 0)), bit for
+ * bit.  status is per job; returns the first error, else SCALIT_OK. */
+typedef struct scalit_qmr_job {
+    scalit_ctx *ctx;
+    int nvec;
+    const double *b;      /* device, nvec right-hand sides, leading dimension ld */
+    double *x;            /* device, nvec solutions (start vectors when ntype > 0) */
+    long ld;
+    const double *shifts; /* host, nvec shifts, or NULL for the context's E0 */
+    int *ret;             /* host, nvec */
+    double *res;          /* host, nvec, may be NULL */
+    int status;
+} scalit_qmr_job;
+int scalit_qmr_multi(int njob, scalit_qmr_job *jobs, int ntype, int qmrmax, double qmrtol);
+
 /* ------------------------------------------------------------------------ */
 /* sequential truncation of a DVR basis (host arrays; eigensolves on the     */
 /* current GPU)                                                              */

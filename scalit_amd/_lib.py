@@ -35,6 +35,12 @@ _d = C.c_double
 FOP = C.CFUNCTYPE(None, _ip, _dp, _dp)
 
 
+class QmrJob(C.Structure):
+    """scalit_qmr_job of include/scalit.h."""
+    _fields_ = [("ctx", C.c_void_p), ("nvec", C.c_int), ("b", C.c_void_p), ("x", C.c_void_p), ("ld", C.c_long),
+                ("shifts", _dp), ("ret", _ip), ("res", _dp), ("status", C.c_int)]
+
+
 class PistJob(C.Structure):
     """scalit_pist_job of include/scalit.h."""
     _fields_ = [("ctx", C.c_void_p), ("nwin", C.c_int), ("e0", _dp), ("x", C.c_void_p), ("ld", C.c_long),
@@ -83,6 +89,7 @@ SIGNATURES = {
     "scalit_small_eig_cx": (_i, [_i, _vp, _i, _vp]),
     "scalit_pist_conv_cx": (_i, [_vp, _d, _d, _i, _vp, _i, _i, _i, _i, _i, _i, _i, _d, _i, _vp, _vp, _vp, _vp]),
     "scalit_crp": (_i, [_vp, _i, _vp, _i, _vp, _i, _i, _d, _i, _vp, _vp]),
+    "scalit_qmr_multi": (_i, [_i, _vp, _i, _i, _d]),
     "scalit_set_qmr_persistent": (_i, [_vp, _i]),
     "scalit_load_data_grid": (_i, [C.c_char_p, _l, _i, _dp]),
     "scalit_save_data_grid": (_i, [C.c_char_p, _l, _i, _dp]),

@@ -163,6 +163,30 @@ class Context:
         self.set_precond(parts, eig0)
 
     @staticmethod
+    def qmr_multi(problems, ntype: int = -1, qmrmax: int = 1000, qmrtol: float = 1e-3):
+        """Several operators' QMR solves in lock step, one graph launch per
+        iteration for all of them (scalit_qmr_multi): `problems` is a list of
+        (context, b tensor of shape (nvec, N), shifts or None).  Returns per
+        problem (ret, x, res)."""
+        jobs = (L.QmrJob * len(problems))()
+        keep = []
+        for j, (ctx, b, shifts) in enumerate(problems):
+            b = b.contiguous()
+            nv = b.shape[0]
+            x = torch.zeros_like(b)
+            sh = None if shifts is None else (C.c_double * nv)(*[float(v) for v in shifts])
+            ret = (C.c_int * nv)()
+            res = (C.c_double * nv)()
+            keep.append((b, x, sh, ret, res))
+            jobs[j].ctx, jobs[j].nvec, jobs[j].ld = ctx._h.value, nv, ctx.N
+            jobs[j].b, jobs[j].x = b.data_ptr(), x.data_ptr()
+            jobs[j].shifts = C.cast(sh, _dp) if sh is not None else None
+            jobs[j].ret, jobs[j].res = C.cast(ret, C.POINTER(C.c_int)), C.cast(res, _dp)
+        check(L.lib().scalit_qmr_multi(len(problems), C.cast(jobs, C.c_void_p), ntype, qmrmax, qmrtol),
+              "scalit_qmr_multi")
+        return [(np.array(k[3][:]), k[1], np.array(k[4][:])) for k in keep]
+
+    @staticmethod
     def pist_conv_multi(problems, etol: float, m: int, m_max: int, startcnt: int, stepcnt: int, stepeig: int = 1,
                         ntype: int = 0, qmr_type: int = -1, qmrmax: int = 1000, qmrtol: float = 1e-3,
                         prec: int = L.FP64):

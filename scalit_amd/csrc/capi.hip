@@ -15,6 +15,7 @@
 #include "solver_state.hpp"
 
 namespace scalit {
+int qmr_multi_f64(int njob, scalit_qmr_job *jobs, int ntype, int qmrmax, double tol);
 int qmr_f64(scalit_ctx *c, int ntype, int qmrmax, double tol, int nvec, const double *b, double *x, long ld,
             const double *shifts, int *ret, double *res, hipStream_t s);
 int qmr_mixed(scalit_ctx *c, int prec, int ntype, int qmrmax, double tol, int inner_max, double inner_tol, int nvec,
@@ -184,6 +185,8 @@ int scalit_create(scalit_ctx **out, int sF, const int *sN, const int *sDEP, int 
     if (device < 0 || device >= ndev) return SCALIT_EINVAL;
     HIP_TRY(hipSetDevice(device));
     scalit_ctx *c = new scalit_ctx();
+    static std::atomic<long> next_uid{1};
+    c->uid = next_uid++;
     c->device = device;
     c->sF = sF;
     long dim = 1;
@@ -475,6 +478,24 @@ int scalit_qmr(scalit_ctx *c, int ntype, int qmrmax, double qmrtol, int nvec, co
     (void)stream;
     HIP_TRY(hipDeviceSynchronize());   // order after whatever the caller enqueued on any stream
     return qmr_f64(c, ntype, qmrmax, qmrtol, nvec, b, x, ld, shifts, ret, res, c->stream);
+}
+
+int scalit_qmr_multi(int njob, scalit_qmr_job *jobs, int ntype, int qmrmax, double qmrtol)
+{
+    if (njob < 1 || !jobs || qmrmax < 0) return SCALIT_EINVAL;
+    for (int j = 0; j < njob; j++) {
+        scalit_qmr_job &J = jobs[j];
+        J.status = SCALIT_EINVAL;
+        if (!J.ctx || J.nvec < 1 || !J.b || !J.x || !J.ret || J.ld < J.ctx->N) return SCALIT_EINVAL;
+        if (J.ctx->device != jobs[0].ctx->device) return SCALIT_EINVAL;
+        for (int i = 0; i < j; i++)
+            if (jobs[i].ctx == J.ctx) return SCALIT_EINVAL;   // one job per context: each owns its workspace
+        SC_TRY(operator_ready(J.ctx));
+        if (!J.ctx->has_precond) return (J.status = SCALIT_ESTATE);
+    }
+    HIP_TRY(hipSetDevice(jobs[0].ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    return qmr_multi_f64(njob, jobs, ntype, qmrmax, qmrtol);
 }
 
 int scalit_qmr_mixed(scalit_ctx *c, int prec, int qmrmax, double qmrtol, int inner_max, double inner_tol, int nvec,

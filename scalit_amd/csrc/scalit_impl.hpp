@@ -153,6 +153,9 @@ struct scalit_ctx {
     // bumped whenever a device buffer that operator kernels read is
     // (re)allocated: recorded graphs of older generations are re-recorded
     long gen = 0;
+    // unique over the process's contexts (a recorded multi-problem graph
+    // names its contexts by it)
+    long uid = 0;
     std::vector<scalit::TuneEntry> hx_tune;
 };
 

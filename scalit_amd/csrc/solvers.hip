@@ -298,13 +298,14 @@ static bool qmr_persist_wanted(const scalit_ctx *c)
 // Solve (H - shift_v) x_v = b_v for the nvec columns already in w.B(); the
 // solution lands in w.X().  shifts: host, nvec values (the QMR shift E0 per
 // vector).  ret / res: host, per vector.
+// The shifts and activity flags of a solve, and its initialisation
+// (qmr.f90:36-50), on stream s.
 template <typename S>
-static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double tol, const double *shifts, int *ret,
-                     double *res, hipStream_t s, HostCb *cb = nullptr)
+static int qmr_prelude(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, const double *shifts, hipStream_t s,
+                       HostCb *cb = nullptr)
 {
     const int nv = w.nvec;
     const long n = w.n;
-    const int conv = std::abs(ntype) == 1 ? 1 : (std::abs(ntype) == 2 ? 2 : 0);
     // the host sources live in the workspace, so the copies need no wait here:
     // they are rewritten only by the next solve, after this one's last wait
     w.hshift.resize(nv);
@@ -313,12 +314,22 @@ static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double t
     w.hones.assign(nv, 1);
     HIP_TRY(hipMemcpyAsync(w.active.p, w.hones.data(), sizeof(int) * nv, hipMemcpyHostToDevice, s));
     QmrVecs<S> v = w.view();
-    // initialisation (qmr.f90:36-50)
     launch_qmr_init<S>(nv, n, ntype <= 0 ? 1 : 0, v, w.part(0), s);
     SC_TRY(op_px<S>(c, w, cb, v.b, v.Z, w.active.p, s));
     launch_qmr_init_z<S>(nv, n, v, w.part(0), w.part(1), w.part(9), s);
     launch_qmr_init_scal(nv, w.part(0), w.part(1), w.scal.p, w.active.p, qmrmax, s);
     HIP_TRY(hipGetLastError());
+    return SCALIT_OK;
+}
+
+template <typename S>
+static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double tol, const double *shifts, int *ret,
+                     double *res, hipStream_t s, HostCb *cb = nullptr)
+{
+    const int nv = w.nvec;
+    const int conv = std::abs(ntype) == 1 ? 1 : (std::abs(ntype) == 2 ? 2 : 0);
+    SC_TRY(qmr_prelude<S>(c, w, ntype, qmrmax, shifts, s, cb));
+    QmrVecs<S> v = w.view();
     if constexpr (std::is_same<S, double>::value) {
         if (cb == nullptr && qmr_persist_wanted(c) && qmrmax >= 1 && qmr_persist_ok(c)) {
             const int rc = qmr_persist_solve(c, nv, v, w.scal.p, w.active.p, w.shifts.p, w.part(9), conv, qmrmax, tol,
@@ -386,7 +397,24 @@ static int qmr_solve(scalit_ctx *c, QmrWS<S> &w, int ntype, int qmrmax, double t
 }
 
 // ---------------------------------------------------------------- per-context solver state
+// A recorded multi-problem QMR iteration (scalit_qmr_multi), kept by the
+// first job's context for the same jobs, workspaces and options.
+struct MultiGraph {
+    std::vector<long> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void drop()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key.clear();
+    }
+};
+
 struct PistState {
+    MultiGraph mg;
     QmrWS<double> q64;
     QmrWS<float> q32;
     QmrWS<bf16_t> q16;
@@ -415,6 +443,7 @@ struct PistState {
     // QMR parameters of the OSB_QMR callback
     int qmr_type = -1, qmr_max = 1000;
     double qmr_tol = 1e-3;
+    void release_multi() { mg.drop(); }
     void release()
     {
         q64.release();
@@ -461,6 +490,7 @@ static PistState &pstate(scalit_ctx *c)
 void release_solvers(scalit_ctx *c)
 {
     if (c->pist) {
+        c->pist->release_multi();
         c->pist->release();
         delete c->pist;
         c->pist = nullptr;
@@ -482,6 +512,126 @@ int qmr_f64(scalit_ctx *c, int ntype, int qmrmax, double tol, int nvec, const do
     SC_TRY(qmr_solve<double>(c, w, ntype, qmrmax, tol, sh.data(), ret, res, s));
     HIP_TRY(hipMemcpy2DAsync(x, ld * 8, w.X(), w.ld * 8, c->N * 8, nvec, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return SCALIT_OK;
+}
+
+// Several operators' FP64 QMR solves (one per context, all on one device)
+// in lock step: one iteration of every solve, each on its context's stream
+// (a branch per problem, forked from and joined to the first context's
+// stream), is recorded as one HIP graph, and each graph launch advances all
+// of them.  A converged or broken-down right-hand side stops updating (its
+// activity flag), so problems that finish early cost empty launches only
+// until the last one converges.
+int qmr_multi_f64(int njob, scalit_qmr_job *jobs, int ntype, int qmrmax, double tol)
+{
+    const int conv = std::abs(ntype) == 1 ? 1 : (std::abs(ntype) == 2 ? 2 : 0);
+    std::vector<QmrWS<double> *> ws(njob);
+    for (int j = 0; j < njob; j++) {
+        scalit_ctx *c = jobs[j].ctx;
+        hipStream_t s = c->stream;
+        PistState &P = pstate(c);
+        SC_TRY(P.q64.ensure(jobs[j].nvec, c->N));
+        auto &w = P.q64;
+        ws[j] = &w;
+        HIP_TRY(hipMemcpy2DAsync(w.B(), w.ld * 8, jobs[j].b, jobs[j].ld * 8, c->N * 8, jobs[j].nvec,
+                                 hipMemcpyDeviceToDevice, s));
+        if (ntype > 0)
+            HIP_TRY(hipMemcpy2DAsync(w.X(), w.ld * 8, jobs[j].x, jobs[j].ld * 8, c->N * 8, jobs[j].nvec,
+                                     hipMemcpyDeviceToDevice, s));
+        std::vector<double> sh(jobs[j].nvec, c->e0);
+        if (jobs[j].shifts)
+            for (int i = 0; i < jobs[j].nvec; i++) sh[i] = jobs[j].shifts[i];
+        SC_TRY(qmr_prelude<double>(c, w, ntype, qmrmax, sh.data(), s));
+        SC_TRY(hx_prepare(c, SCALIT_FP64, jobs[j].nvec, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    hipStream_t s0 = jobs[0].ctx->stream;
+    // the recorded iteration is reused while the jobs, their workspaces, the
+    // operators' buffers and the options are those it was recorded with
+    MultiGraph &mg = pstate(jobs[0].ctx).mg;
+    std::vector<long> key{(long)conv, (long)qmrmax, 0L};
+    std::memcpy(&key[2], &tol, sizeof(double));
+    for (int j = 0; j < njob; j++) {
+        const auto &w = *ws[j];
+        key.insert(key.end(), {jobs[j].ctx->uid, jobs[j].ctx->gen, (long)w.nvec, (long)(intptr_t)w.vecs.p,
+                               (long)(intptr_t)w.parts.p, (long)(intptr_t)w.scal.p, (long)(intptr_t)w.active.p,
+                               (long)(intptr_t)w.shifts.p});
+    }
+    int rc = SCALIT_OK;
+    if (!mg.exec || mg.key != key) {
+        mg.drop();
+        std::vector<hipEvent_t> ev(njob, nullptr);
+        for (auto &e : ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                e = nullptr;
+                rc = SCALIT_EHIP;
+            }
+        if (rc == SCALIT_OK && hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            rc = SCALIT_EHIP;
+        if (rc == SCALIT_OK) {
+            (void)hipEventRecord(ev[0], s0);
+            for (int j = 0; j < njob && rc == SCALIT_OK; j++) {
+                hipStream_t s = jobs[j].ctx->stream;
+                if (j > 0) (void)hipStreamWaitEvent(s, ev[0], 0);
+                rc = qmr_iteration<double>(jobs[j].ctx, *ws[j], conv, qmrmax, tol, s);
+                if (j > 0) {
+                    (void)hipEventRecord(ev[j], s);
+                    (void)hipStreamWaitEvent(s0, ev[j], 0);
+                }
+            }
+            const hipError_t e = hipStreamEndCapture(s0, &mg.graph);
+            if (rc == SCALIT_OK && e != hipSuccess) rc = SCALIT_EHIP;
+            if (rc == SCALIT_OK && hipGraphInstantiate(&mg.exec, mg.graph, nullptr, nullptr, 0) != hipSuccess)
+                rc = SCALIT_EHIP;
+        }
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (rc == SCALIT_OK) mg.key = key;
+        else mg.drop();
+    }
+    hipGraphExec_t exec = mg.exec;
+    for (int done = 0; rc == SCALIT_OK && done < qmrmax;) {
+        const int k = std::min(g_qmr_check_every, qmrmax - done);
+        for (int i = 0; i < k && rc == SCALIT_OK; i++)
+            if (hipGraphLaunch(exec, s0) != hipSuccess) rc = SCALIT_EHIP;
+        done += k;
+        bool any = false;
+        for (int j = 0; j < njob && rc == SCALIT_OK; j++) {
+            auto &w = *ws[j];
+            if (hipMemcpyAsync(w.hact.data(), w.active.p, sizeof(int) * w.nvec, hipMemcpyDeviceToHost, s0) !=
+                    hipSuccess ||
+                hipMemcpyAsync(w.hscal.data(), w.scal.p, sizeof(QmrScal) * w.nvec, hipMemcpyDeviceToHost, s0) !=
+                    hipSuccess)
+                rc = SCALIT_EHIP;
+        }
+        if (rc == SCALIT_OK && hipStreamSynchronize(s0) != hipSuccess) rc = SCALIT_EHIP;
+        for (int j = 0; j < njob; j++)
+            for (int i = 0; i < ws[j]->nvec; i++) any = any || ws[j]->hact[i];
+        if (!any) break;
+    }
+    if (rc == SCALIT_OK && qmrmax < 1)
+        for (int j = 0; j < njob && rc == SCALIT_OK; j++) {
+            auto &w = *ws[j];
+            if (hipMemcpyAsync(w.hscal.data(), w.scal.p, sizeof(QmrScal) * w.nvec, hipMemcpyDeviceToHost, s0) !=
+                hipSuccess)
+                rc = SCALIT_EHIP;
+        }
+    for (int j = 0; j < njob && rc == SCALIT_OK; j++) {
+        auto &w = *ws[j];
+        if (hipMemcpy2DAsync(jobs[j].x, jobs[j].ld * 8, w.X(), w.ld * 8, jobs[j].ctx->N * 8, w.nvec,
+                             hipMemcpyDeviceToDevice, s0) != hipSuccess)
+            rc = SCALIT_EHIP;
+    }
+    if (rc == SCALIT_OK && hipStreamSynchronize(s0) != hipSuccess) rc = SCALIT_EHIP;
+    SC_TRY(rc);
+    for (int j = 0; j < njob; j++) {
+        auto &w = *ws[j];
+        for (int i = 0; i < w.nvec; i++) {
+            jobs[j].ret[i] = w.hscal[i].ret;
+            if (jobs[j].res) jobs[j].res[i] = w.hscal[i].res;
+        }
+        jobs[j].status = SCALIT_OK;
+    }
     return SCALIT_OK;
 }
 

@@ -56,3 +56,38 @@ def test_multi_rejects_shared_context():
     with pytest.raises(Exception):
         Context.pist_conv_multi([(ctx, x0, centres), (ctx, x0, centres)], 1e-10, 2, 120, 12, 4)
     ctx.close()
+
+
+def test_qmr_multi_one_graph_equals_each_alone():
+    """scalit_qmr_multi: FP64 QMR on three operators of uneven grids, 4, 2
+    and 3 right-hand sides at shifts inside each spectrum, every iteration of
+    all three one graph launch.  Each solution and return code equals, bit
+    for bit, the same solve alone on the one-kernel-per-step path, and each
+    solves its system (residual against the oracle's H*X)."""
+    from oracle import oracle as O
+
+    nvs = [4, 2, 3]
+    runs, problems = [], []
+    for i, g in enumerate(GRIDS):
+        prob = P.build_problem(*g, P.jittered_potential(21 + i), seed=21 + i)
+        e0 = float(np.sort(prob.res)[20])
+        ctx = Context.from_problem(prob, e0=e0)
+        ctx.block_diag(1000, 1e-5)
+        ctx.set_qmr_persistent(0)
+        b = P.random_vectors(prob.N, nvs[i], 31 + i)
+        shifts = [e0 + 1e-4 * k for k in range(nvs[i])]
+        runs.append((prob, ctx, b, shifts))
+        problems.append((ctx, ctx.to_device(b), shifts))
+    multi = Context.qmr_multi(problems, ntype=-1, qmrmax=500, qmrtol=1e-9)
+    for (prob, ctx, b, shifts), (ret_m, x_m, res_m) in zip(runs, multi):
+        ret, x, res = ctx.qmr(ctx.to_device(b), ntype=-1, qmrmax=500, qmrtol=1e-9, shifts=shifts)
+        print(prob.sN, "multi", ret_m, "alone", ret, flush=True)
+        assert np.array_equal(np.array(ret), ret_m) and (ret_m > 0).all()
+        assert bool((x == x_m).all())
+        op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+        xm = x_m.cpu().numpy()
+        for v in range(len(shifts)):
+            r = op.hx(xm[v]) - shifts[v] * xm[v] - b[v]
+            assert np.linalg.norm(r) <= 1e-7 * np.linalg.norm(b[v])
+    for _, ctx, _, _ in runs:
+        ctx.close()
