@@ -280,6 +280,25 @@ int scalit_crp(scalit_ctx *ctx, int ncrp, const double *energies, int nm, const 
  * column-major, leading dimension lda) into w (2 n doubles). */
 int scalit_small_eig_cx(int n, const double *a, int lda, double *w);
 
+/* Complex level matrices (sCX = T, src/iterate/module/osb.data.h:31,63):
+ * H0CX of a level (n x n, column-major; its real and imaginary parts as
+ * separate host arrays), DEPCX of a dependent level (myDim values) and, for
+ * an sNDVR operator, OUTHCX (N n_sF values, the layout of OUTH).  They
+ * replace H0 / DEP / OUTH, as loadH0 / loadDEP / loadOutH do under sCX
+ * (src/iterate/module/osb.io.h:45-55,211-236), once scalit_set_scx(ctx, 1)
+ * switches them on: the complex operators (scalit_hx_cx and every complex
+ * solver) then apply MYHX_CX (src/iterate/module/osb.hxcx.h:191-245; TMAPR
+ * takes -APR as mosb3.hxcx.h:92-94 does) and scalit_block_diag runs the sCX
+ * branch of myDiag (osb.diag.h:76-101): HosbDiagDX
+ * (src/iterate/libiterate/diag/hosbdiag_dx.f90:6-73) with MJACOBI_CX
+ * (jacobi_cx.f90:5-96), real VOSB, EIG0 the real parts of the rotated
+ * diagonal.  scalit_set_scx(ctx, 1) returns SCALIT_ESTATE until every level
+ * has its complex data; TOSBW is refused while it is on. */
+int scalit_set_h0_cx(scalit_ctx *ctx, int level, const double *h0_re, const double *h0_im);
+int scalit_set_dep_cx(scalit_ctx *ctx, int level, const double *dep_re, const double *dep_im);
+int scalit_set_outh_cx(scalit_ctx *ctx, const double *outh_re, const double *outh_im);
+int scalit_set_scx(scalit_ctx *ctx, int on);
+
 /* Hellmann-Feynman gradients: dE_k/dp_j = sum_grid psi_k^2 dV/dp_j.
  * psi device N x nk (ld), dvdp device N x np (lddv); out host nk x np
  * (row-major).  Uses the adjoint product's dV = x o g with x = g = psi_k. */

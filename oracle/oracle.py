@@ -13,6 +13,7 @@ libraries built by ``oracle/Makefile``:
 from __future__ import annotations
 
 import ctypes as C
+import types
 import os
 
 import numpy as np
@@ -904,11 +905,14 @@ def lan_dx(apply, x, m: int):
 
 
 def crp_dx(op: "Operator", app, apr, energies, nm: int, x0s, qmr_type=-1, qmrmax=1000, qmrtol=1e-3, sap=True,
-           kind: int = TOSB):
+           kind: int = TOSB, hx=None):
     """osbCRP / calOSBCRP / CRP_HXDX (src/iterate/module/osbw.crp.h:7-131):
     N(E) = 4 sum of the nm Lanczos eigenvalues of APR^1/2 (H - i AP - E)^-1
     APP (H + i AP - E)^-1 APR^1/2, AP = APP + APR, each inverse by QMR_CX
-    with the OSB preconditioner (including the absorbing term when sap)."""
+    with the OSB preconditioner (including the absorbing term when sap).
+    hx(op, u, ntype, aps): the complex H*X (MYHX_DX; an OperatorCX's hx for
+    complex level matrices)."""
+    hx = hx_dx if hx is None else hx
     app, apr = f64(app), f64(apr)
     ap, sq = app + apr, np.sqrt(apr)
     aps = (ap, app, apr)
@@ -919,10 +923,10 @@ def crp_dx(op: "Operator", app, apr, energies, nm: int, x0s, qmr_type=-1, qmrmax
 
         def apply(v):
             y = sq * v
-            t = qmr_cx(lambda u: hx_dx(op, u, TMAP, aps), lambda u: px_dx(op, u, kind, TMAP if sap else TA1, aps), y,
+            t = qmr_cx(lambda u: hx(op, u, TMAP, aps), lambda u: px_dx(op, u, kind, TMAP if sap else TA1, aps), y,
                        qmr_type, qmrmax, qmrtol)[1]
             y = app * t
-            t = qmr_cx(lambda u: hx_dx(op, u, TAAP, aps), lambda u: px_dx(op, u, kind, TAAP if sap else TA1, aps), y,
+            t = qmr_cx(lambda u: hx(op, u, TAAP, aps), lambda u: px_dx(op, u, kind, TAAP if sap else TA1, aps), y,
                        qmr_type, qmrmax, qmrtol)[1]
             return sq * t
 
@@ -930,6 +934,201 @@ def crp_dx(op: "Operator", app, apr, energies, nm: int, x0s, qmr_type=-1, qmrmax
         out.append(4.0 * float(np.sum(ev)) if code > 0 else 0.0)
     op.s.E0 = keep
     return np.array(out)
+
+
+# --------------------------------------------------------------------------
+# complex level matrices (sCX = T): the checker for scalit_set_h0_cx,
+# scalit_block_diag_cx and the complex operators on such an operator
+# --------------------------------------------------------------------------
+class OperatorCX:
+    """The sCX = T operator of module osb (src/iterate/module/osb.data.h:63):
+    complex level matrices H0CX (n x n, H[i, j] = H1(i, j)), complex DEPCX
+    on the dependent levels, optional complex OUTHCX (sNDVR: the outermost
+    level's n x n block per inner point, A(p, i, j) at p + nin (i + n j)),
+    real RES and absorbing potentials.  Carries the attributes the complex
+    preconditioner helpers read (sF, sN, VOSB, EIG0, N, s.E0 / DE / beta)."""
+
+    def __init__(self, sn, h0, res, dep=None, outh=None, e0: float = 0.0):
+        self.sF = len(sn)
+        self.sN = [int(v) for v in sn]
+        self.H0 = [np.asarray(h, dtype=np.complex128) for h in h0]
+        dep = list(dep) if dep is not None else [None] * self.sF
+        if outh is not None:
+            dep[-1] = None
+        self.DEP = [None if d is None else np.asarray(d, dtype=np.complex128).reshape(-1) for d in dep]
+        self.OUTH = None if outh is None else np.asarray(outh, dtype=np.complex128).reshape(-1)
+        self.RES = f64(res).reshape(-1)
+        self.N = int(np.prod(self.sN))
+        self.myDim = [int(np.prod(self.sN[:l])) for l in range(self.sF)]
+        self.myBlk = [int(np.prod(self.sN[l + 1:])) for l in range(self.sF)]
+        assert self.RES.size == self.N
+        self.s = types.SimpleNamespace(E0=e0, DE=1.0, beta=0.0)
+        self.VOSB = None
+        self.EIG0 = None
+
+    def set_shift(self, e0: float, de: float = 1.0, beta: float = 0.0):
+        self.s.E0, self.s.DE, self.s.beta = e0, de, beta
+        self._vdense = getattr(self, "_vdense", None)
+
+    def level(self, l: int, x) -> np.ndarray:
+        """Level l's product: H3X_CX for the sNDVR layer
+        (src/libutil/serial/base/matrix_cx.f90:194-208), else H1X_XYZ_CX /
+        H1X_DEP_CX (src/iterate/libiterate/hpx/hpx_cx.f90:6-39, with
+        H1X_CX / H2X_CX of matrix_cx.f90:97-160): y(d, i, b) = [DEP(d)]
+        sum_j H(i, j) x(d, j, b)."""
+        ndk, n, blk = self.myDim[l], self.sN[l], self.myBlk[l]
+        x = np.asarray(x, dtype=np.complex128)
+        if self.OUTH is not None and l == self.sF - 1:
+            a = self.OUTH.reshape(n, n, ndk)            # [j, i, p]
+            return np.einsum("jip,jp->ip", a, x.reshape(n, ndk)).reshape(-1)
+        y = np.einsum("ij,bjd->bid", self.H0[l], x.reshape(blk, n, ndk))
+        if self.DEP[l] is not None:
+            y = y * self.DEP[l][None, None, :]
+        return y.reshape(-1)
+
+    def hx(self, x, ntype: int, aps=None) -> np.ndarray:
+        """MYHX_CX (src/iterate/module/osb.hxcx.h:191-245): (RES - E0 +- i A) X
+        plus every level's product.  TMAPR takes -APR, as the parallel
+        MYHX_CX does (src/iterate/module/mosb3.hxcx.h:92-94); the serial one
+        (osb.hxcx.h:234-235) takes -APP there."""
+        x = np.asarray(x, dtype=np.complex128)
+        y = (self.RES - _shift_of(self, ntype)) * x
+        a = _imag_diag(ntype, aps) if aps is not None else None
+        if a is not None:
+            y = y + 1j * a * x
+        for l in range(self.sF):
+            y = y + self.level(l, x)
+        return y
+
+    def dense(self, ntype: int = TA0, aps=None) -> np.ndarray:
+        """The full N x N matrix from Kronecker products (tiny grids only)."""
+        m = np.diag(self.RES - _shift_of(self, ntype)).astype(np.complex128)
+        a = _imag_diag(ntype, aps) if aps is not None else None
+        if a is not None:
+            m = m + 1j * np.diag(a)
+        for l in range(self.sF):
+            ndk, n, blk = self.myDim[l], self.sN[l], self.myBlk[l]
+            if self.OUTH is not None and l == self.sF - 1:
+                a3 = self.OUTH.reshape(n, n, ndk)
+                for p in range(ndk):
+                    idx = p + ndk * np.arange(n)
+                    m[np.ix_(idx, idx)] += a3[:, :, p].T
+                continue
+            d = np.ones(ndk) if self.DEP[l] is None else self.DEP[l]
+            m = m + np.kron(np.eye(blk), np.kron(self.H0[l], np.diag(d)))
+        return m
+
+    def diag(self, bjmax: int = 1000, bjtol: float = 1e-5):
+        """The sCX branch of myDiag (src/iterate/module/osb.diag.h:76-101):
+        EIG0 = RES, then for levels sF..1 the level's HOSB (myDiagInitCX,
+        osb.diag.h:36-49: OUTHCX, or HOSBINIT_DEP_CX / HOSBINIT_XYZ_CX of
+        hinitdep.f90:38-63 / hinitxyz.f90:31-52) and HosbDiagDX
+        (src/iterate/libiterate/diag/hosbdiag_dx.f90:6-73).  Returns
+        (VOSB per level, EIG0)."""
+        eig0 = self.RES.copy()
+        vosb = [None] * self.sF
+        for l in range(self.sF - 1, -1, -1):
+            ndk, n, blk = self.myDim[l], self.sN[l], self.myBlk[l]
+            if self.OUTH is not None and l == self.sF - 1:
+                h = self.OUTH.reshape(1, n, n, ndk).copy()          # [b, j, k, d]
+            else:
+                d = np.ones(ndk) if self.DEP[l] is None else self.DEP[l]
+                h = np.broadcast_to(self.H0[l].T[None, :, :, None] * d[None, None, None, :],
+                                    (blk, n, n, ndk)).copy()
+                e = eig0.reshape(blk, n, ndk)
+                for k in range(n):
+                    h[:, k, k, :] += e[:, k, :]
+            v, ev = hosb_diag_dx(h, bjmax, bjtol)
+            vosb[l] = v.reshape(-1)
+            eig0 = ev.reshape(-1)
+        self.VOSB, self.EIG0 = vosb, eig0
+        self._vdense = None
+        return vosb, eig0
+
+
+def _sum_offdiag_cx(h: np.ndarray) -> np.ndarray:
+    """sumOffDiagCX (src/iterate/libiterate/diag/sumdiag.f90:49-66) per block
+    as HosbDiagDX calls it: its dummy HOUT is declared double precision
+    (NIN, NOUT, NOUT) while the actual argument is the complex block, so it
+    reads the first NIN NOUT^2 doubles of the block's interleaved (re, im)
+    storage with real strides: 2 sum_{i>j} sum_d D(d + NIN (j + NOUT i))^2.
+    h: [b, j, k, d] complex (element (d, k, j) of block b)."""
+    blk, n, _, ndk = h.shape
+    jj, ii = np.triu_indices(n, 1)                   # j < i
+    dd = np.ascontiguousarray(h).reshape(blk, -1).view(np.float64)[:, :ndk * n * n].reshape(blk, n, n, ndk)
+    return 2.0 * np.sum(dd[:, ii, jj, :] ** 2, axis=(1, 2))     # dd[b, i, j, d]
+
+
+def _jacobi_angle(a, b, c):
+    """MJACOBI's rotation angle (src/iterate/libiterate/diag/jacobi_cx.f90:39-56)."""
+    pie = 3.14159265358979323846264338328
+    x = np.zeros_like(a)
+    eq = a == c
+    x = np.where(eq & (b != 0.0), -np.copysign(pie / 8.0, b), x)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.arctan(b / (a - c)) / 4.0
+    t = np.where(a > c, np.where(t < 0, t + pie / 4.0, t - pie / 4.0), t)
+    return np.where(eq, x, t)
+
+
+def hosb_diag_dx(h: np.ndarray, bjmax: int, bjtol: float):
+    """HosbDiagDX (src/iterate/libiterate/diag/hosbdiag_dx.f90:6-73) on every
+    block at once: h is [b, j, k, d] complex (HOSB(d, k, j, b)), rotated in
+    place.  Per block, sweeps of MJACOBI_CX (jacobi_cx.f90:5-96: one real
+    angle per pair (p, q) from A = sum |H(p,q)|^2, B = 2 Re sum conj(H(p,q))
+    (H(p,p) - H(q,q)) / 2, C = sum |(H(p,p) - H(q,q)) / 2|^2, a symmetric
+    update of both triangles) over pairs i = 1..MAK-1, j = 1..MAK-i, pair
+    (j, j+i), until |change of sumOffDiagCX| <= tol times its first value or
+    BJMAX sweeps; VOSB accumulates the rotations (VUPDATE); EIG0 = real part
+    of the diagonal.  A block that has stopped takes angle 0, which leaves
+    it unchanged.  Returns (VOSB [b, q, r] = V(r, q), EIG0 [b, k, d])."""
+    blk, n, _, ndk = h.shape
+    v = np.zeros((blk, n, n))
+    v[:, np.arange(n), np.arange(n)] = 1.0
+    initye = _sum_offdiag_cx(h)
+    ye = initye.copy()
+    dye = initye.copy()
+    k = np.ones(blk, dtype=int)
+    active = (np.abs(dye) > initye * bjtol) & (k <= bjmax)
+    pairs = [(j - 1, j + i - 1) for i in range(1, n) for j in range(1, n - i + 1)]
+    while active.any():
+        for p, q in pairs:
+            odg = h[:, q, p, :].copy()
+            ddg = 0.5 * h[:, p, p, :] - 0.5 * h[:, q, q, :]
+            a = np.sum(np.conj(odg) * odg, axis=1).real
+            b = 2.0 * np.sum(np.conj(odg) * ddg, axis=1).real
+            c = np.sum(np.conj(ddg) * ddg, axis=1).real
+            x = np.where(active, _jacobi_angle(a, b, c), 0.0)
+            sn, cs = np.sin(x)[:, None], np.cos(x)[:, None]
+            c2, s2, sc = cs ** 2, sn ** 2, sn * cs
+            hpp, hqq = h[:, p, p, :].copy(), h[:, q, q, :].copy()
+            newp = c2 * hpp + s2 * hqq - 2 * sc * odg
+            newq = c2 * hqq + s2 * hpp + 2 * sc * odg
+            newqp = (c2 - s2) * odg + 2 * sc * ddg
+            h[:, p, p, :] = newp
+            h[:, q, q, :] = newq
+            h[:, q, p, :] = newqp
+            h[:, p, q, :] = newqp
+            r = np.array([t for t in range(n) if t != p and t != q], dtype=int)
+            if r.size:
+                hrp, hrq = h[:, p, r, :].copy(), h[:, q, r, :].copy()    # H(r, p), H(r, q)
+                np_ = cs[:, None, :] * hrp - sn[:, None, :] * hrq
+                nq_ = cs[:, None, :] * hrq + sn[:, None, :] * hrp
+                h[:, p, r, :] = np_
+                h[:, r, p, :] = np_
+                h[:, q, r, :] = nq_
+                h[:, r, q, :] = nq_
+            vp, vq = v[:, p, :].copy(), v[:, q, :].copy()
+            v[:, p, :] = cs * vp - sn * vq
+            v[:, q, :] = cs * vq + sn * vp
+        s = _sum_offdiag_cx(h)
+        d_ = s - ye
+        dye = np.where(active, d_, dye)
+        ye = np.where(active, ye + d_, ye)
+        k = np.where(active, k + 1, k)
+        active = active & (np.abs(dye) > initye * bjtol) & (k <= bjmax)
+    eig = np.stack([h[:, i, i, :].real for i in range(n)], axis=1)     # [b, k, d]
+    return v, eig
 
 
 _CXOP = C.CFUNCTYPE(None, _ip, C.c_void_p, C.c_void_p)
@@ -975,3 +1174,47 @@ def ref_mgs_orth_sx(w, vj: np.ndarray, m: int):
     ok = R.mgs_orth_sx_(C.byref(C.c_int(w.size)), C.c_void_p(w.ctypes.data), C.byref(C.c_int(m)),
                         C.c_void_p(vj.ctypes.data))
     return ok != 0
+
+
+class RefOperatorCX:
+    """An OperatorCX driven through the reference's compiled module osb with
+    sCX = T: MYHX_CX (osb.hxcx.h:191-245) and the sCX branch of myDiag
+    (osb.diag.h:76-101, HosbDiagDX)."""
+
+    def __init__(self, opc: OperatorCX):
+        self.opc = opc
+        real = Operator(opc.sN, [h.real for h in opc.H0], opc.RES,
+                        [None if d is None else d.real for d in opc.DEP], e0=opc.s.E0,
+                        outh=None if opc.OUTH is None else opc.OUTH.real)
+        self.base = RefOperator(real)
+        layer = opc.OUTH is not None
+        h0 = [np.asarray(opc.H0[l], dtype=np.complex128).reshape(-1, order="F")
+              for l in range(opc.sF) if not (layer and l == opc.sF - 1)]
+        dep = [opc.DEP[l] for l in range(opc.sF) if opc.DEP[l] is not None and not (layer and l == opc.sF - 1)]
+        self._h0 = np.ascontiguousarray(np.concatenate(h0) if h0 else np.zeros(1, complex))
+        self._dep = np.ascontiguousarray(np.concatenate(dep) if dep else np.zeros(1, complex))
+        self._outh = np.ascontiguousarray(opc.OUTH if layer else np.zeros(1, complex))
+        ref().refop_set_cx(C.c_void_p(self._h0.ctypes.data), C.c_void_p(self._dep.ctypes.data),
+                           C.c_void_p(self._outh.ctypes.data))
+        ref().refop_set_shift(C.c_double(opc.s.E0), C.c_double(opc.s.DE), C.c_double(opc.s.beta))
+
+    def set_ap(self, which, a):
+        ref().refop_set_ap(C.c_int(which), dptr(f64(a)))
+
+    def hx(self, x, ntype):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        ref().refop_hx_cx(C.c_int(ntype), C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data))
+        return y
+
+    def diag(self, bjmax=1000, bjtol=1e-5):
+        opc = self.opc
+        nv = [opc.myBlk[l] * opc.sN[l] ** 2 for l in range(opc.sF)]
+        vosb = np.zeros(sum(nv))
+        eig0 = np.zeros(opc.N)
+        ref().refop_diag_cx(C.c_int(bjmax), C.c_double(bjtol), dptr(vosb), dptr(eig0))
+        out, o = [], 0
+        for k in nv:
+            out.append(vosb[o:o + k].copy())
+            o += k
+        return out, eig0

@@ -126,9 +126,11 @@ subroutine refop_set_ap(which, a) bind(C, name='refop_set_ap')
   implicit none
   integer(c_int), value :: which
   real(c_double), intent(in) :: a(*)
-  if (.not. allocated(AP)) allocate(AP(myLen))
-  if (.not. allocated(APP)) allocate(APP(myLen))
-  if (.not. allocated(APR)) allocate(APR(myLen))
+  ! (re)sized for the current grid: an earlier operator may have had another
+  if (allocated(AP)) then
+    if (size(AP) /= myLen) deallocate(AP, APP, APR)
+  end if
+  if (.not. allocated(AP)) allocate(AP(myLen), APP(myLen), APR(myLen))
   select case (which)
   case (0)
     AP(1:myLen) = a(1:myLen)
@@ -180,3 +182,73 @@ function refop_qmr_dx(nosbk, hc, pc, ntype, qmax, tol, b, x, qres) bind(C, name=
   sPC = pc
   r = QMR_CX(ntype, qmax, tol, myLen, b, HX_DX, PX_DX, x, qres)
 end function
+
+! sCX = T (complex level matrices): H0CX, DEPCX (dependent levels) and
+! OUTHCX (sNDVR) of module osb, in the module's concatenated layouts, after
+! refop_set has sized them (calOSB).
+subroutine refop_set_cx(ah0, adep, aouth) bind(C, name='refop_set_cx')
+  use iso_c_binding
+  use osb
+  implicit none
+  complex(c_double_complex), intent(in) :: ah0(*), adep(*), aouth(*)
+  sCX = .true.
+  if (allocated(H0CX)) deallocate(H0CX)
+  allocate(H0CX(max(1, myH0%mLen)))
+  if (myH0%mLen > 0) H0CX(1:myH0%mLen) = ah0(1:myH0%mLen)
+  if (allocated(DEPCX)) deallocate(DEPCX)
+  allocate(DEPCX(max(1, myDep%mLen)))
+  if (myDep%mLen > 0) DEPCX(1:myDep%mLen) = adep(1:myDep%mLen)
+  if (allocated(OUTHCX)) deallocate(OUTHCX)
+  allocate(OUTHCX(max(1, outLen)))
+  if (outLen > 0) OUTHCX(1:outLen) = aouth(1:outLen)
+end subroutine
+
+! Module osb's MYHX_CX (src/iterate/module/osb.hxcx.h:191-245).
+subroutine refop_hx_cx(ntype, x, y) bind(C, name='refop_hx_cx')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: ntype
+  complex(c_double_complex), intent(in) :: x(*)
+  complex(c_double_complex), intent(out) :: y(*)
+  call MYHX_CX(ntype, myLen, x, y)
+end subroutine
+
+! The sCX branch of myDiag (src/iterate/module/osb.diag.h:76-101) without
+! its HOSB file: EIG0 = RES, then for levels sF..1 the level's HOSB from
+! myDiagInitCX (osb.diag.h:36-49: OUTHCX, HOSBINIT_DEP_CX or
+! HOSBINIT_XYZ_CX) and HosbDiagDX, which leaves the real rotations in VOSB
+! and the real parts of the rotated diagonal in EIG0.
+subroutine refop_diag_cx(bjmax, bjtol, avosb, aeig0) bind(C, name='refop_diag_cx')
+  use iso_c_binding
+  use osb
+  implicit none
+  integer(c_int), value :: bjmax
+  real(c_double), value :: bjtol
+  real(c_double), intent(out) :: avosb(*), aeig0(*)
+  complex(c_double), allocatable :: hb(:)
+  integer :: level, mx
+  double precision :: tol
+  mx = bjmax
+  tol = bjtol
+  if (allocated(VOSB)) deallocate(VOSB)
+  allocate(VOSB(myVOSB%mLen))
+  if (allocated(EIG0)) deallocate(EIG0)
+  allocate(EIG0(myLen))
+  EIG0(1:myLen) = RES(1:myLen)
+  do level = sF, 1, -1
+    allocate(hb(sN(level) * myLen))
+    if (sNDVR .and. level == sF) then
+      hb(1:sN(sF) * myLen) = OUTHCX(1:sN(sF) * myLen)
+    else if (sDEP(level)) then
+      call HOSBINIT_DEP_CX(myBlk(level), sN(level), myDim(level), H0CX(myH0%mStart(level)), &
+                           DEPCX(myDEP%mStart(level)), EIG0, hb)
+    else
+      call HOSBINIT_XYZ_CX(myBlk(level), sN(level), myDim(level), H0CX(myH0%mStart(level)), EIG0, hb)
+    end if
+    call HosbDiagDX(mx, tol, myBlk(level), sN(level), myDim(level), hb, VOSB(myVOSB%mStart(level)), EIG0)
+    deallocate(hb)
+  end do
+  avosb(1:myVOSB%mLen) = VOSB(1:myVOSB%mLen)
+  aeig0(1:myLen) = EIG0(1:myLen)
+end subroutine

@@ -87,6 +87,10 @@ SIGNATURES = {
     "scalit_qmr_cx": (_i, [_vp, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp]),
     "scalit_pist_conv_sx": (_i, [_vp, _d, _d, _i, _vp, _i, _i, _i, _i, _i, _i, _i, _d, _i, _vp, _vp, _vp, _vp]),
     "scalit_small_eig_cx": (_i, [_i, _vp, _i, _vp]),
+    "scalit_set_h0_cx": (_i, [_vp, _i, _vp, _vp]),
+    "scalit_set_dep_cx": (_i, [_vp, _i, _vp, _vp]),
+    "scalit_set_outh_cx": (_i, [_vp, _vp, _vp]),
+    "scalit_set_scx": (_i, [_vp, _i]),
     "scalit_pist_conv_cx": (_i, [_vp, _d, _d, _i, _vp, _i, _i, _i, _i, _i, _i, _i, _d, _i, _vp, _vp, _vp, _vp]),
     "scalit_crp": (_i, [_vp, _i, _vp, _i, _vp, _i, _i, _d, _i, _vp, _vp]),
     "scalit_qmr_multi": (_i, [_i, _vp, _i, _i, _d]),

@@ -20,7 +20,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["kernels_hx.hip", "kernels_px.hip", "kernels_diag.hip", "kernels_vec.hip", "solvers.hip", "capi.hip",
-           "noalloc.hip", "truncate.hip", "kernels_qmr.hip", "osbw.hip", "complex.hip", "formats.cpp"]
+           "noalloc.hip", "truncate.hip", "kernels_qmr.hip", "osbw.hip", "complex.hip", "complex_h0.hip",
+           "formats.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -7,6 +7,7 @@ provider here, every operation runs in libscalit_hip.so.
 from __future__ import annotations
 
 import ctypes as C
+import re
 
 import numpy as np
 import torch
@@ -91,6 +92,26 @@ class Context:
         return a
 
     @staticmethod
+    def load_file_cx(path: str, n: int, mode: str = "seq") -> np.ndarray:
+        """n complex values from a file in one of the reference's formats
+        (loadData_CX / loadDataDir_CX, src/libutil/serial/io/io_cx.f90:102-126,
+        iodirect_cx.f90:68-90): a binary record holds the (re, im) pairs as
+        2n doubles, so "seq" and "dir" read those; "text" reads list-directed
+        complex constants "(re, im)" (with r* repeat counts)."""
+        if mode == "text":
+            with open(path) as f:
+                text = f.read()
+            num = r"[-+0-9.eEdD]+"
+            vals = []
+            for rep, re_, im_ in re.findall(rf"(?:(\d+)\*)?\(\s*({num})\s*,\s*({num})\s*\)", text):
+                z = complex(float(re_.replace("d", "e").replace("D", "e")), float(im_.replace("d", "e").replace("D", "e")))
+                vals.extend([z] * (int(rep) if rep else 1))
+            if len(vals) < n:
+                raise L.ScalitError(f"load {path}: {len(vals)} complex values, {n} expected")
+            return np.array(vals[:n], dtype=np.complex128)
+        return Context.load_file(path, 2 * int(n), mode).view(np.complex128)
+
+    @staticmethod
     def load_grid_file(path: str, npts: int, ng: int) -> np.ndarray:
         """An OUTH / H0GM file of per-point direct records (P2SaveDataGrid,
         src/libutil/parallel/p_io/p2io_grid.f90:62-83) in the operator's
@@ -140,6 +161,36 @@ class Context:
                 ctx.set_outh(cls.load_file(outh_file, lay["outh_len"], "dir"))
         ctx.set_res(res)
         ctx.set_shift(e0)
+        return ctx
+
+    @classmethod
+    def from_files_cx(cls, sN, sDEP, h0_file: str, res_file: str | None = None, dep_file: str | None = None,
+                      outh_file: str | None = None, bin_mode: bool = True, device: int = 0, e0: float = 0.0):
+        """An sCX = T operator from its files (loadH0 / loadDEP / loadOutH under
+        sCX, osb.io.h:45-55,211-236): H0CX of every level concatenated,
+        sequential (binary when bin_mode, else text); DEPCX and OUTHCX one
+        direct-access record each; RES real (zero with OUTH and no RES file).
+        The context holds only the complex level matrices, with sCX on: its
+        complex operators and set-up run, its real H*X returns SCALIT_ESTATE."""
+        ndvr = outh_file is not None
+        lay = cls.layout(sN, sDEP, ndvr)
+        h0 = cls.load_file_cx(h0_file, lay["h0_len"], "seq" if bin_mode else "text")
+        res = np.zeros(lay["N"]) if (ndvr and res_file is None) else cls.load_file(res_file, lay["N"], "dir")
+        dep = cls.load_file_cx(dep_file, lay["dep_len"], "dir") if lay["dep_len"] else None
+        ctx = cls(sN, sDEP, device)
+        if ndvr:
+            ctx.set_outh_cx(cls.load_file_cx(outh_file, lay["outh_len"], "dir"))
+        for l, n in enumerate(ctx.sN):
+            if ndvr and l == ctx.sF - 1:
+                continue
+            st = lay["h0_start"][l]
+            ctx.set_h0_cx(l + 1, h0[st:st + n * n].reshape(n, n, order="F"))
+            if ctx.sDEP[l]:
+                st = lay["dep_start"][l]
+                ctx.set_dep_cx(l + 1, dep[st:st + ctx.dim[l]])
+        ctx.set_res(res)
+        ctx.set_shift(e0)
+        ctx.set_scx(True)
         return ctx
 
     def save_precond(self, vosb_file: str, eig_file: str, mode: str = "dir"):
@@ -428,8 +479,34 @@ class Context:
         a = np.ascontiguousarray(a, dtype=np.float64)
         check(L.lib().scalit_set_ap(self._h, which, _np_dp(a)), "scalit_set_ap")
 
+    def set_h0_cx(self, level: int, h0: np.ndarray):
+        """H0CX of a level (1-based): an n x n complex matrix, H[i, j] = H1(i, j)
+        (osb.data.h:63; complex symmetric, as the sCX set-up assumes)."""
+        h = np.asarray(h0, dtype=np.complex128)
+        re = np.ascontiguousarray(h.real.reshape(-1, order="F"))
+        im = np.ascontiguousarray(h.imag.reshape(-1, order="F"))
+        check(L.lib().scalit_set_h0_cx(self._h, level, _np_dp(re), _np_dp(im)), "scalit_set_h0_cx")
+
+    def set_dep_cx(self, level: int, dep: np.ndarray):
+        """DEPCX of a dependent level (1-based), myDim(level) complex values."""
+        d = np.asarray(dep, dtype=np.complex128).reshape(-1)
+        re, im = np.ascontiguousarray(d.real), np.ascontiguousarray(d.imag)
+        check(L.lib().scalit_set_dep_cx(self._h, level, _np_dp(re), _np_dp(im)), "scalit_set_dep_cx")
+
+    def set_outh_cx(self, outh: np.ndarray):
+        """OUTHCX of an sNDVR operator: N n_sF complex values in OUTH's layout."""
+        a = np.asarray(outh, dtype=np.complex128).reshape(-1)
+        re, im = np.ascontiguousarray(a.real), np.ascontiguousarray(a.imag)
+        check(L.lib().scalit_set_outh_cx(self._h, _np_dp(re), _np_dp(im)), "scalit_set_outh_cx")
+
+    def set_scx(self, on: bool = True):
+        """sCX: the complex level matrices replace the real ones in the complex
+        operators and in block_diag's OSB set-up."""
+        check(L.lib().scalit_set_scx(self._h, 1 if on else 0), "scalit_set_scx")
+
     def hx_cx(self, x: torch.Tensor, ntype: int = L.TMAP):
-        """MYHX_DX (src/iterate/module/osb.hxdx.h:177-240) on split vectors."""
+        """MYHX_DX (src/iterate/module/osb.hxdx.h:177-240) on split vectors, or
+        MYHX_CX (osb.hxcx.h:191-245) when sCX is on."""
         x = x.contiguous()
         y = torch.empty_like(x)
         check(L.lib().scalit_hx_cx(self._h, ntype, x.shape[0], _ptr(x), _ptr(y), _stream()), "scalit_hx_cx")

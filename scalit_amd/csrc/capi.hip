@@ -356,7 +356,7 @@ static int operator_ready(const scalit_ctx *c)
 int scalit_block_diag(scalit_ctx *c, int bjmax, double bjtol)
 {
     CTX_GUARD(c);
-    SC_TRY(operator_ready(c));
+    if (!c->scx) SC_TRY(operator_ready(c));   // the sCX set-up checks its complex data itself
     if (bjmax < 0) return SCALIT_EINVAL;
     int r = block_diag(c, bjmax, bjtol);
     c->gen++;

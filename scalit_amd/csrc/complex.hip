@@ -267,9 +267,11 @@ static void cx_axpby(long N, cd a, const double *x, cd b, const double *y, doubl
                        y, out);
 }
 
-// MYHX_DX on nvec split complex vectors
+// MYHX_DX on nvec split complex vectors (MYHX_CX when sCX is on)
 static int hx_apply_cx(scalit_ctx *c, int ntype, int nvec, const double *x, double *y, hipStream_t s)
 {
+    if (c->scx) return hx_apply_cxh(c, ntype, nvec, x, y, s);   // complex level matrices: MYHX_CX
+    if (!real_levels_ready(c)) return SCALIT_ESTATE;
     int rc = SCALIT_OK;
     double sign = 0.0;
     const double *a = cx_imag_diag(c, ntype, &sign, &rc);

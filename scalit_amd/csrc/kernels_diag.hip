@@ -42,23 +42,46 @@ __device__ __forceinline__ double jacobi_angle(double A, double B, double C)
     return X;
 }
 
+// Element arithmetic of the set-up: double for a real H0, dcx for complex
+// level matrices (HosbDiagDX, hosbdiag_dx.f90:6-73, with MJACOBI_CX,
+// jacobi_cx.f90:5-96: the same real rotations of a complex symmetric HOSB,
+// the angle from A = sum |H(p,q)|^2, B = 2 Re sum conj(H(p,q)) DDG,
+// C = sum |DDG|^2).  rdot(a, b) is Re(conj(a) b), a term of dot_product.
+__device__ __forceinline__ dcx operator+(dcx a, dcx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ dcx operator-(dcx a, dcx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ dcx operator*(double s, dcx a) { return {s * a.re, s * a.im}; }
+__device__ __forceinline__ double cmul(double a, double b) { return a * b; }
+__device__ __forceinline__ dcx cmul(dcx a, dcx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ double rdot(double a, double b) { return a * b; }
+__device__ __forceinline__ double rdot(dcx a, dcx b) { return a.re * b.re + a.im * b.im; }
+__device__ __forceinline__ double real_of(double a) { return a; }
+__device__ __forceinline__ double real_of(dcx a) { return a.re; }
+__device__ __forceinline__ double add_real(double a, double r) { return a + r; }
+__device__ __forceinline__ dcx add_real(dcx a, double r) { return {a.re + r, a.im}; }
+
 // ---------------------------------------------------------------- level 1
 // One wave per block b.  H (n x n) built in LDS from H0 and EIG0; V written
-// to VOSB (global); EIG0 overwritten with the diagonal at the end.
-__global__ void __launch_bounds__(64) k_osb_jacobi_ndk1(int n, int nblk, const double *__restrict__ h0,
+// to VOSB (global); EIG0 overwritten with the (real part of the) diagonal at
+// the end.  The off-diagonal measure reads the block as doubles at
+// j + n i (j < i): for a real H that is H(j,i); for a complex one it is what
+// sumOffDiagCX (sumdiag.f90:49-66), whose dummy is declared double precision,
+// reads from the complex block's (re, im) storage.
+template <typename E>
+__global__ void __launch_bounds__(64) k_osb_jacobi_ndk1(int n, int nblk, const E *__restrict__ h0,
                                                       double *__restrict__ eig0, double *__restrict__ vosb, int bjmax,
                                                       double bjtol, int *sweeps_out)
 {
     extern __shared__ char smem_raw[];
-    double *H = (double *)smem_raw;   // H(r,c) at r + n c
+    E *H = (E *)smem_raw;   // H(r,c) at r + n c
+    const double *Hd = (const double *)smem_raw;
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
     double *eb = eig0 + (long)b * n;
     double *V = vosb + (long)b * n * n;
     for (int e = lane; e < n * n; e += 64) {
         int r = e % n, c = e / n;
-        double v = h0[e];
-        if (r == c) v = v + eb[r];
+        E v = h0[e];
+        if (r == c) v = add_real(v, eb[r]);
         H[e] = v;
         V[e] = (r == c) ? 1.0 : 0.0;
     }
@@ -67,7 +90,7 @@ __global__ void __launch_bounds__(64) k_osb_jacobi_ndk1(int n, int nblk, const d
         double s = 0.0;
         for (int i = 0; i < n; i++)
             for (int j = 0; j < i; j++) {
-                double h = H[j + n * i];
+                double h = Hd[j + n * i];
                 s = s + (0.0 + h * h);
             }
         return 2.0 * s;
@@ -81,12 +104,12 @@ __global__ void __launch_bounds__(64) k_osb_jacobi_ndk1(int n, int nblk, const d
         for (int i = 1; i <= n - 1; i++) {
             for (int j = 1; j <= n - i; j++) {
                 const int p = j - 1, q = j + i - 1;
-                const double hpq = H[p + n * q], hpp = H[p + n * p], hqq = H[q + n * q];
-                const double odg = hpq;
-                const double ddg = 0.5 * hpp - 0.5 * hqq;
-                const double A = 0.0 + odg * odg;
-                const double B = 2.0 * (0.0 + odg * ddg);
-                const double C = 0.0 + ddg * ddg;
+                const E hpq = H[p + n * q], hpp = H[p + n * p], hqq = H[q + n * q];
+                const E odg = hpq;
+                const E ddg = 0.5 * hpp - 0.5 * hqq;
+                const double A = 0.0 + rdot(odg, odg);
+                const double B = 2.0 * (0.0 + rdot(odg, ddg));
+                const double C = 0.0 + rdot(ddg, ddg);
                 const double X = jacobi_angle(A, B, C);
                 const double SN = sin(X), CS = cos(X);
                 const double C2 = CS * CS, S2 = SN * SN, SC = SN * CS;
@@ -94,15 +117,15 @@ __global__ void __launch_bounds__(64) k_osb_jacobi_ndk1(int n, int nblk, const d
                 if (lane == 0) {
                     H[p + n * p] = C2 * hpp + S2 * hqq - 2 * SC * odg;
                     H[q + n * q] = C2 * hqq + S2 * hpp + 2 * SC * odg;
-                    const double nqp = (C2 - S2) * odg + 2 * SC * ddg;
+                    const E nqp = (C2 - S2) * odg + 2 * SC * ddg;
                     H[p + n * q] = nqp;
                     H[q + n * p] = nqp;
                 }
                 for (int r = lane; r < n; r += 64) {
                     if (r != p && r != q) {
-                        const double hrp = H[r + n * p], hrq = H[r + n * q];
-                        const double np = CS * hrp - SN * hrq;
-                        const double nq = CS * hrq + SN * hrp;
+                        const E hrp = H[r + n * p], hrq = H[r + n * q];
+                        const E np = CS * hrp - SN * hrq;
+                        const E nq = CS * hrq + SN * hrp;
                         H[r + n * p] = np;
                         H[p + n * r] = np;
                         H[r + n * q] = nq;
@@ -123,14 +146,15 @@ __global__ void __launch_bounds__(64) k_osb_jacobi_ndk1(int n, int nblk, const d
         ye = ye + dye;
         k = k + 1;
     }
-    for (int r = lane; r < n; r += 64) eb[r] = H[r + n * r];
+    for (int r = lane; r < n; r += 64) eb[r] = real_of(H[r + n * r]);
     if (lane == 0 && sweeps_out) atomicAdd(sweeps_out, k - 1);
 }
 
 // ---------------------------------------------------------------- NDK > 1
 // HOSB(d,k,j,b) = H0(k,j) * [DEP(d)] + delta_kj EIG0(d,k,b)
-__global__ void k_hosb_init(int ndk, int mak, int nblk, const double *__restrict__ h0, const double *__restrict__ dep,
-                            const double *__restrict__ eig0, double *__restrict__ hosb)
+template <typename E>
+__global__ void k_hosb_init(int ndk, int mak, int nblk, const E *__restrict__ h0, const E *__restrict__ dep,
+                            const double *__restrict__ eig0, E *__restrict__ hosb)
 {
     long total = (long)ndk * mak * mak * nblk;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
@@ -140,9 +164,9 @@ __global__ void k_hosb_init(int ndk, int mak, int nblk, const double *__restrict
         t /= mak;
         int jj = (int)(t % mak);
         long b = t / mak;
-        double v = h0[kk + (long)mak * jj];
-        if (dep) v = v * dep[d];
-        if (kk == jj) v = v + eig0[d + (long)ndk * (kk + (long)mak * b)];
+        E v = h0[kk + (long)mak * jj];
+        if (dep) v = cmul(v, dep[d]);
+        if (kk == jj) v = add_real(v, eig0[d + (long)ndk * (kk + (long)mak * b)]);
         hosb[e] = v;
     }
 }
@@ -199,7 +223,8 @@ __device__ __forceinline__ void block_sum3_256(double &a, double &b, double &c, 
 // their stores (the row and column writes cannot alias the column reads of the
 // same group, but the compiler cannot know that, and would otherwise make
 // every element a full memory round trip).
-__device__ __forceinline__ void jrot_slice(double *hb, double *V, int ndk, int mak, int d0, int p, int q, double CS,
+template <typename E>
+__device__ __forceinline__ void jrot_slice(E *hb, double *V, int ndk, int mak, int d0, int p, int q, double CS,
                                            double SN, bool do_v)
 {
     constexpr int RG = 8;
@@ -209,17 +234,17 @@ __device__ __forceinline__ void jrot_slice(double *hb, double *V, int ndk, int m
     const bool din = d < ndk;
 #define HO(r, c) hb[d + (long)ndk * ((r) + (long)mak * (c))]
     if (din && w == 0) {
-        const double hpp = HO(p, p), hqq = HO(q, q), odg = HO(p, q);
-        const double ddg = 0.5 * hpp - 0.5 * hqq;
+        const E hpp = HO(p, p), hqq = HO(q, q), odg = HO(p, q);
+        const E ddg = 0.5 * hpp - 0.5 * hqq;
         HO(p, p) = C2 * hpp + S2 * hqq - 2 * SC * odg;
         HO(q, q) = C2 * hqq + S2 * hpp + 2 * SC * odg;
-        const double nqp = (C2 - S2) * odg + 2 * SC * ddg;
+        const E nqp = (C2 - S2) * odg + 2 * SC * ddg;
         HO(p, q) = nqp;
         HO(q, p) = nqp;
     }
     if (din)
         for (int r0 = w; r0 < mak; r0 += 4 * RG) {
-            double hp[RG], hq[RG];
+            E hp[RG], hq[RG];
 #pragma unroll
             for (int k = 0; k < RG; k++) {
                 const int r = r0 + 4 * k;
@@ -231,8 +256,8 @@ __device__ __forceinline__ void jrot_slice(double *hb, double *V, int ndk, int m
             for (int k = 0; k < RG; k++) {
                 const int r = r0 + 4 * k;
                 if (r >= mak || r == p || r == q) continue;
-                const double np = CS * hp[k] - SN * hq[k];
-                const double nq = CS * hq[k] + SN * hp[k];
+                const E np = CS * hp[k] - SN * hq[k];
+                const E nq = CS * hq[k] + SN * hp[k];
                 HO(r, p) = np;
                 HO(p, r) = np;
                 HO(r, q) = nq;
@@ -250,23 +275,25 @@ __device__ __forceinline__ void jrot_slice(double *hb, double *V, int ndk, int m
 
 // A/B/C sums of pair (p,q) over the slice of d at d0 (wave 0 reads, the whole
 // workgroup reduces)
-__device__ __forceinline__ void jpair_sums(const double *hb, int ndk, int mak, int d0, int p, int q, double &a,
+template <typename E>
+__device__ __forceinline__ void jpair_sums(const E *hb, int ndk, int mak, int d0, int p, int q, double &a,
                                            double &bb, double &c, double *red)
 {
     const int d = d0 + threadIdx.x;
-    double odg = 0, ddg = 0;
+    E odg{}, ddg{};
     if (threadIdx.x < JD && d < ndk) {
         odg = hb[d + (long)ndk * (p + (long)mak * q)];
         ddg = 0.5 * hb[d + (long)ndk * (p + (long)mak * p)] - 0.5 * hb[d + (long)ndk * (q + (long)mak * q)];
     }
-    a = odg * odg;
-    bb = odg * ddg;
-    c = ddg * ddg;
+    a = rdot(odg, odg);
+    bb = rdot(odg, ddg);
+    c = rdot(ddg, ddg);
     block_sum3_256(a, bb, c, red);
 }
 
 // Partial A/B/C sums of pair (p,q) over each 64-point slice of d.
-__global__ void __launch_bounds__(256) k_jpart(int ndk, int mak, int p, int q, const double *__restrict__ hosb,
+template <typename E>
+__global__ void __launch_bounds__(256) k_jpart(int ndk, int mak, int p, int q, const E *__restrict__ hosb,
                                                JPart *__restrict__ part, const int *active)
 {
     __shared__ double red[3 * 256];
@@ -278,8 +305,9 @@ __global__ void __launch_bounds__(256) k_jpart(int ndk, int mak, int p, int q, c
 }
 
 // One rotation (p,q) of every active block, then the partials of (pn,qn).
+template <typename E>
 __global__ void __launch_bounds__(256) k_jrot(int ndk, int mak, int nrc, int p, int q, int pn, int qn,
-                                              double *__restrict__ hosb, double *__restrict__ vosb,
+                                              E *__restrict__ hosb, double *__restrict__ vosb,
                                               const JPart *__restrict__ part_in, JPart *__restrict__ part_out,
                                               const int *active)
 {
@@ -296,7 +324,7 @@ __global__ void __launch_bounds__(256) k_jrot(int ndk, int mak, int nrc, int p, 
     B = 2.0 * B;
     const double X = jacobi_angle(A, B, C);
     const long m2 = (long)mak * mak;
-    double *hb = hosb + (long)ndk * m2 * b;
+    E *hb = hosb + (long)ndk * m2 * b;
     jrot_slice(hb, vosb + m2 * b, ndk, mak, blockIdx.x * JD, p, q, cos(X), sin(X), blockIdx.x == 0);
     if (pn < 0) return;
     __syncthreads();   // every wave's updates of this slice are in memory
@@ -308,15 +336,16 @@ __global__ void __launch_bounds__(256) k_jrot(int ndk, int mak, int nrc, int p, 
 // A whole sweep for levels with ndk <= 64 (one slice per block): one
 // workgroup per active block applies every rotation of the pair sequence in
 // turn, the sums of each pair reduced in the workgroup.
+template <typename E>
 __global__ void __launch_bounds__(256) k_jsweep(int ndk, int mak, int npair, const int *__restrict__ pq,
-                                                double *__restrict__ hosb, double *__restrict__ vosb,
+                                                E *__restrict__ hosb, double *__restrict__ vosb,
                                                 const int *active)
 {
     __shared__ double red[3 * 256];
     const int b = blockIdx.x;
     if (!active[b]) return;
     const long m2 = (long)mak * mak;
-    double *hb = hosb + (long)ndk * m2 * b;
+    E *hb = hosb + (long)ndk * m2 * b;
     double *V = vosb + m2 * b;
     for (int r = 0; r < npair; r++) {
         const int p = pq[2 * r], q = pq[2 * r + 1];
@@ -328,14 +357,16 @@ __global__ void __launch_bounds__(256) k_jsweep(int ndk, int mak, int npair, con
     }
 }
 
-// per-slice partial of sum_{i>j} H(d,j,i)^2
-__global__ void __launch_bounds__(256) k_offdiag_part(int ndk, int mak, const double *__restrict__ hosb,
+// per-slice partial of sum_{i>j} D(d + ndk (j + mak i))^2, D the block's
+// storage read as doubles (bstride doubles per block): H(d,j,i)^2 for a real
+// HOSB, and for a complex one what sumOffDiagCX reads (see k_osb_jacobi_ndk1)
+__global__ void __launch_bounds__(256) k_offdiag_part(int ndk, int mak, long bstride, const double *__restrict__ hosb,
                                                       double *__restrict__ part)
 {
     __shared__ double red[256];
     const int b = blockIdx.y;
     const int d = blockIdx.x * 256 + threadIdx.x;
-    const double *hb = hosb + (long)ndk * mak * mak * b;
+    const double *hb = hosb + bstride * b;
     double s = 0;
     if (d < ndk)
         for (int i = 0; i < mak; i++)
@@ -380,7 +411,8 @@ __global__ void k_vinit(int mak, int nblk, double *vosb)
     }
 }
 
-__global__ void k_eig_extract(int ndk, int mak, int nblk, const double *__restrict__ hosb, double *__restrict__ eig0)
+template <typename E>
+__global__ void k_eig_extract(int ndk, int mak, int nblk, const E *__restrict__ hosb, double *__restrict__ eig0)
 {
     long total = (long)ndk * mak * nblk;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
@@ -388,19 +420,24 @@ __global__ void k_eig_extract(int ndk, int mak, int nblk, const double *__restri
         long t = e / ndk;
         int i = (int)(t % mak);
         long b = t / mak;
-        eig0[e] = hosb[d + (long)ndk * (i + (long)mak * (i + (long)mak * b))];
+        eig0[e] = real_of(hosb[d + (long)ndk * (i + (long)mak * (i + (long)mak * b))]);
     }
 }
 
 // hosb_src: the level's HOSB given as is (the sNDVR = T layer, OUTH,
-// osb.diag.h:22-23) instead of H0 [x DEP] + diag(EIG0).
-static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const double *hosb_src, hipStream_t s)
+// osb.diag.h:22-23) instead of H0 [x DEP] + diag(EIG0); h0 / dep the
+// level's matrix and DEP (dep null for an independent level).
+template <typename E>
+static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const E *hosb_src, const E *h0,
+                          const E *dep, hipStream_t s)
 {
     Level &L = c->lv[l];
     const int ndk = L.dim, mak = L.n, nblk = L.blk;
     const int nch = (ndk + 255) / 256;    // off-diagonal partials: 256-point slices of d
     const int nrc = (ndk + JD - 1) / JD;  // rotation partials: 64-point slices of d
-    DBuf<double> hosb, part_od, ye, initye;
+    const long bstride = (long)(sizeof(E) / sizeof(double)) * ndk * mak * mak;   // one block, in doubles
+    DBuf<E> hosb;
+    DBuf<double> part_od, ye, initye;
     DBuf<JPart> pa, pb;
     DBuf<int> active, kcount, pq;
     SC_TRY(hosb.alloc((size_t)ndk * mak * mak * nblk));
@@ -414,14 +451,15 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const d
     int rc = SCALIT_OK;
     {
         if (hosb_src) {
-            if (hipMemcpyAsync(hosb.p, hosb_src, sizeof(double) * (size_t)ndk * mak * mak * nblk,
+            if (hipMemcpyAsync(hosb.p, hosb_src, sizeof(E) * (size_t)ndk * mak * mak * nblk,
                                hipMemcpyDeviceToDevice, s) != hipSuccess)
                 rc = SCALIT_EHIP;
         } else
-            hipLaunchKernelGGL(k_hosb_init, dim3(4096), dim3(256), 0, s, ndk, mak, nblk, L.h0.p,
-                               L.dep ? L.depv.p : nullptr, c->eig0.p, hosb.p);
+            hipLaunchKernelGGL(k_hosb_init<E>, dim3(4096), dim3(256), 0, s, ndk, mak, nblk, h0, dep, c->eig0.p,
+                               hosb.p);
         hipLaunchKernelGGL(k_vinit, dim3(1024), dim3(256), 0, s, mak, nblk, L.vosb.p);
-        hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, hosb.p, part_od.p);
+        hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, bstride,
+                           (const double *)hosb.p, part_od.p);
         hipLaunchKernelGGL(k_jconv, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, nch, 1, part_od.p, ye.p, initye.p,
                            kcount.p, active.p, bjmax, bjtol);
         // the pair sequence of one sweep
@@ -458,9 +496,10 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const d
             for (int b = 0; b < nblk; b++) any = any || hact[b];
             if (!any) break;
             if (nrc == 1) {
-                hipLaunchKernelGGL(k_jsweep, dim3(nblk), dim3(256), 0, s, ndk, mak, npair, pq.p, hosb.p, L.vosb.p,
+                hipLaunchKernelGGL(k_jsweep<E>, dim3(nblk), dim3(256), 0, s, ndk, mak, npair, pq.p, hosb.p, L.vosb.p,
                                    active.p);
-                hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, hosb.p, part_od.p);
+                hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, bstride,
+                                   (const double *)hosb.p, part_od.p);
                 hipLaunchKernelGGL(k_jconv, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, nch, 0, part_od.p, ye.p,
                                    initye.p, kcount.p, active.p, bjmax, bjtol);
                 if (hipGetLastError() != hipSuccess) rc = SCALIT_EHIP;
@@ -472,19 +511,20 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const d
                     rc = SCALIT_EHIP;
                     break;
                 }
-                hipLaunchKernelGGL(k_jpart, dim3(nrc, nblk), dim3(256), 0, s, ndk, mak, pp[0], qq[0], hosb.p, pa.p,
-                                   active.p);
+                hipLaunchKernelGGL(k_jpart<E>, dim3(nrc, nblk), dim3(256), 0, s, ndk, mak, pp[0], qq[0], hosb.p,
+                                   pa.p, active.p);
                 JPart *pin = pa.p, *pout = pb.p;
                 for (int r = 0; r < npair; r++) {
                     int pn = (r + 1 < npair) ? pp[r + 1] : -1;
                     int qn = (r + 1 < npair) ? qq[r + 1] : -1;
-                    hipLaunchKernelGGL(k_jrot, dim3(nrc, nblk), dim3(256), 0, s, ndk, mak, nrc, pp[r], qq[r], pn, qn,
-                                       hosb.p, L.vosb.p, pin, pout, active.p);
+                    hipLaunchKernelGGL(k_jrot<E>, dim3(nrc, nblk), dim3(256), 0, s, ndk, mak, nrc, pp[r], qq[r], pn,
+                                       qn, hosb.p, L.vosb.p, pin, pout, active.p);
                     JPart *t = pin;
                     pin = pout;
                     pout = t;
                 }
-                hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, hosb.p, part_od.p);
+                hipLaunchKernelGGL(k_offdiag_part, dim3(nch, nblk), dim3(256), 0, s, ndk, mak, bstride,
+                                   (const double *)hosb.p, part_od.p);
                 hipLaunchKernelGGL(k_jconv, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, nch, 0, part_od.p, ye.p,
                                    initye.p, kcount.p, active.p, bjmax, bjtol);
                 if (hipStreamEndCapture(s, &graph) != hipSuccess ||
@@ -502,7 +542,7 @@ static int diag_level_ndk(scalit_ctx *c, int l, int bjmax, double bjtol, const d
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (rc == SCALIT_OK) {
-            hipLaunchKernelGGL(k_eig_extract, dim3(4096), dim3(256), 0, s, ndk, mak, nblk, hosb.p, c->eig0.p);
+            hipLaunchKernelGGL(k_eig_extract<E>, dim3(4096), dim3(256), 0, s, ndk, mak, nblk, hosb.p, c->eig0.p);
             if (hipStreamSynchronize(s) != hipSuccess) rc = SCALIT_EHIP;
         }
     }
@@ -533,7 +573,7 @@ static int diag_level_dense(scalit_ctx *c, int l, hipStream_t s)
     DBuf<double> hosb, work;
     DBuf<EigvecJob> djobs;
     SC_TRY(hosb.alloc((size_t)nb * n * n));
-    hipLaunchKernelGGL(k_hosb_init, dim3(4096), dim3(256), 0, s, 1, n, nb, L.h0.p, (const double *)nullptr,
+    hipLaunchKernelGGL(k_hosb_init<double>, dim3(4096), dim3(256), 0, s, 1, n, nb, L.h0.p, (const double *)nullptr,
                        c->eig0.p, hosb.p);
     std::vector<EigvecJob> jobs(nb);
     for (int b = 0; b < nb; b++)
@@ -543,32 +583,65 @@ static int diag_level_dense(scalit_ctx *c, int l, hipStream_t s)
     return SCALIT_OK;
 }
 
+// The sCX = T set-up (the complex branch of myDiag, osb.diag.h:76-101): the
+// same levels and rotations on the complex HOSB of H0CX [x DEPCX] (or
+// OUTHCX) + diag(EIG0); VOSB stays real and EIG0 takes the real parts.
+// Level-1 blocks too large for LDS take the global-memory sweep.
+static int block_diag_cx(scalit_ctx *c, int bjmax, double bjtol, hipStream_t s)
+{
+    for (int l = c->sF - 1; l >= 0; l--) {
+        Level &L = c->lv[l];
+        const bool layer = c->ndvr && l == c->sF - 1;
+        const dcx *h0 = (const dcx *)L.h0cx.p;
+        const dcx *dep = L.dep ? (const dcx *)L.depcx.p : nullptr;
+        if (!layer && L.dim == 1 && sizeof(dcx) * (size_t)L.n * L.n <= 160 * 1024) {
+            size_t lds = sizeof(dcx) * (size_t)L.n * L.n;
+            allow_full_lds((const void *)k_osb_jacobi_ndk1<dcx>);
+            hipLaunchKernelGGL(k_osb_jacobi_ndk1<dcx>, dim3(L.blk), dim3(64), lds, s, L.n, L.blk, h0, c->eig0.p,
+                               L.vosb.p, bjmax, bjtol, (int *)nullptr);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(s));
+        } else {
+            SC_TRY(diag_level_ndk<dcx>(c, l, bjmax, bjtol, layer ? (const dcx *)c->outhcx.p : nullptr, h0, dep, s));
+        }
+    }
+    return SCALIT_OK;
+}
+
 int block_diag(scalit_ctx *c, int bjmax, double bjtol)
 {
     hipStream_t s = c->stream;
     for (int l = 0; l < c->sF; l++) {
-        if (!c->lv[l].has_h0 && !(c->ndvr && l == c->sF - 1)) return SCALIT_ESTATE;
-        if (c->lv[l].dep && !c->lv[l].has_dep) return SCALIT_ESTATE;
+        const bool layer = c->ndvr && l == c->sF - 1;
+        if (c->scx) {
+            if (layer ? !c->has_outhcx : (!c->lv[l].has_h0cx || (c->lv[l].dep && !c->lv[l].has_depcx)))
+                return SCALIT_ESTATE;
+        } else {
+            if (!c->lv[l].has_h0 && !layer) return SCALIT_ESTATE;
+            if (c->lv[l].dep && !c->lv[l].has_dep) return SCALIT_ESTATE;
+        }
         SC_TRY(c->lv[l].vosb.alloc((size_t)c->lv[l].blk * c->lv[l].n * c->lv[l].n));
     }
     if (!c->has_res) return SCALIT_ESTATE;
     SC_TRY(c->eig0.alloc(c->N));
     HIP_TRY(hipMemcpyAsync(c->eig0.p, c->res.p, sizeof(double) * c->N, hipMemcpyDeviceToDevice, s));
-    for (int l = c->sF - 1; l >= 0; l--) {
+    for (int l = c->sF - 1; !c->scx && l >= 0; l--) {
         Level &L = c->lv[l];
         if (L.dim == 1 && sizeof(double) * (size_t)L.n * L.n > 160 * 1024) {
             SC_TRY(diag_level_dense(c, l, s));
         } else if (L.dim == 1) {
             size_t lds = sizeof(double) * (size_t)L.n * L.n;
-            allow_full_lds((const void *)k_osb_jacobi_ndk1);
-            hipLaunchKernelGGL(k_osb_jacobi_ndk1, dim3(L.blk), dim3(64), lds, s, L.n, L.blk, L.h0.p, c->eig0.p,
-                               L.vosb.p, bjmax, bjtol, (int *)nullptr);
+            allow_full_lds((const void *)k_osb_jacobi_ndk1<double>);
+            hipLaunchKernelGGL(k_osb_jacobi_ndk1<double>, dim3(L.blk), dim3(64), lds, s, L.n, L.blk, L.h0.p,
+                               c->eig0.p, L.vosb.p, bjmax, bjtol, (int *)nullptr);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(s));
         } else {
-            SC_TRY(diag_level_ndk(c, l, bjmax, bjtol, (c->ndvr && l == c->sF - 1) ? c->outh.p : nullptr, s));
+            SC_TRY(diag_level_ndk<double>(c, l, bjmax, bjtol, (c->ndvr && l == c->sF - 1) ? c->outh.p : nullptr,
+                                          L.h0.p, L.dep ? L.depv.p : nullptr, s));
         }
     }
+    if (c->scx) SC_TRY(block_diag_cx(c, bjmax, bjtol, s));
     c->has_precond = true;
     c->osbw_n = 0;
     // fp32 copies are rebuilt on demand

@@ -186,6 +186,7 @@ extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int
     if (!c || cnt < 1 || cnt > 1024 || !(de > 0.0) || (npc != SCALIT_TA0 && npc != SCALIT_TA1)) return SCALIT_EINVAL;
     if (!c->has_precond || !c->has_res) return SCALIT_ESTATE;
     if (c->ndvr) return SCALIT_EINVAL;   // the level-1 rotation of an sNDVR operator is the outer layer
+    if (c->scx) return SCALIT_EINVAL;    // the window of complex level matrices is complex (not built)
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const long N = c->N;

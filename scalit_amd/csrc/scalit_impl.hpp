@@ -90,6 +90,15 @@ struct Level {
     DBuf<double> vosbt;
     DBuf<float> vosbt_f;
     bool has_h0 = false, has_dep = false;
+    // complex level matrix and DEP (sCX = T), (re, im) adjacent: n x n
+    // column-major and myDim values
+    DBuf<double> h0cx, depcx;
+    bool has_h0cx = false, has_depcx = false;
+};
+
+// a complex value stored as the reference stores double complex
+struct dcx {
+    double re, im;
 };
 
 struct PistState;   // defined in solvers.hip
@@ -122,6 +131,11 @@ struct scalit_ctx {
     bool ndvr = false;
     scalit::DBuf<double> outh;
     scalit::DBuf<float> outh_f;
+    // sCX = T: the complex level matrices (Level::h0cx / depcx, and OUTHCX,
+    // myDim(sF) x n x n complex) replace the real ones in the complex
+    // operators and the OSB set-up (scalit_set_scx)
+    scalit::DBuf<double> outhcx;
+    bool has_outhcx = false, scx = false;
     // fused 3-level fast path is available (sF==3, sDEP=(F,F,T) or (F,F) + OUTH, sizes fit)
     bool fast3 = false;
     scalit::PistState *pist = nullptr;
@@ -200,6 +214,8 @@ void osbw_gather(scalit_ctx *c, int nvec, const double *x, long ldx, const int *
 // cx_imag_diag returns the +-AP / APP / APR diagonal an nType or nPC adds
 // (nullptr when none, SCALIT_ESTATE in *rc when that array is not set).
 int px_apply_cx(scalit_ctx *c, int nosb, int npc, int nvec, const double *x, double *y, hipStream_t s);
+int hx_apply_cxh(scalit_ctx *c, int ntype, int nvec, const double *x, double *y, hipStream_t s);
+bool real_levels_ready(const scalit_ctx *c);
 const double *cx_imag_diag(const scalit_ctx *c, int ntype, double *sign, int *rc);
 void release_complex(scalit_ctx *c);
 void osbw_scatter(scalit_ctx *c, int nvec, double *y, long ldy, const int *active, hipStream_t s);

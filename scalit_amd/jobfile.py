@@ -178,12 +178,16 @@ def pist_params(job: IterateJob, m0: int):
 
 def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: int = 1):
     """The job of an iterate input on this library (progPistEig,
-    src/iterate/module/osbw.progEig.h:6-80, real H0, sCX = F): operator from
-    the job's files, preconditioner from the set-up (or from the VOSB/EIG
-    files when sVOSB < 0; saved there when sVOSB > 0), then
+    src/iterate/module/osbw.progEig.h:6-80): operator from the job's files
+    (complex level matrices when sCX, Context.from_files_cx), preconditioner
+    from the set-up (or from the VOSB/EIG files when sVOSB < 0; saved there
+    when sVOSB > 0), then
 
     * sJOB = 1, bound states: PIST for M0 = conv_num levels around conv_e0;
-      returns (ret, eigenvalues, res);
+      returns (ret, eigenvalues, res).  With sCX, OSB_PISTCONVCX0
+      (mosbp.pistconv.h:77-118): PIST_CONV_CX with H0X = MYHX_CX(TA0) and
+      QMR_CX solves of MYHX_CX(TA1) with P = TA1 (here the TMAP0 / TMAP
+      forms with AP = 0, the same products); returns complex Ritz values;
     * sJOB = 2 / 3, resonances (OSB_PISTCONVDX1 / OSB_PISTCONVDX2): AP = APP
       from its file, the complex-symmetric (2) or Hermitian-product (3) PIST
       for M0 energies of H - i AP whose real parts are nearest conv_e0, with
@@ -201,17 +205,19 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
     from . import _lib as L
     from .context import Context
 
-    if job.sCX or job.sJOB not in (JOB_BOUND, JOB_RES1, JOB_RES2, JOB_CRP, JOB_CRP1, JOB_CRP2):
-        raise ValueError("only real-H0 jobs (sCX = F): bound states (sJOB = 1), resonances (sJOB = 2, 3) and "
-                         "CRP (sJOB = 4-6)")
+    if job.sJOB not in (JOB_BOUND, JOB_RES1, JOB_RES2, JOB_CRP, JOB_CRP1, JOB_CRP2):
+        raise ValueError("sJOB must be bound states (1), resonances (2, 3) or CRP (4-6)")
     path = lambda k: os.path.join(base_dir, job.files[k]) if k in job.files else None
     nosb = {TOSB: L.TOSB, TOSBD1: L.TOSBD1, TOSBD2: L.TOSBD2, TOSBW: L.TOSBW}.get(job.sOSB)
     if nosb is None:
         raise ValueError(f"sOSB = {job.sOSB} is not a preconditioner kernel (1..4)")
     if nosb == L.TOSBW and job.sJOB != JOB_BOUND:
         raise ValueError("the windowed preconditioner (sOSB = 4) is on the bound-state path only")
-    ctx = Context.from_files(job.sN, job.sDEP, h0_file=path("H0"), res_file=path("RES"), dep_file=path("DEP"),
-                             outh_file=path("OUTH"), device=device)
+    if job.sCX and nosb == L.TOSBW:
+        raise ValueError("the windowed preconditioner (sOSB = 4) is not built for complex level matrices (sCX)")
+    load = Context.from_files_cx if job.sCX else Context.from_files
+    ctx = load(job.sN, job.sDEP, h0_file=path("H0"), res_file=path("RES"), dep_file=path("DEP"),
+               outh_file=path("OUTH"), device=device)
     ctx.set_shift(job.osbw_e0, job.osbw_de, job.osbw_beta)
     mode = "dir" if abs(job.sVOSB) <= SEQDIRNUM else "seq"
     if job.sVOSB < 0:
@@ -248,6 +254,18 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
         torch.cuda.synchronize()
         ctx.close()
         return m0, crp, 0.0
+    if job.sCX:
+        # OSB_PISTCONVCX0: sHC = sPC = TA1, no absorbing potential
+        for k in (L.AP, L.APP, L.APR):
+            ctx.set_ap(k, np.zeros(ctx.N))
+        ctx.set_osb(nosb)
+        e0, etol, start, step, stepeig, nmax = pist_params(job, m0)
+        x0 = rng.random(ctx.N) + 1j * rng.random(ctx.N)
+        r, eig, res, _ = ctx.pist_conv_sx(ctx.to_device_cx(x0), e0, etol, m0, nmax, start, step, stepeig,
+                                          qmrmax=job.qmr_max, qmrtol=job.qmr_tol, pc=L.TA1, hermitian=True)
+        torch.cuda.synchronize()
+        ctx.close()
+        return r, eig, res
     if nosb == L.TOSBW:
         # OSB_PISTCONV (osbw.pistconv.h:26-31): sPC = TA1, then initOSBW with
         # the job's window; a window of no states fails the job (returns 0)

@@ -402,6 +402,27 @@ def absorbing_potential(prob: Problem, strength: float = 0.02, start: float = 0.
     return np.ascontiguousarray(np.broadcast_to(full, (n3, n2, n1))).reshape(-1)
 
 
+def complex_levels(prob: Problem, theta: float = 0.05, eps: float = 1e-3, seed: int = 11, outh=None):
+    """Complex level matrices of an sCX = T job (H0CX, DEPCX and OUTHCX of
+    src/iterate/module/osb.data.h:63, which the reference loads from its
+    files): every level's kinetic matrix rotated by exp(-2 i theta), as
+    complex scaling of the coordinates makes it, plus i eps S_l with S_l a
+    seeded symmetric N(0,1) matrix scaled by max |H0_l|; DEP (and OUTH) times
+    exp(-2 i theta).  Every matrix stays complex symmetric, the form the
+    sCX set-up's real rotations assume.  Returns (h0 list, dep list[, outh])."""
+    g = rng(seed)
+    rot = np.exp(-2j * theta)
+    h0 = []
+    for h in prob.h0:
+        h = np.asarray(h, dtype=np.float64)
+        b = g.standard_normal(h.shape)
+        h0.append(rot * h + 1j * eps * np.abs(h).max() * 0.5 * (b + b.T))
+    dep = [None if d is None else rot * np.asarray(d, dtype=np.float64) for d in prob.dep]
+    if outh is None:
+        return h0, dep
+    return h0, dep, rot * np.asarray(outh, dtype=np.float64)
+
+
 def random_vectors(n: int, b: int, seed: int) -> np.ndarray:
     """B standard-normal vectors of length n, shape (b, n)."""
     return rng(seed).standard_normal((b, n))

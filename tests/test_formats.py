@@ -166,3 +166,40 @@ def test_layout_closed_form(sN, sDEP, ndvr):
     assert list(vs) == list(np.cumsum([0] + vos[:-1]))
     assert list(tot) == [N, sum(h0), sum(dep), sum(vos), sN[-1] * N if ndvr else 0]
     assert L.lib().scalit_layout(0, (C.c_int * 1)(4), (C.c_int * 1)(0), 0, None, None, None, tot) == L.SCALIT_EINVAL
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("mode", ["seq", "text", "dir"])
+def test_complex_files_read_as_the_reference_reads_them(tmp_path, mode):
+    """H0CX / DEPCX / OUTHCX files (sCX = T): loadData_CX
+    (src/libutil/serial/io/io_cx.f90:102-126) reads one sequential record of
+    N double complex values (binary) or N list-directed complex constants
+    "(re, im)" (text); loadDataDir_CX one direct record.  A binary record of
+    N complex values is a record of their 2N (re, im) doubles, so the real
+    writers make them: the compiled loadData_CX reads what this library
+    writes bit for bit, and Context.load_file_cx reads it back.  (The
+    reference's saveData_CX declares its data double precision and writes N
+    doubles: it cannot make these files, so none is written with it.)"""
+    from scalit_amd.context import Context
+
+    n = 123
+    z = sample(n, 3) + 1j * sample(n, 4)
+    p = str(tmp_path / "cx.dat")
+    pairs = np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).reshape(-1))
+    if mode == "text":
+        with open(p, "w") as f:
+            for v in z:
+                f.write(f" ({float(v.real)!r},{float(v.imag)!r})\n")
+    else:
+        Context.save_file(p, pairs, mode)
+    got = Context.load_file_cx(p, n, mode)
+    assert np.array_equal(got, z)
+    if mode == "dir":
+        return
+    R = ref_lib()
+    R.loaddata_cx_.restype = C.c_int
+    out = np.zeros(2 * n)
+    b = p.encode()
+    ok = R.loaddata_cx_(C.byref(C.c_int(n)), dptr(out), C.byref(C.c_int(1 if mode == "seq" else 0)), b,
+                        C.c_long(len(b)))
+    assert ok and np.array_equal(out.view(np.complex128), z)

@@ -175,3 +175,149 @@ def test_crp_matches_oracle():
     print("crp", crp, ref, stats, flush=True)
     assert np.abs(crp - ref).max() <= 1e-8 * np.abs(ref).max() and (ref > 0).all()
     ctx.close()
+
+
+# ---------------------------------------------------------------- sCX = T
+def scx_setup(grid, ndvr=False, seed=3):
+    """A context with complex level matrices switched on and the oracle's
+    OperatorCX of the same data."""
+    prob = P.build_problem(*grid, P.gaussian_potential(seed), seed=seed)
+    outh = P.outer_layer(prob, seed + 100) if ndvr else None
+    parts = P.complex_levels(prob, 0.05, 1e-3, 11, outh)
+    opc = O.OperatorCX(prob.sN, parts[0], prob.res, parts[1], outh=parts[2] if ndvr else None)
+    ctx = Context.from_problem(prob, e0=0.0, outh=outh)
+    with pytest.raises(L.ScalitError):
+        ctx.set_scx(True)          # complex data missing
+    for l in range(3):
+        if ndvr and l == 2:
+            continue
+        ctx.set_h0_cx(l + 1, parts[0][l])
+        if parts[1][l] is not None:
+            ctx.set_dep_cx(l + 1, parts[1][l])
+    if ndvr:
+        ctx.set_outh_cx(parts[2])
+    ctx.set_scx(True)
+    aps = (P.absorbing_potential(prob, 0.02), P.absorbing_potential(prob, 0.01, 0.5),
+           P.absorbing_potential(prob, 0.01, 0.5, coord="r"))
+    for k, a in enumerate(aps):
+        ctx.set_ap(k, a)
+    return prob, opc, ctx, aps
+
+
+@pytest.mark.parametrize("grid,ndvr", [((12, 10, 8), False), ((24, 32, 24), False), ((12, 10, 8), True)])
+def test_hx_cx_complex_levels_matches_oracle(grid, ndvr):
+    """MYHX_CX (complex H0CX / DEPCX / OUTHCX) against the oracle's
+    OperatorCX, every nType, to 1e-12 of the result; switched off, the
+    context applies MYHX_DX of its real matrices again."""
+    print(f"hx_cx sCX {grid} ndvr={ndvr}", flush=True)
+    prob, opc, ctx, aps = scx_setup(grid, ndvr)
+    e0 = float(np.sort(prob.res)[prob.N // 5])
+    ctx.set_shift(e0)
+    opc.set_shift(e0)
+    x = rand_cx(prob.N, 2, 31)
+    xd = ctx.to_device_cx(x)
+    for nt in ALL_TYPES:
+        y = ctx.from_device_cx(ctx.hx_cx(xd, nt))
+        for v in range(2):
+            ref = opc.hx(x[v], nt, aps)
+            assert np.abs(y[v] - ref).max() <= 1e-12 * np.abs(ref).max(), nt
+    ctx.set_scx(False)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep, e0=e0,
+                    outh=P.outer_layer(prob, 103) if ndvr else None)
+    y = ctx.from_device_cx(ctx.hx_cx(xd, L.TMAP))[0]
+    ref = O.hx_dx(op, x[0], O.TMAP, aps)
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+    ctx.close()
+
+
+@pytest.mark.parametrize("grid,ndvr", [((12, 10, 8), False), ((24, 32, 24), False), ((12, 10, 8), True)])
+def test_block_diag_cx_matches_oracle(grid, ndvr):
+    """The sCX set-up on the GPU (HosbDiagDX on the complex HOSB, real
+    rotations) against the oracle: the same rotation sequence, so EIG0 to
+    1e-9 and VOSB to 1e-7 (rounding of the A, B, C sums only), every VOSB
+    block orthogonal; TOSBW is refused while sCX is on."""
+    print(f"block_diag sCX {grid} ndvr={ndvr}", flush=True)
+    prob, opc, ctx, aps = scx_setup(grid, ndvr)
+    vosb_o, eig0_o = opc.diag(1000, 1e-5)
+    ctx.block_diag(1000, 1e-5)
+    vosb_g, eig0_g = ctx.get_precond()
+    assert np.abs(eig0_g - eig0_o).max() <= 1e-9 * np.abs(eig0_o).max()
+    for l in range(3):
+        assert np.abs(vosb_g[l] - vosb_o[l]).max() <= 1e-7
+        n = prob.sN[l]
+        v = vosb_g[l].reshape(-1, n, n)
+        for b in range(0, v.shape[0], max(1, v.shape[0] // 7)):
+            assert np.allclose(v[b] @ v[b].T, np.eye(n), atol=1e-12)
+    if not ndvr:
+        with pytest.raises(L.ScalitError):
+            ctx.set_osbw(float(np.median(eig0_g)), 1e-3, 4)
+    ctx.close()
+
+
+@pytest.mark.parametrize("qmr_type", [-1, 0])
+def test_qmr_cx_complex_levels(qmr_type):
+    """QMR_CX on H0CX - E0 - i AP with the sCX preconditioner (the oracle's
+    VOSB / EIG0 loaded into both): the oracle's return code within 1, the
+    solution to 1e-7, and the dense complex-symmetric system solved to 1e-8."""
+    print(f"qmr_cx sCX type={qmr_type}", flush=True)
+    prob, opc, ctx, aps = scx_setup((12, 10, 8))
+    vosb, eig0 = opc.diag(1000, 1e-5)
+    ctx.set_precond(vosb, eig0)
+    srt = np.sort(eig0)
+    e0 = float(0.5 * (srt[100] + srt[101]))
+    de = float(srt[110] - e0)
+    ctx.set_shift(e0, de, 0.3)
+    opc.set_shift(e0, de, 0.3)
+    b = rand_cx(prob.N, 1, 37)
+    r_o, x_o, _ = O.qmr_cx(lambda v: opc.hx(v, O.TMAP, aps), lambda v: O.px_dx(opc, v, O.TOSB, O.TMAP, aps), b[0],
+                           qmr_type, 800, 1e-9)
+    r, x, res = ctx.qmr_cx(ctx.to_device_cx(b), qmr_type, L.TMAP, L.TMAP, 800, 1e-9)
+    assert r_o > 0 and r > 0 and abs(r - r_o) <= 1, (r, r_o)
+    x = ctx.from_device_cx(x)[0]
+    assert np.abs(x - x_o).max() <= 1e-7 * np.abs(x_o).max()
+    dense = opc.dense(O.TMAP, aps)
+    assert np.linalg.norm(dense @ x - b[0]) <= 1e-8 * np.linalg.norm(b[0])
+    ctx.close()
+
+
+@pytest.mark.parametrize("hermitian", [False, True])
+def test_pist_conv_sx_complex_levels(hermitian):
+    """OSB_PISTCONVCX1 / CX2 (mosbp.pistconv.h:226-320) on 8 x 9 x 6 with the
+    preconditioner from the GPU's sCX set-up.  CX1 (PIST_CONV_SX, bilinear
+    products): the 4 eigenvalues of the complex-symmetric H0CX - i AP whose
+    real parts are nearest E0, against a dense eigensolve (1e-10 relative).
+    CX2 (PIST_CONV_CX, Hermitian products) projects a non-Hermitian
+    operator, so its Ritz values are not eigenvalues: they are checked
+    against the oracle's PIST_CONV_CX on the same data (1e-8 relative).
+    CX1 also returns the oracle's return code."""
+    print(f"pist_sx sCX hermitian={hermitian}", flush=True)
+    prob, opc, ctx, aps = scx_setup((8, 9, 6))
+    ctx.block_diag(1000, 1e-5)
+    vosb, eig0 = ctx.get_precond()
+    opc.VOSB, opc.EIG0 = [np.asarray(v) for v in vosb], np.asarray(eig0)
+    ev = np.linalg.eigvals(opc.dense(O.TMAP0, aps))
+    srt = np.sort(ev.real)
+    e0 = float(0.5 * (srt[40] + srt[41]))
+    ctx.set_shift(e0)
+    opc.set_shift(e0)
+    near = ev[np.argsort(np.abs(ev.real - e0), kind="stable")[:4]]
+    near = near[np.argsort(near.real)]
+    x = np.atleast_2d(P.random_vectors(prob.N, 1, 3)[0] + 0j)
+    r, eig, res, stats = ctx.pist_conv_sx(ctx.to_device_cx(x), e0, 1e-12, 4, 160, 16, 4, 2, qmr_type=-1,
+                                          qmrmax=500, qmrtol=1e-10, hermitian=hermitian)
+    print("pist_sx sCX", r, eig, near, res, stats, flush=True)
+    assert r > 0
+    if not hermitian:
+        assert np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
+    lin = lambda v: O.qmr_cx(lambda u: opc.hx(u, O.TMAP, aps), lambda u: O.px_dx(opc, u, O.TOSB, O.TMAP, aps),
+                             v, -1, 500, 1e-10)
+    r_o, eig_o, _ = O.pist_conv_sx(lambda v: opc.hx(v, O.TMAP0, aps), lin, x[0], e0, 1e-12, 0, 4, 160, 16, 4, 2,
+                                   hermitian=hermitian)
+    if hermitian:
+        # as in the sCX bound-state job: the step at which the Ritz values of
+        # the Hermitian projection settle depends on rounding; the values agree
+        print("pist_cx sCX oracle", r_o, eig_o, flush=True)
+        assert r_o > 0 and np.abs(eig - eig_o).max() <= 1e-8 * np.abs(eig_o).max(), (eig, eig_o)
+    else:
+        assert r == r_o, (r, r_o)
+    ctx.close()

@@ -130,13 +130,13 @@ eig0.dat
         assert r3 > 0 and (np.abs(np.sort(eig3) - ref) / np.abs(ref)).max() <= 1e-9
 
 
-def _cx_job_text(prob, files, sjob, conv, extra):
+def _cx_job_text(prob, files, sjob, conv, extra, scx=False):
     import os
 
     return f"""3 {prob.sN[0]} {prob.sN[1]} {prob.sN[2]}
 F F T
 {sjob} 1
-F F T T
+{"T" if scx else "F"} F T T
 1000 1.0D-5 500 1.0D-10
 {conv}
 0.0 1.0 0.0 0
@@ -189,4 +189,95 @@ def test_resonance_and_crp_jobs_from_input_files(tmp_path):
     op.diag()
     ref = O.crp_dx(op, app, apr, [e_start + k * e_step for k in range(3)], 10, list(x0), -1, 500, 1e-10, True)
     print("crp job", crp, ref, flush=True)
+    assert n == 3 and np.abs(crp - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def _pairs(z):
+    return np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).reshape(-1))
+
+
+def test_complex_level_jobs_from_input_files(tmp_path):
+    """sCX = T jobs of 8 x 9 x 6 from iterate input files, H0CX as one
+    sequential record of complex values and DEPCX as a direct record
+    (loadH0 / loadDEP under sCX, osb.io.h:45-55,211-221): resonances
+    (sJOB = 2, OSB_PISTCONVCX1) against a dense eigensolve of H0CX - i APP
+    (1e-10); bound states (sJOB = 1, OSB_PISTCONVCX0) against the oracle's
+    PIST_CONV_CX with MYHX_CX(TA0) and TA1 solves on the same operator and
+    preconditioner (Ritz values to 1e-7); CRP (sJOB = 4) against the oracle's
+    osbCRP with MYHX_CX (1e-8).  The oracle's preconditioner is its own sCX
+    set-up (VOSB within 1e-7 of the job's)."""
+    from scalit_amd import jobfile as J
+
+    prob = P.build_problem(8, 9, 6, P.gaussian_potential(3), seed=3)
+    parts = P.complex_levels(prob, 0.05, 1e-3, 11)
+    lay = Context.layout(prob.sN, [d is not None for d in prob.dep], False)
+    h0 = np.zeros(lay["h0_len"], complex)
+    for l, h in enumerate(parts[0]):
+        h0[lay["h0_start"][l]:lay["h0_start"][l] + h.size] = h.reshape(-1, order="F")
+    dep = np.zeros(lay["dep_len"], complex)
+    for l, d in enumerate(parts[1]):
+        if d is not None:
+            dep[lay["dep_start"][l]:lay["dep_start"][l] + d.size] = d
+    files = {"h0_file": str(tmp_path / "h0cx.dat"), "res_file": str(tmp_path / "res.dat"),
+             "dep_file": str(tmp_path / "depcx.dat")}
+    Context.save_file(files["h0_file"], _pairs(h0), "seq")
+    Context.save_file(files["res_file"], prob.res, "dir")
+    Context.save_file(files["dep_file"], _pairs(dep), "dir")
+    app = P.absorbing_potential(prob, 0.01, 0.6, coord="R")
+    apr = P.absorbing_potential(prob, 0.01, 0.6, coord="r")
+    Context.save_file(str(tmp_path / "app.dat"), app, "dir")
+    Context.save_file(str(tmp_path / "apr.dat"), apr, "dir")
+    opc = O.OperatorCX(prob.sN, parts[0], prob.res, parts[1])
+    opc.diag(1000, 1e-5)
+    # resonances
+    aps = (app, app, np.zeros(prob.N))
+    ev = np.linalg.eigvals(opc.dense(O.TMAP0, aps))
+    srt = np.sort(ev.real)
+    e0 = float(0.5 * (srt[40] + srt[41]))
+    near = ev[np.argsort(np.abs(ev.real - e0), kind="stable")[:4]]
+    near = near[np.argsort(near.real)]
+    (tmp_path / "res.in").write_text(_cx_job_text(prob, files, 2, f"{e0!r} 1.0D-12 16 4 160 4 2", ["app.dat"], True))
+    r, eig, res = J.run_pist_job(J.read_iterate_input(str(tmp_path / "res.in")), base_dir=str(tmp_path))
+    print("sCX resonance job", r, eig, res, flush=True)
+    assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
+    # bound states: Hermitian-product PIST of the complex-symmetric operator
+    z = (0.0 * app, 0.0 * app, 0.0 * app)
+    eb = float(0.5 * (srt[20] + srt[21]))
+    # (the job saves its set-up, sVOSB > 0, and the oracle runs with it)
+    (tmp_path / "bound.in").write_text(_cx_job_text(prob, files, 1, f"{eb!r} 1.0D-10 16 4 160 4 2",
+                                                    ["vosb.dat", "eig0.dat"], True).replace("0 0 0 0 0", "0 42 0 0 0"))
+    job = J.read_iterate_input(str(tmp_path / "bound.in"))
+    assert job.sVOSB == 42
+    r, eig, res = J.run_pist_job(job, base_dir=str(tmp_path), seed=7)
+    rng = np.random.default_rng(7)
+    x0 = rng.random(prob.N) + 1j * rng.random(prob.N)
+    nv = [opc.myBlk[l] * prob.sN[l] ** 2 for l in range(3)]
+    flat = Context.load_file(str(tmp_path / "vosb.dat"), sum(nv), "dir")
+    opc.VOSB = [flat[sum(nv[:l]):sum(nv[:l + 1])] for l in range(3)]
+    opc.EIG0 = Context.load_file(str(tmp_path / "eig0.dat"), prob.N, "dir")
+    opc._vdense = None
+    opc.set_shift(eb)
+    lin = lambda v: O.qmr_cx(lambda u: opc.hx(u, O.TA1, z), lambda u: O.px_dx(opc, u, O.TOSB, O.TA1, z), v, -1,
+                             500, 1e-10)
+    r_o, eig_o, _ = O.pist_conv_sx(lambda v: opc.hx(v, O.TA0, z), lin, x0, eb, 1e-10, 0, 4, 160, 16, 4, 2,
+                                   hermitian=True)
+    print("sCX bound job", r, eig, r_o, eig_o, flush=True)
+    # The Hermitian projection of this non-Hermitian operator has pairs of
+    # Ritz values 3e-6 apart whose distances from E0 nearly tie, so the step
+    # at which the window's values stop moving by 1e-10 depends on rounding
+    # (this job stops at 32 on the GPU and at 16 in the oracle); the values
+    # both return agree to the oracle's earlier stop (2e-8).
+    assert r > 0 and r_o > 0 and np.abs(eig - eig_o).max() <= 1e-7 * np.abs(eig_o).max()
+    # CRP (with the oracle's own set-up again)
+    opc.diag(1000, 1e-5)
+    hs = np.sort(srt)
+    e_start, e_step = float(hs[20] + 1e-4), 2e-4
+    text = _cx_job_text(prob, files, 4, f"{e_start!r} {e_step!r} 0 0 0 3 10", ["app.dat", "apr.dat"], True)
+    (tmp_path / "crp.in").write_text(text)
+    n, crp, _ = J.run_pist_job(J.read_iterate_input(str(tmp_path / "crp.in")), base_dir=str(tmp_path), seed=5)
+    rng = np.random.default_rng(5)
+    x0 = rng.random((3, prob.N)) + 1j * rng.random((3, prob.N))
+    ref = O.crp_dx(opc, app, apr, [e_start + k * e_step for k in range(3)], 10, list(x0), -1, 500, 1e-10, True,
+                   hx=lambda op, u, nt, a: op.hx(u, nt, a))
+    print("sCX crp job", crp, ref, flush=True)
     assert n == 3 and np.abs(crp - ref).max() <= 1e-8 * np.abs(ref).max()

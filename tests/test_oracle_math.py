@@ -500,3 +500,50 @@ def test_crp_operator_and_lanczos_against_dense():
         assert code == nm
         assert abs(crp - 4 * ev.sum()) <= 1e-9 * abs(crp)
         assert 0 < crp <= 4 * np.trace(pm).real * (1 + 1e-12)
+
+
+# ---------------------------------------------------------------- sCX = T
+@pytest.mark.parametrize("ndvr", [False, True])
+def test_hx_cx_equals_kronecker_sum(ndvr):
+    """OperatorCX.hx (MYHX_CX's level loop as batched einsums) against the
+    matrix built independently from Kronecker products I_blk x H0CX x
+    diag(DEPCX) (and the sNDVR layer's per-point blocks placed by index),
+    every nType with its imaginary diagonal."""
+    prob = P.build_problem(5, 6, 4, P.gaussian_potential(3), seed=3)
+    outh = P.outer_layer(prob, 7) if ndvr else None
+    parts = P.complex_levels(prob, 0.07, 2e-3, 5, outh)
+    opc = O.OperatorCX(prob.sN, parts[0], prob.res, parts[1], outh=parts[2] if ndvr else None)
+    opc.set_shift(0.01)
+    aps = (P.absorbing_potential(prob, 0.02), P.absorbing_potential(prob, 0.01, 0.5),
+           P.absorbing_potential(prob, 0.01, 0.5, coord="r"))
+    x = P.random_vectors(opc.N, 1, 4)[0] + 1j * P.random_vectors(opc.N, 1, 5)[0]
+    for nt in (O.TA0, O.TA1, O.TAAP0, O.TMAP0, O.TAAP, O.TAAPP, O.TAAPR, O.TMAP, O.TMAPP, O.TMAPR):
+        y = opc.hx(x, nt, aps)
+        assert np.abs(y - opc.dense(nt, aps) @ x).max() <= 1e-13 * np.abs(y).max(), nt
+    # complex symmetric: the transpose equals the matrix
+    d = opc.dense(O.TA0)
+    assert np.abs(d - d.T).max() <= 1e-15 * np.abs(d).max()
+
+
+@pytest.mark.parametrize("shape", [(7, 9, 1), (3, 8, 5)])
+def test_hosb_diag_dx_is_a_similarity(shape):
+    """HosbDiagDX's rotations are real orthogonal and its updates are
+    H <- J^T H J: on random complex symmetric blocks, V is orthogonal and
+    EIG0 equals the real part of diag(V^T H(d) V) of the unrotated blocks,
+    for every inner point d, to 1e-12; and the off-diagonal mass falls."""
+    blk, n, ndk = shape
+    g = P.rng(8)
+    a = g.standard_normal((blk, n, n, ndk)) + 1j * g.standard_normal((blk, n, n, ndk))
+    h = 0.5 * (a + a.transpose(0, 2, 1, 3))
+    h0 = h.copy()
+    v, eig = O.hosb_diag_dx(h, 200, 1e-10)
+    for b in range(blk):
+        vb = v[b].T                                    # V(r, q)
+        assert np.abs(vb.T @ vb - np.eye(n)).max() <= 1e-12
+        for d in range(ndk):
+            m = h0[b, :, :, d].T                       # H(k, j)
+            r = vb.T @ m @ vb
+            assert np.abs(np.diag(r).real - eig[b, :, d]).max() <= 1e-12 * np.abs(m).max()
+            off0 = np.sum(np.abs(m - np.diag(np.diag(m))) ** 2)
+            off1 = np.sum(np.abs(r - np.diag(np.diag(r))) ** 2)
+            assert off1 < off0

@@ -391,3 +391,57 @@ def test_lan_hij_dx_and_mgs_orth_sx_match_reference():
         vj[:, k] = vh[-1]
     assert O.ref_mgs_orth_cx(w, vj, 3) and O.mgs_orth_cx(w, vh)
     assert np.abs(vj[:, 3] - vh[3]).max() <= 1e-13
+
+
+# ---------------------------------------------------------------- sCX = T
+def _scx_ops(grid=(6, 7, 5), ndvr=False):
+    prob = P.config_problem("pist_fp64", grid=grid)
+    outh = P.outer_layer(prob, 7) if ndvr else None
+    parts = P.complex_levels(prob, 0.05, 1e-3, 11, outh)
+    opc = O.OperatorCX(prob.sN, parts[0], prob.res, parts[1], outh=parts[2] if ndvr else None)
+    srt = np.sort(prob.res)
+    opc.set_shift(float(srt[len(srt) // 4]), 0.01, 0.3)
+    ropc = O.RefOperatorCX(opc)
+    aps = (P.absorbing_potential(prob, 0.02), P.absorbing_potential(prob, 0.015, 0.6),
+           P.absorbing_potential(prob, 0.01, 0.5, coord="r"))
+    for k, a in enumerate(aps):
+        ropc.set_ap(k, a)
+    return prob, opc, ropc, aps
+
+
+@pytest.mark.parametrize("ndvr", [False, True])
+@pytest.mark.parametrize("ntype", [O.TA0, O.TA1, O.TAAP0, O.TMAP0, O.TAAP, O.TAAPP, O.TAAPR, O.TMAP, O.TMAPP,
+                                   O.TMAPR])
+def test_hx_cx_matches_reference(ntype, ndvr):
+    """MYHX_CX of the compiled module osb with sCX = T (osb.hxcx.h:191-245)
+    against the oracle's restatement, every nType, with and without the
+    sNDVR layer, to 1e-14 of the result.  The serial MYHX_CX adds -i APP
+    under TMAPR (osb.hxcx.h:234-235) where the parallel one adds -i APR
+    (mosb3.hxcx.h:92-94, which the oracle follows): there the compiled
+    serial routine is compared with the oracle's TMAPP."""
+    prob, opc, ropc, aps = _scx_ops(ndvr=ndvr)
+    rng = P.rng(15)
+    x = rng.standard_normal(opc.N) + 1j * rng.standard_normal(opc.N)
+    y = opc.hx(x, O.TMAPP if ntype == O.TMAPR else ntype, aps)
+    yr = ropc.hx(x, ntype)
+    assert np.abs(y - yr).max() <= 1e-14 * np.abs(yr).max()
+    if ntype == O.TMAPR:
+        assert np.abs(opc.hx(x, ntype, aps) - yr).max() > 1e-6 * np.abs(yr).max()
+
+
+@pytest.mark.parametrize("grid,ndvr", [((6, 7, 5), False), ((9, 8, 7), False), ((6, 7, 5), True)])
+def test_block_diag_cx_matches_reference(grid, ndvr):
+    """The sCX branch of myDiag (osb.diag.h:76-101: HOSBINIT_*_CX, HosbDiagDX
+    with MJACOBI_CX and sumOffDiagCX), compiled, against the oracle's
+    restatement: EIG0 to 1e-13 and VOSB to 1e-11 (same rotation sequence
+    and sweep counts; the sums of A, B, C differ in order only).  The
+    oracle's off-diagonal measure follows sumOffDiagCX reading the complex
+    block through its double precision dummy; with the complex magnitudes
+    instead the sweep counts change and VOSB moves by about 1e-5, which this
+    bound would catch."""
+    prob, opc, ropc, aps = _scx_ops(grid, ndvr)
+    vr, er = ropc.diag(1000, 1e-5)
+    v, e = opc.diag(1000, 1e-5)
+    assert np.abs(e - er).max() <= 1e-13 * np.abs(er).max()
+    for a, b in zip(v, vr):
+        assert np.abs(a - b).max() <= 1e-11
