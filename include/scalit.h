@@ -237,7 +237,14 @@ int scalit_set_ap(scalit_ctx *ctx, int which, const double *ap);
 int scalit_hx_cx(scalit_ctx *ctx, int ntype, int nvec, const double *x, double *y, scalit_stream stream);
 /* MYPX_DX (src/iterate/module/osb.mypxdx.h): V (kernel)^-1 V^T X with the
  * complex divisor EIG0 - E0 (clamped by TOSBD1 / TOSBD2) +- i A of npc.
- * nosb TOSB, TOSBD1 or TOSBD2 (TOSBW: SCALIT_EINVAL). */
+ * nosb TOSB, TOSBD1, TOSBD2 or TOSBW (after scalit_set_osbw, real level
+ * matrices): the window's components are replaced by HWR^-1 times them,
+ * HWR = H_W + diag(-E0 +- i A(WIND)) (createWH_DXCX, osb.pij.h:65-166;
+ * ZGETRS / ZHETRS in osb.mypxdx.h:127-170), its inverse formed on the host
+ * once per (npc, E0, absorbing potentials, window); TOSBW with npc TAAP0 or
+ * TMAP0 is SCALIT_EINVAL (the reference divides by EIG0 - E0 there but
+ * leaves E0 out of HWR).  The complex solvers below take sOSB from
+ * scalit_set_osb, TOSBW included. */
 int scalit_px_cx(scalit_ctx *ctx, int nosb, int npc, int nvec, const double *x, double *y, scalit_stream stream);
 /* OSB_QMRDX (src/iterate/module/osbw.qmrdx.h:16-27): QMR_CX
  * (src/iterate/libiterate/qmr/qmrcx.f90:4-179) with HX_DX = MYHX_DX(hc) and

@@ -684,12 +684,33 @@ def px_dx_denominator(op: "Operator", nosb: int, npc: int, aps) -> np.ndarray:
     return ade + (0j if a is None else 1j * a)
 
 
-def px_dx(op: "Operator", x, nosb: int, npc: int, aps) -> np.ndarray:
+def px_dx(op: "Operator", x, nosb: int, npc: int, aps, win=None) -> np.ndarray:
     """MYPX_DX (src/iterate/module/osb.mypxdx.h): V (kernel)^-1 V^T X with the
-    real OSB rotations applied to complex X."""
+    real OSB rotations applied to complex X.  nosb = TOSBW takes the window
+    `win` of osbw_setup (px_tosbw_dx)."""
+    if nosb == TOSBW:
+        return px_tosbw_dx(op, x, win, npc, aps)
     v = osb_basis(op)
     z = v.T @ np.asarray(x, dtype=np.complex128)
     return v @ (z / px_dx_denominator(op, nosb, npc, aps))
+
+
+def px_tosbw_dx(op: "Operator", x, win, npc: int, aps) -> np.ndarray:
+    """MYPX_DX's TOSBW case (osb.mypxdx.h:127-170) with createWH_DXCX's window
+    matrix (osb.pij.h:65-166) for a real H0: the rotated vector divided by
+    EIG0 - E0 +- i A, except on the window, whose components are replaced by
+    HWR^-1 times them, HWR = H_W + diag(-E0 +- i A(WIND)) (E0 = 0 under
+    TA0; H_W with EIG0 on its diagonal, unshifted: win[2] of osbw_setup).
+    With z = V^T x: P_TOSB x + V_W (HWR^-1 z_W - z_W / d_W)."""
+    ind, _, hw, _, vw = win
+    x = np.asarray(x, dtype=np.complex128)
+    e0 = _shift_of(op, npc)
+    a = _imag_diag(npc, aps)
+    hwr = hw.astype(np.complex128) + np.diag(np.full(len(ind), -e0) + (0j if a is None else 1j * a[ind]))
+    d = (op.EIG0 - e0 + (0j if a is None else 1j * a))[ind]
+    zw = vw.T @ x
+    corr = np.linalg.solve(hwr, zw) - zw / d
+    return px_dx(op, x, TOSB, npc, aps) + vw @ corr
 
 
 def dot_cx(x, y) -> complex:
@@ -905,7 +926,7 @@ def lan_dx(apply, x, m: int):
 
 
 def crp_dx(op: "Operator", app, apr, energies, nm: int, x0s, qmr_type=-1, qmrmax=1000, qmrtol=1e-3, sap=True,
-           kind: int = TOSB, hx=None):
+           kind: int = TOSB, hx=None, win=None):
     """osbCRP / calOSBCRP / CRP_HXDX (src/iterate/module/osbw.crp.h:7-131):
     N(E) = 4 sum of the nm Lanczos eigenvalues of APR^1/2 (H - i AP - E)^-1
     APP (H + i AP - E)^-1 APR^1/2, AP = APP + APR, each inverse by QMR_CX
@@ -923,11 +944,11 @@ def crp_dx(op: "Operator", app, apr, energies, nm: int, x0s, qmr_type=-1, qmrmax
 
         def apply(v):
             y = sq * v
-            t = qmr_cx(lambda u: hx(op, u, TMAP, aps), lambda u: px_dx(op, u, kind, TMAP if sap else TA1, aps), y,
-                       qmr_type, qmrmax, qmrtol)[1]
+            t = qmr_cx(lambda u: hx(op, u, TMAP, aps),
+                       lambda u: px_dx(op, u, kind, TMAP if sap else TA1, aps, win), y, qmr_type, qmrmax, qmrtol)[1]
             y = app * t
-            t = qmr_cx(lambda u: hx(op, u, TAAP, aps), lambda u: px_dx(op, u, kind, TAAP if sap else TA1, aps), y,
-                       qmr_type, qmrmax, qmrtol)[1]
+            t = qmr_cx(lambda u: hx(op, u, TAAP, aps),
+                       lambda u: px_dx(op, u, kind, TAAP if sap else TA1, aps, win), y, qmr_type, qmrmax, qmrtol)[1]
             return sq * t
 
         code, ev = lan_dx(apply, x, nm)

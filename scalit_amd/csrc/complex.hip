@@ -1106,6 +1106,7 @@ int scalit_set_ap(scalit_ctx *c, int which, const double *ap)
     SC_TRY(c->ap[which].alloc(c->N));
     HIP_TRY(hipMemcpy(c->ap[which].p, ap, sizeof(double) * c->N, hipMemcpyHostToDevice));
     c->has_ap[which] = true;
+    c->ap_ver++;
     c->gen++;
     return SCALIT_OK;
 }
@@ -1131,7 +1132,7 @@ int scalit_qmr_cx(scalit_ctx *c, int qmr_type, int hc, int pc, int qmrmax, doubl
     if (!c || !b || !x || !ret || b == x || qmrmax < 0) return SCALIT_EINVAL;
     HIP_TRY(hipSetDevice(c->device));
     if (!c->has_res || !c->has_precond) return SCALIT_ESTATE;
-    if (c->osb == SCALIT_TOSBW) return SCALIT_EINVAL;
+    if (c->osb == SCALIT_TOSBW && c->osbw_n <= 0) return SCALIT_ESTATE;   // TOSBW needs its window
     HIP_TRY(hipDeviceSynchronize());
     return qmr_cx_run(c, c->osb, hc, pc, qmr_type, qmrmax, qmrtol, b, x, ret, res, c->stream);
 }
@@ -1144,7 +1145,7 @@ int scalit_pist_conv_sx(scalit_ctx *c, double e0, double etol, int ntype, const 
     if (pc != SCALIT_TMAP && pc != SCALIT_TA1) return SCALIT_EINVAL;
     HIP_TRY(hipSetDevice(c->device));
     if (!c->has_res || !c->has_precond || !c->has_ap[SCALIT_AP]) return SCALIT_ESTATE;
-    if (c->osb == SCALIT_TOSBW) return SCALIT_EINVAL;
+    if (c->osb == SCALIT_TOSBW && c->osbw_n <= 0) return SCALIT_ESTATE;   // TOSBW needs its window
     HIP_TRY(hipDeviceSynchronize());
     PistSx p{e0, etol, ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, pc, qmrtol, 0};
     return pist_sx_run(c, p, x0, eig, ret, res, stats, c->stream);
@@ -1158,7 +1159,7 @@ int scalit_pist_conv_cx(scalit_ctx *c, double e0, double etol, int ntype, const 
     if (pc != SCALIT_TMAP && pc != SCALIT_TA1) return SCALIT_EINVAL;
     HIP_TRY(hipSetDevice(c->device));
     if (!c->has_res || !c->has_precond || !c->has_ap[SCALIT_AP]) return SCALIT_ESTATE;
-    if (c->osb == SCALIT_TOSBW) return SCALIT_EINVAL;
+    if (c->osb == SCALIT_TOSBW && c->osbw_n <= 0) return SCALIT_ESTATE;   // TOSBW needs its window
     HIP_TRY(hipDeviceSynchronize());
     PistSx p{e0, etol, ntype, startcnt, stepcnt, stepeig, m, m_max, qmr_type, qmrmax, pc, qmrtol, 1};
     return pist_sx_run(c, p, x0, eig, ret, res, stats, c->stream);
@@ -1170,7 +1171,7 @@ int scalit_crp(scalit_ctx *c, int ncrp, const double *energies, int nm, const do
     if (!c || ncrp < 1 || !energies || nm < 1 || !x0 || !crp) return SCALIT_EINVAL;
     HIP_TRY(hipSetDevice(c->device));
     if (!c->has_res || !c->has_precond || !c->has_ap[SCALIT_APP] || !c->has_ap[SCALIT_APR]) return SCALIT_ESTATE;
-    if (c->osb == SCALIT_TOSBW) return SCALIT_EINVAL;
+    if (c->osb == SCALIT_TOSBW && c->osbw_n <= 0) return SCALIT_ESTATE;   // TOSBW needs its window
     HIP_TRY(hipDeviceSynchronize());
     hipStream_t s = c->stream;
     CxState &X = cxstate(c);
@@ -1185,6 +1186,7 @@ int scalit_crp(scalit_ctx *c, int ncrp, const double *energies, int nm, const do
                        (const double *)c->ap[SCALIT_APR].p, c->ap[SCALIT_AP].p, sq.p);
     HIP_TRY(hipGetLastError());
     c->has_ap[SCALIT_AP] = true;
+    c->ap_ver++;
     const double e_keep = c->e0;
     CrpArgs a{nm, qmr_type, qmrmax, sap ? 1 : 0, qmrtol};
     int its = 0;

@@ -643,10 +643,47 @@ __global__ void k_cx_div(long N, int nvec, const double *__restrict__ eig0, doub
     }
 }
 
+// TOSBW of a complex P*X: the window's coefficients of both planes before
+// the divide (g, nvec x n complex) ...
+__global__ void k_cxw_gather(long N, int n, const int *__restrict__ ind, const double *__restrict__ z,
+                             double *__restrict__ g)
+{
+    const int v = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double *zv = z + 2L * v * N;
+    g[2 * ((long)v * n + k)] = zv[ind[k]];
+    g[2 * ((long)v * n + k) + 1] = zv[N + ind[k]];
+}
+
+// ... and after it, M g in their place (M = HWR^-1, n x n complex)
+__global__ void k_cxw_apply(long N, int n, const int *__restrict__ ind, const dcx *__restrict__ m,
+                            const double *__restrict__ g, double *__restrict__ z)
+{
+    const int v = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double *gv = g + 2L * v * n;
+    double r = 0.0, i = 0.0;
+    for (int j = 0; j < n; j++) {
+        const dcx a = m[k + (long)n * j];
+        const double br = gv[2 * j], bi = gv[2 * j + 1];
+        r += a.re * br - a.im * bi;
+        i += a.re * bi + a.im * br;
+    }
+    double *zv = z + 2L * v * N;
+    zv[ind[k]] = r;
+    zv[N + ind[k]] = i;
+}
+
 int px_apply_cx(scalit_ctx *c, int nosb, int npc, int nvec, const double *x, double *y, hipStream_t s)
 {
     if (!c->has_precond) return SCALIT_ESTATE;
-    if (nosb != SCALIT_TOSB && nosb != SCALIT_TOSBD1 && nosb != SCALIT_TOSBD2) return SCALIT_EINVAL;
+    const bool win = nosb == SCALIT_TOSBW;
+    if (nosb != SCALIT_TOSB && nosb != SCALIT_TOSBD1 && nosb != SCALIT_TOSBD2 && !win) return SCALIT_EINVAL;
+    // TOSBW divides by EIG0 - E0 even for TAAP0 / TMAP0 while its window
+    // matrix takes no E0 there (osb.mypxdx.h:153-160, osb.pij.h:95-105):
+    // those two are refused
+    if (win && (npc == SCALIT_TAAP0 || npc == SCALIT_TMAP0)) return SCALIT_EINVAL;
+    if (win && c->osbw_n <= 0) return SCALIT_ESTATE;
     int rc = SCALIT_OK;
     double sign = 0.0;
     const double *a = cx_imag_diag(c, npc, &sign, &rc);
@@ -662,6 +699,13 @@ int px_apply_cx(scalit_ctx *c, int nosb, int npc, int nvec, const double *x, dou
     }
     SC_TRY(ensure_transposes<double>(c, &Level::vosb, &Level::vosbt, s, true));
     const double e0 = (npc == SCALIT_TA0 || npc == SCALIT_TAAP0 || npc == SCALIT_TMAP0) ? 0.0 : c->e0;
+    const dcx *mw = nullptr;
+    if (win) {
+        SC_TRY(osbw_cx_matrix(c, npc, e0, a, sign, &mw, s));
+        double *old = c->osbw_g.p;
+        SC_TRY(c->osbw_g.alloc(2 * (size_t)c->osbw_n * nvec));
+        if (c->osbw_g.p != old) c->gen++;
+    }
     double *w0 = c->work.p, *w1 = w0 + (long)nr * N, *zt = w1 + (long)nr * N;
     // V^T for levels sF..2 (1-based) on the real and imaginary planes
     const double *cur = x;
@@ -680,8 +724,15 @@ int px_apply_cx(scalit_ctx *c, int nosb, int npc, int nvec, const double *x, dou
     hipLaunchKernelGGL((k_blockrot<double, double, double, false>), g, dim3(256), lds, s, n, nb, nr, L1.vosbt.p,
                        c->eig0.p, (const double *)nullptr, 0.0, SCALIT_TOSB, 0.0, 0.0, cur, N, zt, N,
                        (const int *)nullptr);
+    const dim3 gw((c->osbw_n + 255) / 256, nvec);
+    if (win)
+        hipLaunchKernelGGL(k_cxw_gather, gw, dim3(256), 0, s, N, c->osbw_n, c->osbw_ind.p, (const double *)zt,
+                           c->osbw_g.p);
     hipLaunchKernelGGL(k_cx_div, dim3((unsigned)std::min<long>((N + 255) / 256, 2048), nvec), dim3(256), 0, s, N,
-                       nvec, c->eig0.p, e0, nosb, std::fabs(c->de), c->beta, a, sign, zt);
+                       nvec, c->eig0.p, e0, win ? SCALIT_TOSB : nosb, std::fabs(c->de), c->beta, a, sign, zt);
+    if (win)
+        hipLaunchKernelGGL(k_cxw_apply, gw, dim3(256), 0, s, N, c->osbw_n, c->osbw_ind.p, mw,
+                           (const double *)c->osbw_g.p, zt);
     double *out1 = sF == 1 ? y : dst;
     hipLaunchKernelGGL((k_blockrot<double, double, double, false>), g, dim3(256), lds, s, n, nb, nr, L1.vosb.p,
                        c->eig0.p, (const double *)nullptr, 0.0, SCALIT_TOSB, 0.0, 0.0, (const double *)zt, N, out1,

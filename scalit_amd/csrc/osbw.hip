@@ -23,6 +23,8 @@
 
 #include "scalit_impl.hpp"
 
+#include <complex>
+
 namespace scalit {
 
 struct OsbwLevels {
@@ -181,6 +183,97 @@ void osbw_scatter(scalit_ctx *c, int nvec, double *y, long ldy, const int *activ
 
 using namespace scalit;
 
+namespace scalit {
+
+// a[ind[k]] for the window's states
+__global__ void k_osbw_take(int n, const int *__restrict__ ind, const double *__restrict__ a, double *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = a[ind[k]];
+}
+
+// The window matrix of a complex job (createWH_DXCX, src/iterate/module/
+// osb.pij.h:65-166, for a real H0): HWR = H_W + diag(-e0 + i sign A(WIND)),
+// factorised there by ZGETRF (ZHETRF without an absorbing term) and applied
+// by ZGETRS / ZHETRS in MYPX_DX's TOSBW case (osb.mypxdx.h:127-170).  Here
+// its inverse is formed once per (nPC, e0, absorbing potentials, window) by
+// Gauss-Jordan elimination with partial pivoting on the host (n <= 1024, as
+// LAPACK runs on the host in the reference) and kept on the device, so that
+// P*X applies it as one matrix-vector product.  A cache miss cannot be served
+// under stream capture: the complex solvers' first P*X (eager) fills it.
+int osbw_cx_matrix(scalit_ctx *c, int npc, double e0, const double *a, double sign, const dcx **m, hipStream_t s)
+{
+    const int n = c->osbw_n;
+    if (n <= 0 || (int)c->osbw_ind_host.size() != n) return SCALIT_ESTATE;
+    for (auto &S : c->osbw_cx)
+        if (S.m.p && S.npc == npc && S.e0 == e0 && S.apv == c->ap_ver && S.wv == c->osbw_ver) {
+            *m = (const dcx *)S.m.p;
+            return SCALIT_OK;
+        }
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(s, &st));
+    if (st != hipStreamCaptureStatusNone) return SCALIT_ESTATE;
+    int slot = -1;
+    for (int k = 0; k < 4 && slot < 0; k++)
+        if (c->osbw_cx[k].npc == npc) slot = k;
+    if (slot < 0) slot = (c->osbw_cx_next++) & 3;
+    auto &S = c->osbw_cx[slot];
+    std::vector<double> aw(n, 0.0);
+    if (a) {
+        DBuf<double> t;
+        SC_TRY(t.alloc(n));
+        hipLaunchKernelGGL(k_osbw_take, dim3((n + 255) / 256), dim3(256), 0, s, n, c->osbw_ind.p, a, t.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(aw.data(), t.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    using cd = std::complex<double>;
+    std::vector<cd> A((size_t)n * n), M((size_t)n * n, cd(0.0, 0.0));
+    for (size_t e = 0; e < A.size(); e++) A[e] = c->osbw_hw_host[e];
+    for (int i = 0; i < n; i++) {
+        A[i + (size_t)n * i] += cd(-e0, sign * aw[i]);
+        M[i + (size_t)n * i] = 1.0;
+    }
+    // Gauss-Jordan: rows of [A | M] reduced to [I | A^-1]
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int r = k + 1; r < n; r++)
+            if (std::abs(A[r + (size_t)n * k]) > std::abs(A[p + (size_t)n * k])) p = r;
+        if (A[p + (size_t)n * k] == cd(0.0, 0.0)) return SCALIT_ESTATE;   // singular (ZGETRF's INFO > 0)
+        if (p != k)
+            for (int j = 0; j < n; j++) {
+                std::swap(A[k + (size_t)n * j], A[p + (size_t)n * j]);
+                std::swap(M[k + (size_t)n * j], M[p + (size_t)n * j]);
+            }
+        const cd inv = 1.0 / A[k + (size_t)n * k];
+        for (int j = 0; j < n; j++) {
+            A[k + (size_t)n * j] *= inv;
+            M[k + (size_t)n * j] *= inv;
+        }
+        for (int r = 0; r < n; r++) {
+            if (r == k) continue;
+            const cd f = A[r + (size_t)n * k];
+            if (f == cd(0.0, 0.0)) continue;
+            for (int j = 0; j < n; j++) {
+                A[r + (size_t)n * j] -= f * A[k + (size_t)n * j];
+                M[r + (size_t)n * j] -= f * M[k + (size_t)n * j];
+            }
+        }
+    }
+    const double *old = S.m.p;
+    SC_TRY(S.m.alloc(2 * (size_t)n * n));
+    if (S.m.p != old) c->gen++;
+    HIP_TRY(hipMemcpy(S.m.p, M.data(), sizeof(cd) * M.size(), hipMemcpyHostToDevice));
+    S.npc = npc;
+    S.e0 = e0;
+    S.apv = c->ap_ver;
+    S.wv = c->osbw_ver;
+    *m = (const dcx *)S.m.p;
+    return SCALIT_OK;
+}
+
+}  // namespace scalit
+
 extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int cnt, int *hwlen, double *de_out)
 {
     if (!c || cnt < 1 || cnt > 1024 || !(de > 0.0) || (npc != SCALIT_TA0 && npc != SCALIT_TA1)) return SCALIT_EINVAL;
@@ -246,6 +339,13 @@ extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int
     SC_TRY(z.alloc((size_t)n * n));
     SC_TRY(work.alloc(small_sym_eig_work(n)));
     SC_TRY(small_sym_eig(n, hw.p, n, w.p, z.p, n, work.p, s));
+    // the unshifted H_W and the indices on the host, for the complex jobs'
+    // window matrices (osbw_cx_matrix)
+    c->osbw_hw_host.assign((size_t)n * n, 0.0);
+    HIP_TRY(hipMemcpyAsync(c->osbw_hw_host.data(), hw.p, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++) c->osbw_hw_host[i + (size_t)n * i] += shift;
+    c->osbw_ind_host = ind;
     SC_TRY(c->osbw_m.alloc((size_t)n * n));
     hipLaunchKernelGGL(k_osbw_inverse, dim3((unsigned)(((long)n * n + 255) / 256)), dim3(256), 0, s, n, z.p, w.p,
                        c->osbw_m.p);
@@ -266,6 +366,7 @@ extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int
     c->osbw_n = n;
     c->osbw_nruns = (int)runs.size() - 1;
     c->osbw_shift = shift;
+    c->osbw_ver++;
     c->gen++;
     if (hwlen) *hwlen = n;
     if (de_out) *de_out = de;

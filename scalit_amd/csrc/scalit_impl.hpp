@@ -153,6 +153,23 @@ struct scalit_ctx {
     // (osbw_d, nvec x osbw_n); osbw_n = 0 when no window is set up
     scalit::DBuf<int> osbw_ind, osbw_runs;
     scalit::DBuf<double> osbw_m, osbw_eig0, osbw_d;
+    // complex jobs with TOSBW (real H0): the window's H_W (EIG0 on the
+    // diagonal, unshifted; n x n column-major) and indices on the host, and
+    // per preconditioner type the inverse of H_W + diag(-E0 +- i A_W)
+    // (n x n complex), rebuilt when E0, the absorbing potentials or the
+    // window change; osbw_g holds a complex P*X's gathered window values
+    struct OsbwCx {
+        int npc = -1;
+        double e0 = 0.0;
+        long apv = -1, wv = -1;
+        scalit::DBuf<double> m;
+    };
+    std::vector<double> osbw_hw_host;
+    std::vector<int> osbw_ind_host;
+    OsbwCx osbw_cx[4];
+    int osbw_cx_next = 0;
+    scalit::DBuf<double> osbw_g;
+    long ap_ver = 0, osbw_ver = 0;
     int osbw_n = 0, osbw_nruns = 0;
     double osbw_shift = 0.0;
     // the absorbing potentials AP, APP, APR of resonance and CRP jobs (N
@@ -215,6 +232,7 @@ void osbw_gather(scalit_ctx *c, int nvec, const double *x, long ldx, const int *
 // (nullptr when none, SCALIT_ESTATE in *rc when that array is not set).
 int px_apply_cx(scalit_ctx *c, int nosb, int npc, int nvec, const double *x, double *y, hipStream_t s);
 int hx_apply_cxh(scalit_ctx *c, int ntype, int nvec, const double *x, double *y, hipStream_t s);
+int osbw_cx_matrix(scalit_ctx *c, int npc, double e0, const double *a, double sign, const dcx **m, hipStream_t s);
 bool real_levels_ready(const scalit_ctx *c);
 const double *cx_imag_diag(const scalit_ctx *c, int ntype, double *sign, int *rc);
 void release_complex(scalit_ctx *c);

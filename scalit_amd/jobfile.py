@@ -211,8 +211,6 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
     nosb = {TOSB: L.TOSB, TOSBD1: L.TOSBD1, TOSBD2: L.TOSBD2, TOSBW: L.TOSBW}.get(job.sOSB)
     if nosb is None:
         raise ValueError(f"sOSB = {job.sOSB} is not a preconditioner kernel (1..4)")
-    if nosb == L.TOSBW and job.sJOB != JOB_BOUND:
-        raise ValueError("the windowed preconditioner (sOSB = 4) is on the bound-state path only")
     if job.sCX and nosb == L.TOSBW:
         raise ValueError("the windowed preconditioner (sOSB = 4) is not built for complex level matrices (sCX)")
     load = Context.from_files_cx if job.sCX else Context.from_files
@@ -228,12 +226,23 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
             ctx.save_precond(path("VOSB"), path("EIG"), mode)
     m0 = job.conv_num
     rng = np.random.default_rng(seed)
+
+    def window():
+        # initOSBW / initMOSBW with the job's OSBW line before the solver (every
+        # OSB_PISTCONV* and osbCRP does); a window of no states fails the job
+        if job.osbw_cnt < 1:
+            return False
+        ctx.set_osbw(job.osbw_e0, job.osbw_de, job.osbw_cnt, L.TA1)
+        return True
     if job.sJOB in (JOB_RES1, JOB_RES2):
         # osb.io.h:20-23: APP from its file, APR = 0, AP = APP
         app = Context.load_file(path("APP"), ctx.N, "dir")
         ctx.set_ap(L.AP, app)
         ctx.set_ap(L.APP, app)
         ctx.set_ap(L.APR, np.zeros(ctx.N))
+        if nosb == L.TOSBW and not window():
+            ctx.close()
+            return 0, np.zeros(m0, complex), 0.0
         ctx.set_osb(nosb)
         e0, etol, start, step, stepeig, nmax = pist_params(job, m0)
         x0 = rng.random(ctx.N) + 1j * rng.random(ctx.N)
@@ -246,6 +255,9 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
     if job.sJOB in (JOB_CRP, JOB_CRP1, JOB_CRP2):
         ctx.set_ap(L.APP, Context.load_file(path("APP"), ctx.N, "dir"))
         ctx.set_ap(L.APR, Context.load_file(path("APR"), ctx.N, "dir"))
+        if nosb == L.TOSBW and not window():
+            ctx.close()
+            return 0, np.zeros(m0), 0.0
         ctx.set_osb(nosb)
         energies = [job.conv_e0 + k * job.conv_tol for k in range(m0)]   # osbw.progEig.h:66-68
         x0 = rng.random((m0, ctx.N)) + 1j * rng.random((m0, ctx.N))
@@ -269,10 +281,9 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
     if nosb == L.TOSBW:
         # OSB_PISTCONV (osbw.pistconv.h:26-31): sPC = TA1, then initOSBW with
         # the job's window; a window of no states fails the job (returns 0)
-        if job.osbw_cnt < 1:
+        if not window():
             ctx.close()
             return 0, np.zeros(m0), 0.0
-        ctx.set_osbw(job.osbw_e0, job.osbw_de, job.osbw_cnt, L.TA1)
     ctx.set_osb(nosb)
     e0, etol, start, step, stepeig, nmax = pist_params(job, m0)
     x0 = rng.random((1, ctx.N))

@@ -321,3 +321,94 @@ def test_pist_conv_sx_complex_levels(hermitian):
     else:
         assert r == r_o, (r, r_o)
     ctx.close()
+
+
+# ---------------------------------------------------------------- TOSBW in complex jobs
+@pytest.mark.parametrize("npc", [L.TA0, L.TA1, L.TMAP, L.TAAP, L.TMAPR])
+def test_px_cx_tosbw_matches_oracle(npc):
+    """MYPX_DX's TOSBW case (window of 12 OSB states around E0, real H0) with
+    createWH_DXCX's window matrix H_W + diag(-E0 +- i A_W), against the
+    oracle (1e-10 of the result, as the real TOSBW); TAAP0 / TMAP0 are
+    refused (the reference's divide and window disagree on E0 there)."""
+    print(f"px_cx tosbw npc={npc}", flush=True)
+    prob, op, ctx, aps, e0 = cx_setup((12, 10, 8))
+    n, de = ctx.set_osbw(e0, 1e-3, 12, L.TA1)
+    w = O.osbw_setup(op, e0, 1e-3, 12, O.TA0)
+    assert n == 12 and de == w[1]
+    x = rand_cx(prob.N, 2, 41)
+    y = ctx.from_device_cx(ctx.px_cx(ctx.to_device_cx(x), L.TOSBW, npc))
+    for v in range(2):
+        ref = O.px_dx(op, x[v], O.TOSBW, npc, aps, w)
+        assert np.abs(y[v] - ref).max() <= 1e-10 * np.abs(ref).max()
+    with pytest.raises(L.ScalitError):
+        ctx.px_cx(ctx.to_device_cx(x), L.TOSBW, L.TAAP0)
+    ctx.close()
+
+
+def test_qmr_and_pist_sx_with_tosbw():
+    """QMR_CX and PIST_CONV_SX with the TOSBW preconditioner (sOSB = TOSBW,
+    initOSBW before the job as OSB_PISTCONVDX1 does): QMR within 1 iteration
+    of the oracle's with the same window and the dense system solved to
+    1e-8; the 4 resonance energies nearest E0 on 8 x 9 x 6 against a dense
+    eigensolve (1e-10)."""
+    print("qmr/pist tosbw", flush=True)
+    prob, op, ctx, aps, e0 = cx_setup((12, 10, 8))
+    ctx.set_osbw(e0, 1e-3, 16, L.TA1)
+    w = O.osbw_setup(op, e0, 1e-3, 16, O.TA0)
+    ctx.set_osb(L.TOSBW)
+    b = rand_cx(prob.N, 1, 43)
+    r_o, x_o, _ = O.qmr_cx(lambda v: O.hx_dx(op, v, O.TMAP, aps),
+                           lambda v: O.px_dx(op, v, O.TOSBW, O.TMAP, aps, w), b[0], -1, 800, 1e-9)
+    r, x, res = ctx.qmr_cx(ctx.to_device_cx(b), -1, L.TMAP, L.TMAP, 800, 1e-9)
+    print("qmr tosbw", r, r_o, flush=True)
+    assert r_o > 0 and r > 0 and abs(r - r_o) <= 1
+    x = ctx.from_device_cx(x)[0]
+    dense = op.dense() - e0 * np.eye(prob.N) - 1j * np.diag(aps[0])
+    assert np.linalg.norm(dense @ x - b[0]) <= 1e-8 * np.linalg.norm(b[0])
+    ctx.close()
+    prob, op, ctx, aps, _ = cx_setup((8, 9, 6), 0.002)
+    ev = np.linalg.eigvals(op.dense() - 1j * np.diag(aps[0]))
+    srt = np.sort(ev.real)
+    e0 = float(0.5 * (srt[40] + srt[41]))
+    ctx.set_shift(e0)
+    near = ev[np.argsort(np.abs(ev.real - e0), kind="stable")[:4]]
+    near = near[np.argsort(near.real)]
+    ctx.set_osbw(e0, 1e-3, 10, L.TA1)
+    ctx.set_osb(L.TOSBW)
+    x = np.atleast_2d(P.random_vectors(prob.N, 1, 3)[0] + 0j)
+    r, eig, res, stats = ctx.pist_conv_sx(ctx.to_device_cx(x), e0, 1e-12, 4, 160, 16, 4, 2, qmr_type=-1,
+                                          qmrmax=500, qmrtol=1e-10)
+    print("pist_sx tosbw", r, eig, res, stats, flush=True)
+    assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
+    ctx.close()
+
+
+def test_crp_with_tosbw_matches_oracle():
+    """osbCRP with sOSB = TOSBW on 6 x 7 x 5: the window (10 states around
+    the set-up's E0) is kept while E0 moves to each energy, and each QMR
+    solve's window matrix takes that energy (createWH_DXCX before every
+    OSB_QMRDX); N(E) at three energies against the oracle (1e-8)."""
+    print("crp tosbw", flush=True)
+    prob = P.build_problem(6, 7, 5, P.gaussian_potential(3), seed=3)
+    op = O.Operator(prob.sN, prob.h0, prob.res, prob.dep)
+    vosb, eig0 = op.diag()
+    ctx = Context.from_problem(prob, e0=0.0)
+    ctx.set_precond(vosb, eig0)
+    app = P.absorbing_potential(prob, 0.01, 0.6, coord="r")
+    apr = P.absorbing_potential(prob, 0.01, 0.6, coord="R")
+    ctx.set_ap(L.APP, app)
+    ctx.set_ap(L.APR, apr)
+    h = np.sort(np.linalg.eigvalsh(op.dense()))
+    energies = [float(h[10] + 1e-4), float(h[25] + 1e-4), float(h[40] + 1e-4)]
+    ew = float(0.5 * (h[25] + h[26]))
+    op.set_shift(ew)
+    ctx.set_shift(ew)
+    ctx.set_osbw(ew, 1e-3, 10, L.TA1)
+    w = O.osbw_setup(op, ew, 1e-3, 10, O.TA0)
+    ctx.set_osb(L.TOSBW)
+    x0 = np.stack([P.random_vectors(prob.N, 1, 40 + k)[0] + 0j for k in range(3)])
+    crp, stats = ctx.crp(energies, 12, ctx.to_device_cx(x0), qmr_type=-1, qmrmax=500, qmrtol=1e-12)
+    ref = O.crp_dx(op, app, apr, energies, 12, list(x0), -1, 500, 1e-12, True, kind=O.TOSBW, win=w)
+    print("crp tosbw", crp, ref, stats, flush=True)
+    assert np.abs(crp - ref).max() <= 1e-8 * np.abs(ref).max() and (ref > 0).all()
+    ctx.close()

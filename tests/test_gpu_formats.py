@@ -148,10 +148,10 @@ F F T
 
 
 def test_resonance_and_crp_jobs_from_input_files(tmp_path):
-    """A resonance job (sJOB = 2, APP file) and a CRP job (sJOB = 4, APP and
-    APR files) of 8 x 9 x 6 from iterate input files: the 4 resonance energies
-    whose real parts are nearest the job's E0 against a dense eigensolve of
-    H - i APP (1e-10), and N(E) at 3 energies from 10 Lanczos steps against
+    """A resonance job (sJOB = 2, APP file; also with sOSB = TOSBW) and a CRP
+    job (sJOB = 4, APP and APR files) of 8 x 9 x 6 from iterate input files:
+    the 4 resonance energies whose real parts are nearest the job's E0
+    against a dense eigensolve of H - i APP (1e-10), and N(E) at 3 energies from 10 Lanczos steps against
     the oracle's osbCRP with the same start vectors (1e-8)."""
     from scalit_amd import jobfile as J
 
@@ -177,6 +177,12 @@ def test_resonance_and_crp_jobs_from_input_files(tmp_path):
     job.sJOB = J.JOB_RES2   # the Hermitian-product PIST of OSB_PISTCONVDX2
     r, eig, res = J.run_pist_job(job, base_dir=str(tmp_path))
     print("resonance job (sJOB = 3)", r, eig, res, flush=True)
+    assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
+    # sOSB = TOSBW: the window of the OSBW line (10 states around E0) set up
+    # before the PIST, its matrix taking -E0 - i APP on the diagonal
+    job.sJOB, job.sOSB, job.osbw_e0, job.osbw_de, job.osbw_cnt = J.JOB_RES1, J.TOSBW, e0, 1.0e-3, 10
+    r, eig, res = J.run_pist_job(job, base_dir=str(tmp_path))
+    print("resonance job with TOSBW", r, eig, res, flush=True)
     assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
     hs = np.sort(np.linalg.eigvalsh(h))
     e_start, e_step = float(hs[20] + 1e-4), 2e-4

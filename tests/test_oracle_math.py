@@ -547,3 +547,30 @@ def test_hosb_diag_dx_is_a_similarity(shape):
             off0 = np.sum(np.abs(m - np.diag(np.diag(m))) ** 2)
             off1 = np.sum(np.abs(r - np.diag(np.diag(r))) ** 2)
             assert off1 < off0
+
+
+@pytest.mark.parametrize("npc", [O.TA0, O.TA1, O.TMAP, O.TAAP, O.TMAPR])
+def test_px_tosbw_dx_window_and_complement(npc):
+    """MYPX_DX with TOSBW restated (complex jobs, real H0): on the window's
+    span it inverts HWR = H_W + diag(-E0 +- i A_W) (P V_W HWR a = V_W a, a
+    complex), on the window's orthogonal complement it is MYPX_DX's TOSB, and
+    without an absorbing term it is the real TOSBW P*X on both planes."""
+    prob, op, aps, _, e0 = _cx_problem()
+    w = O.osbw_setup(op, e0, 1e-3, 12, O.TA0)
+    ind, _, hw, _, vw = w
+    g = P.rng(9)
+    a = g.standard_normal(len(ind)) + 1j * g.standard_normal(len(ind))
+    sh = 0.0 if npc == O.TA0 else e0
+    ad = O._imag_diag(npc, aps)
+    hwr = hw + np.diag(np.full(len(ind), -sh) + (0j if ad is None else 1j * ad[ind]))
+    y = O.px_dx(op, vw @ (hwr @ a), O.TOSBW, npc, aps, w)
+    assert np.abs(y - vw @ a).max() <= 1e-10 * np.abs(vw @ a).max()
+    x = P.random_vectors(op.N, 1, 8)[0] + 1j * P.random_vectors(op.N, 1, 9)[0]
+    xc = x - vw @ (vw.T @ x)
+    t = O.px_dx(op, xc, O.TOSB, npc, aps)
+    assert np.abs(O.px_dx(op, xc, O.TOSBW, npc, aps, w) - t).max() <= 1e-12 * np.abs(t).max()
+    if ad is None:
+        wr = O.osbw_setup(op, e0, 1e-3, 12, npc)
+        ref = O.px_tosbw(op, x.real, wr, npc) + 1j * O.px_tosbw(op, x.imag, wr, npc)
+        got = O.px_dx(op, x, O.TOSBW, npc, aps, w)
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
