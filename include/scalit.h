@@ -300,7 +300,10 @@ int scalit_small_eig_cx(int n, const double *a, int lda, double *w);
  * (src/iterate/libiterate/diag/hosbdiag_dx.f90:6-73) with MJACOBI_CX
  * (jacobi_cx.f90:5-96), real VOSB, EIG0 the real parts of the rotated
  * diagonal.  scalit_set_scx(ctx, 1) returns SCALIT_ESTATE until every level
- * has its complex data; TOSBW is refused while it is on. */
+ * has its complex data.  scalit_set_osbw then forms the complex window
+ * Hamiltonian of getVOSBH0_CX (src/iterate/module/osb.hijcx.h:135-164:
+ * v_i . MYHX_CX(TA0) v_j above the diagonal, its conjugate below, EIG0 on
+ * it) for the complex solvers' TOSBW. */
 int scalit_set_h0_cx(scalit_ctx *ctx, int level, const double *h0_re, const double *h0_im);
 int scalit_set_dep_cx(scalit_ctx *ctx, int level, const double *dep_re, const double *dep_im);
 int scalit_set_outh_cx(scalit_ctx *ctx, const double *outh_re, const double *outh_im);

@@ -615,6 +615,21 @@ def osbw_setup(op: "Operator", e0: float, de: float, cnt: int, npc: int = TA1):
     return ind, de, hw, hwr, vw
 
 
+def osbw_setup_cx(op, e0: float, de: float, cnt: int):
+    """initOSBW for complex level matrices (an OperatorCX after diag()):
+    adjustWinSize and getOSBWIndex as osbw_setup, and getVOSBH0_CX
+    (src/iterate/module/osb.hijcx.h:135-164): H_W(i,j) = v_i . MYHX_CX(TA0)
+    v_j for i < j, its conjugate below the diagonal, EIG0 on it.  Returns
+    (indices, width, H_W (complex), None, window vectors)."""
+    de = adjust_win_size(op.EIG0, e0, cnt, de)
+    ind = osbw_index(op.EIG0, e0, de, cnt)
+    vw = np.stack([osb_vector(op, int(k)) for k in ind], axis=1)
+    hv = np.stack([op.hx(vw[:, j].astype(np.complex128), TA0) for j in range(cnt)], axis=1)
+    up = np.triu(vw.T @ hv, 1)
+    hw = up + np.conj(up).T + np.diag(op.EIG0[ind]).astype(np.complex128)
+    return ind, de, hw, None, vw
+
+
 def px_tosbw(op: "Operator", x, win, npc: int = TA1) -> np.ndarray:
     """MYPX with nOSB = TOSBW (src/iterate/module/osb.px.h:191-206): the
     rotated vector divided by EIG0 - E0 except on the window, whose

@@ -193,7 +193,8 @@ __global__ void k_osbw_take(int n, const int *__restrict__ ind, const double *__
 }
 
 // The window matrix of a complex job (createWH_DXCX, src/iterate/module/
-// osb.pij.h:65-166, for a real H0): HWR = H_W + diag(-e0 + i sign A(WIND)),
+// osb.pij.h:65-166; H_W complex for complex level matrices, HWCX):
+// HWR = H_W + diag(-e0 + i sign A(WIND)),
 // factorised there by ZGETRF (ZHETRF without an absorbing term) and applied
 // by ZGETRS / ZHETRS in MYPX_DX's TOSBW case (osb.mypxdx.h:127-170).  Here
 // its inverse is formed once per (nPC, e0, absorbing potentials, window) by
@@ -229,7 +230,8 @@ int osbw_cx_matrix(scalit_ctx *c, int npc, double e0, const double *a, double si
     }
     using cd = std::complex<double>;
     std::vector<cd> A((size_t)n * n), M((size_t)n * n, cd(0.0, 0.0));
-    for (size_t e = 0; e < A.size(); e++) A[e] = c->osbw_hw_host[e];
+    const bool cxw = !c->osbw_hwi_host.empty();   // complex level matrices: a complex H_W
+    for (size_t e = 0; e < A.size(); e++) A[e] = cd(c->osbw_hw_host[e], cxw ? c->osbw_hwi_host[e] : 0.0);
     for (int i = 0; i < n; i++) {
         A[i + (size_t)n * i] += cd(-e0, sign * aw[i]);
         M[i + (size_t)n * i] = 1.0;
@@ -279,7 +281,6 @@ extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int
     if (!c || cnt < 1 || cnt > 1024 || !(de > 0.0) || (npc != SCALIT_TA0 && npc != SCALIT_TA1)) return SCALIT_EINVAL;
     if (!c->has_precond || !c->has_res) return SCALIT_ESTATE;
     if (c->ndvr) return SCALIT_EINVAL;   // the level-1 rotation of an sNDVR operator is the outer layer
-    if (c->scx) return SCALIT_EINVAL;    // the window of complex level matrices is complex (not built)
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const long N = c->N;
@@ -328,13 +329,41 @@ extern "C" int scalit_set_osbw(scalit_ctx *c, int npc, double e0, double de, int
     hipLaunchKernelGGL(k_osbw_basis, dim3((unsigned)((N + 255) / 256), n), dim3(256), 0, s, N, n, L,
                        c->osbw_ind.p, xv.p, ld);
     HIP_TRY(hipGetLastError());
-    SC_TRY(hx_apply_f64(c, nullptr, 0.0, n, xv.p, ld, hv.p, ld, false, nullptr, s));   // HijX: MYHX(TA0)
     SC_TRY(hw.alloc((size_t)n * n));
     // the diagonal is EIG0 (getVOSBH0), less the shift (createWH: TA1)
     const double shift = npc == SCALIT_TA0 ? 0.0 : e0;
-    hipLaunchKernelGGL(k_osbw_gram, dim3(n, n), dim3(256), 0, s, N, n, xv.p, ld, hv.p, ld, c->eig0.p,
-                       c->osbw_ind.p, shift, hw.p);
-    HIP_TRY(hipGetLastError());
+    c->osbw_hwi_host.clear();
+    if (c->scx) {
+        // getVOSBH0_CX (src/iterate/module/osb.hijcx.h:135-164): H_W(i,j) =
+        // v_i . MYHX_CX(TA0) v_j above the diagonal, its conjugate below,
+        // EIG0 on it: the real and imaginary planes' products, the lower
+        // imaginary half negated on the host
+        DBuf<double> xc, hc, hwi;
+        SC_TRY(xc.alloc(2 * (size_t)n * N));
+        SC_TRY(hc.alloc(2 * (size_t)n * N));
+        SC_TRY(hwi.alloc((size_t)n * n));
+        HIP_TRY(hipMemsetAsync(xc.p, 0, sizeof(double) * 2 * (size_t)n * N, s));
+        HIP_TRY(hipMemcpy2DAsync(xc.p, sizeof(double) * 2 * N, xv.p, sizeof(double) * ld, sizeof(double) * N, n,
+                                 hipMemcpyDeviceToDevice, s));
+        SC_TRY(hx_apply_cxh(c, SCALIT_TA0, n, xc.p, hc.p, s));
+        hipLaunchKernelGGL(k_osbw_gram, dim3(n, n), dim3(256), 0, s, N, n, xv.p, ld, (const double *)hc.p, 2 * N,
+                           c->eig0.p, c->osbw_ind.p, shift, hw.p);
+        hipLaunchKernelGGL(k_osbw_gram, dim3(n, n), dim3(256), 0, s, N, n, xv.p, ld, (const double *)hc.p + N,
+                           2 * N, c->eig0.p, c->osbw_ind.p, 0.0, hwi.p);
+        HIP_TRY(hipGetLastError());
+        c->osbw_hwi_host.assign((size_t)n * n, 0.0);
+        HIP_TRY(hipMemcpyAsync(c->osbw_hwi_host.data(), hwi.p, sizeof(double) * (size_t)n * n,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int j = 0; j < n; j++)
+            for (int i = j; i < n; i++)
+                c->osbw_hwi_host[i + (size_t)n * j] = i == j ? 0.0 : -c->osbw_hwi_host[j + (size_t)n * i];
+    } else {
+        SC_TRY(hx_apply_f64(c, nullptr, 0.0, n, xv.p, ld, hv.p, ld, false, nullptr, s));   // HijX: MYHX(TA0)
+        hipLaunchKernelGGL(k_osbw_gram, dim3(n, n), dim3(256), 0, s, N, n, xv.p, ld, hv.p, ld, c->eig0.p,
+                           c->osbw_ind.p, shift, hw.p);
+        HIP_TRY(hipGetLastError());
+    }
     SC_TRY(w.alloc(n));
     SC_TRY(z.alloc((size_t)n * n));
     SC_TRY(work.alloc(small_sym_eig_work(n)));

@@ -164,7 +164,7 @@ struct scalit_ctx {
         long apv = -1, wv = -1;
         scalit::DBuf<double> m;
     };
-    std::vector<double> osbw_hw_host;
+    std::vector<double> osbw_hw_host, osbw_hwi_host;   // hwi: imaginary part (sCX), else empty
     std::vector<int> osbw_ind_host;
     OsbwCx osbw_cx[4];
     int osbw_cx_next = 0;

@@ -211,8 +211,6 @@ def run_pist_job(job: IterateJob, base_dir: str = ".", device: int = 0, seed: in
     nosb = {TOSB: L.TOSB, TOSBD1: L.TOSBD1, TOSBD2: L.TOSBD2, TOSBW: L.TOSBW}.get(job.sOSB)
     if nosb is None:
         raise ValueError(f"sOSB = {job.sOSB} is not a preconditioner kernel (1..4)")
-    if job.sCX and nosb == L.TOSBW:
-        raise ValueError("the windowed preconditioner (sOSB = 4) is not built for complex level matrices (sCX)")
     load = Context.from_files_cx if job.sCX else Context.from_files
     ctx = load(job.sN, job.sDEP, h0_file=path("H0"), res_file=path("RES"), dep_file=path("DEP"),
                outh_file=path("OUTH"), device=device)

@@ -235,7 +235,7 @@ def test_block_diag_cx_matches_oracle(grid, ndvr):
     """The sCX set-up on the GPU (HosbDiagDX on the complex HOSB, real
     rotations) against the oracle: the same rotation sequence, so EIG0 to
     1e-9 and VOSB to 1e-7 (rounding of the A, B, C sums only), every VOSB
-    block orthogonal; TOSBW is refused while sCX is on."""
+    block orthogonal."""
     print(f"block_diag sCX {grid} ndvr={ndvr}", flush=True)
     prob, opc, ctx, aps = scx_setup(grid, ndvr)
     vosb_o, eig0_o = opc.diag(1000, 1e-5)
@@ -248,9 +248,6 @@ def test_block_diag_cx_matches_oracle(grid, ndvr):
         v = vosb_g[l].reshape(-1, n, n)
         for b in range(0, v.shape[0], max(1, v.shape[0] // 7)):
             assert np.allclose(v[b] @ v[b].T, np.eye(n), atol=1e-12)
-    if not ndvr:
-        with pytest.raises(L.ScalitError):
-            ctx.set_osbw(float(np.median(eig0_g)), 1e-3, 4)
     ctx.close()
 
 
@@ -411,4 +408,49 @@ def test_crp_with_tosbw_matches_oracle():
     ref = O.crp_dx(op, app, apr, energies, 12, list(x0), -1, 500, 1e-12, True, kind=O.TOSBW, win=w)
     print("crp tosbw", crp, ref, stats, flush=True)
     assert np.abs(crp - ref).max() <= 1e-8 * np.abs(ref).max() and (ref > 0).all()
+    ctx.close()
+
+
+def test_tosbw_with_complex_levels():
+    """TOSBW on complex level matrices: the window Hamiltonian of
+    getVOSBH0_CX (v_i . MYHX_CX v_j above the diagonal, conjugate below, EIG0
+    on it) and MYPX_DX's TOSBW case against the oracle (1e-10) for three
+    nPC; PIST_CONV_SX with it returns the 4 eigenvalues of the complex
+    symmetric H0CX - i AP nearest E0 on 8 x 9 x 6 (1e-10 against a dense
+    eigensolve)."""
+    print("tosbw sCX", flush=True)
+    prob, opc, ctx, aps = scx_setup((12, 10, 8))
+    ctx.block_diag(1000, 1e-5)
+    vosb, eig0 = ctx.get_precond()
+    opc.VOSB, opc.EIG0 = [np.asarray(v) for v in vosb], np.asarray(eig0)
+    srt = np.sort(eig0)
+    e0 = float(0.5 * (srt[100] + srt[101]))
+    ctx.set_shift(e0)
+    opc.set_shift(e0)
+    n, de = ctx.set_osbw(e0, 1e-3, 12, L.TA1)
+    w = O.osbw_setup_cx(opc, e0, 1e-3, 12)
+    assert n == 12 and de == w[1]
+    x = rand_cx(prob.N, 2, 45)
+    for npc in (L.TA1, L.TMAP, L.TAAPR):
+        y = ctx.from_device_cx(ctx.px_cx(ctx.to_device_cx(x), L.TOSBW, npc))
+        for v in range(2):
+            ref = O.px_dx(opc, x[v], O.TOSBW, npc, aps, w)
+            assert np.abs(y[v] - ref).max() <= 1e-10 * np.abs(ref).max(), npc
+    ctx.close()
+    prob, opc, ctx, aps = scx_setup((8, 9, 6))
+    ctx.block_diag(1000, 1e-5)
+    _, eig0 = ctx.get_precond()
+    ev = np.linalg.eigvals(opc.dense(O.TMAP0, aps))
+    srt = np.sort(ev.real)
+    e0 = float(0.5 * (srt[40] + srt[41]))
+    ctx.set_shift(e0)
+    near = ev[np.argsort(np.abs(ev.real - e0), kind="stable")[:4]]
+    near = near[np.argsort(near.real)]
+    ctx.set_osbw(e0, 1e-3, 10, L.TA1)
+    ctx.set_osb(L.TOSBW)
+    xs = np.atleast_2d(P.random_vectors(prob.N, 1, 3)[0] + 0j)
+    r, eig, res, stats = ctx.pist_conv_sx(ctx.to_device_cx(xs), e0, 1e-12, 4, 160, 16, 4, 2, qmr_type=-1,
+                                          qmrmax=500, qmrtol=1e-10)
+    print("pist_sx sCX tosbw", r, eig, near, res, stats, flush=True)
+    assert r > 0 and np.abs(eig - near).max() <= 1e-10 * np.abs(near).max()
     ctx.close()
