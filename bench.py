@@ -123,7 +123,8 @@ def main():
         print(json.dumps({"qmr_multi": qmr_multi_bench()}), flush=True)
         return
     if args.complex_run:
-        print(json.dumps({"absorbing_potential_24x32x24": complex_bench()}), flush=True)
+        print(json.dumps({"absorbing_potential_24x32x24": complex_bench(),
+                          "complex_levels": complex_levels_bench()}), flush=True)
         return
     prob = P.config_problem("hx_bench")
     ctx = Context.from_problem(prob)
@@ -207,6 +208,7 @@ def main():
         out["gradients_64x96x80"] = gradients_bench(ctx, prob)
         out["uniform_windows"] = config4_bench(["batched_" + c for c in args.config4])
         out["absorbing_potential_24x32x24"] = complex_bench()
+        out["complex_levels"] = complex_levels_bench()
         out["qmr_multi"] = qmr_multi_bench()
     ctx.close()
     print(json.dumps(out))
@@ -460,6 +462,79 @@ def complex_bench():
     out["crp"] = {"energies": energies, "crp": [float(v) for v in crp], "seconds": round(dt, 3),
                   "qmr_iters": int(cst[0]), "solves": int(cst[1]),
                   "us_per_qmr_iteration": round(dt * 1e6 / max(1, int(cst[0])), 1)}
+    ctx.close()
+    return out
+
+
+def _scx_context(prob, theta=0.05):
+    from scalit_amd import problems as P
+    from scalit_amd.context import Context
+
+    h0, dep = P.complex_levels(prob, theta, 1e-3, 11)
+    ctx = Context(prob.sN, [d is not None for d in prob.dep])
+    for l in range(3):
+        ctx.set_h0_cx(l + 1, h0[l])
+        if dep[l] is not None:
+            ctx.set_dep_cx(l + 1, dep[l])
+    ctx.set_res(prob.res)
+    ctx.set_scx(True)
+    return ctx
+
+
+def complex_levels_bench():
+    """Complex level matrices (sCX = T, complex-scaled kinetic matrices,
+    theta = 0.05): MYHX_CX of one complex vector on the 64x96x80 grid (HIP
+    events around 20 applications after 3 warm-up ones; bytes counted as the
+    kernels move them: 40 B per point for the diagonal pass, 48 B per point
+    for each level's read of X and read-modify-write of Y), and on 24x32x24
+    the sCX OSB set-up and the 4 resonance energies nearest a level
+    (PIST_CONV_SX, ETOL 1e-12, QMR 1e-10, absorbing potential in R)."""
+    import torch
+
+    from scalit_amd import _lib as L
+    from scalit_amd import problems as P
+
+    out = {}
+    prob = P.config_problem("hx_bench")
+    ctx = _scx_context(prob)
+    x = ctx.to_device_cx(P.random_vectors(prob.N, 1, 3)[0] + 1j * P.random_vectors(prob.N, 1, 4)[0])
+    for _ in range(3):
+        ctx.hx_cx(x, L.TA0)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(20):
+        ctx.hx_cx(x, L.TA0)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / 20
+    n = prob.N
+    nbytes = n * (40 + 3 * 48)
+    flops = n * (8 * sum(prob.sN) + 8 + 6)
+    out["hx_cx_64x96x80"] = {"ms": round(ms, 4), "GB_per_s": round(nbytes / ms / 1e6, 1),
+                             "TFLOP_per_s": round(flops / ms / 1e9, 2), "bytes": nbytes, "flops": flops}
+    ctx.close()
+    prob = P.config_problem("pist_fp64")
+    ctx = _scx_context(prob)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.block_diag(1000, 1e-5)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    _, eig0 = ctx.get_precond()
+    srt = np.sort(eig0)
+    e0 = float(0.5 * (srt[30] + srt[31]))
+    ctx.set_shift(e0)
+    ctx.set_ap(L.AP, P.absorbing_potential(prob, 0.01))
+    x0 = ctx.to_device_cx(P.random_vectors(prob.N, 1, 5)[0] + 0j)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r, eig, res, st = ctx.pist_conv_sx(x0, e0, 1e-12, 4, 160, 20, 5, 2, qmr_type=-1, qmrmax=3000, qmrtol=1e-10)
+    dt = time.perf_counter() - t0
+    out["resonance_24x32x24"] = {"setup_seconds": round(setup, 3), "ret": int(r), "seconds": round(dt, 3),
+                                 "energies": [[float(z.real), float(z.imag)] for z in eig],
+                                 "qmr_iters": int(st[0]), "solves": int(st[1]),
+                                 "us_per_qmr_iteration": round(dt * 1e6 / max(1, int(st[0])), 1)}
     ctx.close()
     return out
 
