@@ -18,9 +18,9 @@
 // is a batched complex matrix-vector product: one thread per output point
 // (d, i, b), its row of H0CX read by every thread of the same i (one load
 // for a wave walking d), its X column walking d with unit stride.  Every
-// level reads and writes Y once: 16 (2 + n_l) bytes per point, so a level is
-// HBM-bound for n_l below about 20 and otherwise bound by its 8 n_l FLOP per
-// point.
+// level reads and writes Y once; with the level's X columns staged in LDS
+// (levels of up to 256 points) X is read once too: 48 bytes and 8 n_l FLOP
+// per point and vector.
 #include "scalit_impl.hpp"
 #include "solver_state.hpp"
 
@@ -76,6 +76,59 @@ __global__ void __launch_bounds__(256) k_cxh_level(int nin, int n, int blk, cons
         }
         yr[idx] += ar;
         yi[idx] += ai;
+    }
+}
+
+// The same with the X columns staged in LDS.  Column c = (d, b) of a level
+// is the n-vector x(d, ., b) (stride nin); a workgroup stages XC columns
+// (XC consecutive d of one b, or XC consecutive b when nin = 1) once, and its
+// threads, XC apart in the column and 256 / XC apart in the row, form every
+// output of those columns from LDS: each X value is read from HBM once per
+// level instead of once per output row.
+constexpr int XC = 16;
+__global__ void __launch_bounds__(256) k_cxh_level_lds(int nin, int n, int blk, const dcx *__restrict__ h,
+                                                       const dcx *__restrict__ dep, long N,
+                                                       const double *__restrict__ x, double *__restrict__ y)
+{
+    extern __shared__ char smem_raw[];
+    dcx *xt = (dcx *)smem_raw;   // xt[j XC + cc]
+    const int v = blockIdx.y;
+    const double *xr = x + 2L * v * N, *xi = xr + N;
+    double *yr = y + 2L * v * N, *yi = yr + N;
+    const long ncol = (long)nin * blk, c0 = (long)blockIdx.x * XC;
+    for (int e = threadIdx.x; e < n * XC; e += 256) {
+        // nin = 1: a column is contiguous, so walk it first
+        const int cc = nin == 1 ? e / n : e % XC, j = nin == 1 ? e % n : e / XC;
+        const long c = c0 + cc;
+        dcx val{0.0, 0.0};
+        if (c < ncol) {
+            const long off = (c % nin) + (long)nin * (j + (long)n * (c / nin));
+            val = {xr[off], xi[off]};
+        }
+        xt[j * XC + cc] = val;
+    }
+    __syncthreads();
+    constexpr int NG = 256 / XC;
+    const int cc = threadIdx.x % XC, ig = threadIdx.x / XC;
+    const long c = c0 + cc;
+    if (c >= ncol) return;   // no barrier follows
+    const long d = c % nin, b = c / nin;
+    for (int i = ig; i < n; i += NG) {
+        double ar = 0.0, ai = 0.0;
+        for (int j = 0; j < n; j++) {
+            const dcx hv = h[i + (long)n * j], xv = xt[j * XC + cc];
+            ar += hv.re * xv.re - hv.im * xv.im;
+            ai += hv.re * xv.im + hv.im * xv.re;
+        }
+        if (dep) {
+            const dcx dv = dep[d];
+            const double r = dv.re * ar - dv.im * ai;
+            ai = dv.re * ai + dv.im * ar;
+            ar = r;
+        }
+        const long off = d + (long)nin * (i + (long)n * b);
+        yr[off] += ar;
+        yi[off] += ai;
     }
 }
 
@@ -141,6 +194,11 @@ int hx_apply_cxh(scalit_ctx *c, int ntype, int nvec, const double *x, double *y,
         if (c->ndvr && l == c->sF - 1)
             hipLaunchKernelGGL(k_cxh_outh, dim3(cxh_grid((long)L.dim * L.n), nvec), dim3(256), 0, s, L.dim, L.n,
                                (const dcx *)c->outhcx.p, N, x, y);
+        else if (L.n <= 256)
+            hipLaunchKernelGGL(k_cxh_level_lds,
+                               dim3((unsigned)(((long)L.dim * L.blk + XC - 1) / XC), nvec), dim3(256),
+                               sizeof(dcx) * (size_t)XC * L.n, s, L.dim, L.n, L.blk, (const dcx *)L.h0cx.p,
+                               L.dep ? (const dcx *)L.depcx.p : nullptr, N, x, y);
         else
             hipLaunchKernelGGL(k_cxh_level, dim3(cxh_grid(N), nvec), dim3(256), 0, s, L.dim, L.n, L.blk,
                                (const dcx *)L.h0cx.p, L.dep ? (const dcx *)L.depcx.p : nullptr, N, x, y);
