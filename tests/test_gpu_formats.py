@@ -287,3 +287,41 @@ def test_complex_level_jobs_from_input_files(tmp_path):
                    hx=lambda op, u, nt, a: op.hx(u, nt, a))
     print("sCX crp job", crp, ref, flush=True)
     assert n == 3 and np.abs(crp - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def test_complex_levels_ndvr_operator_from_files(tmp_path):
+    """An sCX = T operator with the sNDVR layer from files (H0CX of levels 1
+    and 2 as one sequential record, OUTHCX and RES as direct records):
+    MYHX_CX of the loaded context equals the oracle's OperatorCX to 1e-12 for
+    three nTypes, and its real H*X is refused (no real level matrices)."""
+    from scalit_amd import _lib as L
+
+    prob = P.build_problem(10, 8, 6, P.gaussian_potential(3), seed=3)
+    outh = P.outer_layer(prob, 9)
+    h0c, depc, outhc = P.complex_levels(prob, 0.05, 1e-3, 11, outh)
+    sdep = [False, False, False]
+    lay = Context.layout(prob.sN, sdep, True)
+    h0 = np.zeros(lay["h0_len"], complex)
+    for l in range(2):
+        h0[lay["h0_start"][l]:lay["h0_start"][l] + h0c[l].size] = h0c[l].reshape(-1, order="F")
+    files = {"h0_file": str(tmp_path / "h0cx.dat"), "res_file": str(tmp_path / "res.dat"),
+             "outh_file": str(tmp_path / "outhcx.dat")}
+    Context.save_file(files["h0_file"], _pairs(h0), "seq")
+    Context.save_file(files["res_file"], prob.res, "dir")
+    Context.save_file(files["outh_file"], _pairs(outhc.reshape(-1)), "dir")
+    ctx = Context.from_files_cx(prob.sN, sdep, **files)
+    opc = O.OperatorCX(prob.sN, h0c, prob.res, [None, None, None], outh=outhc)
+    e0 = float(np.sort(prob.res)[prob.N // 4])
+    ctx.set_shift(e0)
+    opc.set_shift(e0)
+    aps = (P.absorbing_potential(prob, 0.02), P.absorbing_potential(prob, 0.01, 0.5), np.zeros(prob.N))
+    for k, a in enumerate(aps):
+        ctx.set_ap(k, a)
+    x = P.random_vectors(prob.N, 1, 5)[0] + 1j * P.random_vectors(prob.N, 1, 6)[0]
+    for nt in (L.TA0, L.TMAP, L.TAAPP):
+        y = ctx.from_device_cx(ctx.hx_cx(ctx.to_device_cx(x), nt))[0]
+        ref = opc.hx(x, nt, aps)
+        assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max(), nt
+    with pytest.raises(L.ScalitError):
+        ctx.hx(ctx.to_device(np.atleast_2d(x.real)))
+    ctx.close()
