@@ -57,6 +57,12 @@ def test_null_context_is_rejected():
     assert lib.scalit_destroy(None) == L.SCALIT_EINVAL
     n = C.c_long()
     assert lib.scalit_get_len(None, C.byref(n)) == L.SCALIT_EINVAL
+    # complex level matrices (sCX): no context, no data
+    z = (C.c_double * 4)()
+    assert lib.scalit_set_h0_cx(None, 1, z, z) == L.SCALIT_EINVAL
+    assert lib.scalit_set_dep_cx(None, 1, z, z) == L.SCALIT_EINVAL
+    assert lib.scalit_set_outh_cx(None, z, z) == L.SCALIT_EINVAL
+    assert lib.scalit_set_scx(None, 1) == L.SCALIT_EINVAL
 
 
 def test_fortran_layer_without_bound_context():
